@@ -1,0 +1,204 @@
+/*
+ * wavi_hip.h - C ABI of libwavi_hip.so, the MI355X (gfx950) velocity solve of
+ * WAVI.jl.
+ *
+ * The library replaces, for one model, the reference's dispatch hooks on
+ * `parallel_spec` (WAVI.jl src/WAVI.jl:17-18, src/Models/Model.jl:1-9):
+ *   update_velocities!(model)            src/Models/update_velocities.jl:11-29
+ *   precondition!(model, spec)           src/Models/update_velocities.jl:48-83
+ *   update_preconditioner!(model, spec)  src/SharedMemorySpec.jl:31
+ * A host binding (Julia `ccall`, Python `ctypes`) keeps the reference's
+ * Model / Simulation objects and moves field data through this interface.
+ *
+ * Conventions
+ *  - Every array is Float64 (or uint8 for masks), column-major with the
+ *    x index fastest: element (i, j) of an nx-by-ny field is at i + nx*j.
+ *    Sigma-grid fields are (nx, ny, nsigma) with the sigma index slowest.
+ *  - Host pointers are borrowed for the duration of the call only.  Device
+ *    memory is owned by the context.
+ *  - Return codes: 0 = success; negative = invalid argument (WAVI_EARG, the
+ *    reference's ArgumentError) or size mismatch (WAVI_EDIM, the reference's
+ *    DimensionMismatch) or invalid state (WAVI_ESTATE); positive = a HIP
+ *    runtime error (WAVI_EHIP).  wavi_last_error() gives the message.
+ *  - Non-convergence of Picard or coarse CG is reported in wavi_solve_report,
+ *    not as an error, as in the reference (update_velocities.jl:20).
+ *  - A context is not re-entrant: use one context per host thread / task.
+ */
+#ifndef WAVI_HIP_H
+#define WAVI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WAVI_OK 0
+#define WAVI_EARG (-1)
+#define WAVI_EDIM (-2)
+#define WAVI_ESTATE (-3)
+#define WAVI_EHIP 1
+
+/* Precision of the solve (argument of wavi_ctx_create). */
+#define WAVI_PRECISION_FP64 0   /* plain FP64: the reference scheme, bit-reproducible */
+#define WAVI_PRECISION_MIXED 1  /* FP32 V-cycle inside FP64 residual refinement */
+
+/* Grid: the reference's Grid(;...) keyword arguments (src/Grid.jl:66-82).
+ * Masks are 0/1 bytes.  u_iszero/v_iszero remove points from the u/v masks
+ * (homogeneous Dirichlet); u_isfixed/v_isfixed mark non-homogeneous Dirichlet
+ * points whose values are taken from the u/v fields (src/Fields/Fields.jl:34-35,
+ * src/Fields/UGrid.jl:76). */
+typedef struct wavi_grid_desc {
+    int32_t nx, ny, nsigma;
+    double dx, dy;
+    const uint8_t *h_mask;      /* nx*ny */
+    const uint8_t *h_isfixed;   /* nx*ny (may be NULL: all false) */
+    const uint8_t *u_iszero;    /* (nx+1)*ny (may be NULL) */
+    const uint8_t *v_iszero;    /* nx*(ny+1) (may be NULL) */
+    const uint8_t *u_isfixed;   /* (nx+1)*ny (may be NULL) */
+    const uint8_t *v_isfixed;   /* nx*(ny+1) (may be NULL) */
+    const double *sigma;        /* nsigma, sigma[0]=0, sigma[nsigma-1]=1 */
+    const double *quadrature_weights; /* nsigma */
+} wavi_grid_desc;
+
+/* Physical parameters used by the solve (src/Params.jl:1-25). */
+typedef struct wavi_phys_params {
+    double dt;                    /* timestep injected by Simulation (Simulation.jl:56-59) */
+    double g;
+    double density_ice;
+    double density_ocean;
+    double glen_n;
+    double glen_reg_strain_rate;
+    double weertman_m;
+    double weertman_reg_speed;
+    double sea_level_wrt_geoid;
+    double minimum_thickness;
+    double uniform_melt_rate;     /* UniformMeltRate(m) (src/MeltRate/MeltRate.jl:37-53) */
+    int32_t evolve_shelves;       /* only 1 is supported (run_simulation.jl:30) */
+} wavi_phys_params;
+
+/* Solver parameters (src/SolverParams.jl:2-14). */
+typedef struct wavi_solver_params {
+    int32_t n_iter_viscosity;     /* must be 2 (SolverParams.jl:3) */
+    double tol_picard;
+    int32_t maxiter_picard;
+    double tol_coarse;
+    int32_t maxiter_coarse;
+    int32_t levels;               /* Haar levels: blocks of 2^(levels+1), 0..4 */
+    double wavelet_threshold;
+    int32_t nsmooth;
+    double smoother_omega;
+    int32_t stencil_margin;       /* colours used to probe the operator diagonal */
+    double super_implicitness;
+} wavi_solver_params;
+
+#define WAVI_REPORT_MAX_PICARD 64
+
+/* Outcome of one update_velocities!. */
+typedef struct wavi_solve_report {
+    int32_t picard_iters;         /* i_picard at loop exit */
+    int32_t converged;            /* rel_resid < tol_picard */
+    double rel_resid;             /* last ||b - A x|| / ||b|| */
+    int32_t coarse_cg_total;      /* sum of coarse CG iterations */
+    int32_t refinements;          /* mixed precision: FP64 residual refinements */
+    int32_t coarse_cg_iters[WAVI_REPORT_MAX_PICARD]; /* per V-cycle (first 64) */
+    double rel_resid_hist[WAVI_REPORT_MAX_PICARD];   /* per Picard iteration */
+} wavi_solve_report;
+
+/* Field identifiers: the arrays of the reference's Fields structs
+ * (src/Fields/HGrid.jl:1-37, UGrid.jl:1-29, VGrid.jl, SigmaGrid.jl:98-109,
+ * src/Wavelets/UWavelets.jl:2-15) plus the two per-cell parameter arrays the
+ * state update reads (Params weertman_c and accumulation_rate, Model.jl:63-74). */
+enum wavi_field {
+    /* h-grid, nx*ny */
+    WAVI_H_B = 0, WAVI_H_H, WAVI_H_S, WAVI_H_DHDT, WAVI_H_ACCUMULATION,
+    WAVI_H_BASAL_MELT, WAVI_H_HAF, WAVI_H_GROUNDED_FRACTION, WAVI_H_DSDH,
+    WAVI_H_SHELF_STRAIN_RATE, WAVI_H_AV_SPEED, WAVI_H_U, WAVI_H_V, WAVI_H_US,
+    WAVI_H_VS, WAVI_H_UB, WAVI_H_VB, WAVI_H_BED_SPEED, WAVI_H_WEERTMAN_C,
+    WAVI_H_BETA, WAVI_H_BETA_EFF, WAVI_H_TAU_BED, WAVI_H_ETA_AV,
+    WAVI_H_QUAD_F1, WAVI_H_QUAD_F2,
+    WAVI_P_WEERTMAN_C, WAVI_P_ACCUMULATION_RATE,
+    /* u-grid, (nx+1)*ny */
+    WAVI_U_S, WAVI_U_H, WAVI_U_GROUNDED_FRACTION, WAVI_U_BETA_EFF, WAVI_U_U,
+    WAVI_U_RESIDUAL,
+    /* v-grid, nx*(ny+1) */
+    WAVI_V_S, WAVI_V_H, WAVI_V_GROUNDED_FRACTION, WAVI_V_BETA_EFF, WAVI_V_V,
+    WAVI_V_RESIDUAL,
+    /* sigma grid, nx*ny*nsigma */
+    WAVI_S_ETA, WAVI_S_GLEN_B,
+    /* wavelet grids (same shapes as u and v grids); masks as 0.0/1.0 and the
+     * coarse warm start in full (uncompacted) layout, zero off the mask */
+    WAVI_WU_WAVELETS, WAVI_WU_MASK, WAVI_WU_CORRECTION_COARSE,
+    WAVI_WV_WAVELETS, WAVI_WV_MASK, WAVI_WV_CORRECTION_COARSE,
+    WAVI_NFIELDS
+};
+
+typedef struct wavi_ctx wavi_ctx;
+
+/* Context life cycle.  Creating the context builds the masks and operators of
+ * setup_fields (src/Fields/Fields.jl:26-115) on the device and initialises
+ * the fields as the HGrid/UGrid/VGrid/UWavelets constructors do
+ * (all-true wavelet masks, quad_f2 = h/(3*eta_av) once h and eta_av are set;
+ * use wavi_init_quad_f2). Corresponds to Model(;...) + update_preconditioner!. */
+int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys,
+                    const wavi_solver_params *solver, int precision_mode,
+                    wavi_ctx **out);
+int wavi_ctx_destroy(wavi_ctx *ctx);
+const char *wavi_last_error(const wavi_ctx *ctx);
+int wavi_set_dt(wavi_ctx *ctx, double dt);
+
+/* Field transfer (host <-> device).  n is the element count of the field. */
+int wavi_field_size(const wavi_ctx *ctx, int field, size_t *n);
+int wavi_set_field(wavi_ctx *ctx, int field, const double *host, size_t n);
+int wavi_get_field(wavi_ctx *ctx, int field, double *host, size_t n);
+/* quad_f2 = h/(3 eta_av) on the h mask (src/Fields/HGrid.jl:229). */
+int wavi_init_quad_f2(wavi_ctx *ctx);
+
+/* Save / restore the state a velocity solve mutates (u, v, eta, beta,
+ * quad_f2, warm start, wavelet masks, ...), device to device.  Used to time
+ * repeated solves from one reset state. */
+int wavi_state_save(wavi_ctx *ctx);
+int wavi_state_restore(wavi_ctx *ctx);
+
+/* update_velocities!(model) (src/Models/update_velocities.jl:11-29):
+ * Picard loop of inner_update! + precondition! on device-resident fields. */
+int wavi_update_velocities(wavi_ctx *ctx, wavi_solve_report *report);
+
+/* inner_update!(model) (update_velocities.jl:32-45). */
+int wavi_inner_update(wavi_ctx *ctx);
+
+/* update_state!(model[, clock]) (src/Models/update_state.jl:7-42) with
+ * UniformMeltRate, including update_wavelets!. */
+int wavi_update_state(wavi_ctx *ctx, wavi_solve_report *report);
+/* update_thickness!(simulation) (src/Simulations/run_simulation.jl:24-42). */
+int wavi_update_thickness(wavi_ctx *ctx, double dt);
+/* nsteps of timestep!: update_state! then update_thickness! (run_simulation.jl:6-17). */
+int wavi_run_steps(wavi_ctx *ctx, int nsteps, wavi_solve_report *last_report);
+/* update_wavelets!(model) (src/Wavelets/update_wavelets.jl:6-32). */
+int wavi_update_wavelets(wavi_ctx *ctx);
+
+/* ---- test hooks: single pieces of the solve on host arrays ---- */
+/* op_fun! (src/utilities.jl:61-145) on full-grid inputs xu ((nx+1)*ny) and
+ * xv (nx*(ny+1)); the output holds the operator at inner points and zero
+ * elsewhere.  sampled=1 masks the inputs to inner points first (the
+ * vecSampled=true path); sampled=0 uses them as given (vecSampled=false). */
+int wavi_op_apply(wavi_ctx *ctx, const double *xu, const double *xv,
+                  double *yu, double *yv, int sampled);
+/* get_rhs (update_velocities.jl:106-182), full layout, zero off inner points. */
+int wavi_rhs(wavi_ctx *ctx, double *bu, double *bv);
+/* get_op_diag (src/Wavelets/update_preconditioners.jl:92-109). */
+int wavi_op_diag(wavi_ctx *ctx, double *du, double *dv);
+/* One V-cycle precondition!(x, p, b) (update_preconditioners.jl:54-83) with
+ * x starting at zero; uses and updates the coarse warm start. */
+int wavi_vcycle(wavi_ctx *ctx, const double *ru, const double *rv,
+                double *eu, double *ev, int32_t *coarse_iters);
+/* 2-D Haar transform of one component (comp 0 = u grid, 1 = v grid).
+ * dir 0: forward dwt (UGrid.jl:89); 1: inverse idwt (UWavelets.jl:48);
+ * 2: transposed inverse idwt^T (UWavelets.jl:49). */
+int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WAVI_HIP_H */
