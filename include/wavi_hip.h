@@ -171,6 +171,10 @@ int wavi_inner_update(wavi_ctx *ctx);
 /* update_state!(model[, clock]) (src/Models/update_state.jl:7-42) with
  * UniformMeltRate, including update_wavelets!. */
 int wavi_update_state(wavi_ctx *ctx, wavi_solve_report *report);
+/* The update_state! steps that precede the velocity solve: surface, geometry
+ * on u/v, height above floatation, grounded fractions, accumulation, melt,
+ * weertman_c, dsdh (update_state.jl:8-15). */
+int wavi_update_geometry(wavi_ctx *ctx);
 /* update_thickness!(simulation) (src/Simulations/run_simulation.jl:24-42). */
 int wavi_update_thickness(wavi_ctx *ctx, double dt);
 /* nsteps of timestep!: update_state! then update_thickness! (run_simulation.jl:6-17). */
@@ -197,6 +201,11 @@ int wavi_vcycle(wavi_ctx *ctx, const double *ru, const double *rv,
  * dir 0: forward dwt (UGrid.jl:89); 1: inverse idwt (UWavelets.jl:48);
  * 2: transposed inverse idwt^T (UWavelets.jl:49). */
 int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out);
+
+/* Device arithmetic probe on host arrays of n elements: fn 0 out = x^y by the
+ * library's deterministic power function, 1 out = sqrt(x), 2 out = x / y.
+ * No reference counterpart: it checks that the device rounds as the CPU does. */
+int wavi_math_probe(int fn, const double *x, const double *y, double *out, size_t n);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,7 @@ def load():
         "om_precondition": (ctypes.c_int, [vp, _dp, ctypes.POINTER(ctypes.c_int)]),
         "om_update_velocities": (ctypes.c_int, [vp, vp]),
         "om_update_state": (None, [vp, vp]),
+        "om_update_geometry": (None, [vp]),
         "om_update_thickness": (None, [vp, ctypes.c_double]),
         "om_run_steps": (None, [vp, ctypes.c_int, vp]),
         "om_update_wavelets": (None, [vp]),
@@ -208,6 +209,9 @@ class OracleModel:
         rep = self._pkg._lib.SolveReport()
         self._lib.om_update_state(self.h, ctypes.addressof(rep))
         return rep.as_dict()
+
+    def update_geometry(self):
+        self._lib.om_update_geometry(self.h)
 
     def update_thickness(self, dt):
         self._lib.om_update_thickness(self.h, dt)

@@ -1158,8 +1158,8 @@ EXPORT void om_update_wavelets(om_model *m)
     free(tu); free(tv);
 }
 
-/* update_state!(model, clock) (update_state.jl:7-21) with UniformMeltRate. */
-EXPORT void om_update_state(om_model *m, wavi_solve_report *rep)
+/* update_state! steps before the velocity solve (update_state.jl:8-15). */
+EXPORT void om_update_geometry(om_model *m)
 {
     const int nx = m->nx, ny = m->ny, nh = m->nh;
     const double ri = m->P.density_ice, rw = m->P.density_ocean, sl = m->P.sea_level_wrt_geoid;
@@ -1201,6 +1201,13 @@ EXPORT void om_update_state(om_model *m, wavi_solve_report *rep)
         m->f[WAVI_H_WEERTMAN_C][k] = m->f[WAVI_P_WEERTMAN_C][k] * m->f[WAVI_H_GROUNDED_FRACTION][k];
         m->f[WAVI_H_DSDH][k] = (1.0 - ri / rw) + (ri / rw) * m->f[WAVI_H_GROUNDED_FRACTION][k];
     }
+}
+
+/* update_state!(model, clock) (update_state.jl:7-21) with UniformMeltRate. */
+EXPORT void om_update_state(om_model *m, wavi_solve_report *rep)
+{
+    const int nx = m->nx, ny = m->ny;
+    om_update_geometry(m);
     om_update_velocities(m, rep);
     /* update_velocities_on_h_grid! (:164-178) */
     const double *U = m->f[WAVI_U_U], *V = m->f[WAVI_V_V];

@@ -1,0 +1,51 @@
+"""Model builders shared by the tests (small grids the oracle runs in
+seconds)."""
+import numpy as np
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def channel(W, nx=48, ny=12, precision="fp64", seed=3, maxiter_picard=3, tol_coarse=1e-8,
+            maxiter_coarse=1000, nσ=4):
+    """A channel single-solve setup on a small grid."""
+    m = W.workloads.channel_solve(nx=nx, ny=ny, precision=precision, seed=seed, nσ=nσ,
+                                  maxiter_picard=maxiter_picard, tol_coarse=tol_coarse)
+    m.solver_params.maxiter_coarse = maxiter_coarse
+    return m
+
+
+def iceberg(W, precision="fp64", nx=20, ny=20):
+    """The reference's iceberg verification setup
+    (test/verification_tests/iceberg_test.jl:50-121): circular shelf with
+    non-homogeneous Dirichlet points on the centre lines."""
+    x0 = y0 = -50000.0
+    dx = dy = 5000.0
+    u_isfixed = np.zeros((nx + 1, ny), bool)
+    u_isfixed[nx // 2, :] = True
+    v_isfixed = np.zeros((nx, ny + 1), bool)
+    v_isfixed[:, ny // 2] = True
+    grid = W.Grid(nx=nx, ny=ny, nσ=4, x0=x0, y0=y0, dx=dx, dy=dy, u_isfixed=u_isfixed, v_isfixed=v_isfixed)
+    h_mask = np.sqrt(grid.xxh ** 2 + grid.yyh ** 2) < 45000
+    grid = W.Grid(nx=nx, ny=ny, nσ=4, x0=x0, y0=y0, dx=dx, dy=dy, u_isfixed=u_isfixed,
+                  v_isfixed=v_isfixed, h_mask=h_mask)
+    thick = np.zeros((nx, ny))
+    thick[h_mask] = 200.0
+    u_drift, v_drift = 5.0, 3.0
+    rate = 2 * np.pi / 10000.0
+    solid_u = u_drift * np.ones((nx + 1, ny)) - rate * grid.yyu
+    solid_v = v_drift * np.ones((nx, ny + 1)) + rate * grid.xxv
+    u_bc = np.zeros((nx + 1, ny))
+    v_bc = np.zeros((nx, ny + 1))
+    u_bc[u_isfixed] = solid_u[u_isfixed]
+    v_bc[v_isfixed] = solid_v[v_isfixed]
+    ic = W.InitialConditions(initial_thickness=thick, initial_u_veloc=u_bc, initial_v_veloc=v_bc)
+    params = W.Params(accumulation_rate=0.3, default_temperature=265.700709)
+    sp = W.SolverParams(maxiter_picard=1)
+    model = W.Model(grid=grid, bed_elevation=-500.0 * np.ones((nx, ny)), params=params,
+                    solver_params=sp, initial_conditions=ic, parallel_spec=W.HIPSpec(precision))
+    return model, solid_u, solid_v

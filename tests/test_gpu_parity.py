@@ -1,0 +1,213 @@
+"""Parity of the HIP library (FP64 path) with the CPU oracle.
+
+The FP64 path evaluates every expression in the oracle's order, reduces with
+the same blocked tree and uses the same deterministic power function, so the
+bar here is bit equality (an ulp-level difference would show up as a failure
+and point at the kernel that broke the order)."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import channel, iceberg, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _step(name):
+    print("[gpu] %s" % name, flush=True)
+    sys.stdout.flush()
+
+
+def _gpu_model(W, model):
+    W.update_preconditioner(model)
+    W.push_fields(model)
+    return model.parallel_spec
+
+
+def _assert_fields_equal(W, model, o, names=None, rtol=0.0):
+    W.pull_fields(model)
+    bad = []
+    for cont, attr, fid in W.model.FIELD_MAP:
+        if names is not None and fid not in names:
+            continue
+        a = np.asarray(model._array(cont, attr), dtype=float)
+        b = o.field(fid).reshape(a.shape, order="F")
+        if rtol == 0.0:
+            ok = np.array_equal(a, b, equal_nan=True)
+        else:
+            ok = rel_l2(a, b) <= rtol
+        if not ok:
+            d = np.abs(a - b)
+            bad.append("%s max|diff| %.3e rel %.3e" % (fid, np.nanmax(d), rel_l2(a, b)))
+    assert not bad, bad
+
+
+def _rand_velocities(model, seed, scale=100.0):
+    rng = np.random.default_rng(seed)
+    f = model.fields
+    f.gu.u[:] = scale * (1.0 + rng.standard_normal(f.gu.u.shape))
+    f.gv.v[:] = 0.2 * scale * rng.standard_normal(f.gv.v.shape)
+
+
+def test_device_math_matches_cpu(W, oracle):
+    _step("math probe")
+    lib = W._lib.load()
+    rng = np.random.default_rng(0)
+    n = 50000
+    x = np.exp(rng.uniform(-30, 30, n))
+    y = rng.choice([-2.0 / 3.0, -0.5, 1.0 / 3.0 - 1.0, 0.0, 2.0], n)
+    out = np.empty(n)
+    assert lib.wavi_math_probe(0, W._lib.dptr(x), W._lib.dptr(y), W._lib.dptr(out), n) == 0
+    ref = np.array([oracle.det_pow(a, b) for a, b in zip(x, y)])
+    assert np.array_equal(out, ref)
+    assert lib.wavi_math_probe(1, W._lib.dptr(x), None, W._lib.dptr(out), n) == 0
+    assert np.array_equal(out, np.sqrt(x))
+    z = rng.standard_normal(n)
+    assert lib.wavi_math_probe(2, W._lib.dptr(x), W._lib.dptr(z), W._lib.dptr(out), n) == 0
+    assert np.array_equal(out, x / z)
+
+
+@pytest.mark.parametrize("shape", [(37, 21), (48, 12), (64, 32)])
+def test_haar_transforms(W, oracle, shape):
+    nx, ny = shape
+    _step("haar %s" % (shape,))
+    m = channel(W, nx=nx, ny=ny)
+    o = oracle.OracleModel(m)
+    _gpu_model(W, m)
+    rng = np.random.default_rng(5)
+    for comp, sh in (("u", (nx + 1, ny)), ("v", (nx, ny + 1))):
+        x = rng.standard_normal(sh)
+        for d in ("forward", "inverse", "inverse_t"):
+            assert np.array_equal(W.haar(m, comp, d, x), o.haar(comp, d, x)), (comp, d)
+
+
+@pytest.mark.parametrize("shape", [(48, 12), (33, 17)])
+def test_geometry_and_inner_update(W, oracle, shape):
+    nx, ny = shape
+    _step("inner update %s" % (shape,))
+    m = channel(W, nx=nx, ny=ny)
+    _rand_velocities(m, 7)
+    o = oracle.OracleModel(m)
+    spec = _gpu_model(W, m)
+    lib = spec._lib
+    assert lib.wavi_update_geometry(spec.ctx) == 0
+    o.update_geometry()
+    _assert_fields_equal(W, m, o)
+    assert lib.wavi_inner_update(spec.ctx) == 0
+    o.inner_update()
+    _assert_fields_equal(W, m, o)
+
+
+def test_operator_rhs_diag(W, oracle):
+    _step("operator/rhs/diag")
+    m, _, _ = iceberg(W)
+    m.params.dt = 0.1
+    _rand_velocities(m, 8, scale=5.0)
+    o = oracle.OracleModel(m)
+    spec = _gpu_model(W, m)
+    lib = spec._lib
+    assert lib.wavi_update_geometry(spec.ctx) == 0
+    assert lib.wavi_inner_update(spec.ctx) == 0
+    o.update_geometry()
+    o.inner_update()
+    rng = np.random.default_rng(9)
+    nx, ny = m.grid.nx, m.grid.ny
+    for sampled in (True, False):
+        xu = rng.standard_normal((nx + 1, ny))
+        xv = rng.standard_normal((nx, ny + 1))
+        yu, yv = W.op_apply(m, xu, xv, sampled=sampled)
+        if sampled:
+            mk = o.masks()
+            xu = np.where(mk["u_inner"], xu, 0.0)
+            xv = np.where(mk["v_inner"], xv, 0.0)
+        ou, ov = o.op_apply(xu, xv)
+        assert np.array_equal(yu, ou) and np.array_equal(yv, ov)
+    bu, bv = W.get_rhs(m)
+    obu, obv = o.rhs()
+    assert np.array_equal(bu, obu) and np.array_equal(bv, obv)
+    du, dv = W.get_op_diag(m)
+    odu, odv = o.op_diag()
+    assert np.array_equal(du, odu) and np.array_equal(dv, odv)
+
+
+def test_vcycle(W, oracle):
+    _step("vcycle")
+    m = channel(W, nx=40, ny=16)
+    _rand_velocities(m, 10)
+    o = oracle.OracleModel(m)
+    spec = _gpu_model(W, m)
+    lib = spec._lib
+    assert lib.wavi_update_geometry(spec.ctx) == 0
+    assert lib.wavi_inner_update(spec.ctx) == 0
+    o.update_geometry()
+    o.inner_update()
+    bu, bv = o.rhs()
+    for rep in range(2):  # the second V-cycle starts from the cached warm start
+        eu, ev, it = W.vcycle(m, bu, bv)
+        oeu, oev, oit = o.vcycle(bu, bv)
+        assert it == oit
+        assert np.array_equal(eu, oeu) and np.array_equal(ev, oev)
+    _assert_fields_equal(W, m, o, names={"WU_CORRECTION_COARSE", "WV_CORRECTION_COARSE"})
+
+
+@pytest.mark.parametrize("shape,picard", [((48, 12), 4), ((40, 24), 3)])
+def test_update_velocities(W, oracle, shape, picard):
+    nx, ny = shape
+    _step("update_velocities %s" % (shape,))
+    m = channel(W, nx=nx, ny=ny, maxiter_picard=picard)
+    o = oracle.OracleModel(m)
+    spec = _gpu_model(W, m)
+    lib = spec._lib
+    assert lib.wavi_update_geometry(spec.ctx) == 0
+    o.update_geometry()
+    rep = W._lib.SolveReport()
+    assert lib.wavi_update_velocities(spec.ctx, ctypes.byref(rep)) == 0
+    orep = o.update_velocities()
+    assert rep.as_dict()["coarse_cg_iters"] == orep["coarse_cg_iters"]
+    assert rep.as_dict()["rel_resid_hist"] == orep["rel_resid_hist"]
+    _assert_fields_equal(W, m, o)
+
+
+def test_iceberg_steps(W, oracle):
+    _step("iceberg steps")
+    m, _, _ = iceberg(W)
+    m.params.dt = 0.1
+    o = oracle.OracleModel(m)
+    sim = W.Simulation(model=m, timestepping_params=W.TimesteppingParams(dt=0.1, n_iter_total=50))
+    o.set_dt(0.1)
+    W.run_simulation(sim)
+    o.run_steps(50)
+    _assert_fields_equal(W, m, o)
+
+
+def test_mismip_plus_fixture_on_gpu(W, oracle):
+    """The reference's regression run (1000 steps) on the GPU: bit-equal to
+    the oracle and within 1e-6 of the reference's v1.8.3 golden."""
+    _step("mismip+ 1000 steps")
+    import os
+    m = W.workloads.mismip_plus_fixture()
+    sim = W.Simulation(model=m, timestepping_params=W.TimesteppingParams(dt=0.1, end_time=100.0))
+    o = oracle.OracleModel(m)
+    W.run_simulation(sim)
+    o.run_steps(1000)
+    _assert_fields_equal(W, m, o)
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "mismip_plus_v1_8_3.npz"))
+    f = m.fields
+    for name, arr in (("h", f.gh.h), ("u", f.gu.u), ("v", f.gv.v), ("viscosity", f.gh.ηav),
+                      ("grounded_fraction", f.gh.grounded_fraction), ("bed_speed", f.gh.bed_speed)):
+        assert rel_l2(arr, d[name]) < 1e-6, name
+
+
+def test_channel_transient_200_steps(W, oracle):
+    """80x10 channel, dt 0.5, 200 steps: thickness within 1e-9 of the oracle
+    (the FP64 path is bit-equal, so this is the looser of the two bars)."""
+    _step("channel transient 200 steps")
+    m = W.workloads.channel_transient()
+    sim = W.Simulation(model=m, timestepping_params=W.TimesteppingParams(dt=0.5, n_iter_total=200))
+    o = oracle.OracleModel(m)
+    W.run_simulation(sim)
+    o.run_steps(200)
+    assert rel_l2(m.fields.gh.h, o.field("H_H")) <= 1e-9
+    _assert_fields_equal(W, m, o)

@@ -112,6 +112,7 @@ SIGNATURES = {
     "wavi_update_velocities": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveReport)]),
     "wavi_inner_update": (ctypes.c_int, [ctypes.c_void_p]),
     "wavi_update_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveReport)]),
+    "wavi_update_geometry": (ctypes.c_int, [ctypes.c_void_p]),
     "wavi_update_thickness": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "wavi_run_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(SolveReport)]),
     "wavi_update_wavelets": (ctypes.c_int, [ctypes.c_void_p]),
@@ -120,6 +121,7 @@ SIGNATURES = {
     "wavi_op_diag": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "wavi_vcycle": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int32)]),
     "wavi_haar": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "wavi_math_probe": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]),
 }
 
 _LIB = None

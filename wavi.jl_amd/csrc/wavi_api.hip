@@ -1,0 +1,1068 @@
+// wavi_api.hip - C ABI of libwavi_hip.so (include/wavi_hip.h): context,
+// device-resident fields and the orchestration of update_velocities!.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/wavi_hip.h"
+// all kernels of the library live in this translation unit
+#include "wavi_kernels.hip"
+
+using namespace wavi;
+
+static thread_local std::string g_create_error;
+
+struct DevVec {
+    double *u = nullptr, *v = nullptr;
+};
+
+struct wavi_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    GridDims g{};
+    wavi_phys_params P{};
+    wavi_solver_params S{};
+    int precision = 0;
+    int nh = 0, nu = 0, nv = 0, nc = 0, ns3 = 0;
+    double rdx = 0, rdy = 0;
+    // masks (uint8, device)
+    uint8_t *hm = nullptr, *hfix = nullptr, *um = nullptr, *vm = nullptr, *cm = nullptr;
+    uint8_t *ui = nullptr, *vi = nullptr, *ufix = nullptr, *vfix = nullptr, *mwu = nullptr, *mwv = nullptr;
+    // fields (double, device); mask fields use mwu/mwv instead
+    double *f[WAVI_NFIELDS] = {};
+    // sigma-level constants
+    double *zeta = nullptr, *qw = nullptr;
+    // rheological coefficients
+    double *heta_h = nullptr, *heta_c = nullptr, *dimp = nullptr;
+    // Picard / V-cycle vectors (full u/v grids)
+    DevVec x, b, r, e, d, rho, t, f3, fixedv;
+    // coarse CG vectors
+    DevVec cr, cp, cq, ct;
+    double *Pr0 = nullptr, *Pr1 = nullptr, *Pd = nullptr, *Pb = nullptr, *Pn = nullptr;
+    double *quad = nullptr;  // 4 * nh pos_fraction quadrants
+    double *norms = nullptr; // device: 2 doubles
+    double *h_norms = nullptr; // pinned host: 2 doubles
+    CGState *st = nullptr;
+    CGState *h_st = nullptr;  // pinned host copy
+    HaarTable *tab = nullptr;
+    // saved state
+    std::vector<double *> saved;
+    bool has_saved = false;
+    uint8_t *saved_mwu = nullptr, *saved_mwv = nullptr;
+    std::string err;
+};
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                   \
+            return WAVI_EHIP;                                                               \
+        }                                                                                   \
+    } while (0)
+
+#define CHK(expr)                 \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != WAVI_OK) return rc_; \
+    } while (0)
+
+static inline dim3 flat(int n) { return dim3((unsigned)((n + 255) / 256)); }
+
+static int field_len(const wavi_ctx *ctx, int id)
+{
+    if (id < 0 || id >= WAVI_NFIELDS) return -1;
+    if (id <= WAVI_P_ACCUMULATION_RATE) return ctx->nh;
+    if (id <= WAVI_U_RESIDUAL) return ctx->nu;
+    if (id <= WAVI_V_RESIDUAL) return ctx->nv;
+    if (id <= WAVI_S_GLEN_B) return ctx->ns3;
+    if (id <= WAVI_WU_CORRECTION_COARSE) return ctx->nu;
+    return ctx->nv;
+}
+
+static OpCoef<double> coef64(const wavi_ctx *ctx)
+{
+    OpCoef<double> c;
+    c.heta_h = ctx->heta_h;
+    c.heta_c = ctx->heta_c;
+    c.dimp = ctx->dimp;
+    c.hu = ctx->f[WAVI_U_H];
+    c.hv = ctx->f[WAVI_V_H];
+    c.bu = ctx->f[WAVI_U_BETA_EFF];
+    c.bv = ctx->f[WAVI_V_BETA_EFF];
+    c.hm = ctx->hm;
+    c.um = ctx->um;
+    c.vm = ctx->vm;
+    c.cm = ctx->cm;
+    c.ui = ctx->ui;
+    c.vi = ctx->vi;
+    c.nx = ctx->g.nx;
+    c.ny = ctx->g.ny;
+    c.rdx = ctx->rdx;
+    c.rdy = ctx->rdy;
+    return c;
+}
+
+// ---------------------------------------------------------------------------
+// Haar tables from wavelet_matrix (src/Wavelets/update_wavelets.jl:38-89).
+static void wavelet_matrix(int n, int levels, bool forward, std::vector<double> &W)
+{
+    W.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) W[(size_t)i * n + i] = 1.0;
+    int step = 1;
+    std::vector<double> tmp(n);
+    for (int jl = levels; jl >= 0; --jl) {
+        step *= 2;
+        int h = step / 2;
+        for (int o = 0; o + 1 <= n - h; o += step) {
+            int e = o + h;
+            if (forward) {
+                for (int c = 0; c < n; ++c) {
+                    double ro = W[(size_t)o * n + c], re = W[(size_t)e * n + c];
+                    tmp[c] = 0.5 * (ro + re);
+                    W[(size_t)e * n + c] = 0.5 * (re - ro);
+                }
+                for (int c = 0; c < n; ++c) W[(size_t)o * n + c] = tmp[c];
+            } else {
+                for (int r = 0; r < n; ++r) {
+                    double co = W[(size_t)r * n + o], ce = W[(size_t)r * n + e];
+                    W[(size_t)r * n + o] = ce + co;
+                    W[(size_t)r * n + e] = ce - co;
+                }
+            }
+        }
+    }
+}
+
+static int build_haar_table(int levels, HaarTable &t, std::string &err)
+{
+    memset(&t, 0, sizeof(t));
+    t.B = 1 << (levels + 1);
+    for (int m = 1; m <= t.B; ++m) {
+        std::vector<double> Wf, Wi;
+        wavelet_matrix(m, levels, true, Wf);
+        wavelet_matrix(m, levels, false, Wi);
+        for (int dir = 0; dir < 3; ++dir)
+            for (int r = 0; r < m; ++r) {
+                int n = 0;
+                for (int c = 0; c < m; ++c) {
+                    double w = dir == 0 ? Wf[(size_t)r * m + c] : dir == 1 ? Wi[(size_t)r * m + c] : Wi[(size_t)c * m + r];
+                    if (w != 0.0) {
+                        if (n >= MAX_HAAR_NNZ) {
+                            err = "Haar table overflow";
+                            return WAVI_EARG;
+                        }
+                        t.col[dir][m][r][n] = (int8_t)c;
+                        t.coef[dir][m][r][n] = (float)w;
+                        if ((double)t.coef[dir][m][r][n] != w) {
+                            err = "Haar coefficient not exact in float";
+                            return WAVI_EARG;
+                        }
+                        ++n;
+                    }
+                }
+                t.nnz[dir][m][r] = (int8_t)n;
+            }
+    }
+    return WAVI_OK;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" {
+#pragma GCC visibility push(default)
+
+const char *wavi_last_error(const wavi_ctx *ctx)
+{
+    if (!ctx) return g_create_error.c_str();
+    return ctx->err.c_str();
+}
+
+int wavi_ctx_destroy(wavi_ctx *ctx);
+
+int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, const wavi_solver_params *solver,
+                    int precision_mode, wavi_ctx **out)
+{
+    g_create_error.clear();
+    if (!grid || !phys || !solver || !out) {
+        g_create_error = "null argument";
+        return WAVI_EARG;
+    }
+    *out = nullptr;
+    if (grid->nx <= 1 || grid->ny <= 1 || grid->nsigma <= 1) {
+        g_create_error = "nx, ny and nsigma must be integers larger than one";
+        return WAVI_EARG;
+    }
+    if (!grid->h_mask || !grid->sigma || !grid->quadrature_weights) {
+        g_create_error = "h_mask, sigma and quadrature_weights are required";
+        return WAVI_EARG;
+    }
+    if (!(grid->dx > 0) || !(grid->dy > 0)) {
+        g_create_error = "dx and dy must be positive";
+        return WAVI_EARG;
+    }
+    if (solver->n_iter_viscosity != 2) {
+        g_create_error = "n_iter_viscosity must be 2";
+        return WAVI_EARG;
+    }
+    if (solver->levels < 0 || solver->levels > 3) {
+        g_create_error = "levels must be in 0..3 (Haar blocks of at most 16)";
+        return WAVI_EARG;
+    }
+    if (solver->nsmooth < 0 || solver->maxiter_picard < 0 || solver->maxiter_coarse < 0) {
+        g_create_error = "iteration counts must be non-negative";
+        return WAVI_EARG;
+    }
+    if (precision_mode != WAVI_PRECISION_FP64 && precision_mode != WAVI_PRECISION_MIXED) {
+        g_create_error = "unknown precision mode";
+        return WAVI_EARG;
+    }
+    if (!phys->evolve_shelves) {
+        g_create_error = "evolve_shelves = false is not supported";
+        return WAVI_EARG;
+    }
+    wavi_ctx *ctx = new wavi_ctx();
+    ctx->P = *phys;
+    ctx->S = *solver;
+    ctx->precision = precision_mode;
+    const int nx = grid->nx, ny = grid->ny;
+    GridDims &g = ctx->g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nxu = nx + 1;
+    g.nyu = ny;
+    g.nxv = nx;
+    g.nyv = ny + 1;
+    g.nbxu = (g.nxu + TILE - 1) / TILE;
+    g.nbyu = (g.nyu + TILE - 1) / TILE;
+    g.nbxv = (g.nxv + TILE - 1) / TILE;
+    g.nbyv = (g.nyv + TILE - 1) / TILE;
+    g.nbx = g.nbxu > g.nbxv ? g.nbxu : g.nbxv;
+    g.nby = g.nbyu > g.nbyv ? g.nbyu : g.nbyv;
+    g.npart = g.nbxu * g.nbyu + g.nbxv * g.nbyv;
+    ctx->nh = nx * ny;
+    ctx->nu = (nx + 1) * ny;
+    ctx->nv = nx * (ny + 1);
+    ctx->nc = (nx - 1) * (ny - 1);
+    ctx->ns3 = nx * ny * grid->nsigma;
+    ctx->rdx = 1.0 / grid->dx;
+    ctx->rdy = 1.0 / grid->dy;
+    int ns = grid->nsigma;
+
+    auto fail = [&](int rc) {
+        g_create_error = ctx->err;
+        wavi_ctx_destroy(ctx);
+        return rc;
+    };
+#define CREATECHK(expr)                                                           \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);         \
+            return fail(WAVI_EHIP);                                               \
+        }                                                                         \
+    } while (0)
+    CREATECHK(hipGetDevice(&ctx->device));
+    CREATECHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+
+    // host masks (setup_fields, src/Fields/Fields.jl:28-35)
+    std::vector<uint8_t> hm(ctx->nh), hfix(ctx->nh, 0), um(ctx->nu), vm(ctx->nv), cm(ctx->nc > 0 ? ctx->nc : 1, 0);
+    std::vector<uint8_t> ui(ctx->nu), vi(ctx->nv), ufix(ctx->nu, 0), vfix(ctx->nv, 0), ones_u(ctx->nu, 1), ones_v(ctx->nv, 1);
+    for (int k = 0; k < ctx->nh; ++k) {
+        hm[k] = grid->h_mask[k] ? 1 : 0;
+        hfix[k] = grid->h_isfixed && grid->h_isfixed[k] ? 1 : 0;
+    }
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i <= nx; ++i) {
+            int k = i + (nx + 1) * j;
+            int m = (i < nx && hm[i + nx * j]) || (i > 0 && hm[(i - 1) + nx * j]);
+            if (grid->u_iszero && grid->u_iszero[k]) m = 0;
+            um[k] = (uint8_t)m;
+            ufix[k] = grid->u_isfixed && grid->u_isfixed[k] ? 1 : 0;
+            ui[k] = (uint8_t)(m && !ufix[k]);
+        }
+    for (int j = 0; j <= ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            int k = i + nx * j;
+            int m = (j < ny && hm[i + nx * j]) || (j > 0 && hm[i + nx * (j - 1)]);
+            if (grid->v_iszero && grid->v_iszero[k]) m = 0;
+            vm[k] = (uint8_t)m;
+            vfix[k] = grid->v_isfixed && grid->v_isfixed[k] ? 1 : 0;
+            vi[k] = (uint8_t)(m && !vfix[k]);
+        }
+    for (int j = 0; j < ny - 1; ++j)
+        for (int i = 0; i < nx - 1; ++i)
+            cm[i + (nx - 1) * j] = hm[i + nx * j] && hm[i + nx * (j + 1)] && hm[(i + 1) + nx * j] && hm[(i + 1) + nx * (j + 1)];
+
+    auto alloc8 = [&](uint8_t **p, const std::vector<uint8_t> &src) -> hipError_t {
+        hipError_t e = hipMalloc(p, src.size());
+        if (e != hipSuccess) return e;
+        return hipMemcpy(*p, src.data(), src.size(), hipMemcpyHostToDevice);
+    };
+    CREATECHK(alloc8(&ctx->hm, hm));
+    CREATECHK(alloc8(&ctx->hfix, hfix));
+    CREATECHK(alloc8(&ctx->um, um));
+    CREATECHK(alloc8(&ctx->vm, vm));
+    CREATECHK(alloc8(&ctx->cm, cm));
+    CREATECHK(alloc8(&ctx->ui, ui));
+    CREATECHK(alloc8(&ctx->vi, vi));
+    CREATECHK(alloc8(&ctx->ufix, ufix));
+    CREATECHK(alloc8(&ctx->vfix, vfix));
+    CREATECHK(alloc8(&ctx->mwu, ones_u));  // UWavelets mask = trues (UWavelets.jl:38)
+    CREATECHK(alloc8(&ctx->mwv, ones_v));
+
+    for (int id = 0; id < WAVI_NFIELDS; ++id) {
+        if (id == WAVI_WU_MASK || id == WAVI_WV_MASK) continue;
+        size_t n = (size_t)field_len(ctx, id);
+        CREATECHK(hipMalloc(&ctx->f[id], n * sizeof(double)));
+        CREATECHK(hipMemset(ctx->f[id], 0, n * sizeof(double)));
+    }
+    // constructor defaults: dsdh = 1, gu/gv grounded_fraction = 1 (HGrid.jl:213, UGrid.jl:94)
+    {
+        std::vector<double> one(std::max(ctx->nh, std::max(ctx->nu, ctx->nv)), 1.0);
+        CREATECHK(hipMemcpy(ctx->f[WAVI_H_DSDH], one.data(), ctx->nh * sizeof(double), hipMemcpyHostToDevice));
+        CREATECHK(hipMemcpy(ctx->f[WAVI_U_GROUNDED_FRACTION], one.data(), ctx->nu * sizeof(double), hipMemcpyHostToDevice));
+        CREATECHK(hipMemcpy(ctx->f[WAVI_V_GROUNDED_FRACTION], one.data(), ctx->nv * sizeof(double), hipMemcpyHostToDevice));
+    }
+    std::vector<double> zeta(ns), qw(ns);
+    for (int k = 0; k < ns; ++k) {
+        zeta[k] = 1.0 - grid->sigma[k];  // Grid.jl:170
+        qw[k] = grid->quadrature_weights[k];
+    }
+    CREATECHK(hipMalloc(&ctx->zeta, ns * sizeof(double)));
+    CREATECHK(hipMalloc(&ctx->qw, ns * sizeof(double)));
+    CREATECHK(hipMemcpy(ctx->zeta, zeta.data(), ns * sizeof(double), hipMemcpyHostToDevice));
+    CREATECHK(hipMemcpy(ctx->qw, qw.data(), ns * sizeof(double), hipMemcpyHostToDevice));
+    auto zalloc = [&](double **p, size_t n) -> hipError_t {
+        hipError_t e = hipMalloc(p, n * sizeof(double));
+        if (e != hipSuccess) return e;
+        return hipMemset(*p, 0, n * sizeof(double));
+    };
+    CREATECHK(zalloc(&ctx->heta_h, ctx->nh));
+    CREATECHK(zalloc(&ctx->heta_c, ctx->nc > 0 ? ctx->nc : 1));
+    CREATECHK(zalloc(&ctx->dimp, ctx->nh));
+    for (DevVec *dv : {&ctx->x, &ctx->b, &ctx->r, &ctx->e, &ctx->d, &ctx->rho, &ctx->t, &ctx->f3, &ctx->fixedv,
+                       &ctx->cr, &ctx->cp, &ctx->cq, &ctx->ct}) {
+        CREATECHK(zalloc(&dv->u, ctx->nu));
+        CREATECHK(zalloc(&dv->v, ctx->nv));
+    }
+    CREATECHK(zalloc(&ctx->Pr0, g.npart));
+    CREATECHK(zalloc(&ctx->Pr1, g.npart));
+    CREATECHK(zalloc(&ctx->Pd, g.npart));
+    CREATECHK(zalloc(&ctx->Pb, g.npart));
+    CREATECHK(zalloc(&ctx->Pn, g.npart));
+    CREATECHK(zalloc(&ctx->quad, 4 * (size_t)ctx->nh));
+    CREATECHK(zalloc(&ctx->norms, 2));
+    CREATECHK(hipHostMalloc((void **)&ctx->h_norms, 2 * sizeof(double), hipHostMallocDefault));
+    CREATECHK(hipMalloc(&ctx->st, sizeof(CGState)));
+    CREATECHK(hipMemset(ctx->st, 0, sizeof(CGState)));
+    CREATECHK(hipHostMalloc((void **)&ctx->h_st, sizeof(CGState), hipHostMallocDefault));
+    HaarTable tab;
+    if (build_haar_table(solver->levels, tab, ctx->err) != WAVI_OK) return fail(WAVI_EARG);
+    CREATECHK(hipMalloc(&ctx->tab, sizeof(HaarTable)));
+    CREATECHK(hipMemcpy(ctx->tab, &tab, sizeof(HaarTable), hipMemcpyHostToDevice));
+#undef CREATECHK
+    *out = ctx;
+    return WAVI_OK;
+}
+
+int wavi_ctx_destroy(wavi_ctx *ctx)
+{
+    if (!ctx) return WAVI_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (uint8_t *p : {ctx->hm, ctx->hfix, ctx->um, ctx->vm, ctx->cm, ctx->ui, ctx->vi, ctx->ufix, ctx->vfix,
+                       ctx->mwu, ctx->mwv, ctx->saved_mwu, ctx->saved_mwv})
+        if (p) (void)hipFree(p);
+    for (int id = 0; id < WAVI_NFIELDS; ++id)
+        if (ctx->f[id]) (void)hipFree(ctx->f[id]);
+    for (double *p : ctx->saved)
+        if (p) (void)hipFree(p);
+    for (double *p : {ctx->zeta, ctx->qw, ctx->heta_h, ctx->heta_c, ctx->dimp, ctx->Pr0, ctx->Pr1, ctx->Pd, ctx->Pb,
+                      ctx->Pn, ctx->quad, ctx->norms})
+        if (p) (void)hipFree(p);
+    for (DevVec *dv : {&ctx->x, &ctx->b, &ctx->r, &ctx->e, &ctx->d, &ctx->rho, &ctx->t, &ctx->f3, &ctx->fixedv,
+                       &ctx->cr, &ctx->cp, &ctx->cq, &ctx->ct}) {
+        if (dv->u) (void)hipFree(dv->u);
+        if (dv->v) (void)hipFree(dv->v);
+    }
+    if (ctx->st) (void)hipFree(ctx->st);
+    if (ctx->tab) (void)hipFree(ctx->tab);
+    if (ctx->h_norms) (void)hipHostFree(ctx->h_norms);
+    if (ctx->h_st) (void)hipHostFree(ctx->h_st);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return WAVI_OK;
+}
+
+int wavi_set_dt(wavi_ctx *ctx, double dt)
+{
+    if (!ctx) return WAVI_EARG;
+    if (!(dt > 0)) {
+        ctx->err = "timestep dt must be a positive number";
+        return WAVI_EARG;
+    }
+    ctx->P.dt = dt;
+    return WAVI_OK;
+}
+
+int wavi_field_size(const wavi_ctx *ctx, int field, size_t *n)
+{
+    if (!ctx || !n) return WAVI_EARG;
+    int len = field_len(ctx, field);
+    if (len < 0) return WAVI_EARG;
+    *n = (size_t)len;
+    return WAVI_OK;
+}
+
+int wavi_set_field(wavi_ctx *ctx, int field, const double *host, size_t n)
+{
+    if (!ctx) return WAVI_EARG;
+    int len = field_len(ctx, field);
+    if (len < 0 || !host) {
+        ctx->err = "unknown field or null pointer";
+        return WAVI_EARG;
+    }
+    if ((size_t)len != n) {
+        ctx->err = "field size mismatch: expected " + std::to_string(len) + ", got " + std::to_string(n);
+        return WAVI_EDIM;
+    }
+    if (field == WAVI_WU_MASK || field == WAVI_WV_MASK) {
+        double *tmp = nullptr;
+        HIPCHK(hipMalloc(&tmp, n * sizeof(double)));
+        HIPCHK(hipMemcpy(tmp, host, n * sizeof(double), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_mask_from_double, flat(len), dim3(256), 0, ctx->stream, tmp,
+                           field == WAVI_WU_MASK ? ctx->mwu : ctx->mwv, len);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipFree(tmp));
+        return WAVI_OK;
+    }
+    HIPCHK(hipMemcpy(ctx->f[field], host, n * sizeof(double), hipMemcpyHostToDevice));
+    return WAVI_OK;
+}
+
+int wavi_get_field(wavi_ctx *ctx, int field, double *host, size_t n)
+{
+    if (!ctx) return WAVI_EARG;
+    int len = field_len(ctx, field);
+    if (len < 0 || !host) {
+        ctx->err = "unknown field or null pointer";
+        return WAVI_EARG;
+    }
+    if ((size_t)len != n) {
+        ctx->err = "field size mismatch: expected " + std::to_string(len) + ", got " + std::to_string(n);
+        return WAVI_EDIM;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (field == WAVI_WU_MASK || field == WAVI_WV_MASK) {
+        double *tmp = nullptr;
+        HIPCHK(hipMalloc(&tmp, n * sizeof(double)));
+        hipLaunchKernelGGL(k_mask_to_double, flat(len), dim3(256), 0, ctx->stream,
+                           field == WAVI_WU_MASK ? ctx->mwu : ctx->mwv, tmp, len);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy(host, tmp, n * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipFree(tmp));
+        return WAVI_OK;
+    }
+    HIPCHK(hipMemcpy(host, ctx->f[field], n * sizeof(double), hipMemcpyDeviceToHost));
+    return WAVI_OK;
+}
+
+int wavi_init_quad_f2(wavi_ctx *ctx)
+{
+    if (!ctx) return WAVI_EARG;
+    hipLaunchKernelGGL(k_init_quad_f2, flat(ctx->nh), dim3(256), 0, ctx->stream, ctx->nh, ctx->hm,
+                       ctx->f[WAVI_H_H], ctx->f[WAVI_H_ETA_AV], ctx->f[WAVI_H_QUAD_F2]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAVI_OK;
+}
+
+int wavi_state_save(wavi_ctx *ctx)
+{
+    if (!ctx) return WAVI_EARG;
+    if (!ctx->has_saved) {
+        ctx->saved.assign(WAVI_NFIELDS, nullptr);
+        for (int id = 0; id < WAVI_NFIELDS; ++id) {
+            if (!ctx->f[id]) continue;
+            HIPCHK(hipMalloc(&ctx->saved[id], field_len(ctx, id) * sizeof(double)));
+        }
+        HIPCHK(hipMalloc(&ctx->saved_mwu, ctx->nu));
+        HIPCHK(hipMalloc(&ctx->saved_mwv, ctx->nv));
+        ctx->has_saved = true;
+    }
+    for (int id = 0; id < WAVI_NFIELDS; ++id)
+        if (ctx->f[id])
+            HIPCHK(hipMemcpyAsync(ctx->saved[id], ctx->f[id], field_len(ctx, id) * sizeof(double),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->saved_mwu, ctx->mwu, ctx->nu, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->saved_mwv, ctx->mwv, ctx->nv, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAVI_OK;
+}
+
+int wavi_state_restore(wavi_ctx *ctx)
+{
+    if (!ctx) return WAVI_EARG;
+    if (!ctx->has_saved) {
+        ctx->err = "no saved state";
+        return WAVI_ESTATE;
+    }
+    for (int id = 0; id < WAVI_NFIELDS; ++id)
+        if (ctx->f[id])
+            HIPCHK(hipMemcpyAsync(ctx->f[id], ctx->saved[id], field_len(ctx, id) * sizeof(double),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->mwu, ctx->saved_mwu, ctx->nu, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->mwv, ctx->saved_mwv, ctx->nv, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAVI_OK;
+}
+
+#pragma GCC visibility pop
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// pieces of the solve (device resident)
+
+static InnerArgs inner_args(wavi_ctx *ctx)
+{
+    InnerArgs p;
+    p.hm = ctx->hm;
+    p.um = ctx->um;
+    p.vm = ctx->vm;
+    p.cm = ctx->cm;
+    p.U = ctx->f[WAVI_U_U];
+    p.V = ctx->f[WAVI_V_V];
+    p.h = ctx->f[WAVI_H_H];
+    p.wc = ctx->f[WAVI_H_WEERTMAN_C];
+    p.gfh = ctx->f[WAVI_H_GROUNDED_FRACTION];
+    p.gfu = ctx->f[WAVI_U_GROUNDED_FRACTION];
+    p.gfv = ctx->f[WAVI_V_GROUNDED_FRACTION];
+    p.dsdh = ctx->f[WAVI_H_DSDH];
+    p.glen_b = ctx->f[WAVI_S_GLEN_B];
+    p.zeta = ctx->zeta;
+    p.qw = ctx->qw;
+    p.sr = ctx->f[WAVI_H_SHELF_STRAIN_RATE];
+    p.avs = ctx->f[WAVI_H_AV_SPEED];
+    p.bs = ctx->f[WAVI_H_BED_SPEED];
+    p.beta = ctx->f[WAVI_H_BETA];
+    p.tau = ctx->f[WAVI_H_TAU_BED];
+    p.eta = ctx->f[WAVI_S_ETA];
+    p.etaav = ctx->f[WAVI_H_ETA_AV];
+    p.f1 = ctx->f[WAVI_H_QUAD_F1];
+    p.f2 = ctx->f[WAVI_H_QUAD_F2];
+    p.beff = ctx->f[WAVI_H_BETA_EFF];
+    p.bu = ctx->f[WAVI_U_BETA_EFF];
+    p.bv = ctx->f[WAVI_V_BETA_EFF];
+    p.heta_h = ctx->heta_h;
+    p.heta_c = ctx->heta_c;
+    p.dimp = ctx->dimp;
+    p.rdx = ctx->rdx;
+    p.rdy = ctx->rdy;
+    p.wexp = 1.0 / ctx->P.weertman_m - 1.0;
+    p.wreg2 = ctx->P.weertman_reg_speed * ctx->P.weertman_reg_speed;
+    p.gexp = 1.0 / ctx->P.glen_n - 1.0;
+    p.greg2 = ctx->P.glen_reg_strain_rate * ctx->P.glen_reg_strain_rate;
+    p.c0 = (((-ctx->P.density_ice) * ctx->P.g) * ctx->S.super_implicitness) * ctx->P.dt;
+    p.ns = ctx->ns3 / ctx->nh;
+    p.niter_visc = ctx->S.n_iter_viscosity;
+    p.xfirst_c2h = ctx->g.nx > ctx->g.ny ? 1 : 0;  // KroneckerProduct branch for gc.cent
+    p.yfirst_h2c = ctx->g.ny <= ctx->g.nx ? 1 : 0; // and for gh.cent_xy
+    return p;
+}
+
+static int do_inner_update(wavi_ctx *ctx)
+{
+    InnerArgs p = inner_args(ctx);
+    hipLaunchKernelGGL(k_inner_h, flat(ctx->nh), dim3(256), 0, ctx->stream, ctx->g, p);
+    hipLaunchKernelGGL(k_inner_uvc, flat(ctx->nu + ctx->nv + ctx->nc), dim3(256), 0, ctx->stream, ctx->g, p);
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+static int do_op(wavi_ctx *ctx, const double *xu, const double *xv, double *yu, double *yv)
+{
+    hipLaunchKernelGGL(k_op<double>, flat(ctx->nu + ctx->nv), dim3(256), 0, ctx->stream, ctx->g, coef64(ctx), xu, xv,
+                       yu, yv);
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+// get_rhs into ctx->b
+static int do_rhs(wavi_ctx *ctx)
+{
+    hipLaunchKernelGGL((k_select<double, double>), flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_U_U],
+                       ctx->ufix, ctx->fixedv.u, ctx->nu);
+    hipLaunchKernelGGL((k_select<double, double>), flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->f[WAVI_V_V],
+                       ctx->vfix, ctx->fixedv.v, ctx->nv);
+    CHK(do_op(ctx, ctx->fixedv.u, ctx->fixedv.v, ctx->f3.u, ctx->f3.v));
+    RhsArgs a;
+    a.hm = ctx->hm;
+    a.ui = ctx->ui;
+    a.vi = ctx->vi;
+    a.s = ctx->f[WAVI_H_S];
+    a.dsdh = ctx->f[WAVI_H_DSDH];
+    a.acc = ctx->f[WAVI_H_ACCUMULATION];
+    a.mel = ctx->f[WAVI_H_BASAL_MELT];
+    a.hu = ctx->f[WAVI_U_H];
+    a.su = ctx->f[WAVI_U_S];
+    a.hv = ctx->f[WAVI_V_H];
+    a.sv = ctx->f[WAVI_V_S];
+    a.f3u = ctx->f3.u;
+    a.f3v = ctx->f3.v;
+    a.bu = ctx->b.u;
+    a.bv = ctx->b.v;
+    a.rdx = ctx->rdx;
+    a.rdy = ctx->rdy;
+    a.g = ctx->P.g;
+    a.ri = ctx->P.density_ice;
+    a.rw = ctx->P.density_ocean;
+    a.sl = ctx->P.sea_level_wrt_geoid;
+    a.tdt = ctx->S.super_implicitness * ctx->P.dt;
+    hipLaunchKernelGGL(k_rhs, flat(ctx->nu + ctx->nv), dim3(256), 0, ctx->stream, ctx->g, a);
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+static int do_diag(wavi_ctx *ctx)
+{
+    hipLaunchKernelGGL(k_diag<double>, flat(ctx->nu + ctx->nv), dim3(256), 0, ctx->stream, ctx->g, coef64(ctx),
+                       ctx->d.u, ctx->d.v);
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+static int do_smoother(wavi_ctx *ctx, const DevVec &x, const DevVec &b, const int order[4])
+{
+    const GridDims &g = ctx->g;
+    int nthreads_u = ((g.nxu + 1) / 2) * g.nyu, nthreads_v = ((g.nxv + 1) / 2) * g.nyv;
+    OpCoef<double> c = coef64(ctx);
+    for (int it = 0; it < ctx->S.nsmooth; ++it)
+        for (int q = 0; q < 4; ++q) {
+            int col = order[q];
+            int n = col <= 2 ? nthreads_u : nthreads_v;
+            hipLaunchKernelGGL(k_colour<double>, flat(n), dim3(256), 0, ctx->stream, g, c, x.u, x.v, b.u, b.v,
+                               ctx->d.u, ctx->d.v, ctx->S.smoother_omega, col);
+        }
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+static CGArgs<double> cg_args(wavi_ctx *ctx)
+{
+    CGArgs<double> a;
+    a.g = ctx->g;
+    a.c = coef64(ctx);
+    a.tab = ctx->tab;
+    a.mwu = ctx->mwu;
+    a.mwv = ctx->mwv;
+    a.ui = ctx->ui;
+    a.vi = ctx->vi;
+    a.xu = ctx->f[WAVI_WU_CORRECTION_COARSE];
+    a.xv = ctx->f[WAVI_WV_CORRECTION_COARSE];
+    a.ru = ctx->cr.u;
+    a.rv = ctx->cr.v;
+    a.pu = ctx->cp.u;
+    a.pv = ctx->cp.v;
+    a.qu = ctx->cq.u;
+    a.qv = ctx->cq.v;
+    a.tu = ctx->ct.u;
+    a.tv = ctx->ct.v;
+    a.Pr[0] = ctx->Pr0;
+    a.Pr[1] = ctx->Pr1;
+    a.Pd = ctx->Pd;
+    a.Pb = ctx->Pb;
+    a.st = ctx->st;
+    a.tol_coarse = ctx->S.tol_coarse;
+    a.reltol = std::sqrt(2.220446049250313e-16);  // IterativeSolvers' default sqrt(eps)
+    a.maxiter = ctx->S.maxiter_coarse;
+    return a;
+}
+
+// coarse CG on b_c held in ctx->cr; solution in the warm start fields
+static int do_coarse_cg(wavi_ctx *ctx, int *iters)
+{
+    const GridDims &g = ctx->g;
+    dim3 tg(g.nbx, g.nby);
+    CGArgs<double> a = cg_args(ctx);
+    // t = P x0
+    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, 1, a.xu, a.xv,
+                       ctx->mwu, ctx->mwv, a.tu, a.tv, ctx->ui, ctx->vi);
+    hipLaunchKernelGGL(k_cg_setup<double>, tg, dim3(256), 0, ctx->stream, a);
+    HIPCHK(hipGetLastError());
+    const int CHUNK = 32;
+    int q = 0;
+    const int qmax = ctx->S.maxiter_coarse + 1;
+    while (true) {
+        for (int s = 0; s < CHUNK; ++s, ++q) {
+            int par = q & 1;
+            hipLaunchKernelGGL(k_cg_dir<double>, tg, dim3(256), 0, ctx->stream, a, par);
+            hipLaunchKernelGGL(k_cg_apply<double>, tg, dim3(256), 0, ctx->stream, a);
+            hipLaunchKernelGGL(k_cg_update<double>, tg, dim3(256), 0, ctx->stream, a, par);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(ctx->h_st, ctx->st, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (ctx->h_st->done) break;
+        if (q > qmax + CHUNK) {
+            ctx->err = "coarse CG did not terminate";
+            return WAVI_ESTATE;
+        }
+    }
+    *iters = ctx->h_st->iters;
+    return WAVI_OK;
+}
+
+// precondition!(x, p, b) with x = 0: b in ctx->r, correction into ctx->e
+static int do_vcycle(wavi_ctx *ctx, int *cg_iters)
+{
+    static const int fwd[4] = {1, 3, 2, 4}, rev[4] = {4, 2, 3, 1};
+    const GridDims &g = ctx->g;
+    dim3 tg(g.nbx, g.nby);
+    CHK(do_diag(ctx));
+    hipLaunchKernelGGL(k_fill<double>, flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->e.u, 0.0, ctx->nu);
+    hipLaunchKernelGGL(k_fill<double>, flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->e.v, 0.0, ctx->nv);
+    CHK(do_smoother(ctx, ctx->e, ctx->r, fwd));
+    // rho = r - A e
+    hipLaunchKernelGGL(k_tile_resid<double>, tg, dim3(256), 0, ctx->stream, g, coef64(ctx), ctx->e.u, ctx->e.v,
+                       ctx->r.u, ctx->r.v, ctx->rho.u, ctx->rho.v, ctx->Pn, (double *)nullptr);
+    // b_c = R rho, into the CG residual buffer
+    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, 2, ctx->rho.u,
+                       ctx->rho.v, ctx->ui, ctx->vi, ctx->cr.u, ctx->cr.v, ctx->mwu, ctx->mwv);
+    HIPCHK(hipGetLastError());
+    CHK(do_coarse_cg(ctx, cg_iters));
+    // e += P x_c
+    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, 1,
+                       ctx->f[WAVI_WU_CORRECTION_COARSE], ctx->f[WAVI_WV_CORRECTION_COARSE], ctx->mwu, ctx->mwv,
+                       ctx->t.u, ctx->t.v, ctx->ui, ctx->vi);
+    hipLaunchKernelGGL(k_add_masked<double>, flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->e.u, ctx->t.u, ctx->ui,
+                       ctx->nu);
+    hipLaunchKernelGGL(k_add_masked<double>, flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->e.v, ctx->t.v, ctx->vi,
+                       ctx->nv);
+    CHK(do_smoother(ctx, ctx->e, ctx->r, rev));
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+// precondition!(model, ::BasicParallelSpec) (update_velocities.jl:50-83)
+static int do_precondition(wavi_ctx *ctx, int *converged, double *rel, int *cg_iters)
+{
+    const GridDims &g = ctx->g;
+    dim3 tg(g.nbx, g.nby);
+    hipLaunchKernelGGL((k_select<double, double>), flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_U_U],
+                       ctx->ui, ctx->x.u, ctx->nu);
+    hipLaunchKernelGGL((k_select<double, double>), flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->f[WAVI_V_V],
+                       ctx->vi, ctx->x.v, ctx->nv);
+    CHK(do_rhs(ctx));
+    hipLaunchKernelGGL(k_tile_resid<double>, tg, dim3(256), 0, ctx->stream, g, coef64(ctx), ctx->x.u, ctx->x.v,
+                       ctx->b.u, ctx->b.v, ctx->r.u, ctx->r.v, ctx->Pr0, ctx->Pb);
+    hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, ctx->stream, ctx->Pr0, ctx->Pb, g.npart, ctx->norms);
+    // set_residual! (update_velocities.jl:428-433): residual fields at inner points
+    hipLaunchKernelGGL(k_copy_masked<double>, flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_U_RESIDUAL],
+                       ctx->r.u, ctx->ui, ctx->nu);
+    hipLaunchKernelGGL(k_copy_masked<double>, flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->f[WAVI_V_RESIDUAL],
+                       ctx->r.v, ctx->vi, ctx->nv);
+    HIPCHK(hipMemcpyAsync(ctx->h_norms, ctx->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *rel = ctx->h_norms[0] / ctx->h_norms[1];
+    *converged = *rel < ctx->S.tol_picard;
+    *cg_iters = 0;
+    if (!*converged) {
+        CHK(do_vcycle(ctx, cg_iters));
+        hipLaunchKernelGGL(k_set_velocities, flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_U_U], ctx->x.u,
+                           ctx->e.u, ctx->ui, ctx->nu);
+        hipLaunchKernelGGL(k_set_velocities, flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->f[WAVI_V_V], ctx->x.v,
+                           ctx->e.v, ctx->vi, ctx->nv);
+        HIPCHK(hipGetLastError());
+    }
+    return WAVI_OK;
+}
+
+static int do_update_velocities(wavi_ctx *ctx, wavi_solve_report *rep)
+{
+    wavi_solve_report local;
+    if (!rep) rep = &local;
+    memset(rep, 0, sizeof(*rep));
+    int converged = 0, it = 0, total = 0;
+    double rel = INFINITY;
+    while (!converged && it < ctx->S.maxiter_picard) {
+        it = it + 1;
+        CHK(do_inner_update(ctx));
+        int cg = 0;
+        CHK(do_precondition(ctx, &converged, &rel, &cg));
+        total += cg;
+        if (it <= WAVI_REPORT_MAX_PICARD) {
+            rep->coarse_cg_iters[it - 1] = cg;
+            rep->rel_resid_hist[it - 1] = rel;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    rep->picard_iters = it;
+    rep->converged = converged;
+    rep->rel_resid = rel;
+    rep->coarse_cg_total = total;
+    return WAVI_OK;
+}
+
+static GeomArgs geom_args(wavi_ctx *ctx)
+{
+    GeomArgs a;
+    a.hm = ctx->hm;
+    a.um = ctx->um;
+    a.vm = ctx->vm;
+    a.b = ctx->f[WAVI_H_B];
+    a.h = ctx->f[WAVI_H_H];
+    a.s = ctx->f[WAVI_H_S];
+    a.haf = ctx->f[WAVI_H_HAF];
+    a.q = ctx->quad;
+    a.gfh = ctx->f[WAVI_H_GROUNDED_FRACTION];
+    a.gfu = ctx->f[WAVI_U_GROUNDED_FRACTION];
+    a.gfv = ctx->f[WAVI_V_GROUNDED_FRACTION];
+    a.hu = ctx->f[WAVI_U_H];
+    a.su = ctx->f[WAVI_U_S];
+    a.hv = ctx->f[WAVI_V_H];
+    a.sv = ctx->f[WAVI_V_S];
+    a.wc = ctx->f[WAVI_H_WEERTMAN_C];
+    a.dsdh = ctx->f[WAVI_H_DSDH];
+    a.acc = ctx->f[WAVI_H_ACCUMULATION];
+    a.mel = ctx->f[WAVI_H_BASAL_MELT];
+    a.wc_param = ctx->f[WAVI_P_WEERTMAN_C];
+    a.acc_param = ctx->f[WAVI_P_ACCUMULATION_RATE];
+    a.sl = ctx->P.sea_level_wrt_geoid;
+    a.ri = ctx->P.density_ice;
+    a.rw = ctx->P.density_ocean;
+    a.melt = ctx->P.uniform_melt_rate;
+    return a;
+}
+
+static int do_update_geometry(wavi_ctx *ctx)
+{
+    GeomArgs a = geom_args(ctx);
+    hipLaunchKernelGGL(k_surface_haf, flat(ctx->nh), dim3(256), 0, ctx->stream, ctx->g, a);
+    hipLaunchKernelGGL(k_gf_h, flat(ctx->nh), dim3(256), 0, ctx->stream, ctx->g, a);
+    hipLaunchKernelGGL(k_geom_uv, flat(ctx->nu + ctx->nv), dim3(256), 0, ctx->stream, ctx->g, a);
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+static int do_update_wavelets(wavi_ctx *ctx)
+{
+    dim3 tg(ctx->g.nbx, ctx->g.nby);
+    hipLaunchKernelGGL(k_tile_wavelets, tg, dim3(256), 0, ctx->stream, ctx->g, ctx->tab, ctx->f[WAVI_U_U],
+                       ctx->f[WAVI_V_V], ctx->um, ctx->vm, ctx->f[WAVI_WU_WAVELETS], ctx->f[WAVI_WV_WAVELETS],
+                       ctx->mwu, ctx->mwv, ctx->f[WAVI_WU_CORRECTION_COARSE], ctx->f[WAVI_WV_CORRECTION_COARSE],
+                       ctx->S.wavelet_threshold);
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+static int do_update_state(wavi_ctx *ctx, wavi_solve_report *rep)
+{
+    CHK(do_update_geometry(ctx));
+    CHK(do_update_velocities(ctx, rep));
+    hipLaunchKernelGGL(k_post_solve, flat(ctx->nh), dim3(256), 0, ctx->stream, ctx->g, ctx->hm, ctx->um, ctx->vm,
+                       ctx->f[WAVI_U_U], ctx->f[WAVI_V_V], ctx->f[WAVI_H_BETA], ctx->f[WAVI_H_QUAD_F1],
+                       ctx->f[WAVI_H_QUAD_F2], ctx->f[WAVI_U_H], ctx->f[WAVI_V_H], ctx->f[WAVI_H_ACCUMULATION],
+                       ctx->f[WAVI_H_BASAL_MELT], ctx->f[WAVI_H_U], ctx->f[WAVI_H_V], ctx->f[WAVI_H_UB],
+                       ctx->f[WAVI_H_VB], ctx->f[WAVI_H_US], ctx->f[WAVI_H_VS], ctx->f[WAVI_H_DHDT], ctx->rdx,
+                       ctx->rdy);
+    HIPCHK(hipGetLastError());
+    CHK(do_update_wavelets(ctx));
+    return WAVI_OK;
+}
+
+static int do_update_thickness(wavi_ctx *ctx, double dt)
+{
+    hipLaunchKernelGGL(k_update_thickness, flat(ctx->nh), dim3(256), 0, ctx->stream, ctx->nh, ctx->hm, ctx->hfix,
+                       ctx->f[WAVI_H_H], ctx->f[WAVI_H_DHDT], ctx->P.minimum_thickness, dt);
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" {
+#pragma GCC visibility push(default)
+
+int wavi_update_velocities(wavi_ctx *ctx, wavi_solve_report *report)
+{
+    if (!ctx) return WAVI_EARG;
+    CHK(do_update_velocities(ctx, report));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAVI_OK;
+}
+
+int wavi_inner_update(wavi_ctx *ctx)
+{
+    if (!ctx) return WAVI_EARG;
+    CHK(do_inner_update(ctx));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAVI_OK;
+}
+
+int wavi_update_geometry(wavi_ctx *ctx)
+{
+    if (!ctx) return WAVI_EARG;
+    CHK(do_update_geometry(ctx));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAVI_OK;
+}
+
+int wavi_update_state(wavi_ctx *ctx, wavi_solve_report *report)
+{
+    if (!ctx) return WAVI_EARG;
+    CHK(do_update_state(ctx, report));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAVI_OK;
+}
+
+int wavi_update_thickness(wavi_ctx *ctx, double dt)
+{
+    if (!ctx) return WAVI_EARG;
+    CHK(do_update_thickness(ctx, dt));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAVI_OK;
+}
+
+int wavi_run_steps(wavi_ctx *ctx, int nsteps, wavi_solve_report *last_report)
+{
+    if (!ctx) return WAVI_EARG;
+    if (nsteps < 0) {
+        ctx->err = "nsteps must be non-negative";
+        return WAVI_EARG;
+    }
+    for (int n = 0; n < nsteps; ++n) {
+        CHK(do_update_state(ctx, last_report));
+        CHK(do_update_thickness(ctx, ctx->P.dt));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAVI_OK;
+}
+
+int wavi_update_wavelets(wavi_ctx *ctx)
+{
+    if (!ctx) return WAVI_EARG;
+    CHK(do_update_wavelets(ctx));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAVI_OK;
+}
+
+static int upload2(wavi_ctx *ctx, const double *hu, const double *hv, DevVec &d)
+{
+    HIPCHK(hipMemcpy(d.u, hu, ctx->nu * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d.v, hv, ctx->nv * sizeof(double), hipMemcpyHostToDevice));
+    return WAVI_OK;
+}
+
+static int download2(wavi_ctx *ctx, const DevVec &d, double *hu, double *hv)
+{
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(hu, d.u, ctx->nu * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hv, d.v, ctx->nv * sizeof(double), hipMemcpyDeviceToHost));
+    return WAVI_OK;
+}
+
+int wavi_op_apply(wavi_ctx *ctx, const double *xu, const double *xv, double *yu, double *yv, int sampled)
+{
+    if (!ctx) return WAVI_EARG;
+    if (!xu || !xv || !yu || !yv) {
+        ctx->err = "null pointer";
+        return WAVI_EARG;
+    }
+    CHK(upload2(ctx, xu, xv, ctx->fixedv));
+    if (sampled) {
+        hipLaunchKernelGGL((k_select<double, double>), flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->fixedv.u,
+                           ctx->ui, ctx->fixedv.u, ctx->nu);
+        hipLaunchKernelGGL((k_select<double, double>), flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->fixedv.v,
+                           ctx->vi, ctx->fixedv.v, ctx->nv);
+    }
+    CHK(do_op(ctx, ctx->fixedv.u, ctx->fixedv.v, ctx->t.u, ctx->t.v));
+    return download2(ctx, ctx->t, yu, yv);
+}
+
+int wavi_rhs(wavi_ctx *ctx, double *bu, double *bv)
+{
+    if (!ctx) return WAVI_EARG;
+    if (!bu || !bv) {
+        ctx->err = "null pointer";
+        return WAVI_EARG;
+    }
+    CHK(do_rhs(ctx));
+    return download2(ctx, ctx->b, bu, bv);
+}
+
+int wavi_op_diag(wavi_ctx *ctx, double *du, double *dv)
+{
+    if (!ctx) return WAVI_EARG;
+    if (!du || !dv) {
+        ctx->err = "null pointer";
+        return WAVI_EARG;
+    }
+    CHK(do_diag(ctx));
+    return download2(ctx, ctx->d, du, dv);
+}
+
+int wavi_vcycle(wavi_ctx *ctx, const double *ru, const double *rv, double *eu, double *ev, int32_t *coarse_iters)
+{
+    if (!ctx) return WAVI_EARG;
+    if (!ru || !rv || !eu || !ev) {
+        ctx->err = "null pointer";
+        return WAVI_EARG;
+    }
+    CHK(upload2(ctx, ru, rv, ctx->r));
+    int it = 0;
+    CHK(do_vcycle(ctx, &it));
+    if (coarse_iters) *coarse_iters = it;
+    return download2(ctx, ctx->e, eu, ev);
+}
+
+int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out)
+{
+    if (!ctx) return WAVI_EARG;
+    if (!in || !out || comp < 0 || comp > 1 || dir < 0 || dir > 2) {
+        ctx->err = "invalid argument";
+        return WAVI_EARG;
+    }
+    const int n = comp == 0 ? ctx->nu : ctx->nv;
+    double *din = comp == 0 ? ctx->fixedv.u : ctx->fixedv.v;
+    double *dout = comp == 0 ? ctx->t.u : ctx->t.v;
+    HIPCHK(hipMemcpy(din, in, n * sizeof(double), hipMemcpyHostToDevice));
+    dim3 tg(ctx->g.nbx, ctx->g.nby);
+    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, ctx->g, ctx->tab, dir,
+                       ctx->fixedv.u, ctx->fixedv.v, (const uint8_t *)nullptr, (const uint8_t *)nullptr, ctx->t.u,
+                       ctx->t.v, (const uint8_t *)nullptr, (const uint8_t *)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost));
+    return WAVI_OK;
+}
+
+int wavi_math_probe(int fn, const double *x, const double *y, double *out, size_t n)
+{
+    if (fn < 0 || fn > 2 || !x || !out || (fn != 1 && !y) || n > (size_t)(1 << 28)) {
+        g_create_error = "invalid argument";
+        return WAVI_EARG;
+    }
+    if (n == 0) return WAVI_OK;
+    double *d = nullptr;
+    hipError_t e = hipMalloc(&d, 3 * n * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(d, x, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && y) e = hipMemcpy(d + n, y, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_math_probe, flat((int)n), dim3(256), 0, 0, fn, d, d + n, d + 2 * n, (int)n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (d) (void)hipFree(d);
+    if (e != hipSuccess) {
+        g_create_error = std::string("wavi_math_probe: ") + hipGetErrorString(e);
+        return WAVI_EHIP;
+    }
+    return WAVI_OK;
+}
+
+#pragma GCC visibility pop
+}  // extern "C"

@@ -1,0 +1,274 @@
+// wavi_device.h - device-side building blocks of the gfx950 velocity solve.
+//
+// Everything here is written for 64-wide wavefronts and 256-thread
+// workgroups that own one 16 x 16 block of the u grid and the v grid.  All
+// FP64 arithmetic is written in the order in which WAVI.jl's sparse and
+// Kronecker products accumulate it (src/KroneckerProduct.jl:57-73,
+// src/utilities.jl:61-145); the library is compiled with -ffp-contract=off
+// so every a*b+c keeps two roundings.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wavi {
+
+constexpr int TILE = 16;              // reduction / transform tile edge
+constexpr int TPB = TILE * TILE;      // threads per block-tile workgroup
+constexpr int MAX_HAAR_NNZ = 16;      // a row of the forward or transposed inverse touches a whole block
+
+// ---------------------------------------------------------------------------
+// Geometry of the staggered grids.
+struct GridDims {
+    int nx, ny;           // h grid
+    int nxu, nyu;         // u grid (nx+1, ny)
+    int nxv, nyv;         // v grid (nx, ny+1)
+    int nbxu, nbyu;       // 16x16 blocks of the u grid
+    int nbxv, nbyv;       // 16x16 blocks of the v grid
+    int nbx, nby;         // launch grid of block-tile kernels: max of u and v
+    int npart;            // nbxu*nbyu + nbxv*nbyv partials of a [u; v] reduction
+};
+
+// ---------------------------------------------------------------------------
+// Deterministic x^y (double-double log and exp; see the oracle's det_pow for
+// the derivation).  Uses only IEEE +,-,*,/, fma, frexp, ldexp and rint, so it
+// returns the same bits on the CPU and the GPU.
+__device__ __forceinline__ double det_pow(double x, double y)
+{
+    const double LN2_HI = 0x1.62e42fefa39efp-1;
+    const double LN2_LO = 0x1.abc9e3b39803fp-56;
+    const double INV_LN2 = 0x1.71547652b82fep+0;
+    if (y == 0.0) return 1.0;
+    if (__builtin_isnan(x) || __builtin_isnan(y)) return __builtin_nan("");
+    if (x == 0.0) return y > 0.0 ? 0.0 : __builtin_inf();
+    if (x < 0.0) return __builtin_nan("");
+    if (__builtin_isinf(x)) return y > 0.0 ? __builtin_inf() : 0.0;
+    int k;
+    double m = frexp(x, &k);
+    if (m < 0.70710678118654752440) {
+        m = m * 2.0;
+        k = k - 1;
+    }
+    double f = m - 1.0;
+    double d_hi = 2.0 + f;
+    double d_lo = f - (d_hi - 2.0);
+    double s_hi = f / d_hi;
+    double e = fma(-s_hi, d_hi, f);
+    e = e - s_hi * d_lo;
+    double s_lo = e / d_hi;
+    double t = s_hi * s_hi;
+    double p = 1.0 / 23.0;
+    p = p * t + 1.0 / 21.0;
+    p = p * t + 1.0 / 19.0;
+    p = p * t + 1.0 / 17.0;
+    p = p * t + 1.0 / 15.0;
+    p = p * t + 1.0 / 13.0;
+    p = p * t + 1.0 / 11.0;
+    p = p * t + 1.0 / 9.0;
+    p = p * t + 1.0 / 7.0;
+    p = p * t + 1.0 / 5.0;
+    p = p * t + 1.0 / 3.0;
+    p = p * t;
+    double lm_hi = 2.0 * s_hi;
+    double lm_lo = 2.0 * s_lo + 2.0 * (s_hi * p);
+    double dk = (double)k;
+    double kh = dk * LN2_HI;
+    double kl = fma(dk, LN2_HI, -kh) + dk * LN2_LO;
+    double sh = kh + lm_hi;
+    double bb = sh - kh;
+    double sl = (kh - (sh - bb)) + (lm_hi - bb);
+    sl = sl + (kl + lm_lo);
+    double lh = sh + sl;
+    double ll = sl - (lh - sh);
+    double zh = y * lh;
+    double zl = fma(y, lh, -zh) + y * ll;
+    if (zh > 709.78) return __builtin_inf();
+    if (zh < -745.2) return 0.0;
+    double n = rint(zh * INV_LN2);
+    double r = fma(-n, LN2_HI, zh);
+    r = r - n * LN2_LO;
+    r = r + zl;
+    double q = 1.0 / 6227020800.0;
+    q = q * r + 1.0 / 479001600.0;
+    q = q * r + 1.0 / 39916800.0;
+    q = q * r + 1.0 / 3628800.0;
+    q = q * r + 1.0 / 362880.0;
+    q = q * r + 1.0 / 40320.0;
+    q = q * r + 1.0 / 5040.0;
+    q = q * r + 1.0 / 720.0;
+    q = q * r + 1.0 / 120.0;
+    q = q * r + 1.0 / 24.0;
+    q = q * r + 1.0 / 6.0;
+    q = q * r + 0.5;
+    double em1 = r + (r * r) * q;
+    double ee = 1.0 + em1;
+    return ldexp(ee, (int)n);
+}
+
+// ---------------------------------------------------------------------------
+// Canonical reduction trees.  A 256-thread workgroup reduces its tile with
+// t[e] += t[e+s] for s = 128..1: s = 128, 64 through LDS, s = 32..1 with
+// wavefront shuffles (lane e reads lane e+s, the same pairing).
+template <typename T>
+__device__ __forceinline__ T tree256(T v, T *lds)
+{
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    if (t < 128) lds[t] = lds[t] + lds[t + 128];
+    __syncthreads();
+    T r = 0;
+    if (t < 64) {
+        r = lds[t] + lds[t + 64];
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1) r = r + __shfl_down(r, s, 64);
+    }
+    __syncthreads();
+    return r;  // valid in thread 0
+}
+
+// Fold `np` block partials into 256 lanes (lane l sums P[l], P[l+256], ...
+// in order) and sum the lanes with tree256.  Every workgroup that calls this
+// obtains the same bits; the result is broadcast to all threads.
+__device__ __forceinline__ double lanes_sum(const double *P, int np, double *lds)
+{
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    for (int q = t; q < np; q += 256) acc = acc + P[q];
+    double s = tree256(acc, lds);
+    if (t == 0) lds[0] = s;
+    __syncthreads();
+    double out = lds[0];
+    __syncthreads();
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// The momentum operator op_fun! (src/utilities.jl:61-145), evaluated at one
+// u point or one v point from an input accessor.  Intermediates (r_xx, r_yy,
+// r_xy, extra) are recomputed by each output that needs them with the same
+// expression, so a point's value does not depend on which kernel computes it.
+template <typename T>
+struct OpCoef {
+    const T *heta_h;    // h*eta_av on the h mask, 0 off it          (gh.dneghηav)
+    const T *heta_c;    // corner average of h*eta_av on the c mask   (gc.dneghηav)
+    const T *dimp;      // -rho_i g theta dt dsdh on the h mask       (gh.dimplicit)
+    const T *hu, *hv;   // thickness on u and v points                (gu.h, gv.h)
+    const T *bu, *bv;   // beta_eff on u and v points                 (gu.βeff, gv.βeff)
+    const uint8_t *hm, *um, *vm, *cm, *ui, *vi;
+    int nx, ny;
+    T rdx, rdy;
+};
+
+template <typename T>
+struct GlobalIn {
+    const T *U, *V;
+    int nxu, nx;
+    __device__ __forceinline__ T u(int i, int j) const { return U[i + nxu * j]; }
+    __device__ __forceinline__ T v(int i, int j) const { return V[i + nx * j]; }
+};
+
+// Unit vector at one u (or v) point: the probe vector of get_op_diag seen
+// from inside the stencil's reach.
+template <typename T>
+struct DeltaIn {
+    int i0, j0, comp;  // comp 0: u point, 1: v point
+    __device__ __forceinline__ T u(int i, int j) const { return (comp == 0 && i == i0 && j == j0) ? T(1) : T(0); }
+    __device__ __forceinline__ T v(int i, int j) const { return (comp == 1 && i == i0 && j == j0) ? T(1) : T(0); }
+};
+
+template <typename T, class In>
+__device__ __forceinline__ T op_rxx(const OpCoef<T> &c, const In &in, int i, int j)
+{
+    const int k = i + c.nx * j;
+    if (!c.hm[k]) return T(0);
+    T dudx = c.rdx * in.u(i + 1, j) - c.rdx * in.u(i, j);
+    T dvdy = c.rdy * in.v(i, j + 1) - c.rdy * in.v(i, j);
+    return T(2) * (c.heta_h[k] * (T(2) * dudx + dvdy));
+}
+
+template <typename T, class In>
+__device__ __forceinline__ T op_ryy(const OpCoef<T> &c, const In &in, int i, int j)
+{
+    const int k = i + c.nx * j;
+    if (!c.hm[k]) return T(0);
+    T dudx = c.rdx * in.u(i + 1, j) - c.rdx * in.u(i, j);
+    T dvdy = c.rdy * in.v(i, j + 1) - c.rdy * in.v(i, j);
+    return T(2) * (c.heta_h[k] * (T(2) * dvdy + dudx));
+}
+
+// corner (i, j) of the c grid, i < nx-1, j < ny-1
+template <typename T, class In>
+__device__ __forceinline__ T op_rxy(const OpCoef<T> &c, const In &in, int i, int j)
+{
+    const int k = i + (c.nx - 1) * j;
+    if (!c.cm[k]) return T(0);
+    T dudy = c.rdy * in.u(i + 1, j + 1) - c.rdy * in.u(i + 1, j);
+    T dvdx = c.rdx * in.v(i + 1, j + 1) - c.rdx * in.v(i, j + 1);
+    return c.heta_c[k] * (dudy + dvdx);
+}
+
+template <typename T, class In>
+__device__ __forceinline__ T op_extra(const OpCoef<T> &c, const In &in, int i, int j)
+{
+    const int nxu = c.nx + 1;
+    const int k = i + c.nx * j;
+    T qx1 = c.um[(i + 1) + nxu * j] ? c.hu[(i + 1) + nxu * j] * in.u(i + 1, j) : T(0);
+    T qx0 = c.um[i + nxu * j] ? c.hu[i + nxu * j] * in.u(i, j) : T(0);
+    T qy1 = c.vm[i + c.nx * (j + 1)] ? c.hv[i + c.nx * (j + 1)] * in.v(i, j + 1) : T(0);
+    T qy0 = c.vm[i + c.nx * j] ? c.hv[i + c.nx * j] * in.v(i, j) : T(0);
+    T divq = (c.rdx * qx1 - c.rdx * qx0) + (c.rdy * qy1 - c.rdy * qy0);
+    return c.dimp[k] * divq;
+}
+
+// fx at u point (i, j), 0 <= i <= nx, 0 <= j < ny (caller checks inner mask)
+template <typename T, class In>
+__device__ __forceinline__ T op_u(const OpCoef<T> &c, const In &in, int i, int j)
+{
+    const int nx = c.nx, ny = c.ny;
+    const int k = i + (nx + 1) * j;
+    T a1 = i < nx ? op_rxx(c, in, i, j) : T(0);
+    T a0 = i > 0 ? op_rxx(c, in, i - 1, j) : T(0);
+    T d_rxx_dx = c.rdx * a1 - c.rdx * a0;
+    T c1 = (i >= 1 && i <= nx - 1 && j <= ny - 2) ? op_rxy(c, in, i - 1, j) : T(0);
+    T c0 = (i >= 1 && i <= nx - 1 && j >= 1) ? op_rxy(c, in, i - 1, j - 1) : T(0);
+    T d_rxy_dy = c.rdy * c1 - c.rdy * c0;
+    T taubx = c.um[k] ? c.bu[k] * in.u(i, j) : T(0);
+    T e1 = i < nx ? op_extra(c, in, i, j) : T(0);
+    T e0 = i > 0 ? op_extra(c, in, i - 1, j) : T(0);
+    T d_extra_dx = c.rdx * e1 - c.rdx * e0;
+    return ((d_rxx_dx + d_rxy_dy) - taubx) - c.hu[k] * d_extra_dx;
+}
+
+// fy at v point (i, j), 0 <= i < nx, 0 <= j <= ny
+template <typename T, class In>
+__device__ __forceinline__ T op_v(const OpCoef<T> &c, const In &in, int i, int j)
+{
+    const int nx = c.nx, ny = c.ny;
+    const int k = i + nx * j;
+    T a1 = j < ny ? op_ryy(c, in, i, j) : T(0);
+    T a0 = j > 0 ? op_ryy(c, in, i, j - 1) : T(0);
+    T d_ryy_dy = c.rdy * a1 - c.rdy * a0;
+    T c1 = (j >= 1 && j <= ny - 1 && i <= nx - 2) ? op_rxy(c, in, i, j - 1) : T(0);
+    T c0 = (j >= 1 && j <= ny - 1 && i >= 1) ? op_rxy(c, in, i - 1, j - 1) : T(0);
+    T d_rxy_dx = c.rdx * c1 - c.rdx * c0;
+    T tauby = c.vm[k] ? c.bv[k] * in.v(i, j) : T(0);
+    T e1 = j < ny ? op_extra(c, in, i, j) : T(0);
+    T e0 = j > 0 ? op_extra(c, in, i, j - 1) : T(0);
+    T d_extra_dy = c.rdy * e1 - c.rdy * e0;
+    return ((d_ryy_dy + d_rxy_dx) - tauby) - c.hv[k] * d_extra_dy;
+}
+
+// ---------------------------------------------------------------------------
+// Haar tables: the composed 1-D matrix of wavelet_matrix(m, levels, dir)
+// (src/Wavelets/update_wavelets.jl:38-89) for every block length m <= B,
+// row r: nnz[m][r] terms (col, coef), columns increasing.  dir 0 forward,
+// 1 inverse, 2 transposed inverse.  Coefficients are +-2^-k (k <= 4), exact in
+// float.  Filled on the host at context creation.
+struct HaarTable {
+    int B;
+    int8_t nnz[3][17][16];
+    int8_t col[3][17][16][MAX_HAAR_NNZ];
+    float coef[3][17][16][MAX_HAAR_NNZ];
+};
+
+}  // namespace wavi

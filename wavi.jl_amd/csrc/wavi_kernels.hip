@@ -1,0 +1,1114 @@
+// wavi_kernels.hip - gfx950 kernels of the WAVI.jl velocity solve.
+//
+// Layout: every field is a full grid in HBM, x fastest (u: (nx+1) x ny,
+// v: nx x (ny+1), h: nx x ny, c: (nx-1) x (ny-1), sigma: nx x ny x nsigma
+// with sigma slowest).  Vectors of the reference's "inner" unknowns are full
+// u/v grids that are zero off the inner masks, so no gather/scatter index
+// maps exist on the device.
+//
+// Kernels named k_tile_* run one 256-thread workgroup per 16 x 16 block of
+// the u grid and of the v grid (grid (nbx, nby)); they own the Haar blocks
+// (levels <= 3) and write one reduction partial per block.  Other kernels
+// are flat (256 threads, one point each).
+#include "wavi_device.h"
+
+namespace wavi {
+
+// ===========================================================================
+// small utilities
+
+template <typename T>
+__global__ void k_fill(T *a, T v, int n)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = v;
+}
+
+// out = mask ? in : 0
+template <typename Tin, typename Tout>
+__global__ void k_select(const Tin *in, const uint8_t *mask, Tout *out, int n)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = mask[k] ? (Tout)in[k] : Tout(0);
+}
+
+// a = a + b where mask (inner points)
+template <typename T>
+__global__ void k_add_masked(T *a, const T *b, const uint8_t *mask, int n)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && mask[k]) a[k] = a[k] + b[k];
+}
+
+// dst = src where mask
+template <typename T>
+__global__ void k_copy_masked(T *dst, const T *src, const uint8_t *mask, int n)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && mask[k]) dst[k] = src[k];
+}
+
+// U[inner] = x + e (set_velocities! after precondition!, update_velocities.jl:79-80)
+__global__ void k_set_velocities(double *U, const double *x, const double *e, const uint8_t *inner, int n)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && inner[k]) U[k] = x[k] + e[k];
+}
+
+// mask as 0.0/1.0 <-> uint8
+__global__ void k_mask_from_double(const double *m, uint8_t *out, int n)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = m[k] > 0.5 ? 1 : 0;
+}
+
+__global__ void k_mask_to_double(const uint8_t *m, double *out, int n)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = m[k] ? 1.0 : 0.0;
+}
+
+// ===========================================================================
+// Canonical [u; v] dot-product partials: one per 16x16 block (u blocks then
+// v blocks).  b == nullptr gives sum of squares.
+template <typename T>
+__global__ void k_tile_dot(GridDims g, const T *au, const T *av, const T *bu, const T *bv, double *P)
+{
+    __shared__ double lds[256];
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int bx = blockIdx.x, by = blockIdx.y;
+    const int i = bx * TILE + lx, j = by * TILE + ly;
+    if (bx < g.nbxu && by < g.nbyu) {
+        double v = 0.0;
+        if (i < g.nxu && j < g.nyu) {
+            int k = i + g.nxu * j;
+            v = bu ? (double)au[k] * (double)bu[k] : (double)au[k] * (double)au[k];
+        }
+        double s = tree256(v, lds);
+        if (threadIdx.x == 0) P[bx + g.nbxu * by] = s;
+    }
+    if (bx < g.nbxv && by < g.nbyv) {
+        double v = 0.0;
+        if (i < g.nxv && j < g.nyv) {
+            int k = i + g.nxv * j;
+            v = bv ? (double)av[k] * (double)bv[k] : (double)av[k] * (double)av[k];
+        }
+        double s = tree256(v, lds);
+        if (threadIdx.x == 0) P[g.nbxu * g.nbyu + bx + g.nbxv * by] = s;
+    }
+}
+
+// out[slot] = sqrt(sum of partials) for each of `count` partial arrays.
+__global__ void k_finish_norms(const double *P0, const double *P1, int np, double *out)
+{
+    __shared__ double lds[256];
+    double a = lanes_sum(P0, np, lds);
+    double b = P1 ? lanes_sum(P1, np, lds) : 0.0;
+    if (threadIdx.x == 0) {
+        out[0] = sqrt(a);
+        out[1] = sqrt(b);
+    }
+}
+
+// ===========================================================================
+// update_state! steps before the solve (src/Models/update_state.jl:51-147).
+
+struct GeomArgs {
+    const uint8_t *hm, *um, *vm;
+    const double *b, *h;
+    double *s, *haf, *q;  // q: 4 quadrant fractions, 4*nh
+    double *gfh, *gfu, *gfv;
+    double *hu, *su, *hv, *sv;
+    double *wc, *dsdh, *acc, *mel;
+    const double *wc_param, *acc_param;
+    double sl, ri, rw, melt;
+};
+
+// update_surface_elevation! (:51-57), update_height_above_floatation! (:80-85)
+__global__ void k_surface_haf(GridDims g, GeomArgs a)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.nx * g.ny) return;
+    double h = a.h[k], b = a.b[k];
+    if (a.hm[k]) {
+        double s1 = b + h, s2 = a.sl + h * (1.0 - a.ri / a.rw);
+        double s = s1 > s2 ? s1 : s2;
+        if (__builtin_isnan(s1) || __builtin_isnan(s2)) s = __builtin_nan("");
+        a.s[k] = s;
+    }
+    a.haf[k] = h - (a.rw / a.ri) * (a.sl - b);
+}
+
+// pos_fraction quadrant areas (src/utilities.jl:263-360) for one cell.
+__device__ double pos_fraction_quadrant(int M, int N, const double *z1, const uint8_t *mask, int i, int j, int quad)
+{
+    int i2, j2, i3, j3;
+    if (quad == 1) { i2 = i + 1 < M ? i + 1 : M - 1; j2 = j; i3 = i; j3 = j + 1 < N ? j + 1 : N - 1; }
+    else if (quad == 2) { i2 = i; j2 = j + 1 < N ? j + 1 : N - 1; i3 = i > 0 ? i - 1 : 0; j3 = j; }
+    else if (quad == 3) { i2 = i > 0 ? i - 1 : 0; j2 = j; i3 = i; j3 = j > 0 ? j - 1 : 0; }
+    else { i2 = i; j2 = j > 0 ? j - 1 : 0; i3 = i + 1 < M ? i + 1 : M - 1; j3 = j; }
+    double v1 = z1[i + M * j];
+    double v2 = mask[i2 + M * j2] ? z1[i2 + M * j2] : v1;
+    double v3 = mask[i3 + M * j3] ? z1[i3 + M * j3] : v1;
+    double s1 = (v1 > 0) - (v1 < 0), s2 = (v2 > 0) - (v2 < 0), s3 = (v3 > 0) - (v3 < 0);
+    double area = 0.0;
+    if (!((s1 == s2) && (s1 == s3))) {
+        bool flip = fabs(v3 - v1) < fabs(v2 - v1);
+        double a, b;
+        if (!flip) {
+            a = -(v2 - v1) / (v3 - v1);
+            b = -2.0 * v1 / (v3 - v1);
+        } else {
+            a = -(v3 - v1) / (v2 - v1);
+            b = -2.0 * v1 / (v2 - v1);
+        }
+        double a1 = 0.5 * (b * b) / a;
+        double a2 = 0.5 * ((1.0 - b) * (1.0 - b)) / a;
+        double a3 = 0.5 * a + b;
+        int t1 = (b > 0.0) + (b > 1.0);
+        double ab = a + b;
+        int t2 = (ab > 0.0) + (ab > 1.0);
+        int ix = 1 + t1 + 3 * t2;
+        switch (ix) {
+        case 1: area = 0.0; break;
+        case 2: area = -a1; break;
+        case 3: area = a2 - a1; break;
+        case 4: area = a1 + a3; break;
+        case 5: area = a3; break;
+        case 6: area = a2 + a3; break;
+        case 7: area = (1.0 - a2) + a1; break;
+        case 8: area = 1.0 - a2; break;
+        default: area = 1.0; break;
+        }
+        if (b < 0.0) area = 1.0 - area;
+        if (v1 < 0.0) area = 1.0 - area;
+    }
+    if (v1 > 0.0 && v2 > 0.0 && v3 > 0.0) area = 1.0;
+    if (v1 <= 0.0 && v2 <= 0.0 && v3 <= 0.0) area = 0.0;
+    if (!mask[i + M * j]) area = 0.0;
+    return area;
+}
+
+// update_grounded_fraction (:92-99) on h, then accumulation, melt,
+// weertman_c and dsdh (:106-147)
+__global__ void k_gf_h(GridDims g, GeomArgs a)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nh = g.nx * g.ny;
+    if (k >= nh) return;
+    int i = k % g.nx, j = k / g.nx;
+    double af = 0.0;
+    for (int quad = 1; quad <= 4; ++quad) {
+        double q = pos_fraction_quadrant(g.nx, g.ny, a.haf, a.hm, i, j, quad);
+        a.q[(quad - 1) * nh + k] = q;
+        af = af + 0.25 * q;
+    }
+    a.gfh[k] = af;
+    a.acc[k] = a.acc_param[k];
+    a.mel[k] = a.melt;
+    a.wc[k] = a.wc_param[k] * af;
+    a.dsdh[k] = (1.0 - a.ri / a.rw) + (a.ri / a.rw) * af;
+}
+
+// masked h -> face averages of update_geometry_on_uv_grids! (:65-73) and the
+// u/v grounded fractions of pos_fraction (utilities.jl:367-390)
+__global__ void k_geom_uv(GridDims g, GeomArgs a)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu = g.nxu * g.nyu, nv = g.nxv * g.nyv, nh = g.nx * g.ny;
+    const int nx = g.nx, ny = g.ny;
+    if (t < nu) {
+        int i = t % g.nxu, j = t / g.nxu;
+        // grounded fraction on u: quadrants 1,4 from cell i-1, 2,3 from cell i
+        double gf = 0.0;
+        double q1 = i >= 1 ? a.q[0 * nh + (i - 1) + nx * j] : 0.0;
+        double q2 = i < nx ? a.q[1 * nh + i + nx * j] : 0.0;
+        double q3 = i < nx ? a.q[2 * nh + i + nx * j] : 0.0;
+        double q4 = i >= 1 ? a.q[3 * nh + (i - 1) + nx * j] : 0.0;
+        gf = gf + 0.25 * q1;
+        gf = gf + 0.25 * q2;
+        gf = gf + 0.25 * q3;
+        gf = gf + 0.25 * q4;
+        a.gfu[t] = gf;
+        if (a.um[t]) {
+            double den = 0.0, nh_ = 0.0, ns_ = 0.0;
+            if (i >= 1) {
+                int kk = (i - 1) + nx * j;
+                den = den + 0.5 * (double)a.hm[kk];
+                nh_ = nh_ + 0.5 * (a.hm[kk] ? a.h[kk] : 0.0);
+                ns_ = ns_ + 0.5 * (a.hm[kk] ? a.s[kk] : 0.0);
+            }
+            if (i <= nx - 1) {
+                int kk = i + nx * j;
+                den = den + 0.5 * (double)a.hm[kk];
+                nh_ = nh_ + 0.5 * (a.hm[kk] ? a.h[kk] : 0.0);
+                ns_ = ns_ + 0.5 * (a.hm[kk] ? a.s[kk] : 0.0);
+            }
+            a.hu[t] = nh_ / den;
+            a.su[t] = ns_ / den;
+        }
+    } else if (t < nu + nv) {
+        int tv = t - nu;
+        int i = tv % g.nxv, j = tv / g.nxv;
+        double gf = 0.0;
+        double q1 = j >= 1 ? a.q[0 * nh + i + nx * (j - 1)] : 0.0;
+        double q2 = j >= 1 ? a.q[1 * nh + i + nx * (j - 1)] : 0.0;
+        double q3 = j < ny ? a.q[2 * nh + i + nx * j] : 0.0;
+        double q4 = j < ny ? a.q[3 * nh + i + nx * j] : 0.0;
+        gf = gf + 0.25 * q1;
+        gf = gf + 0.25 * q2;
+        gf = gf + 0.25 * q3;
+        gf = gf + 0.25 * q4;
+        a.gfv[tv] = gf;
+        if (a.vm[tv]) {
+            double den = 0.0, nh_ = 0.0, ns_ = 0.0;
+            if (j >= 1) {
+                int kk = i + nx * (j - 1);
+                den = den + 0.5 * (double)a.hm[kk];
+                nh_ = nh_ + 0.5 * (a.hm[kk] ? a.h[kk] : 0.0);
+                ns_ = ns_ + 0.5 * (a.hm[kk] ? a.s[kk] : 0.0);
+            }
+            if (j <= ny - 1) {
+                int kk = i + nx * j;
+                den = den + 0.5 * (double)a.hm[kk];
+                nh_ = nh_ + 0.5 * (a.hm[kk] ? a.h[kk] : 0.0);
+                ns_ = ns_ + 0.5 * (a.hm[kk] ? a.s[kk] : 0.0);
+            }
+            a.hv[tv] = nh_ / den;
+            a.sv[tv] = ns_ / den;
+        }
+    }
+}
+
+// ===========================================================================
+// inner_update! (src/Models/update_velocities.jl:32-45, 202-383)
+
+struct InnerArgs {
+    const uint8_t *hm, *um, *vm, *cm;
+    const double *U, *V;
+    const double *h, *wc, *gfh, *gfu, *gfv, *dsdh, *glen_b;
+    const double *zeta, *qw;       // nsigma
+    double *sr, *avs, *bs, *beta, *tau, *eta, *etaav, *f1, *f2, *beff;
+    double *bu, *bv;
+    double *heta_h, *heta_c, *dimp;
+    double rdx, rdy;
+    double wexp, wreg2, gexp, greg2, c0;
+    int ns, niter_visc, xfirst_c2h, yfirst_h2c;
+};
+
+// shear strain (uy + vx) on the c mask at corner (a, b), 0 outside the c grid
+__device__ __forceinline__ double corner_shear(const GridDims &g, const InnerArgs &p, int a, int b)
+{
+    const int nxc = g.nx - 1, nyc = g.ny - 1;
+    if (a < 0 || a >= nxc || b < 0 || b >= nyc) return 0.0;
+    if (!p.cm[a + nxc * b]) return 0.0;
+    const int nxu = g.nxu, nx = g.nx;
+    double u1 = p.um[(a + 1) + nxu * (b + 1)] ? p.U[(a + 1) + nxu * (b + 1)] : 0.0;
+    double u0 = p.um[(a + 1) + nxu * b] ? p.U[(a + 1) + nxu * b] : 0.0;
+    double v1 = p.vm[(a + 1) + nx * (b + 1)] ? p.V[(a + 1) + nx * (b + 1)] : 0.0;
+    double v0 = p.vm[a + nx * (b + 1)] ? p.V[a + nx * (b + 1)] : 0.0;
+    double dudy = p.rdy * u1 - p.rdy * u0;
+    double dvdx = p.rdx * v1 - p.rdx * v0;
+    return dudy + dvdx;
+}
+
+// gc.cent (CGrid.jl:45) at h cell (i, j) in the reference's factor order
+__device__ __forceinline__ double cent_c_to_h(const GridDims &g, const InnerArgs &p, int i, int j)
+{
+    const int nxc = g.nx - 1, nyc = g.ny - 1;
+    double out = 0.0;
+    if (p.xfirst_c2h) {
+        if (j - 1 >= 0 && j - 1 < nyc) {
+            double t0 = 0.0;
+            if (i - 1 >= 0 && i - 1 < nxc) t0 = t0 + 0.5 * corner_shear(g, p, i - 1, j - 1);
+            if (i >= 0 && i < nxc) t0 = t0 + 0.5 * corner_shear(g, p, i, j - 1);
+            out = out + 0.5 * t0;
+        }
+        if (j >= 0 && j < nyc) {
+            double t1 = 0.0;
+            if (i - 1 >= 0 && i - 1 < nxc) t1 = t1 + 0.5 * corner_shear(g, p, i - 1, j);
+            if (i >= 0 && i < nxc) t1 = t1 + 0.5 * corner_shear(g, p, i, j);
+            out = out + 0.5 * t1;
+        }
+    } else {
+        if (i - 1 >= 0 && i - 1 < nxc) {
+            double t0 = 0.0;
+            if (j - 1 >= 0 && j - 1 < nyc) t0 = t0 + 0.5 * corner_shear(g, p, i - 1, j - 1);
+            if (j >= 0 && j < nyc) t0 = t0 + 0.5 * corner_shear(g, p, i - 1, j);
+            out = out + 0.5 * t0;
+        }
+        if (i >= 0 && i < nxc) {
+            double t1 = 0.0;
+            if (j - 1 >= 0 && j - 1 < nyc) t1 = t1 + 0.5 * corner_shear(g, p, i, j - 1);
+            if (j >= 0 && j < nyc) t1 = t1 + 0.5 * corner_shear(g, p, i, j);
+            out = out + 0.5 * t1;
+        }
+    }
+    return out;
+}
+
+// All h-cell updates of inner_update!: strain rate, speeds, beta, basal drag,
+// viscosity on the sigma levels, quadratures, beta_eff and the h-grid
+// rheological coefficients.
+__global__ void k_inner_h(GridDims g, InnerArgs p)
+{
+    const int nh = g.nx * g.ny;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nh) return;
+    const int i = k % g.nx, j = k / g.nx;
+    const int nxu = g.nxu, nx = g.nx;
+    const bool hm = p.hm[k];
+    // update_shelf_strain_rate! (:202-209)
+    double ux = 0.0, vy = 0.0, sh = 0.0;
+    double ub = 0.0, vb = 0.0;
+    {
+        double u0 = p.um[i + nxu * j] ? p.U[i + nxu * j] : 0.0;
+        double u1 = p.um[(i + 1) + nxu * j] ? p.U[(i + 1) + nxu * j] : 0.0;
+        double v0 = p.vm[i + nx * j] ? p.V[i + nx * j] : 0.0;
+        double v1 = p.vm[i + nx * (j + 1)] ? p.V[i + nx * (j + 1)] : 0.0;
+        if (hm) {
+            ux = p.rdx * u1 - p.rdx * u0;
+            vy = p.rdy * v1 - p.rdy * v0;
+            sh = cent_c_to_h(g, p, i, j);
+            ub = 0.5 * u0 + 0.5 * u1;
+            vb = 0.5 * v0 + 0.5 * v1;
+        }
+    }
+    double sr = sqrt(((ux * ux + vy * vy) + ux * vy) + 0.25 * (sh * sh));
+    p.sr[k] = sr;
+    // update_av_speed! (:216-220)
+    double avs = sqrt(ub * ub + vb * vb);
+    p.avs[k] = avs;
+    // update_bed_speed!, update_β!, update_basal_drag! (:227-255)
+    double f2_old = p.f2[k], beta_old = p.beta[k];
+    double bs = avs / (1.0 + f2_old * beta_old);
+    double beta = p.wc[k] * det_pow(sqrt(bs * bs + p.wreg2), p.wexp);
+    double tau = beta * bs;
+    p.bs[k] = bs;
+    p.beta[k] = beta;
+    p.tau[k] = tau;
+    // inner_update_viscosity!, update_av_viscosity!, update_quadrature_falpha! (:264-328)
+    double av = 0.0, q1 = 0.0, q2 = 0.0;
+    const double hk = p.h[k];
+    if (hm) {
+        for (int ks = 0; ks < p.ns; ++ks) {
+            const int k3 = k + nh * ks;
+            double eta = p.eta[k3];
+            const double zeta = p.zeta[ks];
+            const double gb = p.glen_b[k3];
+            for (int it = 0; it < p.niter_visc; ++it) {
+                double q = (tau * zeta) / eta;
+                double arg = sqrt((sr * sr + 0.25 * (q * q)) + p.greg2);
+                eta = (0.5 * gb) * det_pow(arg, p.gexp);
+            }
+            p.eta[k3] = eta;
+            const double w = p.qw[ks];
+            av = av + w * eta;
+            q1 = q1 + ((w * hk) * zeta) / eta;
+            q2 = q2 + ((w * hk) * (zeta * zeta)) / eta;
+        }
+    }
+    p.etaav[k] = av;
+    p.f1[k] = q1;
+    p.f2[k] = q2;
+    // update_βeff! (:335-339)
+    p.beff[k] = beta / (1.0 + q2 * beta);
+    // update_rheological_operators! (:374-383), h-grid parts
+    p.heta_h[k] = hm ? hk * av : 0.0;
+    p.dimp[k] = hm ? p.c0 * p.dsdh[k] : 0.0;
+}
+
+// update_βeff_on_uv_grids! (:348-366) and the c-grid coefficient
+// gc.dneghηav (:378)
+__global__ void k_inner_uvc(GridDims g, InnerArgs p)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu = g.nxu * g.nyu, nv = g.nxv * g.nyv, nc = (g.nx - 1) * (g.ny - 1);
+    const int nx = g.nx;
+    if (t < nu) {
+        if (!p.um[t]) return;
+        int i = t % g.nxu, j = t / g.nxu;
+        double den = 0.0, nb = 0.0, ng = 0.0;
+        if (i >= 1) {
+            int kk = (i - 1) + nx * j;
+            den = den + 0.5 * (double)p.hm[kk];
+            nb = nb + 0.5 * (p.hm[kk] ? p.beff[kk] : 0.0);
+            ng = ng + 0.5 * (p.hm[kk] ? p.gfh[kk] : 0.0);
+        }
+        if (i <= nx - 1) {
+            int kk = i + nx * j;
+            den = den + 0.5 * (double)p.hm[kk];
+            nb = nb + 0.5 * (p.hm[kk] ? p.beff[kk] : 0.0);
+            ng = ng + 0.5 * (p.hm[kk] ? p.gfh[kk] : 0.0);
+        }
+        double b = nb / den;
+        double ip = ng / den;
+        if (ip > 0.0) b = (b * p.gfu[t]) / ip;
+        p.bu[t] = b;
+    } else if (t < nu + nv) {
+        int tv = t - nu;
+        if (!p.vm[tv]) return;
+        int i = tv % g.nxv, j = tv / g.nxv;
+        double den = 0.0, nb = 0.0, ng = 0.0;
+        if (j >= 1) {
+            int kk = i + nx * (j - 1);
+            den = den + 0.5 * (double)p.hm[kk];
+            nb = nb + 0.5 * (p.hm[kk] ? p.beff[kk] : 0.0);
+            ng = ng + 0.5 * (p.hm[kk] ? p.gfh[kk] : 0.0);
+        }
+        if (j <= g.ny - 1) {
+            int kk = i + nx * j;
+            den = den + 0.5 * (double)p.hm[kk];
+            nb = nb + 0.5 * (p.hm[kk] ? p.beff[kk] : 0.0);
+            ng = ng + 0.5 * (p.hm[kk] ? p.gfh[kk] : 0.0);
+        }
+        double b = nb / den;
+        double ip = ng / den;
+        if (ip > 0.0) b = (b * p.gfv[tv]) / ip;
+        p.bv[tv] = b;
+    } else if (t < nu + nv + nc) {
+        int tc = t - nu - nv;
+        int i = tc % (nx - 1), j = tc / (nx - 1);
+        if (!p.cm[tc]) {
+            p.heta_c[tc] = 0.0;
+            return;
+        }
+        // gh.cent_xy (HGrid.jl:203) of h*eta_av in the reference's factor order
+        double a00 = p.h[i + nx * j] * p.etaav[i + nx * j];
+        double a10 = p.h[(i + 1) + nx * j] * p.etaav[(i + 1) + nx * j];
+        double a01 = p.h[i + nx * (j + 1)] * p.etaav[i + nx * (j + 1)];
+        double a11 = p.h[(i + 1) + nx * (j + 1)] * p.etaav[(i + 1) + nx * (j + 1)];
+        double out;
+        if (p.yfirst_h2c) {
+            double t0 = 0.5 * a00 + 0.5 * a01;
+            double t1 = 0.5 * a10 + 0.5 * a11;
+            out = 0.5 * t0 + 0.5 * t1;
+        } else {
+            double t0 = 0.5 * a00 + 0.5 * a10;
+            double t1 = 0.5 * a01 + 0.5 * a11;
+            out = 0.5 * t0 + 0.5 * t1;
+        }
+        p.heta_c[tc] = out;
+    }
+}
+
+// ===========================================================================
+// get_rhs (update_velocities.jl:106-182): f1 + f2 + f3 with f3 = -A(u_fixed)
+// passed in as the operator applied to the fixed values.
+struct RhsArgs {
+    const uint8_t *hm, *ui, *vi;
+    const double *s, *dsdh, *acc, *mel;
+    const double *hu, *su, *hv, *sv;
+    const double *f3u, *f3v;   // A applied to the fixed velocities
+    double *bu, *bv;
+    double rdx, rdy, g, ri, rw, sl, tdt;
+};
+
+__device__ __forceinline__ double surf_adj(const RhsArgs &a, int k)
+{
+    if (!a.hm[k]) return 0.0;
+    return a.s[k] + (a.tdt * a.dsdh[k]) * (a.acc[k] - a.mel[k]);
+}
+
+__global__ void k_rhs(GridDims g, RhsArgs a)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu = g.nxu * g.nyu, nv = g.nxv * g.nyv;
+    const int nx = g.nx, ny = g.ny;
+    const double rg = a.ri * a.g, hri = 0.5 * a.ri, hrw = 0.5 * a.rw;
+    if (t < nu) {
+        if (!a.ui[t]) { a.bu[t] = 0.0; return; }
+        int i = t % g.nxu, j = t / g.nxu;
+        double a1 = i < nx ? surf_adj(a, i + nx * j) : 0.0;
+        double a0 = i > 0 ? surf_adj(a, (i - 1) + nx * j) : 0.0;
+        double tt = a.rdx * a1 - a.rdx * a0;
+        double f1 = (rg * a.hu[t]) * tt;
+        double m1 = i < nx ? (a.hm[i + nx * j] ? 1.0 : 0.0) : 0.0;
+        double m0 = i > 0 ? (a.hm[(i - 1) + nx * j] ? 1.0 : 0.0) : 0.0;
+        double tm = a.rdx * m1 - a.rdx * m0;
+        double h = a.hu[t], s = a.su[t];
+        double d = h - (s - a.sl);
+        if (!(d > 0.0)) d = 0.0;
+        double X = (hri * (h * h) - hrw * (d * d)) - (a.ri * h) * s;
+        double f2 = (tm * a.g) * X;
+        a.bu[t] = (f1 + f2) + (-a.f3u[t]);
+    } else if (t < nu + nv) {
+        int tv = t - nu;
+        if (!a.vi[tv]) { a.bv[tv] = 0.0; return; }
+        int i = tv % g.nxv, j = tv / g.nxv;
+        double a1 = j < ny ? surf_adj(a, i + nx * j) : 0.0;
+        double a0 = j > 0 ? surf_adj(a, i + nx * (j - 1)) : 0.0;
+        double tt = a.rdy * a1 - a.rdy * a0;
+        double f1 = (rg * a.hv[tv]) * tt;
+        double m1 = j < ny ? (a.hm[i + nx * j] ? 1.0 : 0.0) : 0.0;
+        double m0 = j > 0 ? (a.hm[i + nx * (j - 1)] ? 1.0 : 0.0) : 0.0;
+        double tm = a.rdy * m1 - a.rdy * m0;
+        double h = a.hv[tv], s = a.sv[tv];
+        double d = h - (s - a.sl);
+        if (!(d > 0.0)) d = 0.0;
+        double X = (hri * (h * h) - hrw * (d * d)) - (a.ri * h) * s;
+        double f2 = (tm * a.g) * X;
+        a.bv[tv] = (f1 + f2) + (-a.f3v[tv]);
+    }
+}
+
+// ===========================================================================
+// Operator application, residual, diagonal, colour sweeps.
+
+// y = A x at inner points, 0 elsewhere (flat over u then v points)
+template <typename T>
+__global__ void k_op(GridDims g, OpCoef<T> c, const T *xu, const T *xv, T *yu, T *yv)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu = g.nxu * g.nyu, nv = g.nxv * g.nyv;
+    GlobalIn<T> in{xu, xv, g.nxu, g.nx};
+    if (t < nu) {
+        int i = t % g.nxu, j = t / g.nxu;
+        yu[t] = c.ui[t] ? op_u(c, in, i, j) : T(0);
+    } else if (t < nu + nv) {
+        int tv = t - nu;
+        int i = tv % g.nxv, j = tv / g.nxv;
+        yv[tv] = c.vi[tv] ? op_v(c, in, i, j) : T(0);
+    }
+}
+
+// r = b - A x at inner points (get_resid!, utilities.jl:549-552); partials of
+// sum r^2 into Pr and, if Pb != nullptr, of sum b^2 into Pb.
+template <typename T>
+__global__ void k_tile_resid(GridDims g, OpCoef<T> c, const T *xu, const T *xv, const T *bu, const T *bv,
+                             T *ru, T *rv, double *Pr, double *Pb)
+{
+    __shared__ double lds[256];
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int bx = blockIdx.x, by = blockIdx.y;
+    const int i = bx * TILE + lx, j = by * TILE + ly;
+    GlobalIn<T> in{xu, xv, g.nxu, g.nx};
+    if (bx < g.nbxu && by < g.nbyu) {
+        double r2 = 0.0, b2 = 0.0;
+        if (i < g.nxu && j < g.nyu) {
+            int k = i + g.nxu * j;
+            T r = c.ui[k] ? bu[k] - op_u(c, in, i, j) : T(0);
+            ru[k] = r;
+            r2 = (double)r * (double)r;
+            b2 = (double)bu[k] * (double)bu[k];
+        }
+        double s = tree256(r2, lds);
+        if (threadIdx.x == 0) Pr[bx + g.nbxu * by] = s;
+        if (Pb) {
+            s = tree256(b2, lds);
+            if (threadIdx.x == 0) Pb[bx + g.nbxu * by] = s;
+        }
+    }
+    if (bx < g.nbxv && by < g.nbyv) {
+        double r2 = 0.0, b2 = 0.0;
+        if (i < g.nxv && j < g.nyv) {
+            int k = i + g.nxv * j;
+            T r = c.vi[k] ? bv[k] - op_v(c, in, i, j) : T(0);
+            rv[k] = r;
+            r2 = (double)r * (double)r;
+            b2 = (double)bv[k] * (double)bv[k];
+        }
+        const int o = g.nbxu * g.nbyu + bx + g.nbxv * by;
+        double s = tree256(r2, lds);
+        if (threadIdx.x == 0) Pr[o] = s;
+        if (Pb) {
+            s = tree256(b2, lds);
+            if (threadIdx.x == 0) Pb[o] = s;
+        }
+    }
+}
+
+// diag(A) at inner points: the stencil evaluated on a unit vector, which is
+// what get_op_diag's colour probes (update_preconditioners.jl:92-109) see.
+template <typename T>
+__global__ void k_diag(GridDims g, OpCoef<T> c, T *du, T *dv)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu = g.nxu * g.nyu, nv = g.nxv * g.nyv;
+    if (t < nu) {
+        int i = t % g.nxu, j = t / g.nxu;
+        DeltaIn<T> in{i, j, 0};
+        du[t] = c.ui[t] ? op_u(c, in, i, j) : T(0);
+    } else if (t < nu + nv) {
+        int tv = t - nu;
+        int i = tv % g.nxv, j = tv / g.nxv;
+        DeltaIn<T> in{i, j, 1};
+        dv[tv] = c.vi[tv] ? op_v(c, in, i, j) : T(0);
+    }
+}
+
+// One colour of gauss_seidel_smoother! (update_preconditioners.jl:133-150):
+// x[c] += omega * (b - A x)[c] / diag[c].  Colours 1,2 are u points with
+// 1 + mod(i-j, 2) == colour, 3,4 are v points with 3 + mod(i-j, 2).  No point
+// of a colour is in the stencil of another point of the same colour, so the
+// residual evaluated here equals the reference's full recompute.
+template <typename T>
+__global__ void k_colour(GridDims g, OpCoef<T> c, T *xu, T *xv, const T *bu, const T *bv,
+                         const T *du, const T *dv, T omega, int colour)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    GlobalIn<T> in{xu, xv, g.nxu, g.nx};
+    if (colour <= 2) {
+        // half of the u points: row j, every other i
+        const int half = (g.nxu + 1) / 2;
+        int j = t / half, ii = t % half;
+        if (j >= g.nyu) return;
+        int par = (colour - 1);  // mod(i - j, 2) == par
+        int i = 2 * ii + ((j + par) & 1);
+        if (i >= g.nxu) return;
+        int k = i + g.nxu * j;
+        if (!c.ui[k]) return;
+        T r = bu[k] - op_u(c, in, i, j);
+        xu[k] = xu[k] + (omega * r) / du[k];
+    } else {
+        const int half = (g.nxv + 1) / 2;
+        int j = t / half, ii = t % half;
+        if (j >= g.nyv) return;
+        int par = (colour - 3);
+        int i = 2 * ii + ((j + par) & 1);
+        if (i >= g.nxv) return;
+        int k = i + g.nxv * j;
+        if (!c.vi[k]) return;
+        T r = bv[k] - op_v(c, in, i, j);
+        xv[k] = xv[k] + (omega * r) / dv[k];
+    }
+}
+
+// ===========================================================================
+// Haar transforms on 16x16 tiles in LDS.  dir: 0 forward, 1 inverse,
+// 2 transposed inverse.  y factor first, then x (the order KroneckerProduct
+// mul! takes for square factors); each output sums its terms in increasing
+// source index.
+
+template <typename T>
+__device__ __forceinline__ void haar_tile(const HaarTable *tab, int dir, T (*tile)[TILE + 1], T (*tmp)[TILE + 1],
+                                          int i0, int j0, int nxw, int nyw)
+{
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int B = tab->B;
+    const int i = i0 + lx, j = j0 + ly;
+    // y pass
+    if (i < nxw && j < nyw) {
+        int jb = ly - (ly % B);
+        int m = nyw - (j0 + jb);
+        m = m < B ? m : B;
+        int r = ly - jb;
+        int n = tab->nnz[dir][m][r];
+        T acc = T(0);
+        for (int q = 0; q < n; ++q)
+            acc = acc + (T)tab->coef[dir][m][r][q] * tile[jb + tab->col[dir][m][r][q]][lx];
+        tmp[ly][lx] = acc;
+    }
+    __syncthreads();
+    // x pass
+    if (i < nxw && j < nyw) {
+        int ib = lx - (lx % B);
+        int m = nxw - (i0 + ib);
+        m = m < B ? m : B;
+        int r = lx - ib;
+        int n = tab->nnz[dir][m][r];
+        T acc = T(0);
+        for (int q = 0; q < n; ++q)
+            acc = acc + (T)tab->coef[dir][m][r][q] * tmp[ly][ib + tab->col[dir][m][r][q]];
+        tile[ly][lx] = acc;
+    }
+    __syncthreads();
+}
+
+// out = mask_out ? H(mask_in ? in : 0) : 0 on the u grid (comp 0) and v grid
+template <typename Tin, typename Tout>
+__global__ void k_tile_haar(GridDims g, const HaarTable *tab, int dir,
+                            const Tin *inu, const Tin *inv, const uint8_t *min_u, const uint8_t *min_v,
+                            Tout *outu, Tout *outv, const uint8_t *mout_u, const uint8_t *mout_v)
+{
+    __shared__ Tout tile[TILE][TILE + 1];
+    __shared__ Tout tmp[TILE][TILE + 1];
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int bx = blockIdx.x, by = blockIdx.y;
+    const int i0 = bx * TILE, j0 = by * TILE;
+    const int i = i0 + lx, j = j0 + ly;
+    if (bx < g.nbxu && by < g.nbyu) {
+        int k = i + g.nxu * j;
+        bool ok = i < g.nxu && j < g.nyu;
+        tile[ly][lx] = (ok && (!min_u || min_u[k])) ? (Tout)inu[k] : Tout(0);
+        __syncthreads();
+        haar_tile<Tout>(tab, dir, tile, tmp, i0, j0, g.nxu, g.nyu);
+        if (ok) outu[k] = (!mout_u || mout_u[k]) ? tile[ly][lx] : Tout(0);
+        __syncthreads();
+    }
+    if (bx < g.nbxv && by < g.nbyv) {
+        int k = i + g.nxv * j;
+        bool ok = i < g.nxv && j < g.nyv;
+        tile[ly][lx] = (ok && (!min_v || min_v[k])) ? (Tout)inv[k] : Tout(0);
+        __syncthreads();
+        haar_tile<Tout>(tab, dir, tile, tmp, i0, j0, g.nxv, g.nyv);
+        if (ok) outv[k] = (!mout_v || mout_v[k]) ? tile[ly][lx] : Tout(0);
+        __syncthreads();
+    }
+}
+
+// update_wavelets! (src/Wavelets/update_wavelets.jl:6-32): dwt of the masked
+// velocities, threshold mask, zero warm start.
+__global__ void k_tile_wavelets(GridDims g, const HaarTable *tab, const double *U, const double *V,
+                                const uint8_t *um, const uint8_t *vm, double *Wu, double *Wv,
+                                uint8_t *mwu, uint8_t *mwv, double *cu, double *cv, double thr)
+{
+    __shared__ double tile[TILE][TILE + 1];
+    __shared__ double tmp[TILE][TILE + 1];
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int bx = blockIdx.x, by = blockIdx.y;
+    const int i0 = bx * TILE, j0 = by * TILE;
+    const int i = i0 + lx, j = j0 + ly;
+    if (bx < g.nbxu && by < g.nbyu) {
+        int k = i + g.nxu * j;
+        bool ok = i < g.nxu && j < g.nyu;
+        tile[ly][lx] = (ok && um[k]) ? U[k] : 0.0;
+        __syncthreads();
+        haar_tile<double>(tab, 0, tile, tmp, i0, j0, g.nxu, g.nyu);
+        if (ok) {
+            double w = tile[ly][lx];
+            Wu[k] = w;
+            mwu[k] = fabs(w) >= thr ? 1 : 0;
+            cu[k] = 0.0;
+        }
+        __syncthreads();
+    }
+    if (bx < g.nbxv && by < g.nbyv) {
+        int k = i + g.nxv * j;
+        bool ok = i < g.nxv && j < g.nyv;
+        tile[ly][lx] = (ok && vm[k]) ? V[k] : 0.0;
+        __syncthreads();
+        haar_tile<double>(tab, 0, tile, tmp, i0, j0, g.nxv, g.nyv);
+        if (ok) {
+            double w = tile[ly][lx];
+            Wv[k] = w;
+            mwv[k] = fabs(w) >= thr ? 1 : 0;
+            cv[k] = 0.0;
+        }
+        __syncthreads();
+    }
+}
+
+// ===========================================================================
+// Coarse CG: IterativeSolvers.cg! on A_c = R A P (update_preconditioners.jl:73)
+// with x0 the cached warm start.  Scalars live on the device; the loop is
+// three kernels per iteration and no host synchronisation.  Every workgroup
+// folds the block partials itself, so all of them hold the same scalars.
+
+struct CGState {
+    double tol;       // max(reltol ||r0||, abstol)
+    int done;         // sticky: set by the first k_cg_dir that sees convergence
+    int iters;        // iterations performed
+    int it[2];        // iteration counter, double-buffered by parity
+};
+
+template <typename T>
+struct CGArgs {
+    GridDims g;
+    OpCoef<T> c;
+    const HaarTable *tab;
+    const uint8_t *mwu, *mwv, *ui, *vi;
+    T *xu, *xv;       // solution (warm start in, result out), coarse layout
+    T *ru, *rv;       // residual
+    T *pu, *pv;       // search direction u
+    T *qu, *qv;       // c = A_c u
+    T *tu, *tv;       // P u on the fine grid
+    double *Pr[2];    // sum r^2 partials by parity
+    double *Pd;       // dot(u, c) partials
+    double *Pb;       // sum b^2 partials
+    CGState *st;
+    double tol_coarse, reltol;
+    int maxiter;
+};
+
+// A at own tile points of the fine field (tu, tv), result in LDS, then the
+// transposed inverse Haar (restriction) masked by the wavelet mask.
+template <typename T>
+__device__ __forceinline__ void tile_APR(const CGArgs<T> &a, int comp, T (*tile)[TILE + 1], T (*tmp)[TILE + 1])
+{
+    const GridDims &g = a.g;
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int i0 = blockIdx.x * TILE, j0 = blockIdx.y * TILE;
+    const int i = i0 + lx, j = j0 + ly;
+    GlobalIn<T> in{a.tu, a.tv, g.nxu, g.nx};
+    if (comp == 0) {
+        T val = T(0);
+        if (i < g.nxu && j < g.nyu && a.ui[i + g.nxu * j]) val = op_u(a.c, in, i, j);
+        tile[ly][lx] = val;
+        __syncthreads();
+        haar_tile<T>(a.tab, 2, tile, tmp, i0, j0, g.nxu, g.nyu);
+        if (i < g.nxu && j < g.nyu && !a.mwu[i + g.nxu * j]) tile[ly][lx] = T(0);
+    } else {
+        T val = T(0);
+        if (i < g.nxv && j < g.nyv && a.vi[i + g.nxv * j]) val = op_v(a.c, in, i, j);
+        tile[ly][lx] = val;
+        __syncthreads();
+        haar_tile<T>(a.tab, 2, tile, tmp, i0, j0, g.nxv, g.nyv);
+        if (i < g.nxv && j < g.nyv && !a.mwv[i + g.nxv * j]) tile[ly][lx] = T(0);
+    }
+    __syncthreads();
+}
+
+// setup: t = P x0 is computed by k_tile_haar beforehand; here c0 = R A t,
+// r = b - c0 (b is passed in ru/rv and overwritten), u = 0, partials of
+// ||r0||^2 into Pr[0] and of ||b||^2 into Pb.
+template <typename T>
+__global__ void k_cg_setup(CGArgs<T> a)
+{
+    __shared__ T tile[TILE][TILE + 1];
+    __shared__ T tmp[TILE][TILE + 1];
+    __shared__ double lds[256];
+    const GridDims &g = a.g;
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int bx = blockIdx.x, by = blockIdx.y;
+    const int i = bx * TILE + lx, j = by * TILE + ly;
+    if (bx == 0 && by == 0 && threadIdx.x == 0) {
+        a.st->done = 0;
+        a.st->iters = 0;
+        a.st->it[0] = 0;
+        a.st->it[1] = 0;
+    }
+    if (bx < g.nbxu && by < g.nbyu) {
+        tile_APR<T>(a, 0, tile, tmp);
+        double r2 = 0.0, b2 = 0.0;
+        if (i < g.nxu && j < g.nyu) {
+            int k = i + g.nxu * j;
+            T b = a.ru[k];
+            T r = b - tile[ly][lx];
+            a.ru[k] = r;
+            a.pu[k] = T(0);
+            r2 = (double)r * (double)r;
+            b2 = (double)b * (double)b;
+        }
+        double s = tree256(r2, lds);
+        if (threadIdx.x == 0) a.Pr[0][bx + g.nbxu * by] = s;
+        s = tree256(b2, lds);
+        if (threadIdx.x == 0) a.Pb[bx + g.nbxu * by] = s;
+    }
+    if (bx < g.nbxv && by < g.nbyv) {
+        tile_APR<T>(a, 1, tile, tmp);
+        double r2 = 0.0, b2 = 0.0;
+        if (i < g.nxv && j < g.nyv) {
+            int k = i + g.nxv * j;
+            T b = a.rv[k];
+            T r = b - tile[ly][lx];
+            a.rv[k] = r;
+            a.pv[k] = T(0);
+            r2 = (double)r * (double)r;
+            b2 = (double)b * (double)b;
+        }
+        const int o = g.nbxu * g.nbyu + bx + g.nbxv * by;
+        double s = tree256(r2, lds);
+        if (threadIdx.x == 0) a.Pr[0][o] = s;
+        s = tree256(b2, lds);
+        if (threadIdx.x == 0) a.Pb[o] = s;
+    }
+}
+
+// iteration, part 1: convergence test, beta, u = r + beta u, t = P u
+template <typename T>
+__global__ void k_cg_dir(CGArgs<T> a, int par)
+{
+    __shared__ T tile[TILE][TILE + 1];
+    __shared__ T tmp[TILE][TILE + 1];
+    __shared__ double lds[256];
+    if (a.st->done) return;
+    const GridDims &g = a.g;
+    const int it = a.st->it[par];
+    double residual = sqrt(lanes_sum(a.Pr[par], g.npart, lds));
+    double tol;
+    if (it == 0) {
+        double bn = sqrt(lanes_sum(a.Pb, g.npart, lds));
+        double abstol = a.tol_coarse * bn;
+        tol = a.reltol * residual;
+        if (abstol > tol) tol = abstol;
+    } else {
+        tol = a.st->tol;
+    }
+    const bool done = (it >= a.maxiter) || (residual <= tol);
+    const int bx = blockIdx.x, by = blockIdx.y;
+    if (bx == 0 && by == 0 && threadIdx.x == 0) {
+        if (it == 0) a.st->tol = tol;
+        if (done) {
+            a.st->done = 1;
+            a.st->iters = it;
+        }
+    }
+    if (done) return;
+    double prev = it == 0 ? 1.0 : sqrt(lanes_sum(a.Pr[1 - par], g.npart, lds));
+    const T beta = (T)((residual * residual) / (prev * prev));
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int i0 = bx * TILE, j0 = by * TILE;
+    const int i = i0 + lx, j = j0 + ly;
+    if (bx < g.nbxu && by < g.nbyu) {
+        int k = i + g.nxu * j;
+        bool ok = i < g.nxu && j < g.nyu;
+        T un = T(0);
+        if (ok) {
+            un = a.ru[k] + beta * a.pu[k];
+            a.pu[k] = un;
+        }
+        tile[ly][lx] = (ok && a.mwu[k]) ? un : T(0);
+        __syncthreads();
+        haar_tile<T>(a.tab, 1, tile, tmp, i0, j0, g.nxu, g.nyu);
+        if (ok) a.tu[k] = a.ui[k] ? tile[ly][lx] : T(0);
+        __syncthreads();
+    }
+    if (bx < g.nbxv && by < g.nbyv) {
+        int k = i + g.nxv * j;
+        bool ok = i < g.nxv && j < g.nyv;
+        T un = T(0);
+        if (ok) {
+            un = a.rv[k] + beta * a.pv[k];
+            a.pv[k] = un;
+        }
+        tile[ly][lx] = (ok && a.mwv[k]) ? un : T(0);
+        __syncthreads();
+        haar_tile<T>(a.tab, 1, tile, tmp, i0, j0, g.nxv, g.nyv);
+        if (ok) a.tv[k] = a.vi[k] ? tile[ly][lx] : T(0);
+        __syncthreads();
+    }
+}
+
+// iteration, part 2: c = R A t, partials of dot(u, c)
+template <typename T>
+__global__ void k_cg_apply(CGArgs<T> a)
+{
+    __shared__ T tile[TILE][TILE + 1];
+    __shared__ T tmp[TILE][TILE + 1];
+    __shared__ double lds[256];
+    if (a.st->done) return;
+    const GridDims &g = a.g;
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int bx = blockIdx.x, by = blockIdx.y;
+    const int i = bx * TILE + lx, j = by * TILE + ly;
+    if (bx < g.nbxu && by < g.nbyu) {
+        tile_APR<T>(a, 0, tile, tmp);
+        double d = 0.0;
+        if (i < g.nxu && j < g.nyu) {
+            int k = i + g.nxu * j;
+            T cval = tile[ly][lx];
+            a.qu[k] = cval;
+            d = (double)(a.pu[k] * cval);
+        }
+        double s = tree256(d, lds);
+        if (threadIdx.x == 0) a.Pd[bx + g.nbxu * by] = s;
+    }
+    if (bx < g.nbxv && by < g.nbyv) {
+        tile_APR<T>(a, 1, tile, tmp);
+        double d = 0.0;
+        if (i < g.nxv && j < g.nyv) {
+            int k = i + g.nxv * j;
+            T cval = tile[ly][lx];
+            a.qv[k] = cval;
+            d = (double)(a.pv[k] * cval);
+        }
+        double s = tree256(d, lds);
+        if (threadIdx.x == 0) a.Pd[g.nbxu * g.nbyu + bx + g.nbxv * by] = s;
+    }
+}
+
+// iteration, part 3: alpha, x += alpha u, r -= alpha c, partials of ||r||^2
+template <typename T>
+__global__ void k_cg_update(CGArgs<T> a, int par)
+{
+    __shared__ double lds[256];
+    if (a.st->done) return;
+    const GridDims &g = a.g;
+    double residual = sqrt(lanes_sum(a.Pr[par], g.npart, lds));
+    double dot = lanes_sum(a.Pd, g.npart, lds);
+    const T alpha = (T)((residual * residual) / dot);
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int bx = blockIdx.x, by = blockIdx.y;
+    const int i = bx * TILE + lx, j = by * TILE + ly;
+    double *Pn = a.Pr[1 - par];
+    if (bx == 0 && by == 0 && threadIdx.x == 0) a.st->it[1 - par] = a.st->it[par] + 1;
+    if (bx < g.nbxu && by < g.nbyu) {
+        double r2 = 0.0;
+        if (i < g.nxu && j < g.nyu) {
+            int k = i + g.nxu * j;
+            a.xu[k] = a.xu[k] + alpha * a.pu[k];
+            T r = a.ru[k] - alpha * a.qu[k];
+            a.ru[k] = r;
+            r2 = (double)r * (double)r;
+        }
+        double s = tree256(r2, lds);
+        if (threadIdx.x == 0) Pn[bx + g.nbxu * by] = s;
+    }
+    if (bx < g.nbxv && by < g.nbyv) {
+        double r2 = 0.0;
+        if (i < g.nxv && j < g.nyv) {
+            int k = i + g.nxv * j;
+            a.xv[k] = a.xv[k] + alpha * a.pv[k];
+            T r = a.rv[k] - alpha * a.qv[k];
+            a.rv[k] = r;
+            r2 = (double)r * (double)r;
+        }
+        double s = tree256(r2, lds);
+        if (threadIdx.x == 0) Pn[g.nbxu * g.nbyu + bx + g.nbxv * by] = s;
+    }
+}
+
+// update_velocities_on_h_grid! (update_state.jl:164-178) and update_dhdt!
+// (:184-189)
+__global__ void k_post_solve(GridDims g, const uint8_t *hm, const uint8_t *um, const uint8_t *vm,
+                             const double *U, const double *V, const double *beta, const double *f1,
+                             const double *f2, const double *hu, const double *hv, const double *acc,
+                             const double *mel, double *uh, double *vh, double *ubo, double *vbo,
+                             double *uso, double *vso, double *dhdt, double rdx, double rdy)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.nx * g.ny) return;
+    const int i = k % g.nx, j = k / g.nx;
+    const int nxu = g.nxu, nx = g.nx;
+    double u_ = 0.5 * U[i + nxu * j] + 0.5 * U[(i + 1) + nxu * j];
+    double v_ = 0.5 * V[i + nx * j] + 0.5 * V[i + nx * (j + 1)];
+    uh[k] = u_;
+    vh[k] = v_;
+    double ub = u_ / (1.0 + beta[k] * f2[k]);
+    double vb = v_ / (1.0 + beta[k] * f2[k]);
+    ubo[k] = ub;
+    vbo[k] = vb;
+    uso[k] = ub * (1.0 + beta[k] * f1[k]);
+    vso[k] = vb * (1.0 + beta[k] * f1[k]);
+    if (hm[k]) {
+        int u1 = (i + 1) + nxu * j, u0 = i + nxu * j, v1 = i + nx * (j + 1), v0 = i + nx * j;
+        double q1 = um[u1] ? hu[u1] * U[u1] : 0.0;
+        double q0 = um[u0] ? hu[u0] * U[u0] : 0.0;
+        double p1 = vm[v1] ? hv[v1] * V[v1] : 0.0;
+        double p0 = vm[v0] ? hv[v0] * V[v0] : 0.0;
+        double div = (rdx * q1 - rdx * q0) + (rdy * p1 - rdy * p0);
+        dhdt[k] = (acc[k] - mel[k]) - div;
+    }
+}
+
+// update_thickness! (run_simulation.jl:24-42), evolveShelves = true
+__global__ void k_update_thickness(int nh, const uint8_t *hm, const uint8_t *hfix, double *h, const double *dhdt,
+                                   double hmin, double dt)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nh || !hm[k]) return;
+    double a = hmin - h[k], b = dt * dhdt[k];
+    double upd = a > b ? a : b;
+    if (hfix[k]) upd = 0.0;
+    h[k] = h[k] + upd;
+}
+
+// quad_f2 = h/(3 eta_av) on the h mask (src/Fields/HGrid.jl:229)
+__global__ void k_init_quad_f2(int nh, const uint8_t *hm, const double *h, const double *etaav, double *f2)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nh && hm[k]) f2[k] = h[k] / (3.0 * etaav[k]);
+}
+
+// device math probe: fn 0 det_pow(x, y), 1 sqrt(x), 2 x / y
+__global__ void k_math_probe(int fn, const double *x, const double *y, double *out, int n)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (fn == 0) out[k] = det_pow(x[k], y[k]);
+    else if (fn == 1) out[k] = sqrt(x[k]);
+    else out[k] = x[k] / y[k];
+}
+
+}  // namespace wavi
