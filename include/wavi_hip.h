@@ -202,6 +202,23 @@ int wavi_vcycle(wavi_ctx *ctx, const double *ru, const double *rv,
  * 2: transposed inverse idwt^T (UWavelets.jl:49). */
 int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out);
 
+/* Kernel timing with HIP events on the context's stream, for measurement.
+ * kernel: WAVI_KT_CG_APPLY (coarse CG: A then R on one tile, dot partials),
+ * WAVI_KT_CG_DIR (coarse CG: direction update and prolongation),
+ * WAVI_KT_CG_UPDATE (coarse CG: axpys and norm partials), WAVI_KT_COLOUR
+ * (one smoother colour sweep).  enable=1 starts (and clears) the
+ * accumulation; every 16th launch of each kernel is timed, and
+ * wavi_profile_read returns the total milliseconds and the number of timed
+ * launches.
+ * No reference counterpart. */
+#define WAVI_KT_CG_APPLY 0
+#define WAVI_KT_CG_DIR 1
+#define WAVI_KT_CG_UPDATE 2
+#define WAVI_KT_COLOUR 3
+#define WAVI_KT_COUNT 4
+int wavi_profile_enable(wavi_ctx *ctx, int enable);
+int wavi_profile_read(wavi_ctx *ctx, int kernel, double *total_ms, int64_t *launches);
+
 /* Device arithmetic probe on host arrays of n elements: fn 0 out = x^y by the
  * library's deterministic power function, 1 out = sqrt(x), 2 out = x / y.
  * No reference counterpart: it checks that the device rounds as the CPU does. */

@@ -21,6 +21,8 @@ PRECISION = {"fp64": 0, "mixed": 1}
 
 REPORT_MAX_PICARD = 64
 
+KT = {"cg_apply": 0, "cg_dir": 1, "cg_update": 2, "colour": 3}
+
 # enum wavi_field, in header order
 FIELD_NAMES = [
     "H_B", "H_H", "H_S", "H_DHDT", "H_ACCUMULATION", "H_BASAL_MELT", "H_HAF",
@@ -121,6 +123,8 @@ SIGNATURES = {
     "wavi_op_diag": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "wavi_vcycle": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int32)]),
     "wavi_haar": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "wavi_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "wavi_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "wavi_math_probe": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]),
 }
 

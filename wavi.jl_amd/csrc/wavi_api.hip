@@ -46,15 +46,63 @@ struct wavi_ctx {
     double *quad = nullptr;  // 4 * nh pos_fraction quadrants
     double *norms = nullptr; // device: 2 doubles
     double *h_norms = nullptr; // pinned host: 2 doubles
-    CGState *st = nullptr;
-    CGState *h_st = nullptr;  // pinned host copy
+    CGScalars *st = nullptr;
+    CGScalars *h_st = nullptr;  // pinned host copy
+    int nblk = 0;               // workgroups of the XCD-ordered tile kernels
     HaarTable *tab = nullptr;
     // saved state
     std::vector<double *> saved;
     bool has_saved = false;
     uint8_t *saved_mwu = nullptr, *saved_mwv = nullptr;
+    // kernel timing (wavi_profile_*)
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_pending;  // (kernel id, first event index)
+    double prof_ms[WAVI_KT_COUNT] = {};
+    int64_t prof_n[WAVI_KT_COUNT] = {};
+    int64_t prof_seen[WAVI_KT_COUNT] = {};  // launches seen; every 16th is timed
     std::string err;
 };
+
+// Timed launch: when profiling is on, record events around the launch.
+struct ProfScope {
+    wavi_ctx *ctx;
+    int kid;
+    int idx = -1;
+    ProfScope(wavi_ctx *c, int k) : ctx(c), kid(k)
+    {
+        if (!ctx->prof) return;
+        if ((ctx->prof_seen[kid]++ & 15) != 0) return;
+        size_t need = (ctx->ev_pending.size() + 1) * 2;
+        while (ctx->ev_pool.size() < need) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            ctx->ev_pool.push_back(e);
+        }
+        idx = (int)ctx->ev_pending.size() * 2;
+        (void)hipEventRecord(ctx->ev_pool[idx], ctx->stream);
+    }
+    ~ProfScope()
+    {
+        if (idx < 0) return;
+        (void)hipEventRecord(ctx->ev_pool[idx + 1], ctx->stream);
+        ctx->ev_pending.push_back({kid, idx});
+    }
+};
+
+// After a stream synchronisation: fold the recorded event pairs.
+static void prof_collect(wavi_ctx *ctx)
+{
+    if (ctx->ev_pending.empty()) return;
+    for (auto &p : ctx->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev_pool[p.second], ctx->ev_pool[p.second + 1]) == hipSuccess) {
+            ctx->prof_ms[p.first] += ms;
+            ctx->prof_n[p.first] += 1;
+        }
+    }
+    ctx->ev_pending.clear();
+}
 
 #define HIPCHK(expr)                                                                        \
     do {                                                                                    \
@@ -169,6 +217,32 @@ static int build_haar_table(int levels, HaarTable &t, std::string &err)
             }
     }
     return WAVI_OK;
+}
+
+// +-1 Haar rows as bit masks for levels 0..3 (inverse, transposed inverse)
+static void build_haar_masks(HaarMasks tabs[4])
+{
+    memset(tabs, 0, 4 * sizeof(HaarMasks));
+    for (int lev = 0; lev < 4; ++lev) {
+        const int B = 2 << lev;
+        for (int m = 1; m <= B; ++m) {
+            std::vector<double> Wi;
+            wavelet_matrix(m, lev, false, Wi);
+            for (int dir = 0; dir < 2; ++dir)
+                for (int r = 0; r < m; ++r) {
+                    uint16_t nz = 0, neg = 0;
+                    for (int c = 0; c < m; ++c) {
+                        double w = dir == 0 ? Wi[(size_t)r * m + c] : Wi[(size_t)c * m + r];
+                        if (w != 0.0) {
+                            nz |= (uint16_t)(1u << c);
+                            if (w < 0.0) neg |= (uint16_t)(1u << c);
+                        }
+                    }
+                    tabs[lev].nz[dir][m][r] = nz;
+                    tabs[lev].neg[dir][m][r] = neg;
+                }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -357,9 +431,15 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
     CREATECHK(zalloc(&ctx->quad, 4 * (size_t)ctx->nh));
     CREATECHK(zalloc(&ctx->norms, 2));
     CREATECHK(hipHostMalloc((void **)&ctx->h_norms, 2 * sizeof(double), hipHostMallocDefault));
-    CREATECHK(hipMalloc(&ctx->st, sizeof(CGState)));
-    CREATECHK(hipMemset(ctx->st, 0, sizeof(CGState)));
-    CREATECHK(hipHostMalloc((void **)&ctx->h_st, sizeof(CGState), hipHostMallocDefault));
+    CREATECHK(hipMalloc(&ctx->st, sizeof(CGScalars)));
+    CREATECHK(hipMemset(ctx->st, 0, sizeof(CGScalars)));
+    CREATECHK(hipHostMalloc((void **)&ctx->h_st, sizeof(CGScalars), hipHostMallocDefault));
+    ctx->nblk = ((g.nbx * g.nby + 7) / 8) * 8;
+    {
+        HaarMasks hm_tab[4];
+        build_haar_masks(hm_tab);
+        CREATECHK(hipMemcpyToSymbol(HIP_SYMBOL(c_haar), hm_tab, sizeof(hm_tab), 0, hipMemcpyHostToDevice));
+    }
     HaarTable tab;
     if (build_haar_table(solver->levels, tab, ctx->err) != WAVI_OK) return fail(WAVI_EARG);
     CREATECHK(hipMalloc(&ctx->tab, sizeof(HaarTable)));
@@ -392,6 +472,7 @@ int wavi_ctx_destroy(wavi_ctx *ctx)
     if (ctx->tab) (void)hipFree(ctx->tab);
     if (ctx->h_norms) (void)hipHostFree(ctx->h_norms);
     if (ctx->h_st) (void)hipHostFree(ctx->h_st);
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return WAVI_OK;
@@ -643,6 +724,7 @@ static int do_smoother(wavi_ctx *ctx, const DevVec &x, const DevVec &b, const in
         for (int q = 0; q < 4; ++q) {
             int col = order[q];
             int n = col <= 2 ? nthreads_u : nthreads_v;
+            ProfScope ps(ctx, WAVI_KT_COLOUR);
             hipLaunchKernelGGL(k_colour<double>, flat(n), dim3(256), 0, ctx->stream, g, c, x.u, x.v, b.u, b.v,
                                ctx->d.u, ctx->d.v, ctx->S.smoother_omega, col);
         }
@@ -655,7 +737,7 @@ static CGArgs<double> cg_args(wavi_ctx *ctx)
     CGArgs<double> a;
     a.g = ctx->g;
     a.c = coef64(ctx);
-    a.tab = ctx->tab;
+    a.levels = ctx->S.levels;
     a.mwu = ctx->mwu;
     a.mwv = ctx->mwv;
     a.ui = ctx->ui;
@@ -670,14 +752,14 @@ static CGArgs<double> cg_args(wavi_ctx *ctx)
     a.qv = ctx->cq.v;
     a.tu = ctx->ct.u;
     a.tv = ctx->ct.v;
-    a.Pr[0] = ctx->Pr0;
-    a.Pr[1] = ctx->Pr1;
+    a.Pr = ctx->Pr0;
     a.Pd = ctx->Pd;
     a.Pb = ctx->Pb;
     a.st = ctx->st;
     a.tol_coarse = ctx->S.tol_coarse;
     a.reltol = std::sqrt(2.220446049250313e-16);  // IterativeSolvers' default sqrt(eps)
     a.maxiter = ctx->S.maxiter_coarse;
+    a.nblk = ctx->nblk;
     return a;
 }
 
@@ -685,26 +767,36 @@ static CGArgs<double> cg_args(wavi_ctx *ctx)
 static int do_coarse_cg(wavi_ctx *ctx, int *iters)
 {
     const GridDims &g = ctx->g;
-    dim3 tg(g.nbx, g.nby);
+    dim3 tg(g.nbx, g.nby), tb(ctx->nblk);
     CGArgs<double> a = cg_args(ctx);
     // t = P x0
-    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, 1, a.xu, a.xv,
-                       ctx->mwu, ctx->mwv, a.tu, a.tv, ctx->ui, ctx->vi);
-    hipLaunchKernelGGL(k_cg_setup<double>, tg, dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1,
+                       a.xu, a.xv, ctx->mwu, ctx->mwv, a.tu, a.tv, ctx->ui, ctx->vi);
+    hipLaunchKernelGGL(k_cg_setup<double>, tb, dim3(256), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
     const int CHUNK = 32;
     int q = 0;
     const int qmax = ctx->S.maxiter_coarse + 1;
     while (true) {
         for (int s = 0; s < CHUNK; ++s, ++q) {
-            int par = q & 1;
-            hipLaunchKernelGGL(k_cg_dir<double>, tg, dim3(256), 0, ctx->stream, a, par);
-            hipLaunchKernelGGL(k_cg_apply<double>, tg, dim3(256), 0, ctx->stream, a);
-            hipLaunchKernelGGL(k_cg_update<double>, tg, dim3(256), 0, ctx->stream, a, par);
+            const int par = q & 1;
+            {
+                ProfScope ps(ctx, WAVI_KT_CG_DIR);
+                hipLaunchKernelGGL(k_cg_dir<double>, tb, dim3(256), 0, ctx->stream, a, par);
+            }
+            {
+                ProfScope ps(ctx, WAVI_KT_CG_APPLY);
+                hipLaunchKernelGGL(k_cg_apply<double>, tb, dim3(256), 0, ctx->stream, a);
+            }
+            {
+                ProfScope ps(ctx, WAVI_KT_CG_UPDATE);
+                hipLaunchKernelGGL(k_cg_update<double>, tb, dim3(256), 0, ctx->stream, a, par);
+            }
         }
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(ctx->h_st, ctx->st, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->h_st, ctx->st, sizeof(CGScalars), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        prof_collect(ctx);
         if (ctx->h_st->done) break;
         if (q > qmax + CHUNK) {
             ctx->err = "coarse CG did not terminate";
@@ -729,12 +821,12 @@ static int do_vcycle(wavi_ctx *ctx, int *cg_iters)
     hipLaunchKernelGGL(k_tile_resid<double>, tg, dim3(256), 0, ctx->stream, g, coef64(ctx), ctx->e.u, ctx->e.v,
                        ctx->r.u, ctx->r.v, ctx->rho.u, ctx->rho.v, ctx->Pn, (double *)nullptr);
     // b_c = R rho, into the CG residual buffer
-    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, 2, ctx->rho.u,
+    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 2, ctx->rho.u,
                        ctx->rho.v, ctx->ui, ctx->vi, ctx->cr.u, ctx->cr.v, ctx->mwu, ctx->mwv);
     HIPCHK(hipGetLastError());
     CHK(do_coarse_cg(ctx, cg_iters));
     // e += P x_c
-    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, 1,
+    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1,
                        ctx->f[WAVI_WU_CORRECTION_COARSE], ctx->f[WAVI_WV_CORRECTION_COARSE], ctx->mwu, ctx->mwv,
                        ctx->t.u, ctx->t.v, ctx->ui, ctx->vi);
     hipLaunchKernelGGL(k_add_masked<double>, flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->e.u, ctx->t.u, ctx->ui,
@@ -767,6 +859,7 @@ static int do_precondition(wavi_ctx *ctx, int *converged, double *rel, int *cg_i
     HIPCHK(hipMemcpyAsync(ctx->h_norms, ctx->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
     *rel = ctx->h_norms[0] / ctx->h_norms[1];
     *converged = *rel < ctx->S.tol_picard;
     *cg_iters = 0;
@@ -1030,12 +1123,37 @@ int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out)
     double *dout = comp == 0 ? ctx->t.u : ctx->t.v;
     HIPCHK(hipMemcpy(din, in, n * sizeof(double), hipMemcpyHostToDevice));
     dim3 tg(ctx->g.nbx, ctx->g.nby);
-    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, ctx->g, ctx->tab, dir,
+    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, ctx->g, ctx->tab, ctx->S.levels, dir,
                        ctx->fixedv.u, ctx->fixedv.v, (const uint8_t *)nullptr, (const uint8_t *)nullptr, ctx->t.u,
                        ctx->t.v, (const uint8_t *)nullptr, (const uint8_t *)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost));
+    return WAVI_OK;
+}
+
+int wavi_profile_enable(wavi_ctx *ctx, int enable)
+{
+    if (!ctx) return WAVI_EARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    ctx->prof = enable != 0;
+    if (enable)
+        for (int k = 0; k < WAVI_KT_COUNT; ++k) {
+            ctx->prof_ms[k] = 0.0;
+            ctx->prof_n[k] = 0;
+            ctx->prof_seen[k] = 0;
+        }
+    return WAVI_OK;
+}
+
+int wavi_profile_read(wavi_ctx *ctx, int kernel, double *total_ms, int64_t *launches)
+{
+    if (!ctx || kernel < 0 || kernel >= WAVI_KT_COUNT || !total_ms || !launches) return WAVI_EARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    *total_ms = ctx->prof_ms[kernel];
+    *launches = ctx->prof_n[kernel];
     return WAVI_OK;
 }
 

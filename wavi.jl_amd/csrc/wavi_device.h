@@ -126,6 +126,33 @@ __device__ __forceinline__ T tree256(T v, T *lds)
     return r;  // valid in thread 0
 }
 
+// Two independent trees in one pass (same pairing for each component).
+__device__ __forceinline__ void tree256x2(double &a, double &b, double *lds)
+{
+    const int t = threadIdx.x;
+    lds[t] = a;
+    lds[t + 256] = b;
+    __syncthreads();
+    if (t < 128) {
+        lds[t] = lds[t] + lds[t + 128];
+        lds[t + 256] = lds[t + 256] + lds[t + 384];
+    }
+    __syncthreads();
+    double ra = 0.0, rb = 0.0;
+    if (t < 64) {
+        ra = lds[t] + lds[t + 64];
+        rb = lds[t + 256] + lds[t + 320];
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1) {
+            ra = ra + __shfl_down(ra, s, 64);
+            rb = rb + __shfl_down(rb, s, 64);
+        }
+    }
+    __syncthreads();
+    a = ra;
+    b = rb;  // valid in thread 0
+}
+
 // Fold `np` block partials into 256 lanes (lane l sums P[l], P[l+256], ...
 // in order) and sum the lanes with tree256.  Every workgroup that calls this
 // obtains the same bits; the result is broadcast to all threads.

@@ -715,38 +715,6 @@ __device__ __forceinline__ void haar_tile(const HaarTable *tab, int dir, T (*til
     __syncthreads();
 }
 
-// out = mask_out ? H(mask_in ? in : 0) : 0 on the u grid (comp 0) and v grid
-template <typename Tin, typename Tout>
-__global__ void k_tile_haar(GridDims g, const HaarTable *tab, int dir,
-                            const Tin *inu, const Tin *inv, const uint8_t *min_u, const uint8_t *min_v,
-                            Tout *outu, Tout *outv, const uint8_t *mout_u, const uint8_t *mout_v)
-{
-    __shared__ Tout tile[TILE][TILE + 1];
-    __shared__ Tout tmp[TILE][TILE + 1];
-    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int bx = blockIdx.x, by = blockIdx.y;
-    const int i0 = bx * TILE, j0 = by * TILE;
-    const int i = i0 + lx, j = j0 + ly;
-    if (bx < g.nbxu && by < g.nbyu) {
-        int k = i + g.nxu * j;
-        bool ok = i < g.nxu && j < g.nyu;
-        tile[ly][lx] = (ok && (!min_u || min_u[k])) ? (Tout)inu[k] : Tout(0);
-        __syncthreads();
-        haar_tile<Tout>(tab, dir, tile, tmp, i0, j0, g.nxu, g.nyu);
-        if (ok) outu[k] = (!mout_u || mout_u[k]) ? tile[ly][lx] : Tout(0);
-        __syncthreads();
-    }
-    if (bx < g.nbxv && by < g.nbyv) {
-        int k = i + g.nxv * j;
-        bool ok = i < g.nxv && j < g.nyv;
-        tile[ly][lx] = (ok && (!min_v || min_v[k])) ? (Tout)inv[k] : Tout(0);
-        __syncthreads();
-        haar_tile<Tout>(tab, dir, tile, tmp, i0, j0, g.nxv, g.nyv);
-        if (ok) outv[k] = (!mout_v || mout_v[k]) ? tile[ly][lx] : Tout(0);
-        __syncthreads();
-    }
-}
-
 // update_wavelets! (src/Wavelets/update_wavelets.jl:6-32): dwt of the masked
 // velocities, threshold mask, zero warm start.
 __global__ void k_tile_wavelets(GridDims g, const HaarTable *tab, const double *U, const double *V,
@@ -790,262 +758,495 @@ __global__ void k_tile_wavelets(GridDims g, const HaarTable *tab, const double *
 }
 
 // ===========================================================================
-// Coarse CG: IterativeSolvers.cg! on A_c = R A P (update_preconditioners.jl:73)
-// with x0 the cached warm start.  Scalars live on the device; the loop is
-// three kernels per iteration and no host synchronisation.  Every workgroup
-// folds the block partials itself, so all of them hold the same scalars.
+// Haar transforms with +-1 coefficients (the inverse and its transpose): each
+// composed row is a pair of 16-bit masks, bit c of `nz` marking a term in
+// column c and bit c of `neg` its sign, one table per `levels` (0..3).
+// Terms are summed in increasing column order, as the oracle sums them.
+struct HaarMasks {
+    uint16_t nz[2][17][16];
+    uint16_t neg[2][17][16];
+};
+__constant__ HaarMasks c_haar[4];
 
-struct CGState {
+template <typename T>
+__device__ __forceinline__ T haar_pm_row(uint32_t nz, uint32_t neg, const T *src, int stride)
+{
+    T acc = T(0);
+    while (nz) {
+        const int c = __builtin_ctz(nz);
+        const T v = src[c * stride];
+        acc = ((neg >> c) & 1u) ? acc - v : acc + v;
+        nz &= nz - 1u;
+    }
+    return acc;
+}
+
+// tile[ly][lx] <- (Wy (x) Wx) tile with W the inverse (dir 0) or its
+// transpose (dir 1): y pass into tmp, x pass back into tile.
+template <typename T>
+__device__ __forceinline__ void haar_pm_tile(int levels, int dir, T (*tile)[TILE + 1], T (*tmp)[TILE + 1], int i0,
+                                             int j0, int nxw, int nyw)
+{
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int B = 2 << levels;
+    const int i = i0 + lx, j = j0 + ly;
+    if (i < nxw && j < nyw) {
+        const int jb = ly & ~(B - 1);
+        const int m = min(nyw - (j0 + jb), B);
+        const int r = ly - jb;
+        tmp[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][r], c_haar[levels].neg[dir][m][r], &tile[jb][lx],
+                                     TILE + 1);
+    }
+    __syncthreads();
+    if (i < nxw && j < nyw) {
+        const int ib = lx & ~(B - 1);
+        const int m = min(nxw - (i0 + ib), B);
+        const int r = lx - ib;
+        tile[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][r], c_haar[levels].neg[dir][m][r], &tmp[ly][ib], 1);
+    }
+    __syncthreads();
+}
+
+// The same transform on the u tile and the v tile of one block at once.
+template <typename T>
+__device__ __forceinline__ void haar_pm_tile2(int levels, int dir, T (*tu)[TILE + 1], T (*mu)[TILE + 1],
+                                              T (*tv)[TILE + 1], T (*mv)[TILE + 1], int i0, int j0, const GridDims &g,
+                                              bool has_u, bool has_v)
+{
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int B = 2 << levels;
+    const int i = i0 + lx, j = j0 + ly;
+    const int jb = ly & ~(B - 1), ib = lx & ~(B - 1);
+    const int ry = ly - jb, rx = lx - ib;
+    if (has_u && i < g.nxu && j < g.nyu) {
+        const int m = min(g.nyu - (j0 + jb), B);
+        mu[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][ry], c_haar[levels].neg[dir][m][ry], &tu[jb][lx], TILE + 1);
+    }
+    if (has_v && i < g.nxv && j < g.nyv) {
+        const int m = min(g.nyv - (j0 + jb), B);
+        mv[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][ry], c_haar[levels].neg[dir][m][ry], &tv[jb][lx], TILE + 1);
+    }
+    __syncthreads();
+    if (has_u && i < g.nxu && j < g.nyu) {
+        const int m = min(g.nxu - (i0 + ib), B);
+        tu[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][rx], c_haar[levels].neg[dir][m][rx], &mu[ly][ib], 1);
+    }
+    if (has_v && i < g.nxv && j < g.nyv) {
+        const int m = min(g.nxv - (i0 + ib), B);
+        tv[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][rx], c_haar[levels].neg[dir][m][rx], &mv[ly][ib], 1);
+    }
+    __syncthreads();
+}
+
+// ===========================================================================
+// The operator on one 16x16 tile of u and v outputs, every intermediate
+// computed once per tile in LDS.  U and V are staged with a one-point halo
+// (18 x 18, s.U[y][x] = U(i0-1+x, j0-1+y)); r_xx, r_yy and the implicit
+// term on the 17 x 17 h cells and r_xy on the 17 x 17 c corners starting one
+// point before the tile.  Arithmetic is that of op_u / op_v (same expressions,
+// same order, absent terms zero), so results are bit-identical.
+template <typename T>
+struct OpTile {
+    T U[18][19];
+    T V[18][19];
+    T rxx[17][18];
+    T ryy[17][18];
+    T ext[17][18];
+    T rxy[17][18];
+};
+
+template <typename T>
+__device__ __forceinline__ void optile_load(const GridDims &g, const T *U, const T *V, int i0, int j0, OpTile<T> &s)
+{
+    for (int q = threadIdx.x; q < 18 * 18; q += 256) {
+        const int x = q % 18, y = q / 18;
+        const int i = i0 - 1 + x, j = j0 - 1 + y;
+        s.U[y][x] = (i >= 0 && i < g.nxu && j >= 0 && j < g.nyu) ? U[i + g.nxu * j] : T(0);
+        s.V[y][x] = (i >= 0 && i < g.nxv && j >= 0 && j < g.nyv) ? V[i + g.nxv * j] : T(0);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void optile_mid(const GridDims &g, const OpCoef<T> &c, int i0, int j0, OpTile<T> &s)
+{
+    const int nx = g.nx, ny = g.ny, nxu = g.nxu;
+    for (int q = threadIdx.x; q < 17 * 17; q += 256) {
+        const int x = q % 17, y = q / 17;
+        const int a = i0 - 1 + x, b = j0 - 1 + y;
+        T rxx = T(0), ryy = T(0), ext = T(0), rxy = T(0);
+        if (a >= 0 && a < nx && b >= 0 && b < ny) {
+            const int k = a + nx * b;
+            const T u0 = s.U[y][x], u1 = s.U[y][x + 1], v0 = s.V[y][x], v1 = s.V[y + 1][x];
+            if (c.hm[k]) {
+                const T dudx = c.rdx * u1 - c.rdx * u0;
+                const T dvdy = c.rdy * v1 - c.rdy * v0;
+                rxx = T(2) * (c.heta_h[k] * (T(2) * dudx + dvdy));
+                ryy = T(2) * (c.heta_h[k] * (T(2) * dvdy + dudx));
+            }
+            const int ku0 = a + nxu * b, kv0 = a + nx * b, kv1 = kv0 + nx;
+            const T qx1 = c.um[ku0 + 1] ? c.hu[ku0 + 1] * u1 : T(0);
+            const T qx0 = c.um[ku0] ? c.hu[ku0] * u0 : T(0);
+            const T qy1 = c.vm[kv1] ? c.hv[kv1] * v1 : T(0);
+            const T qy0 = c.vm[kv0] ? c.hv[kv0] * v0 : T(0);
+            const T divq = (c.rdx * qx1 - c.rdx * qx0) + (c.rdy * qy1 - c.rdy * qy0);
+            ext = c.dimp[k] * divq;
+        }
+        if (a >= 0 && a < nx - 1 && b >= 0 && b < ny - 1) {
+            const int kc = a + (nx - 1) * b;
+            if (c.cm[kc]) {
+                const T dudy = c.rdy * s.U[y + 1][x + 1] - c.rdy * s.U[y][x + 1];
+                const T dvdx = c.rdx * s.V[y + 1][x + 1] - c.rdx * s.V[y + 1][x];
+                rxy = c.heta_c[kc] * (dudy + dvdx);
+            }
+        }
+        s.rxx[y][x] = rxx;
+        s.ryy[y][x] = ryy;
+        s.ext[y][x] = ext;
+        s.rxy[y][x] = rxy;
+    }
+}
+
+// fx at u point (i, j) = (i0 + x, j0 + y)
+template <typename T>
+__device__ __forceinline__ T optile_u(const GridDims &g, const OpCoef<T> &c, const OpTile<T> &s, int i, int j, int x,
+                                      int y)
+{
+    const int k = i + g.nxu * j;
+    const T d_rxx_dx = c.rdx * s.rxx[y + 1][x + 1] - c.rdx * s.rxx[y + 1][x];
+    const T d_rxy_dy = c.rdy * s.rxy[y + 1][x] - c.rdy * s.rxy[y][x];
+    const T taubx = c.um[k] ? c.bu[k] * s.U[y + 1][x + 1] : T(0);
+    const T d_extra_dx = c.rdx * s.ext[y + 1][x + 1] - c.rdx * s.ext[y + 1][x];
+    return ((d_rxx_dx + d_rxy_dy) - taubx) - c.hu[k] * d_extra_dx;
+}
+
+// fy at v point (i, j) = (i0 + x, j0 + y)
+template <typename T>
+__device__ __forceinline__ T optile_v(const GridDims &g, const OpCoef<T> &c, const OpTile<T> &s, int i, int j, int x,
+                                      int y)
+{
+    const int k = i + g.nxv * j;
+    const T d_ryy_dy = c.rdy * s.ryy[y + 1][x + 1] - c.rdy * s.ryy[y][x + 1];
+    const T d_rxy_dx = c.rdx * s.rxy[y][x + 1] - c.rdx * s.rxy[y][x];
+    const T tauby = c.vm[k] ? c.bv[k] * s.V[y + 1][x + 1] : T(0);
+    const T d_extra_dy = c.rdy * s.ext[y + 1][x + 1] - c.rdy * s.ext[y][x + 1];
+    return ((d_ryy_dy + d_rxy_dx) - tauby) - c.hv[k] * d_extra_dy;
+}
+
+// ===========================================================================
+// Coarse CG: IterativeSolvers.cg! on A_c = R A P (update_preconditioners.jl:73)
+// with x0 the cached warm start: three kernels per iteration and no host
+// round trip.  Kernels that end in a reduction write one partial per 16x16
+// block; the next kernel folds them with the canonical tree in every
+// workgroup (all get the same bits), so no workgroup ever waits for another
+// inside a kernel.  Scalars that a kernel publishes are double-buffered by
+// iteration parity; every workgroup writes the same value.
+
+struct CGScalars {
+    double res[2];    // residual norm by iteration parity
     double tol;       // max(reltol ||r0||, abstol)
-    int done;         // sticky: set by the first k_cg_dir that sees convergence
-    int iters;        // iterations performed
-    int it[2];        // iteration counter, double-buffered by parity
+    int it[2];        // iterations done, by parity
+    int done;         // sticky convergence / maxiter flag
+    int iters;        // iterations at termination
 };
 
 template <typename T>
 struct CGArgs {
     GridDims g;
     OpCoef<T> c;
-    const HaarTable *tab;
+    int levels;
     const uint8_t *mwu, *mwv, *ui, *vi;
-    T *xu, *xv;       // solution (warm start in, result out), coarse layout
-    T *ru, *rv;       // residual
+    T *xu, *xv;       // solution (warm start in, result out)
+    T *ru, *rv;       // residual (right-hand side on entry to setup)
     T *pu, *pv;       // search direction u
     T *qu, *qv;       // c = A_c u
     T *tu, *tv;       // P u on the fine grid
-    double *Pr[2];    // sum r^2 partials by parity
-    double *Pd;       // dot(u, c) partials
-    double *Pb;       // sum b^2 partials
-    CGState *st;
+    double *Pr, *Pd, *Pb;
+    CGScalars *st;
     double tol_coarse, reltol;
     int maxiter;
+    int nblk;         // launched workgroups, a multiple of 8
 };
 
-// A at own tile points of the fine field (tu, tv), result in LDS, then the
-// transposed inverse Haar (restriction) masked by the wavelet mask.
+// Workgroup-uniform read of a flag that other workgroups of the same kernel
+// may be writing: thread 0 reads it once and broadcasts it through LDS.
+__device__ __forceinline__ int uniform_flag(const int *p, int *slot)
+{
+    if (threadIdx.x == 0) *slot = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return *slot;
+}
+
+// XCD-aware tile order: workgroups are dispatched round-robin over the 8
+// XCDs, so workgroup w takes tile (w % 8) * (nblk / 8) + w / 8 and each XCD
+// walks one contiguous band of tiles.
+__device__ __forceinline__ int xcd_tile(int nblk)
+{
+    const int w = blockIdx.x;
+    return (w & 7) * (nblk >> 3) + (w >> 3);
+}
+
+// A t on the tile's u and v points into tu / tv (zero off the inner masks),
+// then R = transposed inverse Haar, zero off the wavelet masks.
 template <typename T>
-__device__ __forceinline__ void tile_APR(const CGArgs<T> &a, int comp, T (*tile)[TILE + 1], T (*tmp)[TILE + 1])
+__device__ __forceinline__ void tile_apply_restrict2(const CGArgs<T> &a, const OpTile<T> &s, int i0, int j0,
+                                                     bool has_u, bool has_v, T (*tu)[TILE + 1], T (*mu)[TILE + 1],
+                                                     T (*tv)[TILE + 1], T (*mv)[TILE + 1])
 {
     const GridDims &g = a.g;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int i0 = blockIdx.x * TILE, j0 = blockIdx.y * TILE;
     const int i = i0 + lx, j = j0 + ly;
-    GlobalIn<T> in{a.tu, a.tv, g.nxu, g.nx};
-    if (comp == 0) {
-        T val = T(0);
-        if (i < g.nxu && j < g.nyu && a.ui[i + g.nxu * j]) val = op_u(a.c, in, i, j);
-        tile[ly][lx] = val;
-        __syncthreads();
-        haar_tile<T>(a.tab, 2, tile, tmp, i0, j0, g.nxu, g.nyu);
-        if (i < g.nxu && j < g.nyu && !a.mwu[i + g.nxu * j]) tile[ly][lx] = T(0);
-    } else {
-        T val = T(0);
-        if (i < g.nxv && j < g.nyv && a.vi[i + g.nxv * j]) val = op_v(a.c, in, i, j);
-        tile[ly][lx] = val;
-        __syncthreads();
-        haar_tile<T>(a.tab, 2, tile, tmp, i0, j0, g.nxv, g.nyv);
-        if (i < g.nxv && j < g.nyv && !a.mwv[i + g.nxv * j]) tile[ly][lx] = T(0);
-    }
+    const bool oku = has_u && i < g.nxu && j < g.nyu;
+    const bool okv = has_v && i < g.nxv && j < g.nyv;
+    tu[ly][lx] = (oku && a.ui[i + g.nxu * j]) ? optile_u(g, a.c, s, i, j, lx, ly) : T(0);
+    tv[ly][lx] = (okv && a.vi[i + g.nxv * j]) ? optile_v(g, a.c, s, i, j, lx, ly) : T(0);
+    __syncthreads();
+    haar_pm_tile2<T>(a.levels, 1, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
+    if (oku && !a.mwu[i + g.nxu * j]) tu[ly][lx] = T(0);
+    if (okv && !a.mwv[i + g.nxv * j]) tv[ly][lx] = T(0);
     __syncthreads();
 }
 
-// setup: t = P x0 is computed by k_tile_haar beforehand; here c0 = R A t,
-// r = b - c0 (b is passed in ru/rv and overwritten), u = 0, partials of
-// ||r0||^2 into Pr[0] and of ||b||^2 into Pb.
+// setup (t = P x0 computed beforehand): r = b - R A t (b arrives in r),
+// u = 0, partials of ||r0||^2 and ||b||^2; resets the scalars.
 template <typename T>
-__global__ void k_cg_setup(CGArgs<T> a)
+__global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
 {
-    __shared__ T tile[TILE][TILE + 1];
-    __shared__ T tmp[TILE][TILE + 1];
-    __shared__ double lds[256];
+    __shared__ OpTile<T> s;
+    __shared__ T tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
+    __shared__ double lds[512];
     const GridDims &g = a.g;
-    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int bx = blockIdx.x, by = blockIdx.y;
-    const int i = bx * TILE + lx, j = by * TILE + ly;
-    if (bx == 0 && by == 0 && threadIdx.x == 0) {
+    const int L = xcd_tile(a.nblk);
+    if (L >= g.nbx * g.nby) return;
+    if (L == 0 && threadIdx.x == 0) {
         a.st->done = 0;
         a.st->iters = 0;
         a.st->it[0] = 0;
         a.st->it[1] = 0;
     }
-    if (bx < g.nbxu && by < g.nbyu) {
-        tile_APR<T>(a, 0, tile, tmp);
-        double r2 = 0.0, b2 = 0.0;
-        if (i < g.nxu && j < g.nyu) {
-            int k = i + g.nxu * j;
-            T b = a.ru[k];
-            T r = b - tile[ly][lx];
-            a.ru[k] = r;
-            a.pu[k] = T(0);
-            r2 = (double)r * (double)r;
-            b2 = (double)b * (double)b;
-        }
-        double s = tree256(r2, lds);
-        if (threadIdx.x == 0) a.Pr[0][bx + g.nbxu * by] = s;
-        s = tree256(b2, lds);
-        if (threadIdx.x == 0) a.Pb[bx + g.nbxu * by] = s;
+    const int bx = L % g.nbx, by = L / g.nbx;
+    const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
+    const int i0 = bx * TILE, j0 = by * TILE;
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int i = i0 + lx, j = j0 + ly;
+    const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
+    const int ku = i + g.nxu * j, kv = i + g.nxv * j;
+    optile_load(g, a.tu, a.tv, i0, j0, s);
+    __syncthreads();
+    optile_mid(g, a.c, i0, j0, s);
+    __syncthreads();
+    tile_apply_restrict2<T>(a, s, i0, j0, has_u, has_v, tu, mu, tv, mv);
+    double r2u = 0.0, b2u = 0.0, r2v = 0.0, b2v = 0.0;
+    if (oku) {
+        const T b = a.ru[ku];
+        const T r = b - tu[ly][lx];
+        a.ru[ku] = r;
+        a.pu[ku] = T(0);
+        r2u = (double)r * (double)r;
+        b2u = (double)b * (double)b;
     }
-    if (bx < g.nbxv && by < g.nbyv) {
-        tile_APR<T>(a, 1, tile, tmp);
-        double r2 = 0.0, b2 = 0.0;
-        if (i < g.nxv && j < g.nyv) {
-            int k = i + g.nxv * j;
-            T b = a.rv[k];
-            T r = b - tile[ly][lx];
-            a.rv[k] = r;
-            a.pv[k] = T(0);
-            r2 = (double)r * (double)r;
-            b2 = (double)b * (double)b;
+    if (okv) {
+        const T b = a.rv[kv];
+        const T r = b - tv[ly][lx];
+        a.rv[kv] = r;
+        a.pv[kv] = T(0);
+        r2v = (double)r * (double)r;
+        b2v = (double)b * (double)b;
+    }
+    tree256x2(r2u, r2v, lds);
+    tree256x2(b2u, b2v, lds);
+    if (threadIdx.x == 0) {
+        if (has_u) {
+            a.Pr[bx + g.nbxu * by] = r2u;
+            a.Pb[bx + g.nbxu * by] = b2u;
         }
-        const int o = g.nbxu * g.nbyu + bx + g.nbxv * by;
-        double s = tree256(r2, lds);
-        if (threadIdx.x == 0) a.Pr[0][o] = s;
-        s = tree256(b2, lds);
-        if (threadIdx.x == 0) a.Pb[o] = s;
+        if (has_v) {
+            const int o = g.nbxu * g.nbyu + bx + g.nbxv * by;
+            a.Pr[o] = r2v;
+            a.Pb[o] = b2v;
+        }
     }
 }
 
-// iteration, part 1: convergence test, beta, u = r + beta u, t = P u
+// part 1 (iteration parity par): residual norm and convergence test, then
+// u = r + beta u with beta = res^2 / prev^2, t = P u
 template <typename T>
-__global__ void k_cg_dir(CGArgs<T> a, int par)
+__global__ void __launch_bounds__(256) k_cg_dir(CGArgs<T> a, int par)
 {
-    __shared__ T tile[TILE][TILE + 1];
-    __shared__ T tmp[TILE][TILE + 1];
+    __shared__ T tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
     __shared__ double lds[256];
-    if (a.st->done) return;
     const GridDims &g = a.g;
+    const int L = xcd_tile(a.nblk);
+    if (L >= g.nbx * g.nby) return;
+    const int bx = L % g.nbx, by = L / g.nbx;
+    const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int i0 = bx * TILE, j0 = by * TILE;
+    const int i = i0 + lx, j = j0 + ly;
+    const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
+    const int ku = i + g.nxu * j, kv = i + g.nxv * j;
+    __shared__ int flag;
+    // independent loads first
     const int it = a.st->it[par];
-    double residual = sqrt(lanes_sum(a.Pr[par], g.npart, lds));
+    const T ru = oku ? a.ru[ku] : T(0), pu = oku ? a.pu[ku] : T(0);
+    const T rv = okv ? a.rv[kv] : T(0), pv = okv ? a.pv[kv] : T(0);
+    const bool mwu = oku && a.mwu[ku], mwv = okv && a.mwv[kv];
+    const bool iu = oku && a.ui[ku], iv = okv && a.vi[kv];
+    if (uniform_flag(&a.st->done, &flag)) return;
+    const double res = sqrt(lanes_sum(a.Pr, g.npart, lds));
     double tol;
     if (it == 0) {
-        double bn = sqrt(lanes_sum(a.Pb, g.npart, lds));
-        double abstol = a.tol_coarse * bn;
-        tol = a.reltol * residual;
+        const double abstol = a.tol_coarse * sqrt(lanes_sum(a.Pb, g.npart, lds));
+        tol = a.reltol * res;
         if (abstol > tol) tol = abstol;
     } else {
         tol = a.st->tol;
     }
-    const bool done = (it >= a.maxiter) || (residual <= tol);
-    const int bx = blockIdx.x, by = blockIdx.y;
-    if (bx == 0 && by == 0 && threadIdx.x == 0) {
+    const bool done = (it >= a.maxiter) || (res <= tol);
+    if (threadIdx.x == 0) {
         if (it == 0) a.st->tol = tol;
+        a.st->res[par] = res;
         if (done) {
             a.st->done = 1;
             a.st->iters = it;
         }
     }
     if (done) return;
-    double prev = it == 0 ? 1.0 : sqrt(lanes_sum(a.Pr[1 - par], g.npart, lds));
-    const T beta = (T)((residual * residual) / (prev * prev));
+    const double prev = it == 0 ? 1.0 : a.st->res[1 - par];
+    const T beta = (T)((res * res) / (prev * prev));
+    T unu = T(0), unv = T(0);
+    if (oku) {
+        unu = ru + beta * pu;
+        a.pu[ku] = unu;
+    }
+    if (okv) {
+        unv = rv + beta * pv;
+        a.pv[kv] = unv;
+    }
+    tu[ly][lx] = mwu ? unu : T(0);
+    tv[ly][lx] = mwv ? unv : T(0);
+    __syncthreads();
+    haar_pm_tile2<T>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
+    if (oku) a.tu[ku] = iu ? tu[ly][lx] : T(0);
+    if (okv) a.tv[kv] = iv ? tv[ly][lx] : T(0);
+}
+
+// part 2: c = R A t and partials of dot(u, c)
+template <typename T>
+__global__ void __launch_bounds__(256) k_cg_apply(CGArgs<T> a)
+{
+    __shared__ OpTile<T> s;
+    __shared__ T tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
+    __shared__ double lds[512];
+    const GridDims &g = a.g;
+    const int L = xcd_tile(a.nblk);
+    if (L >= g.nbx * g.nby) return;
+    const int bx = L % g.nbx, by = L / g.nbx;
+    const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
+    const int i0 = bx * TILE, j0 = by * TILE;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int i = i0 + lx, j = j0 + ly;
+    const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
+    const int ku = i + g.nxu * j, kv = i + g.nxv * j;
+    __shared__ int flag;
+    const T pu = oku ? a.pu[ku] : T(0), pv = okv ? a.pv[kv] : T(0);
+    optile_load(g, a.tu, a.tv, i0, j0, s);
+    if (uniform_flag(&a.st->done, &flag)) return;
+    optile_mid(g, a.c, i0, j0, s);
+    __syncthreads();
+    tile_apply_restrict2<T>(a, s, i0, j0, has_u, has_v, tu, mu, tv, mv);
+    double du = 0.0, dv = 0.0;
+    if (oku) {
+        const T cv = tu[ly][lx];
+        a.qu[ku] = cv;
+        du = (double)(pu * cv);
+    }
+    if (okv) {
+        const T cv = tv[ly][lx];
+        a.qv[kv] = cv;
+        dv = (double)(pv * cv);
+    }
+    tree256x2(du, dv, lds);
+    if (threadIdx.x == 0) {
+        if (has_u) a.Pd[bx + g.nbxu * by] = du;
+        if (has_v) a.Pd[g.nbxu * g.nbyu + bx + g.nbxv * by] = dv;
+    }
+}
+
+// part 3: alpha = res^2 / dot(u, c), x += alpha u, r -= alpha c, partials of
+// ||r||^2 and the iteration count
+template <typename T>
+__global__ void __launch_bounds__(256) k_cg_update(CGArgs<T> a, int par)
+{
+    __shared__ double lds[512];
+    const GridDims &g = a.g;
+    const int L = xcd_tile(a.nblk);
+    if (L >= g.nbx * g.nby) return;
+    const int bx = L % g.nbx, by = L / g.nbx;
+    const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int i = bx * TILE + lx, j = by * TILE + ly;
+    const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
+    const int ku = i + g.nxu * j, kv = i + g.nxv * j;
+    __shared__ int flag;
+    const int it = a.st->it[par];
+    const double res = a.st->res[par];
+    const T xu = oku ? a.xu[ku] : T(0), pu = oku ? a.pu[ku] : T(0), ru = oku ? a.ru[ku] : T(0), qu = oku ? a.qu[ku] : T(0);
+    const T xv = okv ? a.xv[kv] : T(0), pv = okv ? a.pv[kv] : T(0), rv = okv ? a.rv[kv] : T(0), qv = okv ? a.qv[kv] : T(0);
+    if (uniform_flag(&a.st->done, &flag)) return;
+    const double dot = lanes_sum(a.Pd, g.npart, lds);
+    const T alpha = (T)((res * res) / dot);
+    if (threadIdx.x == 0) a.st->it[1 - par] = it + 1;
+    double r2u = 0.0, r2v = 0.0;
+    if (oku) {
+        a.xu[ku] = xu + alpha * pu;
+        const T r = ru - alpha * qu;
+        a.ru[ku] = r;
+        r2u = (double)r * (double)r;
+    }
+    if (okv) {
+        a.xv[kv] = xv + alpha * pv;
+        const T r = rv - alpha * qv;
+        a.rv[kv] = r;
+        r2v = (double)r * (double)r;
+    }
+    tree256x2(r2u, r2v, lds);
+    if (threadIdx.x == 0) {
+        if (has_u) a.Pr[bx + g.nbxu * by] = r2u;
+        if (has_v) a.Pr[g.nbxu * g.nbyu + bx + g.nbxv * by] = r2v;
+    }
+}
+
+// out = mask_out ? H(mask_in ? in : 0) : 0 on the u grid (comp 0) and v grid
+template <typename Tin, typename Tout>
+__global__ void k_tile_haar(GridDims g, const HaarTable *tab, int levels, int dir,
+                            const Tin *inu, const Tin *inv, const uint8_t *min_u, const uint8_t *min_v,
+                            Tout *outu, Tout *outv, const uint8_t *mout_u, const uint8_t *mout_v)
+{
+    __shared__ Tout tile[TILE][TILE + 1];
+    __shared__ Tout tmp[TILE][TILE + 1];
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int bx = blockIdx.x, by = blockIdx.y;
     const int i0 = bx * TILE, j0 = by * TILE;
     const int i = i0 + lx, j = j0 + ly;
     if (bx < g.nbxu && by < g.nbyu) {
         int k = i + g.nxu * j;
         bool ok = i < g.nxu && j < g.nyu;
-        T un = T(0);
-        if (ok) {
-            un = a.ru[k] + beta * a.pu[k];
-            a.pu[k] = un;
-        }
-        tile[ly][lx] = (ok && a.mwu[k]) ? un : T(0);
+        tile[ly][lx] = (ok && (!min_u || min_u[k])) ? (Tout)inu[k] : Tout(0);
         __syncthreads();
-        haar_tile<T>(a.tab, 1, tile, tmp, i0, j0, g.nxu, g.nyu);
-        if (ok) a.tu[k] = a.ui[k] ? tile[ly][lx] : T(0);
+        if (dir == 0) haar_tile<Tout>(tab, 0, tile, tmp, i0, j0, g.nxu, g.nyu);
+        else haar_pm_tile<Tout>(levels, dir - 1, tile, tmp, i0, j0, g.nxu, g.nyu);
+        if (ok) outu[k] = (!mout_u || mout_u[k]) ? tile[ly][lx] : Tout(0);
         __syncthreads();
     }
     if (bx < g.nbxv && by < g.nbyv) {
         int k = i + g.nxv * j;
         bool ok = i < g.nxv && j < g.nyv;
-        T un = T(0);
-        if (ok) {
-            un = a.rv[k] + beta * a.pv[k];
-            a.pv[k] = un;
-        }
-        tile[ly][lx] = (ok && a.mwv[k]) ? un : T(0);
+        tile[ly][lx] = (ok && (!min_v || min_v[k])) ? (Tout)inv[k] : Tout(0);
         __syncthreads();
-        haar_tile<T>(a.tab, 1, tile, tmp, i0, j0, g.nxv, g.nyv);
-        if (ok) a.tv[k] = a.vi[k] ? tile[ly][lx] : T(0);
+        if (dir == 0) haar_tile<Tout>(tab, 0, tile, tmp, i0, j0, g.nxv, g.nyv);
+        else haar_pm_tile<Tout>(levels, dir - 1, tile, tmp, i0, j0, g.nxv, g.nyv);
+        if (ok) outv[k] = (!mout_v || mout_v[k]) ? tile[ly][lx] : Tout(0);
         __syncthreads();
-    }
-}
-
-// iteration, part 2: c = R A t, partials of dot(u, c)
-template <typename T>
-__global__ void k_cg_apply(CGArgs<T> a)
-{
-    __shared__ T tile[TILE][TILE + 1];
-    __shared__ T tmp[TILE][TILE + 1];
-    __shared__ double lds[256];
-    if (a.st->done) return;
-    const GridDims &g = a.g;
-    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int bx = blockIdx.x, by = blockIdx.y;
-    const int i = bx * TILE + lx, j = by * TILE + ly;
-    if (bx < g.nbxu && by < g.nbyu) {
-        tile_APR<T>(a, 0, tile, tmp);
-        double d = 0.0;
-        if (i < g.nxu && j < g.nyu) {
-            int k = i + g.nxu * j;
-            T cval = tile[ly][lx];
-            a.qu[k] = cval;
-            d = (double)(a.pu[k] * cval);
-        }
-        double s = tree256(d, lds);
-        if (threadIdx.x == 0) a.Pd[bx + g.nbxu * by] = s;
-    }
-    if (bx < g.nbxv && by < g.nbyv) {
-        tile_APR<T>(a, 1, tile, tmp);
-        double d = 0.0;
-        if (i < g.nxv && j < g.nyv) {
-            int k = i + g.nxv * j;
-            T cval = tile[ly][lx];
-            a.qv[k] = cval;
-            d = (double)(a.pv[k] * cval);
-        }
-        double s = tree256(d, lds);
-        if (threadIdx.x == 0) a.Pd[g.nbxu * g.nbyu + bx + g.nbxv * by] = s;
-    }
-}
-
-// iteration, part 3: alpha, x += alpha u, r -= alpha c, partials of ||r||^2
-template <typename T>
-__global__ void k_cg_update(CGArgs<T> a, int par)
-{
-    __shared__ double lds[256];
-    if (a.st->done) return;
-    const GridDims &g = a.g;
-    double residual = sqrt(lanes_sum(a.Pr[par], g.npart, lds));
-    double dot = lanes_sum(a.Pd, g.npart, lds);
-    const T alpha = (T)((residual * residual) / dot);
-    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int bx = blockIdx.x, by = blockIdx.y;
-    const int i = bx * TILE + lx, j = by * TILE + ly;
-    double *Pn = a.Pr[1 - par];
-    if (bx == 0 && by == 0 && threadIdx.x == 0) a.st->it[1 - par] = a.st->it[par] + 1;
-    if (bx < g.nbxu && by < g.nbyu) {
-        double r2 = 0.0;
-        if (i < g.nxu && j < g.nyu) {
-            int k = i + g.nxu * j;
-            a.xu[k] = a.xu[k] + alpha * a.pu[k];
-            T r = a.ru[k] - alpha * a.qu[k];
-            a.ru[k] = r;
-            r2 = (double)r * (double)r;
-        }
-        double s = tree256(r2, lds);
-        if (threadIdx.x == 0) Pn[bx + g.nbxu * by] = s;
-    }
-    if (bx < g.nbxv && by < g.nbyv) {
-        double r2 = 0.0;
-        if (i < g.nxv && j < g.nyv) {
-            int k = i + g.nxv * j;
-            a.xv[k] = a.xv[k] + alpha * a.pv[k];
-            T r = a.rv[k] - alpha * a.qv[k];
-            a.rv[k] = r;
-            r2 = (double)r * (double)r;
-        }
-        double s = tree256(r2, lds);
-        if (threadIdx.x == 0) Pn[g.nbxu * g.nbyu + bx + g.nbxv * by] = s;
     }
 }
 
