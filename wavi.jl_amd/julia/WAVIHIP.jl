@@ -1,0 +1,225 @@
+"""
+    WAVIHIP
+
+MI355X velocity solve for WAVI.jl models.  `HIPSpec` is a new
+`AbstractParallelSpec` (WAVI.jl `src/WAVI.jl:17`); models built with
+`parallel_spec = HIPSpec()` dispatch `update_velocities!` and
+`update_state!` to `libwavi_hip.so` through `ccall`.  The fields stay in the
+model's `Fields` struct and travel to the device before a call and back after
+it; `run_steps!` keeps them resident over many time steps.
+
+This file is written against the C ABI in `include/wavi_hip.h`.  It has not
+been run: the build image has no Julia.
+"""
+module WAVIHIP
+
+using WAVI
+import WAVI: update_velocities!, update_state!, AbstractParallelSpec, AbstractModel
+
+const LIB = get(ENV, "WAVI_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libwavi_hip.so"))
+
+# --- C structs (layout of include/wavi_hip.h) ------------------------------
+struct GridDesc
+    nx::Int32
+    ny::Int32
+    nsigma::Int32
+    dx::Float64
+    dy::Float64
+    h_mask::Ptr{UInt8}
+    h_isfixed::Ptr{UInt8}
+    u_iszero::Ptr{UInt8}
+    v_iszero::Ptr{UInt8}
+    u_isfixed::Ptr{UInt8}
+    v_isfixed::Ptr{UInt8}
+    sigma::Ptr{Float64}
+    quadrature_weights::Ptr{Float64}
+end
+
+struct PhysParams
+    dt::Float64
+    g::Float64
+    density_ice::Float64
+    density_ocean::Float64
+    glen_n::Float64
+    glen_reg_strain_rate::Float64
+    weertman_m::Float64
+    weertman_reg_speed::Float64
+    sea_level_wrt_geoid::Float64
+    minimum_thickness::Float64
+    uniform_melt_rate::Float64
+    evolve_shelves::Int32
+end
+
+struct SolverParamsC
+    n_iter_viscosity::Int32
+    tol_picard::Float64
+    maxiter_picard::Int32
+    tol_coarse::Float64
+    maxiter_coarse::Int32
+    levels::Int32
+    wavelet_threshold::Float64
+    nsmooth::Int32
+    smoother_omega::Float64
+    stencil_margin::Int32
+    super_implicitness::Float64
+end
+
+struct SolveReport
+    picard_iters::Int32
+    converged::Int32
+    rel_resid::Float64
+    coarse_cg_total::Int32
+    refinements::Int32
+    coarse_cg_iters::NTuple{64,Int32}
+    rel_resid_hist::NTuple{64,Float64}
+end
+
+const PRECISION = Dict(:fp64 => Int32(0), :mixed => Int32(1))
+
+"""
+    HIPSpec(; precision = :fp64)
+
+Parallel spec that runs the velocity solve on the GPU.  `precision = :mixed`
+runs the V-cycle in FP32 inside FP64 residual refinement.
+"""
+mutable struct HIPSpec <: AbstractParallelSpec
+    precision::Int32
+    ctx::Ptr{Cvoid}
+    dt::Float64
+    function HIPSpec(; precision::Symbol = :fp64)
+        haskey(PRECISION, precision) || throw(ArgumentError("precision must be :fp64 or :mixed"))
+        spec = new(PRECISION[precision], C_NULL, NaN)
+        finalizer(spec) do s
+            s.ctx == C_NULL || ccall((:wavi_ctx_destroy, LIB), Cint, (Ptr{Cvoid},), s.ctx)
+            s.ctx = C_NULL
+        end
+        return spec
+    end
+end
+
+function check(ctx, rc)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:wavi_last_error, LIB), Cstring, (Ptr{Cvoid},), ctx))
+    rc == -2 && throw(DimensionMismatch(msg))
+    rc == -1 && throw(ArgumentError(msg))
+    error("libwavi_hip: $msg (code $rc)")
+end
+
+# (field id, accessor) in the order of enum wavi_field
+fieldlist(m) = begin
+    f = m.fields; gh, gu, gv, g3d, wu, wv = f.gh, f.gu, f.gv, f.g3d, f.wu, f.wv
+    Any[gh.b, gh.h, gh.s, gh.dhdt, gh.accumulation, gh.basal_melt, gh.haf, gh.grounded_fraction,
+        gh.dsdh, gh.shelf_strain_rate, gh.av_speed, gh.u, gh.v, gh.us, gh.vs, gh.ub, gh.vb,
+        gh.bed_speed, gh.weertman_c, gh.β, gh.βeff, gh.τbed, gh.ηav, gh.quad_f1, gh.quad_f2,
+        m.params.weertman_c, m.params.accumulation_rate,
+        gu.s, gu.h, gu.grounded_fraction, gu.βeff, gu.u, gu.residual,
+        gv.s, gv.h, gv.grounded_fraction, gv.βeff, gv.v, gv.residual,
+        g3d.η, g3d.glen_b]
+end
+
+"""
+    update_preconditioner!(model)
+
+Dispatch hook (`src/SharedMemorySpec.jl:31`): creates the device context on
+first use and keeps its dt in step with `model.params.dt`.
+"""
+function WAVI.update_preconditioner!(model::AbstractModel{T,N,M,<:HIPSpec}) where {T,N,M}
+    spec = model.parallel_spec
+    if spec.ctx == C_NULL
+        g, p, s = model.grid, model.params, model.solver_params
+        σ = collect(Float64, g.σ); w = collect(Float64, g.quadrature_weights)
+        masks = map(x -> collect(UInt8, x), (g.h_mask, g.h_isfixed, g.u_iszero, g.v_iszero, g.u_isfixed, g.v_isfixed))
+        GC.@preserve σ w masks begin
+            desc = GridDesc(g.nx, g.ny, g.nσ, g.dx, g.dy, map(pointer, masks)..., pointer(σ), pointer(w))
+            melt = model.melt_rate isa UniformMeltRate ? model.melt_rate.m :
+                   throw(ArgumentError("HIPSpec supports UniformMeltRate"))
+            phys = PhysParams(p.dt, p.g, p.density_ice, p.density_ocean, p.glen_n, p.glen_reg_strain_rate,
+                              p.weertman_m, p.weertman_reg_speed, p.sea_level_wrt_geoid, p.minimum_thickness,
+                              melt, p.evolveShelves ? 1 : 0)
+            solver = SolverParamsC(s.n_iter_viscosity, s.tol_picard, s.maxiter_picard, s.tol_coarse,
+                                   s.maxiter_coarse, s.levels, s.wavelet_threshold, s.nsmooth,
+                                   s.smoother_omega, s.stencil_margin, s.super_implicitness)
+            out = Ref{Ptr{Cvoid}}(C_NULL)
+            rc = ccall((:wavi_ctx_create, LIB), Cint,
+                       (Ref{GridDesc}, Ref{PhysParams}, Ref{SolverParamsC}, Cint, Ref{Ptr{Cvoid}}),
+                       desc, phys, solver, spec.precision, out)
+            check(C_NULL, rc)
+            spec.ctx = out[]
+            spec.dt = p.dt
+        end
+    elseif spec.dt != model.params.dt
+        check(spec.ctx, ccall((:wavi_set_dt, LIB), Cint, (Ptr{Cvoid}, Cdouble), spec.ctx, model.params.dt))
+        spec.dt = model.params.dt
+    end
+    return model
+end
+
+function push_fields!(model)
+    ctx = model.parallel_spec.ctx
+    for (id, a) in enumerate(fieldlist(model))
+        check(ctx, ccall((:wavi_set_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t),
+                         ctx, id - 1, a, length(a)))
+    end
+    wu, wv = model.fields.wu, model.fields.wv
+    for (id, a) in ((41, wu.wavelets), (42, Float64.(wu.mask)), (44, wv.wavelets), (45, Float64.(wv.mask)))
+        check(ctx, ccall((:wavi_set_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t), ctx, id, a, length(a)))
+    end
+    return model
+end
+
+function pull_fields!(model)
+    ctx = model.parallel_spec.ctx
+    for (id, a) in enumerate(fieldlist(model))
+        check(ctx, ccall((:wavi_get_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t),
+                         ctx, id - 1, a, length(a)))
+    end
+    wu, wv = model.fields.wu, model.fields.wv
+    for (id, w, mask) in ((41, wu, 42), (44, wv, 45))
+        check(ctx, ccall((:wavi_get_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t), ctx, id, w.wavelets, length(w.wavelets)))
+        m = similar(w.wavelets)
+        check(ctx, ccall((:wavi_get_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t), ctx, mask, m, length(m)))
+        w.mask .= m .> 0.5
+        w.n[] = count(w.mask)
+    end
+    return model
+end
+
+"""
+    update_velocities!(model::AbstractModel{T,N,M,<:HIPSpec})
+
+`update_velocities!` (`src/Models/update_velocities.jl:11-29`) on the GPU.
+"""
+function WAVI.update_velocities!(model::AbstractModel{T,N,M,<:HIPSpec}) where {T,N,M}
+    WAVI.update_preconditioner!(model)
+    push_fields!(model)
+    rep = Ref{SolveReport}()
+    check(model.parallel_spec.ctx, ccall((:wavi_update_velocities, LIB), Cint, (Ptr{Cvoid}, Ref{SolveReport}),
+                                         model.parallel_spec.ctx, rep))
+    pull_fields!(model)
+    println("Solved momentum equation on the GPU with residual ", round(rep[].rel_resid, sigdigits=3),
+            " at iteration ", rep[].picard_iters)
+    return model
+end
+
+"""
+    run_steps!(simulation, nsteps)
+
+`nsteps` of `timestep!` (`src/Simulations/run_simulation.jl:6-17`) with the
+state resident on the device.
+"""
+function run_steps!(simulation, nsteps::Integer)
+    model = simulation.model
+    WAVI.update_preconditioner!(model)
+    push_fields!(model)
+    rep = Ref{SolveReport}()
+    check(model.parallel_spec.ctx, ccall((:wavi_run_steps, LIB), Cint, (Ptr{Cvoid}, Cint, Ref{SolveReport}),
+                                         model.parallel_spec.ctx, nsteps, rep))
+    pull_fields!(model)
+    simulation.clock.n_iter += nsteps
+    simulation.clock.time += nsteps * simulation.timestepping_params.dt
+    return simulation
+end
+
+export HIPSpec, run_steps!
+
+end # module

@@ -183,6 +183,8 @@ class HIPSpec:
     FP64 residual refinement)."""
 
     def __init__(self, precision="fp64"):
+        self.ctx = None
+        self._lib = None
         if precision not in _lib.PRECISION:
             raise ArgumentError("precision must be one of %s" % list(_lib.PRECISION))
         self.precision = precision
@@ -194,7 +196,7 @@ class HIPSpec:
         self.close()
 
     def close(self):
-        if self.ctx is not None and self._lib is not None:
+        if getattr(self, "ctx", None) is not None and getattr(self, "_lib", None) is not None:
             self._lib.wavi_ctx_destroy(self.ctx)
             self.ctx = None
 
