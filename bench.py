@@ -49,8 +49,39 @@ def cg_apply_bytes(nx, ny, w=8):
     return w * (5 * (nu + nv) + 2 * nh + nc) + 3 * (nu + nv) + nh + nc
 
 
-def setup(W, nx, ny, precision):
-    m = W.workloads.channel_solve(nx=nx, ny=ny, precision=precision)
+def measured_traffic(nx, ny, precision):
+    """HBM bytes per k_cg_apply launch from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary for this workload, scaled by the copy-kernel
+    calibration recorded there; None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", "k_cg_apply_%dx%d_%s_traffic.md" % (nx, ny, precision))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch k_cg_apply:"):
+            return int(line.split(":")[1])
+    return None
+
+
+def calibration_copy(mib):
+    """One device-to-device copy of known size (a plain copy kernel), the
+    reference point that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), n) == 0 and hip.hipMalloc(ctypes.byref(b), n) == 0
+    assert hip.hipMemset(a, 1, n) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def setup(W, nx, ny, precision, picard=None, maxiter_coarse=None):
+    kw = {} if picard is None else {"maxiter_picard": picard}
+    m = W.workloads.channel_solve(nx=nx, ny=ny, precision=precision, **kw)
+    if maxiter_coarse is not None:
+        m.solver_params.maxiter_coarse = maxiter_coarse
     W.update_preconditioner(m)
     W.push_fields(m)
     spec = m.parallel_spec
@@ -111,6 +142,12 @@ def main():
     ap.add_argument("--nx", type=int, default=1280)
     ap.add_argument("--ny", type=int, default=160)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--picard", type=int, default=None,
+                    help="profiling runs only: cap the Picard iterations")
+    ap.add_argument("--maxiter-coarse", type=int, default=None,
+                    help="profiling runs only: cap the coarse CG iterations")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="profiling runs only: issue one device-to-device copy of this size first")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the velocity solve here runs on a single MI355X; "
@@ -120,7 +157,9 @@ def main():
     W_lib = W._lib
     nx, ny = args.nx, args.ny
     log("[bench] setup %dx%dx8 precision=%s" % (nx, ny, args.precision))
-    m, spec, lib = setup(W, nx, ny, args.precision)
+    m, spec, lib = setup(W, nx, ny, args.precision, args.picard, args.maxiter_coarse)
+    if args.calibration_copy_mib:
+        calibration_copy(args.calibration_copy_mib)
     for k in range(args.warmup):
         assert lib.wavi_state_restore(spec.ctx) == 0
         t0 = time.perf_counter()
@@ -164,7 +203,8 @@ def main():
                    "picard_iters": rep["picard_iters"], "coarse_cg_total": rep["coarse_cg_total"],
                    "rel_resid": rep["rel_resid"]},
         "roofline": {"bound": "hbm", "kernel": "k_cg_apply", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(nx, ny, args.precision),
                      "bytes_per_launch": B, "avg_launch_us": avg_s * 1e6, "launches": n.value},
     }
     if not args.no_cpu_baseline:
