@@ -99,6 +99,17 @@ def solve(spec, lib):
     return rep.as_dict()
 
 
+def velocities(W, m, spec):
+    """u and v of the model's device state."""
+    lib = spec._lib
+    nx, ny = m.grid.nx, m.grid.ny
+    u = np.empty((nx + 1) * ny)
+    v = np.empty(nx * (ny + 1))
+    assert lib.wavi_get_field(spec.ctx, W._lib.FIELD["U_U"], W._lib.dptr(u), u.size) == 0
+    assert lib.wavi_get_field(spec.ctx, W._lib.FIELD["V_V"], W._lib.dptr(v), v.size) == 0
+    return u, v
+
+
 def cpu_baseline(W, nx, ny, gpu_report):
     """The oracle (a single-threaded C restatement of the reference algorithm)
     on a bounded sample of the same solve: the first Picard iteration with the
@@ -207,6 +218,20 @@ def main():
                      "traffic": measured_traffic(nx, ny, args.precision),
                      "bytes_per_launch": B, "avg_launch_us": avg_s * 1e6, "launches": n.value},
     }
+    if args.precision == "mixed":
+        log("[bench] FP64 solve of the same state for the error of the mixed solve")
+        mu, mv = velocities(W, m, spec)
+        m64, spec64, lib64 = setup(W, nx, ny, "fp64", args.picard, args.maxiter_coarse)
+        solve(spec64, lib64)
+        ru, rv = velocities(W, m64, spec64)
+        ref = np.concatenate([ru.ravel(), rv.ravel()])
+        got = np.concatenate([mu.ravel(), mv.ravel()])
+        out["config"]["error_vs_fp64"] = {
+            "rel_l2": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+            "max_abs": float(np.max(np.abs(got - ref))),
+            "max_rel_to_max": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))),
+        }
+        out["config"]["refinements"] = rep["refinements"]
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle sample)")
         out["cpu_baseline"] = cpu_baseline(W, nx, ny, rep)

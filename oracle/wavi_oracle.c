@@ -261,10 +261,20 @@ EXPORT void om_haar(om_model *m, int comp, int dir, const double *in, double *ou
 /* ------------------------------------------------------------------ */
 /* Canonical reduction: sum over the stacked [u; v] vector of a_k * b_k.
  * u then v grids are cut into 16 x 16 blocks (x fastest); each block's 256
- * products (zero outside the grid) are summed by the pairwise tree
- * t[e] += t[e+s], s = 128, 64, ..., 1; block partials P[0..np) are folded
- * into 256 lanes (lane l sums P[l], P[l+256], ... in order) and the lanes
- * are summed by the same pairwise tree.  The GPU kernels use this tree. */
+ * products (zero outside the grid) are summed by tree256: within each run of
+ * 64 elements t[e] += t[e+s] for s = 32, 16, ..., 1, then the four run sums
+ * combine as (r0 + r2) + (r1 + r3).  Block partials P[0..np) are folded into
+ * 256 lanes (lane l sums P[l], P[l+256], ... in order) and the lanes are
+ * summed by the same tree.  The GPU kernels use this tree (a 64-wide
+ * wavefront reduces one run with shuffles). */
+
+static double tree256(double *t)
+{
+    for (int c = 0; c < 4; ++c)
+        for (int s = 32; s > 0; s >>= 1)
+            for (int e = 0; e < s; ++e) t[64 * c + e] = t[64 * c + e] + t[64 * c + e + s];
+    return (t[0] + t[128]) + (t[64] + t[192]);
+}
 
 static double block_tree(const double *a, const double *b, int nxg, int nyg, int bx, int by)
 {
@@ -279,9 +289,7 @@ static double block_tree(const double *a, const double *b, int nxg, int nyg, int
             }
             t[lx + 16 * ly] = v;
         }
-    for (int s = 128; s > 0; s >>= 1)
-        for (int e = 0; e < s; ++e) t[e] = t[e] + t[e + s];
-    return t[0];
+    return tree256(t);
 }
 
 static double lanes_tree(const double *P, int np)
@@ -292,9 +300,7 @@ static double lanes_tree(const double *P, int np)
         for (int q = l; q < np; q += 256) acc = acc + P[q];
         t[l] = acc;
     }
-    for (int s = 128; s > 0; s >>= 1)
-        for (int e = 0; e < s; ++e) t[e] = t[e] + t[e + s];
-    return t[0];
+    return tree256(t);
 }
 
 static double canonical_dot(const om_model *m, const double *au, const double *av,

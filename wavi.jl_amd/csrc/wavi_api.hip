@@ -20,6 +20,20 @@ struct DevVec {
     double *u = nullptr, *v = nullptr;
 };
 
+template <typename T>
+struct DevVecT {
+    T *u = nullptr, *v = nullptr;
+};
+
+// Buffers of one V-cycle precision: operator coefficients, smoother and CG
+// vectors, coarse warm start.  The FP64 set aliases the context's fields;
+// the FP32 set (mixed precision) has its own copies.
+template <typename T>
+struct VSet {
+    T *heta_h = nullptr, *heta_c = nullptr, *dimp = nullptr;
+    DevVecT<T> h, beff, e, r, d, rho, t, cr, cp, cq, ct, xc;
+};
+
 struct wavi_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -42,6 +56,7 @@ struct wavi_ctx {
     DevVec x, b, r, e, d, rho, t, f3, fixedv;
     // coarse CG vectors
     DevVec cr, cp, cq, ct;
+    VSet<float> v32;
     double *Pr0 = nullptr, *Pr1 = nullptr, *Pd = nullptr, *Pb = nullptr, *Pn = nullptr;
     double *quad = nullptr;  // 4 * nh pos_fraction quadrants
     double *norms = nullptr; // device: 2 doubles
@@ -435,6 +450,21 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
     CREATECHK(hipMemset(ctx->st, 0, sizeof(CGScalars)));
     CREATECHK(hipHostMalloc((void **)&ctx->h_st, sizeof(CGScalars), hipHostMallocDefault));
     ctx->nblk = ((g.nbx * g.nby + 7) / 8) * 8;
+    if (precision_mode == WAVI_PRECISION_MIXED) {
+        VSet<float> &w = ctx->v32;
+        auto falloc = [&](float **p, size_t n) -> hipError_t {
+            hipError_t e = hipMalloc(p, n * sizeof(float));
+            if (e != hipSuccess) return e;
+            return hipMemset(*p, 0, n * sizeof(float));
+        };
+        CREATECHK(falloc(&w.heta_h, ctx->nh));
+        CREATECHK(falloc(&w.heta_c, ctx->nc > 0 ? ctx->nc : 1));
+        CREATECHK(falloc(&w.dimp, ctx->nh));
+        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cr, &w.cp, &w.cq, &w.ct, &w.xc}) {
+            CREATECHK(falloc(&dv->u, ctx->nu));
+            CREATECHK(falloc(&dv->v, ctx->nv));
+        }
+    }
     {
         HaarMasks hm_tab[4];
         build_haar_masks(hm_tab);
@@ -467,6 +497,15 @@ int wavi_ctx_destroy(wavi_ctx *ctx)
                        &ctx->cr, &ctx->cp, &ctx->cq, &ctx->ct}) {
         if (dv->u) (void)hipFree(dv->u);
         if (dv->v) (void)hipFree(dv->v);
+    }
+    {
+        VSet<float> &w = ctx->v32;
+        for (float *p : {w.heta_h, w.heta_c, w.dimp})
+            if (p) (void)hipFree(p);
+        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cr, &w.cp, &w.cq, &w.ct, &w.xc}) {
+            if (dv->u) (void)hipFree(dv->u);
+            if (dv->v) (void)hipFree(dv->v);
+        }
     }
     if (ctx->st) (void)hipFree(ctx->st);
     if (ctx->tab) (void)hipFree(ctx->tab);
@@ -707,51 +746,100 @@ static int do_rhs(wavi_ctx *ctx)
     return WAVI_OK;
 }
 
-static int do_diag(wavi_ctx *ctx)
+template <typename T>
+static OpCoef<T> coef_of(const wavi_ctx *ctx, const VSet<T> &w)
 {
-    hipLaunchKernelGGL(k_diag<double>, flat(ctx->nu + ctx->nv), dim3(256), 0, ctx->stream, ctx->g, coef64(ctx),
-                       ctx->d.u, ctx->d.v);
+    OpCoef<T> c;
+    c.heta_h = w.heta_h;
+    c.heta_c = w.heta_c;
+    c.dimp = w.dimp;
+    c.hu = w.h.u;
+    c.hv = w.h.v;
+    c.bu = w.beff.u;
+    c.bv = w.beff.v;
+    c.hm = ctx->hm;
+    c.um = ctx->um;
+    c.vm = ctx->vm;
+    c.cm = ctx->cm;
+    c.ui = ctx->ui;
+    c.vi = ctx->vi;
+    c.nx = ctx->g.nx;
+    c.ny = ctx->g.ny;
+    c.rdx = (T)ctx->rdx;
+    c.rdy = (T)ctx->rdy;
+    return c;
+}
+
+// the FP64 buffer set: views of the context's fields and vectors
+static VSet<double> vset64(wavi_ctx *ctx)
+{
+    VSet<double> w;
+    w.heta_h = ctx->heta_h;
+    w.heta_c = ctx->heta_c;
+    w.dimp = ctx->dimp;
+    w.h = {ctx->f[WAVI_U_H], ctx->f[WAVI_V_H]};
+    w.beff = {ctx->f[WAVI_U_BETA_EFF], ctx->f[WAVI_V_BETA_EFF]};
+    w.e = {ctx->e.u, ctx->e.v};
+    w.r = {ctx->r.u, ctx->r.v};
+    w.d = {ctx->d.u, ctx->d.v};
+    w.rho = {ctx->rho.u, ctx->rho.v};
+    w.t = {ctx->t.u, ctx->t.v};
+    w.cr = {ctx->cr.u, ctx->cr.v};
+    w.cp = {ctx->cp.u, ctx->cp.v};
+    w.cq = {ctx->cq.u, ctx->cq.v};
+    w.ct = {ctx->ct.u, ctx->ct.v};
+    w.xc = {ctx->f[WAVI_WU_CORRECTION_COARSE], ctx->f[WAVI_WV_CORRECTION_COARSE]};
+    return w;
+}
+
+template <typename T>
+static int do_diag(wavi_ctx *ctx, const VSet<T> &w)
+{
+    hipLaunchKernelGGL(k_diag<T>, flat(ctx->nu + ctx->nv), dim3(256), 0, ctx->stream, ctx->g, coef_of(ctx, w),
+                       w.d.u, w.d.v);
     HIPCHK(hipGetLastError());
     return WAVI_OK;
 }
 
-static int do_smoother(wavi_ctx *ctx, const DevVec &x, const DevVec &b, const int order[4])
+template <typename T>
+static int do_smoother(wavi_ctx *ctx, const VSet<T> &w, const int order[4])
 {
     const GridDims &g = ctx->g;
     int nthreads_u = ((g.nxu + 1) / 2) * g.nyu, nthreads_v = ((g.nxv + 1) / 2) * g.nyv;
-    OpCoef<double> c = coef64(ctx);
+    OpCoef<T> c = coef_of(ctx, w);
     for (int it = 0; it < ctx->S.nsmooth; ++it)
         for (int q = 0; q < 4; ++q) {
             int col = order[q];
             int n = col <= 2 ? nthreads_u : nthreads_v;
             ProfScope ps(ctx, WAVI_KT_COLOUR);
-            hipLaunchKernelGGL(k_colour<double>, flat(n), dim3(256), 0, ctx->stream, g, c, x.u, x.v, b.u, b.v,
-                               ctx->d.u, ctx->d.v, ctx->S.smoother_omega, col);
+            hipLaunchKernelGGL(k_colour<T>, flat(n), dim3(256), 0, ctx->stream, g, c, w.e.u, w.e.v, w.r.u, w.r.v,
+                               w.d.u, w.d.v, (T)ctx->S.smoother_omega, col);
         }
     HIPCHK(hipGetLastError());
     return WAVI_OK;
 }
 
-static CGArgs<double> cg_args(wavi_ctx *ctx)
+template <typename T>
+static CGArgs<T> cg_args(wavi_ctx *ctx, const VSet<T> &w)
 {
-    CGArgs<double> a;
+    CGArgs<T> a;
     a.g = ctx->g;
-    a.c = coef64(ctx);
+    a.c = coef_of(ctx, w);
     a.levels = ctx->S.levels;
     a.mwu = ctx->mwu;
     a.mwv = ctx->mwv;
     a.ui = ctx->ui;
     a.vi = ctx->vi;
-    a.xu = ctx->f[WAVI_WU_CORRECTION_COARSE];
-    a.xv = ctx->f[WAVI_WV_CORRECTION_COARSE];
-    a.ru = ctx->cr.u;
-    a.rv = ctx->cr.v;
-    a.pu = ctx->cp.u;
-    a.pv = ctx->cp.v;
-    a.qu = ctx->cq.u;
-    a.qv = ctx->cq.v;
-    a.tu = ctx->ct.u;
-    a.tv = ctx->ct.v;
+    a.xu = w.xc.u;
+    a.xv = w.xc.v;
+    a.ru = w.cr.u;
+    a.rv = w.cr.v;
+    a.pu = w.cp.u;
+    a.pv = w.cp.v;
+    a.qu = w.cq.u;
+    a.qv = w.cq.v;
+    a.tu = w.ct.u;
+    a.tv = w.ct.v;
     a.Pr = ctx->Pr0;
     a.Pd = ctx->Pd;
     a.Pb = ctx->Pb;
@@ -763,16 +851,17 @@ static CGArgs<double> cg_args(wavi_ctx *ctx)
     return a;
 }
 
-// coarse CG on b_c held in ctx->cr; solution in the warm start fields
-static int do_coarse_cg(wavi_ctx *ctx, int *iters)
+// coarse CG on b_c held in w.cr; solution in the warm start w.xc
+template <typename T>
+static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
 {
     const GridDims &g = ctx->g;
     dim3 tg(g.nbx, g.nby), tb(ctx->nblk);
-    CGArgs<double> a = cg_args(ctx);
+    CGArgs<T> a = cg_args(ctx, w);
     // t = P x0
-    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1,
-                       a.xu, a.xv, ctx->mwu, ctx->mwv, a.tu, a.tv, ctx->ui, ctx->vi);
-    hipLaunchKernelGGL(k_cg_setup<double>, tb, dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_tile_haar<T, T>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1, a.xu, a.xv,
+                       ctx->mwu, ctx->mwv, a.tu, a.tv, ctx->ui, ctx->vi);
+    hipLaunchKernelGGL(k_cg_setup<T>, tb, dim3(256), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
     const int CHUNK = 32;
     int q = 0;
@@ -782,15 +871,15 @@ static int do_coarse_cg(wavi_ctx *ctx, int *iters)
             const int par = q & 1;
             {
                 ProfScope ps(ctx, WAVI_KT_CG_DIR);
-                hipLaunchKernelGGL(k_cg_dir<double>, tb, dim3(256), 0, ctx->stream, a, par);
+                hipLaunchKernelGGL(k_cg_dir<T>, tb, dim3(256), 0, ctx->stream, a, par);
             }
             {
                 ProfScope ps(ctx, WAVI_KT_CG_APPLY);
-                hipLaunchKernelGGL(k_cg_apply<double>, tb, dim3(256), 0, ctx->stream, a);
+                hipLaunchKernelGGL(k_cg_apply<T>, tb, dim3(256), 0, ctx->stream, a);
             }
             {
                 ProfScope ps(ctx, WAVI_KT_CG_UPDATE);
-                hipLaunchKernelGGL(k_cg_update<double>, tb, dim3(256), 0, ctx->stream, a, par);
+                hipLaunchKernelGGL(k_cg_update<T>, tb, dim3(256), 0, ctx->stream, a, par);
             }
         }
         HIPCHK(hipGetLastError());
@@ -807,33 +896,67 @@ static int do_coarse_cg(wavi_ctx *ctx, int *iters)
     return WAVI_OK;
 }
 
-// precondition!(x, p, b) with x = 0: b in ctx->r, correction into ctx->e
-static int do_vcycle(wavi_ctx *ctx, int *cg_iters)
+// precondition!(x, p, b) with x = 0, in the arithmetic of T: b in w.r,
+// correction into w.e (update_preconditioners.jl:54-83)
+template <typename T>
+static int do_vcycle_t(wavi_ctx *ctx, const VSet<T> &w, int *cg_iters)
 {
     static const int fwd[4] = {1, 3, 2, 4}, rev[4] = {4, 2, 3, 1};
     const GridDims &g = ctx->g;
     dim3 tg(g.nbx, g.nby);
-    CHK(do_diag(ctx));
-    hipLaunchKernelGGL(k_fill<double>, flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->e.u, 0.0, ctx->nu);
-    hipLaunchKernelGGL(k_fill<double>, flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->e.v, 0.0, ctx->nv);
-    CHK(do_smoother(ctx, ctx->e, ctx->r, fwd));
+    CHK(do_diag(ctx, w));
+    hipLaunchKernelGGL(k_fill<T>, flat(ctx->nu), dim3(256), 0, ctx->stream, w.e.u, T(0), ctx->nu);
+    hipLaunchKernelGGL(k_fill<T>, flat(ctx->nv), dim3(256), 0, ctx->stream, w.e.v, T(0), ctx->nv);
+    CHK(do_smoother(ctx, w, fwd));
     // rho = r - A e
-    hipLaunchKernelGGL(k_tile_resid<double>, tg, dim3(256), 0, ctx->stream, g, coef64(ctx), ctx->e.u, ctx->e.v,
-                       ctx->r.u, ctx->r.v, ctx->rho.u, ctx->rho.v, ctx->Pn, (double *)nullptr);
+    hipLaunchKernelGGL(k_tile_resid<T>, tg, dim3(256), 0, ctx->stream, g, coef_of(ctx, w), w.e.u, w.e.v, w.r.u,
+                       w.r.v, w.rho.u, w.rho.v, ctx->Pn, (double *)nullptr);
     // b_c = R rho, into the CG residual buffer
-    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 2, ctx->rho.u,
-                       ctx->rho.v, ctx->ui, ctx->vi, ctx->cr.u, ctx->cr.v, ctx->mwu, ctx->mwv);
+    hipLaunchKernelGGL((k_tile_haar<T, T>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 2, w.rho.u,
+                       w.rho.v, ctx->ui, ctx->vi, w.cr.u, w.cr.v, ctx->mwu, ctx->mwv);
     HIPCHK(hipGetLastError());
-    CHK(do_coarse_cg(ctx, cg_iters));
+    CHK(do_coarse_cg(ctx, w, cg_iters));
     // e += P x_c
-    hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1,
-                       ctx->f[WAVI_WU_CORRECTION_COARSE], ctx->f[WAVI_WV_CORRECTION_COARSE], ctx->mwu, ctx->mwv,
-                       ctx->t.u, ctx->t.v, ctx->ui, ctx->vi);
-    hipLaunchKernelGGL(k_add_masked<double>, flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->e.u, ctx->t.u, ctx->ui,
-                       ctx->nu);
-    hipLaunchKernelGGL(k_add_masked<double>, flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->e.v, ctx->t.v, ctx->vi,
-                       ctx->nv);
-    CHK(do_smoother(ctx, ctx->e, ctx->r, rev));
+    hipLaunchKernelGGL((k_tile_haar<T, T>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1, w.xc.u,
+                       w.xc.v, ctx->mwu, ctx->mwv, w.t.u, w.t.v, ctx->ui, ctx->vi);
+    hipLaunchKernelGGL(k_add_masked<T>, flat(ctx->nu), dim3(256), 0, ctx->stream, w.e.u, w.t.u, ctx->ui, ctx->nu);
+    hipLaunchKernelGGL(k_add_masked<T>, flat(ctx->nv), dim3(256), 0, ctx->stream, w.e.v, w.t.v, ctx->vi, ctx->nv);
+    CHK(do_smoother(ctx, w, rev));
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+template <typename Tin, typename Tout>
+static void convert(wavi_ctx *ctx, const Tin *in, Tout *out, int n)
+{
+    hipLaunchKernelGGL((k_convert<Tin, Tout>), flat(n), dim3(256), 0, ctx->stream, in, out, n);
+}
+
+// V-cycle on the FP64 Picard residual ctx->r; correction into ctx->e.  In
+// mixed precision the operator coefficients, the residual and the warm start
+// are rounded to FP32, the V-cycle runs in FP32 (reductions accumulated in
+// FP64) and the correction and warm start are widened back.
+static int do_vcycle(wavi_ctx *ctx, int *cg_iters)
+{
+    if (ctx->precision == WAVI_PRECISION_FP64) return do_vcycle_t<double>(ctx, vset64(ctx), cg_iters);
+    VSet<float> &w = ctx->v32;
+    convert(ctx, ctx->heta_h, w.heta_h, ctx->nh);
+    convert(ctx, ctx->heta_c, w.heta_c, ctx->nc > 0 ? ctx->nc : 1);
+    convert(ctx, ctx->dimp, w.dimp, ctx->nh);
+    convert(ctx, ctx->f[WAVI_U_H], w.h.u, ctx->nu);
+    convert(ctx, ctx->f[WAVI_V_H], w.h.v, ctx->nv);
+    convert(ctx, ctx->f[WAVI_U_BETA_EFF], w.beff.u, ctx->nu);
+    convert(ctx, ctx->f[WAVI_V_BETA_EFF], w.beff.v, ctx->nv);
+    convert(ctx, ctx->r.u, w.r.u, ctx->nu);
+    convert(ctx, ctx->r.v, w.r.v, ctx->nv);
+    convert(ctx, ctx->f[WAVI_WU_CORRECTION_COARSE], w.xc.u, ctx->nu);
+    convert(ctx, ctx->f[WAVI_WV_CORRECTION_COARSE], w.xc.v, ctx->nv);
+    HIPCHK(hipGetLastError());
+    CHK(do_vcycle_t<float>(ctx, w, cg_iters));
+    convert(ctx, w.e.u, ctx->e.u, ctx->nu);
+    convert(ctx, w.e.v, ctx->e.v, ctx->nv);
+    convert(ctx, w.xc.u, ctx->f[WAVI_WU_CORRECTION_COARSE], ctx->nu);
+    convert(ctx, w.xc.v, ctx->f[WAVI_WV_CORRECTION_COARSE], ctx->nv);
     HIPCHK(hipGetLastError());
     return WAVI_OK;
 }
@@ -897,6 +1020,8 @@ static int do_update_velocities(wavi_ctx *ctx, wavi_solve_report *rep)
     rep->converged = converged;
     rep->rel_resid = rel;
     rep->coarse_cg_total = total;
+    // mixed precision: every Picard iteration is one FP64 residual refinement
+    rep->refinements = ctx->precision == WAVI_PRECISION_MIXED ? it : 0;
     return WAVI_OK;
 }
 
@@ -1093,7 +1218,7 @@ int wavi_op_diag(wavi_ctx *ctx, double *du, double *dv)
         ctx->err = "null pointer";
         return WAVI_EARG;
     }
-    CHK(do_diag(ctx));
+    CHK(do_diag(ctx, vset64(ctx)));
     return download2(ctx, ctx->d, du, dv);
 }
 

@@ -105,68 +105,61 @@ __device__ __forceinline__ double det_pow(double x, double y)
 }
 
 // ---------------------------------------------------------------------------
-// Canonical reduction trees.  A 256-thread workgroup reduces its tile with
-// t[e] += t[e+s] for s = 128..1: s = 128, 64 through LDS, s = 32..1 with
-// wavefront shuffles (lane e reads lane e+s, the same pairing).
+// Canonical reduction trees.  A 256-thread workgroup reduces one value per
+// thread: each 64-wide wavefront sums its lanes with t[e] += t[e+s],
+// s = 32..1 (shuffles, lane e reads lane e+s), and the four wavefront sums
+// combine as (w0 + w2) + (w1 + w3).  The oracle sums in the same order.
 template <typename T>
 __device__ __forceinline__ T tree256(T v, T *lds)
 {
-    const int t = threadIdx.x;
-    lds[t] = v;
-    __syncthreads();
-    if (t < 128) lds[t] = lds[t] + lds[t + 128];
-    __syncthreads();
-    T r = 0;
-    if (t < 64) {
-        r = lds[t] + lds[t + 64];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-        for (int s = 32; s > 0; s >>= 1) r = r + __shfl_down(r, s, 64);
-    }
+    for (int s = 32; s > 0; s >>= 1) v = v + __shfl_down(v, s, 64);
+    if (l == 0) lds[w] = v;
     __syncthreads();
-    return r;  // valid in thread 0
+    const T r = (lds[0] + lds[2]) + (lds[1] + lds[3]);
+    __syncthreads();
+    return r;
 }
 
-// Two independent trees in one pass (same pairing for each component).
+// Two independent trees in one pass.
 __device__ __forceinline__ void tree256x2(double &a, double &b, double *lds)
 {
-    const int t = threadIdx.x;
-    lds[t] = a;
-    lds[t + 256] = b;
-    __syncthreads();
-    if (t < 128) {
-        lds[t] = lds[t] + lds[t + 128];
-        lds[t + 256] = lds[t + 256] + lds[t + 384];
-    }
-    __syncthreads();
-    double ra = 0.0, rb = 0.0;
-    if (t < 64) {
-        ra = lds[t] + lds[t + 64];
-        rb = lds[t + 256] + lds[t + 320];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-        for (int s = 32; s > 0; s >>= 1) {
-            ra = ra + __shfl_down(ra, s, 64);
-            rb = rb + __shfl_down(rb, s, 64);
-        }
+    for (int s = 32; s > 0; s >>= 1) {
+        a = a + __shfl_down(a, s, 64);
+        b = b + __shfl_down(b, s, 64);
+    }
+    if (l == 0) {
+        lds[w] = a;
+        lds[4 + w] = b;
     }
     __syncthreads();
-    a = ra;
-    b = rb;  // valid in thread 0
+    a = (lds[0] + lds[2]) + (lds[1] + lds[3]);
+    b = (lds[4] + lds[6]) + (lds[5] + lds[7]);
+    __syncthreads();
 }
 
-// Fold `np` block partials into 256 lanes (lane l sums P[l], P[l+256], ...
-// in order) and sum the lanes with tree256.  Every workgroup that calls this
-// obtains the same bits; the result is broadcast to all threads.
+// Fold `np` block partials into 256 lanes (lane L = thread L sums P[L],
+// P[L+256], ... in order; missing terms add +0) and sum the lanes with
+// tree256.  Loads are issued eight at a time.  Every workgroup that calls
+// this obtains the same bits, in all threads.
 __device__ __forceinline__ double lanes_sum(const double *P, int np, double *lds)
 {
     const int t = threadIdx.x;
     double acc = 0.0;
-    for (int q = t; q < np; q += 256) acc = acc + P[q];
-    double s = tree256(acc, lds);
-    if (t == 0) lds[0] = s;
-    __syncthreads();
-    double out = lds[0];
-    __syncthreads();
-    return out;
+    for (int q0 = 0; q0 < np; q0 += 8 * 256) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int q = q0 + t + 256 * k;
+            v[k] = q < np ? P[q] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc = acc + v[k];
+    }
+    return tree256(acc, lds);
 }
 
 // ---------------------------------------------------------------------------

@@ -32,6 +32,14 @@ __global__ void k_select(const Tin *in, const uint8_t *mask, Tout *out, int n)
     if (k < n) out[k] = mask[k] ? (Tout)in[k] : Tout(0);
 }
 
+// out = (Tout)in
+template <typename Tin, typename Tout>
+__global__ void k_convert(const Tin *in, Tout *out, int n)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = (Tout)in[k];
+}
+
 // a = a + b where mask (inner points)
 template <typename T>
 __global__ void k_add_masked(T *a, const T *b, const uint8_t *mask, int n)
@@ -866,70 +874,195 @@ __device__ __forceinline__ void optile_load(const GridDims &g, const T *U, const
     }
 }
 
+// Coefficients of one h cell and its c corner (the cell (i0-1+x, j0-1+y) of
+// the 17 x 17 window, q = x + 17 y), loaded before they are needed so that a
+// tile costs one round trip to memory.
 template <typename T>
-__device__ __forceinline__ void optile_mid(const GridDims &g, const OpCoef<T> &c, int i0, int j0, OpTile<T> &s)
+struct CellCoef {
+    T heta, dimp, hu0, hu1, hv0, hv1, hetac;
+    uint8_t hm, um0, um1, vm0, vm1, cm;
+    bool cell, corner;
+};
+
+template <typename T>
+__device__ __forceinline__ void cell_prefetch(const GridDims &g, const OpCoef<T> &c, int i0, int j0, int q,
+                                              CellCoef<T> &cc)
 {
     const int nx = g.nx, ny = g.ny, nxu = g.nxu;
-    for (int q = threadIdx.x; q < 17 * 17; q += 256) {
-        const int x = q % 17, y = q / 17;
-        const int a = i0 - 1 + x, b = j0 - 1 + y;
-        T rxx = T(0), ryy = T(0), ext = T(0), rxy = T(0);
-        if (a >= 0 && a < nx && b >= 0 && b < ny) {
-            const int k = a + nx * b;
-            const T u0 = s.U[y][x], u1 = s.U[y][x + 1], v0 = s.V[y][x], v1 = s.V[y + 1][x];
-            if (c.hm[k]) {
-                const T dudx = c.rdx * u1 - c.rdx * u0;
-                const T dvdy = c.rdy * v1 - c.rdy * v0;
-                rxx = T(2) * (c.heta_h[k] * (T(2) * dudx + dvdy));
-                ryy = T(2) * (c.heta_h[k] * (T(2) * dvdy + dudx));
-            }
-            const int ku0 = a + nxu * b, kv0 = a + nx * b, kv1 = kv0 + nx;
-            const T qx1 = c.um[ku0 + 1] ? c.hu[ku0 + 1] * u1 : T(0);
-            const T qx0 = c.um[ku0] ? c.hu[ku0] * u0 : T(0);
-            const T qy1 = c.vm[kv1] ? c.hv[kv1] * v1 : T(0);
-            const T qy0 = c.vm[kv0] ? c.hv[kv0] * v0 : T(0);
-            const T divq = (c.rdx * qx1 - c.rdx * qx0) + (c.rdy * qy1 - c.rdy * qy0);
-            ext = c.dimp[k] * divq;
-        }
-        if (a >= 0 && a < nx - 1 && b >= 0 && b < ny - 1) {
-            const int kc = a + (nx - 1) * b;
-            if (c.cm[kc]) {
-                const T dudy = c.rdy * s.U[y + 1][x + 1] - c.rdy * s.U[y][x + 1];
-                const T dvdx = c.rdx * s.V[y + 1][x + 1] - c.rdx * s.V[y + 1][x];
-                rxy = c.heta_c[kc] * (dudy + dvdx);
-            }
-        }
-        s.rxx[y][x] = rxx;
-        s.ryy[y][x] = ryy;
-        s.ext[y][x] = ext;
-        s.rxy[y][x] = rxy;
+    const int x = q % 17, y = q / 17;
+    const int a = i0 - 1 + x, b = j0 - 1 + y;
+    cc.cell = q < 289 && a >= 0 && a < nx && b >= 0 && b < ny;
+    cc.corner = q < 289 && a >= 0 && a < nx - 1 && b >= 0 && b < ny - 1;
+    if (cc.cell) {
+        const int k = a + nx * b, ku0 = a + nxu * b, kv0 = a + nx * b, kv1 = kv0 + nx;
+        cc.hm = c.hm[k];
+        cc.heta = c.heta_h[k];
+        cc.dimp = c.dimp[k];
+        cc.um0 = c.um[ku0];
+        cc.um1 = c.um[ku0 + 1];
+        cc.hu0 = c.hu[ku0];
+        cc.hu1 = c.hu[ku0 + 1];
+        cc.vm0 = c.vm[kv0];
+        cc.vm1 = c.vm[kv1];
+        cc.hv0 = c.hv[kv0];
+        cc.hv1 = c.hv[kv1];
+    }
+    if (cc.corner) {
+        const int kc = a + (nx - 1) * b;
+        cc.cm = c.cm[kc];
+        cc.hetac = c.heta_c[kc];
     }
 }
 
-// fx at u point (i, j) = (i0 + x, j0 + y)
+// Intermediates of one cell / corner from the staged inputs (same
+// expressions as op_rxx, op_ryy, op_extra, op_rxy).
 template <typename T>
-__device__ __forceinline__ T optile_u(const GridDims &g, const OpCoef<T> &c, const OpTile<T> &s, int i, int j, int x,
-                                      int y)
+__device__ __forceinline__ void cell_compute(const OpCoef<T> &c, const CellCoef<T> &cc, int q, OpTile<T> &s)
 {
-    const int k = i + g.nxu * j;
-    const T d_rxx_dx = c.rdx * s.rxx[y + 1][x + 1] - c.rdx * s.rxx[y + 1][x];
-    const T d_rxy_dy = c.rdy * s.rxy[y + 1][x] - c.rdy * s.rxy[y][x];
-    const T taubx = c.um[k] ? c.bu[k] * s.U[y + 1][x + 1] : T(0);
-    const T d_extra_dx = c.rdx * s.ext[y + 1][x + 1] - c.rdx * s.ext[y + 1][x];
-    return ((d_rxx_dx + d_rxy_dy) - taubx) - c.hu[k] * d_extra_dx;
+    if (q >= 289) return;
+    const int x = q % 17, y = q / 17;
+    T rxx = T(0), ryy = T(0), ext = T(0), rxy = T(0);
+    if (cc.cell) {
+        const T u0 = s.U[y][x], u1 = s.U[y][x + 1], v0 = s.V[y][x], v1 = s.V[y + 1][x];
+        if (cc.hm) {
+            const T dudx = c.rdx * u1 - c.rdx * u0;
+            const T dvdy = c.rdy * v1 - c.rdy * v0;
+            rxx = T(2) * (cc.heta * (T(2) * dudx + dvdy));
+            ryy = T(2) * (cc.heta * (T(2) * dvdy + dudx));
+        }
+        const T qx1 = cc.um1 ? cc.hu1 * u1 : T(0);
+        const T qx0 = cc.um0 ? cc.hu0 * u0 : T(0);
+        const T qy1 = cc.vm1 ? cc.hv1 * v1 : T(0);
+        const T qy0 = cc.vm0 ? cc.hv0 * v0 : T(0);
+        const T divq = (c.rdx * qx1 - c.rdx * qx0) + (c.rdy * qy1 - c.rdy * qy0);
+        ext = cc.dimp * divq;
+    }
+    if (cc.corner && cc.cm) {
+        const T dudy = c.rdy * s.U[y + 1][x + 1] - c.rdy * s.U[y][x + 1];
+        const T dvdx = c.rdx * s.V[y + 1][x + 1] - c.rdx * s.V[y + 1][x];
+        rxy = cc.hetac * (dudy + dvdx);
+    }
+    s.rxx[y][x] = rxx;
+    s.ryy[y][x] = ryy;
+    s.ext[y][x] = ext;
+    s.rxy[y][x] = rxy;
 }
 
-// fy at v point (i, j) = (i0 + x, j0 + y)
+// Coefficients of the thread's own u and v output points.
 template <typename T>
-__device__ __forceinline__ T optile_v(const GridDims &g, const OpCoef<T> &c, const OpTile<T> &s, int i, int j, int x,
-                                      int y)
+struct PointCoef {
+    T bu, hu, bv, hv;
+    bool oku, okv, ui, um, vi, vm, mwu, mwv;
+    int ku, kv;
+};
+
+template <typename T>
+__device__ __forceinline__ void point_prefetch(const GridDims &g, const OpCoef<T> &c, const uint8_t *mwu,
+                                               const uint8_t *mwv, int i, int j, bool has_u, bool has_v,
+                                               PointCoef<T> &pc)
 {
-    const int k = i + g.nxv * j;
+    pc.oku = has_u && i < g.nxu && j < g.nyu;
+    pc.okv = has_v && i < g.nxv && j < g.nyv;
+    pc.ku = i + g.nxu * j;
+    pc.kv = i + g.nxv * j;
+    pc.ui = pc.um = pc.vi = pc.vm = pc.mwu = pc.mwv = false;
+    pc.bu = pc.hu = pc.bv = pc.hv = T(0);
+    if (pc.oku) {
+        pc.ui = c.ui[pc.ku];
+        pc.um = c.um[pc.ku];
+        pc.bu = c.bu[pc.ku];
+        pc.hu = c.hu[pc.ku];
+        pc.mwu = mwu[pc.ku];
+    }
+    if (pc.okv) {
+        pc.vi = c.vi[pc.kv];
+        pc.vm = c.vm[pc.kv];
+        pc.bv = c.bv[pc.kv];
+        pc.hv = c.hv[pc.kv];
+        pc.mwv = mwv[pc.kv];
+    }
+}
+
+// fx at the u point (i0 + x, j0 + y)
+template <typename T>
+__device__ __forceinline__ T optile_u(const OpCoef<T> &c, const OpTile<T> &s, const PointCoef<T> &pc, int x, int y)
+{
+    const T d_rxx_dx = c.rdx * s.rxx[y + 1][x + 1] - c.rdx * s.rxx[y + 1][x];
+    const T d_rxy_dy = c.rdy * s.rxy[y + 1][x] - c.rdy * s.rxy[y][x];
+    const T taubx = pc.um ? pc.bu * s.U[y + 1][x + 1] : T(0);
+    const T d_extra_dx = c.rdx * s.ext[y + 1][x + 1] - c.rdx * s.ext[y + 1][x];
+    return ((d_rxx_dx + d_rxy_dy) - taubx) - pc.hu * d_extra_dx;
+}
+
+// fy at the v point (i0 + x, j0 + y)
+template <typename T>
+__device__ __forceinline__ T optile_v(const OpCoef<T> &c, const OpTile<T> &s, const PointCoef<T> &pc, int x, int y)
+{
     const T d_ryy_dy = c.rdy * s.ryy[y + 1][x + 1] - c.rdy * s.ryy[y][x + 1];
     const T d_rxy_dx = c.rdx * s.rxy[y][x + 1] - c.rdx * s.rxy[y][x];
-    const T tauby = c.vm[k] ? c.bv[k] * s.V[y + 1][x + 1] : T(0);
+    const T tauby = pc.vm ? pc.bv * s.V[y + 1][x + 1] : T(0);
     const T d_extra_dy = c.rdy * s.ext[y + 1][x + 1] - c.rdy * s.ext[y][x + 1];
-    return ((d_ryy_dy + d_rxy_dx) - tauby) - c.hv[k] * d_extra_dy;
+    return ((d_ryy_dy + d_rxy_dx) - tauby) - pc.hv * d_extra_dy;
+}
+
+// Stage one tile of a fine vector: each thread loads its (at most two)
+// staging entries into registers; optile_store puts them in LDS.
+template <typename T>
+struct StageRegs {
+    T u0, v0, u1, v1;
+};
+
+template <typename T>
+__device__ __forceinline__ void optile_fetch(const GridDims &g, const T *U, const T *V, int i0, int j0,
+                                             StageRegs<T> &r)
+{
+    const int q0 = threadIdx.x, q1 = threadIdx.x + 256;
+    {
+        const int x = q0 % 18, y = q0 / 18, i = i0 - 1 + x, j = j0 - 1 + y;
+        r.u0 = (i >= 0 && i < g.nxu && j >= 0 && j < g.nyu) ? U[i + g.nxu * j] : T(0);
+        r.v0 = (i >= 0 && i < g.nxv && j >= 0 && j < g.nyv) ? V[i + g.nxv * j] : T(0);
+    }
+    r.u1 = r.v1 = T(0);
+    if (q1 < 324) {
+        const int x = q1 % 18, y = q1 / 18, i = i0 - 1 + x, j = j0 - 1 + y;
+        r.u1 = (i >= 0 && i < g.nxu && j >= 0 && j < g.nyu) ? U[i + g.nxu * j] : T(0);
+        r.v1 = (i >= 0 && i < g.nxv && j >= 0 && j < g.nyv) ? V[i + g.nxv * j] : T(0);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void optile_store(const StageRegs<T> &r, OpTile<T> &s)
+{
+    const int q0 = threadIdx.x, q1 = threadIdx.x + 256;
+    s.U[q0 / 18][q0 % 18] = r.u0;
+    s.V[q0 / 18][q0 % 18] = r.v0;
+    if (q1 < 324) {
+        s.U[q1 / 18][q1 % 18] = r.u1;
+        s.V[q1 / 18][q1 % 18] = r.v1;
+    }
+}
+
+// The whole tile operator with prefetch: stage t, compute intermediates,
+// outputs into tu / tv (zero off the inner masks).  Ends with a barrier.
+template <typename T>
+__device__ __forceinline__ void optile_apply(const GridDims &g, const OpCoef<T> &c, const T *tU, const T *tV,
+                                             int i0, int j0, const PointCoef<T> &pc, OpTile<T> &s,
+                                             T (*tu)[TILE + 1], T (*tv)[TILE + 1])
+{
+    StageRegs<T> sr;
+    CellCoef<T> c0, c1;
+    optile_fetch(g, tU, tV, i0, j0, sr);
+    cell_prefetch(g, c, i0, j0, threadIdx.x, c0);
+    cell_prefetch(g, c, i0, j0, threadIdx.x + 256, c1);
+    optile_store(sr, s);
+    __syncthreads();
+    cell_compute(c, c0, threadIdx.x, s);
+    cell_compute(c, c1, threadIdx.x + 256, s);
+    __syncthreads();
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    tu[ly][lx] = (pc.oku && pc.ui) ? optile_u(c, s, pc, lx, ly) : T(0);
+    tv[ly][lx] = (pc.okv && pc.vi) ? optile_v(c, s, pc, lx, ly) : T(0);
+    __syncthreads();
 }
 
 // ===========================================================================
@@ -937,15 +1070,16 @@ __device__ __forceinline__ T optile_v(const GridDims &g, const OpCoef<T> &c, con
 // with x0 the cached warm start: three kernels per iteration and no host
 // round trip.  Kernels that end in a reduction write one partial per 16x16
 // block; the next kernel folds them with the canonical tree in every
-// workgroup (all get the same bits), so no workgroup ever waits for another
-// inside a kernel.  Scalars that a kernel publishes are double-buffered by
-// iteration parity; every workgroup writes the same value.
+// workgroup (all get the same bits), so no workgroup waits for another inside
+// a kernel.  Scalars a kernel publishes are double-buffered by iteration
+// parity and every workgroup writes the same value; a kernel never reads a
+// scalar that another workgroup of the same kernel writes.
 
 struct CGScalars {
-    double res[2];    // residual norm by iteration parity
+    double res[2];    // residual norm by iteration parity (written by k_cg_dir)
     double tol;       // max(reltol ||r0||, abstol)
     int it[2];        // iterations done, by parity
-    int done;         // sticky convergence / maxiter flag
+    int done;         // sticky stop flag, written only by k_cg_dir
     int iters;        // iterations at termination
 };
 
@@ -967,15 +1101,6 @@ struct CGArgs {
     int nblk;         // launched workgroups, a multiple of 8
 };
 
-// Workgroup-uniform read of a flag that other workgroups of the same kernel
-// may be writing: thread 0 reads it once and broadcasts it through LDS.
-__device__ __forceinline__ int uniform_flag(const int *p, int *slot)
-{
-    if (threadIdx.x == 0) *slot = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    return *slot;
-}
-
 // XCD-aware tile order: workgroups are dispatched round-robin over the 8
 // XCDs, so workgroup w takes tile (w % 8) * (nblk / 8) + w / 8 and each XCD
 // walks one contiguous band of tiles.
@@ -985,25 +1110,15 @@ __device__ __forceinline__ int xcd_tile(int nblk)
     return (w & 7) * (nblk >> 3) + (w >> 3);
 }
 
-// A t on the tile's u and v points into tu / tv (zero off the inner masks),
-// then R = transposed inverse Haar, zero off the wavelet masks.
+// c = R A t for the tile: A into tu / tv, then the transposed inverse Haar,
+// zero off the wavelet masks.
 template <typename T>
-__device__ __forceinline__ void tile_apply_restrict2(const CGArgs<T> &a, const OpTile<T> &s, int i0, int j0,
-                                                     bool has_u, bool has_v, T (*tu)[TILE + 1], T (*mu)[TILE + 1],
-                                                     T (*tv)[TILE + 1], T (*mv)[TILE + 1])
+__device__ __forceinline__ void tile_apply_restrict(const CGArgs<T> &a, int i0, int j0, bool has_u, bool has_v,
+                                                    const PointCoef<T> &pc, OpTile<T> &s, T (*tu)[TILE + 1],
+                                                    T (*mu)[TILE + 1], T (*tv)[TILE + 1], T (*mv)[TILE + 1])
 {
-    const GridDims &g = a.g;
-    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int i = i0 + lx, j = j0 + ly;
-    const bool oku = has_u && i < g.nxu && j < g.nyu;
-    const bool okv = has_v && i < g.nxv && j < g.nyv;
-    tu[ly][lx] = (oku && a.ui[i + g.nxu * j]) ? optile_u(g, a.c, s, i, j, lx, ly) : T(0);
-    tv[ly][lx] = (okv && a.vi[i + g.nxv * j]) ? optile_v(g, a.c, s, i, j, lx, ly) : T(0);
-    __syncthreads();
-    haar_pm_tile2<T>(a.levels, 1, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
-    if (oku && !a.mwu[i + g.nxu * j]) tu[ly][lx] = T(0);
-    if (okv && !a.mwv[i + g.nxv * j]) tv[ly][lx] = T(0);
-    __syncthreads();
+    optile_apply(a.g, a.c, a.tu, a.tv, i0, j0, pc, s, tu, tv);
+    haar_pm_tile2<T>(a.levels, 1, tu, mu, tv, mv, i0, j0, a.g, has_u, has_v);
 }
 
 // setup (t = P x0 computed beforehand): r = b - R A t (b arrives in r),
@@ -1013,7 +1128,7 @@ __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
 {
     __shared__ OpTile<T> s;
     __shared__ T tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
-    __shared__ double lds[512];
+    __shared__ double lds[8];
     const GridDims &g = a.g;
     const int L = xcd_tile(a.nblk);
     if (L >= g.nbx * g.nby) return;
@@ -1027,30 +1142,24 @@ __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
     const int i0 = bx * TILE, j0 = by * TILE;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int i = i0 + lx, j = j0 + ly;
-    const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
-    const int ku = i + g.nxu * j, kv = i + g.nxv * j;
-    optile_load(g, a.tu, a.tv, i0, j0, s);
-    __syncthreads();
-    optile_mid(g, a.c, i0, j0, s);
-    __syncthreads();
-    tile_apply_restrict2<T>(a, s, i0, j0, has_u, has_v, tu, mu, tv, mv);
+    PointCoef<T> pc;
+    point_prefetch(g, a.c, a.mwu, a.mwv, i0 + lx, j0 + ly, has_u, has_v, pc);
+    const T bu = pc.oku ? a.ru[pc.ku] : T(0), bv = pc.okv ? a.rv[pc.kv] : T(0);
+    tile_apply_restrict<T>(a, i0, j0, has_u, has_v, pc, s, tu, mu, tv, mv);
     double r2u = 0.0, b2u = 0.0, r2v = 0.0, b2v = 0.0;
-    if (oku) {
-        const T b = a.ru[ku];
-        const T r = b - tu[ly][lx];
-        a.ru[ku] = r;
-        a.pu[ku] = T(0);
+    if (pc.oku) {
+        const T r = bu - (pc.mwu ? tu[ly][lx] : T(0));
+        a.ru[pc.ku] = r;
+        a.pu[pc.ku] = T(0);
         r2u = (double)r * (double)r;
-        b2u = (double)b * (double)b;
+        b2u = (double)bu * (double)bu;
     }
-    if (okv) {
-        const T b = a.rv[kv];
-        const T r = b - tv[ly][lx];
-        a.rv[kv] = r;
-        a.pv[kv] = T(0);
+    if (pc.okv) {
+        const T r = bv - (pc.mwv ? tv[ly][lx] : T(0));
+        a.rv[pc.kv] = r;
+        a.pv[pc.kv] = T(0);
         r2v = (double)r * (double)r;
-        b2v = (double)b * (double)b;
+        b2v = (double)bv * (double)bv;
     }
     tree256x2(r2u, r2v, lds);
     tree256x2(b2u, b2v, lds);
@@ -1067,13 +1176,15 @@ __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
     }
 }
 
-// part 1 (iteration parity par): residual norm and convergence test, then
-// u = r + beta u with beta = res^2 / prev^2, t = P u
+// part 1 (iteration parity par): residual norm and stop test, then
+// u = r + beta u with beta = res^2 / prev^2, t = P u.  The stop decision is
+// recomputed from the partials each time, so it needs no flag: after a stop
+// the partials are frozen and it[1 - par] repeats the count.
 template <typename T>
 __global__ void __launch_bounds__(256) k_cg_dir(CGArgs<T> a, int par)
 {
     __shared__ T tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
-    __shared__ double lds[256];
+    __shared__ double lds[8];
     const GridDims &g = a.g;
     const int L = xcd_tile(a.nblk);
     if (L >= g.nbx * g.nby) return;
@@ -1084,14 +1195,13 @@ __global__ void __launch_bounds__(256) k_cg_dir(CGArgs<T> a, int par)
     const int i = i0 + lx, j = j0 + ly;
     const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
     const int ku = i + g.nxu * j, kv = i + g.nxv * j;
-    __shared__ int flag;
     // independent loads first
     const int it = a.st->it[par];
+    const double told = a.st->tol, prev_res = a.st->res[1 - par];
     const T ru = oku ? a.ru[ku] : T(0), pu = oku ? a.pu[ku] : T(0);
     const T rv = okv ? a.rv[kv] : T(0), pv = okv ? a.pv[kv] : T(0);
     const bool mwu = oku && a.mwu[ku], mwv = okv && a.mwv[kv];
     const bool iu = oku && a.ui[ku], iv = okv && a.vi[kv];
-    if (uniform_flag(&a.st->done, &flag)) return;
     const double res = sqrt(lanes_sum(a.Pr, g.npart, lds));
     double tol;
     if (it == 0) {
@@ -1099,7 +1209,7 @@ __global__ void __launch_bounds__(256) k_cg_dir(CGArgs<T> a, int par)
         tol = a.reltol * res;
         if (abstol > tol) tol = abstol;
     } else {
-        tol = a.st->tol;
+        tol = told;
     }
     const bool done = (it >= a.maxiter) || (res <= tol);
     if (threadIdx.x == 0) {
@@ -1108,10 +1218,11 @@ __global__ void __launch_bounds__(256) k_cg_dir(CGArgs<T> a, int par)
         if (done) {
             a.st->done = 1;
             a.st->iters = it;
+            a.st->it[1 - par] = it;
         }
     }
     if (done) return;
-    const double prev = it == 0 ? 1.0 : a.st->res[1 - par];
+    const double prev = it == 0 ? 1.0 : prev_res;
     const T beta = (T)((res * res) / (prev * prev));
     T unu = T(0), unv = T(0);
     if (oku) {
@@ -1136,33 +1247,28 @@ __global__ void __launch_bounds__(256) k_cg_apply(CGArgs<T> a)
 {
     __shared__ OpTile<T> s;
     __shared__ T tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
-    __shared__ double lds[512];
+    __shared__ double lds[8];
     const GridDims &g = a.g;
     const int L = xcd_tile(a.nblk);
     if (L >= g.nbx * g.nby) return;
+    if (a.st->done) return;  // written only by earlier kernels
     const int bx = L % g.nbx, by = L / g.nbx;
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
     const int i0 = bx * TILE, j0 = by * TILE;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int i = i0 + lx, j = j0 + ly;
-    const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
-    const int ku = i + g.nxu * j, kv = i + g.nxv * j;
-    __shared__ int flag;
-    const T pu = oku ? a.pu[ku] : T(0), pv = okv ? a.pv[kv] : T(0);
-    optile_load(g, a.tu, a.tv, i0, j0, s);
-    if (uniform_flag(&a.st->done, &flag)) return;
-    optile_mid(g, a.c, i0, j0, s);
-    __syncthreads();
-    tile_apply_restrict2<T>(a, s, i0, j0, has_u, has_v, tu, mu, tv, mv);
+    PointCoef<T> pc;
+    point_prefetch(g, a.c, a.mwu, a.mwv, i0 + lx, j0 + ly, has_u, has_v, pc);
+    const T pu = pc.oku ? a.pu[pc.ku] : T(0), pv = pc.okv ? a.pv[pc.kv] : T(0);
+    tile_apply_restrict<T>(a, i0, j0, has_u, has_v, pc, s, tu, mu, tv, mv);
     double du = 0.0, dv = 0.0;
-    if (oku) {
-        const T cv = tu[ly][lx];
-        a.qu[ku] = cv;
+    if (pc.oku) {
+        const T cv = pc.mwu ? tu[ly][lx] : T(0);
+        a.qu[pc.ku] = cv;
         du = (double)(pu * cv);
     }
-    if (okv) {
-        const T cv = tv[ly][lx];
-        a.qv[kv] = cv;
+    if (pc.okv) {
+        const T cv = pc.mwv ? tv[ly][lx] : T(0);
+        a.qv[pc.kv] = cv;
         dv = (double)(pv * cv);
     }
     tree256x2(du, dv, lds);
@@ -1177,22 +1283,21 @@ __global__ void __launch_bounds__(256) k_cg_apply(CGArgs<T> a)
 template <typename T>
 __global__ void __launch_bounds__(256) k_cg_update(CGArgs<T> a, int par)
 {
-    __shared__ double lds[512];
+    __shared__ double lds[8];
     const GridDims &g = a.g;
     const int L = xcd_tile(a.nblk);
     if (L >= g.nbx * g.nby) return;
+    if (a.st->done) return;  // written only by earlier kernels
     const int bx = L % g.nbx, by = L / g.nbx;
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     const int i = bx * TILE + lx, j = by * TILE + ly;
     const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
     const int ku = i + g.nxu * j, kv = i + g.nxv * j;
-    __shared__ int flag;
     const int it = a.st->it[par];
     const double res = a.st->res[par];
     const T xu = oku ? a.xu[ku] : T(0), pu = oku ? a.pu[ku] : T(0), ru = oku ? a.ru[ku] : T(0), qu = oku ? a.qu[ku] : T(0);
     const T xv = okv ? a.xv[kv] : T(0), pv = okv ? a.pv[kv] : T(0), rv = okv ? a.rv[kv] : T(0), qv = okv ? a.qv[kv] : T(0);
-    if (uniform_flag(&a.st->done, &flag)) return;
     const double dot = lanes_sum(a.Pd, g.npart, lds);
     const T alpha = (T)((res * res) / dot);
     if (threadIdx.x == 0) a.st->it[1 - par] = it + 1;
