@@ -49,15 +49,33 @@ def cg_apply_bytes(nx, ny, w=8):
     return w * (5 * (nu + nv) + 2 * nh + nc) + 3 * (nu + nv) + nh + nc
 
 
+def cgm_iter_bytes(nx, ny, w=4):
+    """Algorithmic HBM bytes of one k_cgm_iter launch (one whole iteration of
+    the mixed-precision coarse CG): r, w, p, s, x on the tile and the images
+    P r, P s, P w in; p, s, x, r, w and the three images out; beta_eff and
+    thickness on u and v, h*eta_av on h and c, dimp on h (w bytes each);
+    masks (one byte each): inner u, v; wavelet u, v; u, v masks; h and c
+    masks."""
+    nu, nv, nh, nc = (nx + 1) * ny, nx * (ny + 1), nx * ny, (nx - 1) * (ny - 1)
+    return w * (18 * (nu + nv) + 2 * nh + nc) + 3 * (nu + nv) + nh + nc
+
+
+DOMINANT = {  # precision -> (kernel, timing id, bytes per launch)
+    "fp64": ("k_cg_apply", "cg_apply", lambda nx, ny: cg_apply_bytes(nx, ny, 8)),
+    "mixed": ("k_cgm_iter", "cg_iter_mixed", lambda nx, ny: cgm_iter_bytes(nx, ny, 4)),
+}
+
+
 def measured_traffic(nx, ny, precision):
-    """HBM bytes per k_cg_apply launch from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary for this workload, scaled by the copy-kernel
-    calibration recorded there; None when no summary matches."""
-    path = os.path.join(ROOT, "profiles", "k_cg_apply_%dx%d_%s_traffic.md" % (nx, ny, precision))
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary for this workload, scaled by the
+    copy-kernel calibration recorded there; None when no summary matches."""
+    kernel = DOMINANT[precision][0]
+    path = os.path.join(ROOT, "profiles", "%s_%dx%d_%s_traffic.md" % (kernel, nx, ny, precision))
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch k_cg_apply:"):
+        if line.startswith("traffic_bytes_per_launch %s:" % kernel):
             return int(line.split(":")[1])
     return None
 
@@ -189,10 +207,11 @@ def main():
             k, dt, rep["picard_iters"], rep["coarse_cg_total"], rep["rel_resid"]))
     ms = ctypes.c_double(0.0)
     n = ctypes.c_int64(0)
-    lib.wavi_profile_read(spec.ctx, W._lib.KT["cg_apply"], ctypes.byref(ms), ctypes.byref(n))
+    kernel, kid, nbytes = DOMINANT[args.precision]
+    lib.wavi_profile_read(spec.ctx, W._lib.KT[kid], ctypes.byref(ms), ctypes.byref(n))
     lib.wavi_profile_enable(spec.ctx, 0)
     avg_s = ms.value / 1e3 / max(n.value, 1)
-    B = cg_apply_bytes(nx, ny, 8 if args.precision == "fp64" else 4)
+    B = nbytes(nx, ny)
     achieved = B / avg_s / 1e9
     total = float(sum(times))
     value = args.steps / total
@@ -213,7 +232,7 @@ def main():
                    "grid": [nx, ny, 8], "precision": args.precision,
                    "picard_iters": rep["picard_iters"], "coarse_cg_total": rep["coarse_cg_total"],
                    "rel_resid": rep["rel_resid"]},
-        "roofline": {"bound": "hbm", "kernel": "k_cg_apply", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(nx, ny, args.precision),
                      "bytes_per_launch": B, "avg_launch_us": avg_s * 1e6, "launches": n.value},

@@ -206,7 +206,8 @@ int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out);
  * kernel: WAVI_KT_CG_APPLY (coarse CG: A then R on one tile, dot partials),
  * WAVI_KT_CG_DIR (coarse CG: direction update and prolongation),
  * WAVI_KT_CG_UPDATE (coarse CG: axpys and norm partials), WAVI_KT_COLOUR
- * (one smoother colour sweep).  enable=1 starts (and clears) the
+ * (one smoother colour sweep), WAVI_KT_CG_ITER_MIXED (mixed-precision coarse
+ * CG: one whole iteration per launch).  enable=1 starts (and clears) the
  * accumulation; every 16th launch of each kernel is timed, and
  * wavi_profile_read returns the total milliseconds and the number of timed
  * launches.
@@ -215,7 +216,8 @@ int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out);
 #define WAVI_KT_CG_DIR 1
 #define WAVI_KT_CG_UPDATE 2
 #define WAVI_KT_COLOUR 3
-#define WAVI_KT_COUNT 4
+#define WAVI_KT_CG_ITER_MIXED 4
+#define WAVI_KT_COUNT 5
 int wavi_profile_enable(wavi_ctx *ctx, int enable);
 int wavi_profile_read(wavi_ctx *ctx, int kernel, double *total_ms, int64_t *launches);
 

@@ -21,7 +21,7 @@ PRECISION = {"fp64": 0, "mixed": 1}
 
 REPORT_MAX_PICARD = 64
 
-KT = {"cg_apply": 0, "cg_dir": 1, "cg_update": 2, "colour": 3}
+KT = {"cg_apply": 0, "cg_dir": 1, "cg_update": 2, "colour": 3, "cg_iter_mixed": 4}
 
 # enum wavi_field, in header order
 FIELD_NAMES = [

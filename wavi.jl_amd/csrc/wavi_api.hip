@@ -57,6 +57,12 @@ struct wavi_ctx {
     // coarse CG vectors
     DevVec cr, cp, cq, ct;
     VSet<float> v32;
+    // mixed-precision coarse CG: s, w, the parity-buffered images P r, P s,
+    // P w and partials of (r, r) and (w, r)
+    DevVecT<float> ms, mw, mtr[2], mts[2], mtw[2];
+    double *Pg[2] = {}, *Pdm[2] = {};
+    CGMScalars *stm = nullptr;
+    CGMScalars *h_stm = nullptr;  // pinned host copy
     double *Pr0 = nullptr, *Pr1 = nullptr, *Pd = nullptr, *Pb = nullptr, *Pn = nullptr;
     double *quad = nullptr;  // 4 * nh pos_fraction quadrants
     double *norms = nullptr; // device: 2 doubles
@@ -460,10 +466,19 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
         CREATECHK(falloc(&w.heta_h, ctx->nh));
         CREATECHK(falloc(&w.heta_c, ctx->nc > 0 ? ctx->nc : 1));
         CREATECHK(falloc(&w.dimp, ctx->nh));
-        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cr, &w.cp, &w.cq, &w.ct, &w.xc}) {
+        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cr, &w.cp, &w.cq, &w.ct, &w.xc,
+                                   &ctx->ms, &ctx->mw, &ctx->mtr[0], &ctx->mtr[1], &ctx->mts[0], &ctx->mts[1],
+                                   &ctx->mtw[0], &ctx->mtw[1]}) {
             CREATECHK(falloc(&dv->u, ctx->nu));
             CREATECHK(falloc(&dv->v, ctx->nv));
         }
+        for (int p = 0; p < 2; ++p) {
+            CREATECHK(zalloc(&ctx->Pg[p], g.npart));
+            CREATECHK(zalloc(&ctx->Pdm[p], g.npart));
+        }
+        CREATECHK(hipMalloc(&ctx->stm, sizeof(CGMScalars)));
+        CREATECHK(hipMemset(ctx->stm, 0, sizeof(CGMScalars)));
+        CREATECHK(hipHostMalloc((void **)&ctx->h_stm, sizeof(CGMScalars), hipHostMallocDefault));
     }
     {
         HaarMasks hm_tab[4];
@@ -502,11 +517,19 @@ int wavi_ctx_destroy(wavi_ctx *ctx)
         VSet<float> &w = ctx->v32;
         for (float *p : {w.heta_h, w.heta_c, w.dimp})
             if (p) (void)hipFree(p);
-        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cr, &w.cp, &w.cq, &w.ct, &w.xc}) {
+        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cr, &w.cp, &w.cq, &w.ct, &w.xc,
+                                   &ctx->ms, &ctx->mw, &ctx->mtr[0], &ctx->mtr[1], &ctx->mts[0], &ctx->mts[1],
+                                   &ctx->mtw[0], &ctx->mtw[1]}) {
             if (dv->u) (void)hipFree(dv->u);
             if (dv->v) (void)hipFree(dv->v);
         }
+        for (int p = 0; p < 2; ++p) {
+            if (ctx->Pg[p]) (void)hipFree(ctx->Pg[p]);
+            if (ctx->Pdm[p]) (void)hipFree(ctx->Pdm[p]);
+        }
     }
+    if (ctx->stm) (void)hipFree(ctx->stm);
+    if (ctx->h_stm) (void)hipHostFree(ctx->h_stm);
     if (ctx->st) (void)hipFree(ctx->st);
     if (ctx->tab) (void)hipFree(ctx->tab);
     if (ctx->h_norms) (void)hipHostFree(ctx->h_norms);
@@ -893,6 +916,92 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
         }
     }
     *iters = ctx->h_st->iters;
+    return WAVI_OK;
+}
+
+static CGMArgs cgm_args(wavi_ctx *ctx)
+{
+    const VSet<float> &w = ctx->v32;
+    CGMArgs a;
+    a.g = ctx->g;
+    a.c = coef_of(ctx, w);
+    a.levels = ctx->S.levels;
+    a.mwu = ctx->mwu;
+    a.mwv = ctx->mwv;
+    a.ui = ctx->ui;
+    a.vi = ctx->vi;
+    a.xu = w.xc.u;
+    a.xv = w.xc.v;
+    a.ru = w.cr.u;
+    a.rv = w.cr.v;
+    a.pu = w.cp.u;
+    a.pv = w.cp.v;
+    a.su = ctx->ms.u;
+    a.sv = ctx->ms.v;
+    a.wu = ctx->mw.u;
+    a.wv = ctx->mw.v;
+    a.t0u = w.ct.u;
+    a.t0v = w.ct.v;
+    for (int p = 0; p < 2; ++p) {
+        a.tru[p] = ctx->mtr[p].u;
+        a.trv[p] = ctx->mtr[p].v;
+        a.tsu[p] = ctx->mts[p].u;
+        a.tsv[p] = ctx->mts[p].v;
+        a.twu[p] = ctx->mtw[p].u;
+        a.twv[p] = ctx->mtw[p].v;
+        a.Pg[p] = ctx->Pg[p];
+        a.Pd[p] = ctx->Pdm[p];
+    }
+    a.Pb = ctx->Pb;
+    a.st = ctx->stm;
+    a.tol_coarse = ctx->S.tol_coarse;
+    a.reltol = std::sqrt(2.220446049250313e-16);  // IterativeSolvers' default sqrt(eps)
+    a.maxiter = ctx->S.maxiter_coarse;
+    a.nblk = ctx->nblk;
+    return a;
+}
+
+// Mixed-precision coarse CG (Chronopoulos-Gear form, one launch per
+// iteration) on b_c held in v32.cr; solution in the warm start v32.xc.  The
+// images P r and P s are recomputed after every chunk of 32 launches.
+template <>
+int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
+{
+    const GridDims &g = ctx->g;
+    dim3 tg(g.nbx, g.nby), tb(ctx->nblk);
+    CGMArgs a = cgm_args(ctx);
+    // P x0
+    hipLaunchKernelGGL((k_tile_haar<float, float>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1,
+                       a.xu, a.xv, ctx->mwu, ctx->mwv, w.ct.u, w.ct.v, ctx->ui, ctx->vi);
+    hipLaunchKernelGGL(k_cgm_setup, tb, dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_cgm_iter<0, true>), tb, dim3(256), 0, ctx->stream, a);
+    HIPCHK(hipGetLastError());
+    const int CHUNK = 32;
+    int q = 1;  // launch index; launch q reads the set of parity q & 1
+    const int qmax = ctx->S.maxiter_coarse + 2;
+    while (true) {
+        for (int s = 0; s < CHUNK; ++s, ++q) {
+            ProfScope ps(ctx, WAVI_KT_CG_ITER_MIXED);
+            if (q & 1)
+                hipLaunchKernelGGL((k_cgm_iter<1, false>), tb, dim3(256), 0, ctx->stream, a);
+            else
+                hipLaunchKernelGGL((k_cgm_iter<0, false>), tb, dim3(256), 0, ctx->stream, a);
+        }
+        if (q & 1)
+            hipLaunchKernelGGL(k_cgm_resync<1>, tb, dim3(256), 0, ctx->stream, a);
+        else
+            hipLaunchKernelGGL(k_cgm_resync<0>, tb, dim3(256), 0, ctx->stream, a);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(ctx->h_stm, ctx->stm, sizeof(CGMScalars), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        prof_collect(ctx);
+        if (ctx->h_stm->done) break;
+        if (q > qmax + CHUNK) {
+            ctx->err = "coarse CG did not terminate";
+            return WAVI_ESTATE;
+        }
+    }
+    *iters = ctx->h_stm->iters;
     return WAVI_OK;
 }
 

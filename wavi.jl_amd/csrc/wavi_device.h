@@ -162,6 +162,31 @@ __device__ __forceinline__ double lanes_sum(const double *P, int np, double *lds
     return tree256(acc, lds);
 }
 
+// lanes_sum of two partial arrays of the same length in one pass.
+__device__ __forceinline__ void lanes_sum2(const double *P, const double *Q, int np, double *lds, double &a,
+                                           double &b)
+{
+    const int t = threadIdx.x;
+    double x = 0.0, y = 0.0;
+    for (int q0 = 0; q0 < np; q0 += 8 * 256) {
+        double v[8], w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int q = q0 + t + 256 * k;
+            v[k] = q < np ? P[q] : 0.0;
+            w[k] = q < np ? Q[q] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            x = x + v[k];
+            y = y + w[k];
+        }
+    }
+    tree256x2(x, y, lds);
+    a = x;
+    b = y;
+}
+
 // ---------------------------------------------------------------------------
 // The momentum operator op_fun! (src/utilities.jl:61-145), evaluated at one
 // u point or one v point from an input accessor.  Intermediates (r_xx, r_yy,

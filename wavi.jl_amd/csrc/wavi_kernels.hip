@@ -803,7 +803,7 @@ __device__ __forceinline__ void haar_pm_tile(int levels, int dir, T (*tile)[TILE
         const int m = min(nyw - (j0 + jb), B);
         const int r = ly - jb;
         tmp[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][r], c_haar[levels].neg[dir][m][r], &tile[jb][lx],
-                                     TILE + 1);
+                                     TILE + 1, B);
     }
     __syncthreads();
     if (i < nxw && j < nyw) {
@@ -1319,6 +1319,331 @@ __global__ void __launch_bounds__(256) k_cg_update(CGArgs<T> a, int par)
         if (has_u) a.Pr[bx + g.nbxu * by] = r2u;
         if (has_v) a.Pr[g.nbxu * g.nbyu + bx + g.nbxv * by] = r2v;
     }
+}
+
+// ===========================================================================
+// Mixed-precision coarse CG: the Chronopoulos-Gear form of CG (one combined
+// reduction per iteration) on FP32 vectors with FP64 reductions and scalars,
+// one kernel per iteration.  In exact arithmetic it produces the iterates of
+// the CG above; it computes
+//     p = r + beta p,  s = w + beta s,  x += alpha p,  r -= alpha s,
+//     w = A_c r,  gamma = (r, r),  delta = (w, r),
+//     beta = gamma / gamma_old,  alpha = gamma / (delta - beta gamma / alpha_old).
+// w = R A P r needs P r on a one-point halo around the tile.  The fine-grid
+// images T_r = P r, T_s = P s and T_w = P w are carried by the recurrences of
+// r and s (T_s = T_w + beta T_s, T_r = T_r - alpha T_s), so every halo value
+// is an elementwise function of buffers that the previous launch wrote.  T_w
+// is computed from w in every iteration; T_r and T_s are recomputed from r and
+// s every 32 iterations (k_cgm_resync) so that rounding in the recurrences
+// cannot accumulate.  The images, the partials and the scalars are
+// double-buffered: the launch of parity PAR reads set PAR and writes set
+// 1 - PAR, so no workgroup reads what another workgroup of the same launch
+// writes.
+
+struct CGMState {
+    double gamma;  // ||r||^2 at the start of the launch that wrote this state
+    double alpha;  // that launch's step length
+    double tol;    // stopping tolerance, fixed at iteration 0
+    int it;        // iterations done
+    int done;      // stop decided
+};
+
+struct CGMScalars {
+    CGMState s[2];  // by launch parity
+    int done;       // sticky stop flag, read by the host only
+    int iters;      // iterations at the stop
+};
+
+struct CGMArgs {
+    GridDims g;
+    OpCoef<float> c;
+    int levels;
+    const uint8_t *mwu, *mwv, *ui, *vi;
+    float *xu, *xv;          // solution (warm start on entry)
+    float *ru, *rv;          // residual (right-hand side on entry to setup)
+    float *pu, *pv;          // search direction p
+    float *su, *sv;          // s = A_c p
+    float *wu, *wv;          // w = A_c r
+    const float *t0u, *t0v;  // P x0 (setup only)
+    float *tru[2], *trv[2];  // T_r = P r by parity
+    float *tsu[2], *tsv[2];  // T_s = P s
+    float *twu[2], *twv[2];  // T_w = P w
+    double *Pg[2], *Pd[2];   // partials of (r, r) and (w, r) by parity
+    double *Pb;              // partials of ||b||^2
+    CGMScalars *st;
+    double tol_coarse, reltol;
+    int maxiter;
+    int nblk;
+};
+
+// Write the tile's own entries of a staged 18 x 18 window (entries at window
+// coordinates 1..16) to a fine-grid u/v vector pair.
+__device__ __forceinline__ void stage_write_own(const GridDims &g, const StageRegs<float> &r, int i0, int j0,
+                                                float *U, float *V)
+{
+    const int q0 = threadIdx.x, q1 = threadIdx.x + 256;
+    {
+        const int x = q0 % 18, y = q0 / 18, i = i0 - 1 + x, j = j0 - 1 + y;
+        if (x >= 1 && x <= 16 && y >= 1 && y <= 16) {
+            if (i < g.nxu && j < g.nyu) U[i + g.nxu * j] = r.u0;
+            if (i < g.nxv && j < g.nyv) V[i + g.nxv * j] = r.v0;
+        }
+    }
+    if (q1 < 324) {
+        const int x = q1 % 18, y = q1 / 18, i = i0 - 1 + x, j = j0 - 1 + y;
+        if (x >= 1 && x <= 16 && y >= 1 && y <= 16) {
+            if (i < g.nxu && j < g.nyu) U[i + g.nxu * j] = r.u1;
+            if (i < g.nxv && j < g.nyv) V[i + g.nxv * j] = r.v1;
+        }
+    }
+}
+
+// setup (P x0 computed beforehand): r0 = b - R A P x0 (b arrives in r),
+// p = s = 0, T_r[0] = P r0, partials of ||b||^2; resets the scalars.
+__global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
+{
+    __shared__ OpTile<float> s;
+    __shared__ float tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
+    __shared__ double lds[8];
+    const GridDims &g = a.g;
+    const int L = xcd_tile(a.nblk);
+    if (L >= g.nbx * g.nby) return;
+    if (L == 0 && threadIdx.x == 0) {
+        for (int p = 0; p < 2; ++p) {
+            a.st->s[p].gamma = 0.0;
+            a.st->s[p].alpha = 0.0;
+            a.st->s[p].tol = 0.0;
+            a.st->s[p].it = 0;
+            a.st->s[p].done = 0;
+        }
+        a.st->done = 0;
+        a.st->iters = 0;
+    }
+    const int bx = L % g.nbx, by = L / g.nbx;
+    const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
+    const int i0 = bx * TILE, j0 = by * TILE;
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    PointCoef<float> pc;
+    point_prefetch(g, a.c, a.mwu, a.mwv, i0 + lx, j0 + ly, has_u, has_v, pc);
+    const float bu = pc.oku ? a.ru[pc.ku] : 0.f, bv = pc.okv ? a.rv[pc.kv] : 0.f;
+    optile_apply(g, a.c, a.t0u, a.t0v, i0, j0, pc, s, tu, tv);
+    haar_pm_tile2<float>(a.levels, 1, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
+    float ru = 0.f, rv = 0.f;
+    double b2u = 0.0, b2v = 0.0;
+    if (pc.oku) {
+        ru = bu - (pc.mwu ? tu[ly][lx] : 0.f);
+        a.ru[pc.ku] = ru;
+        a.pu[pc.ku] = 0.f;
+        a.su[pc.ku] = 0.f;
+        b2u = (double)bu * (double)bu;
+    }
+    if (pc.okv) {
+        rv = bv - (pc.mwv ? tv[ly][lx] : 0.f);
+        a.rv[pc.kv] = rv;
+        a.pv[pc.kv] = 0.f;
+        a.sv[pc.kv] = 0.f;
+        b2v = (double)bv * (double)bv;
+    }
+    // each thread rewrites only its own entry; the transform ended with a barrier
+    tu[ly][lx] = pc.mwu ? ru : 0.f;
+    tv[ly][lx] = pc.mwv ? rv : 0.f;
+    __syncthreads();
+    haar_pm_tile2<float>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
+    if (pc.oku) a.tru[0][pc.ku] = pc.ui ? tu[ly][lx] : 0.f;
+    if (pc.okv) a.trv[0][pc.kv] = pc.vi ? tv[ly][lx] : 0.f;
+    tree256x2(b2u, b2v, lds);
+    if (threadIdx.x == 0) {
+        if (has_u) a.Pb[bx + g.nbxu * by] = b2u;
+        if (has_v) a.Pb[g.nbxu * g.nbyu + bx + g.nbxv * by] = b2v;
+    }
+}
+
+// One iteration (INIT: the launch before the first iteration, which only
+// computes w0 = A_c r0, T_w = P w0 and the partials, and copies T_r[0] into
+// set 1 with T_s = 0).
+template <int PAR, bool INIT>
+__global__ void __launch_bounds__(256) k_cgm_iter(CGMArgs a)
+{
+    static_assert(!INIT || PAR == 0, "the initial launch reads set 0");
+    __shared__ OpTile<float> s;
+    __shared__ float tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
+    __shared__ double lds[8];
+    const GridDims &g = a.g;
+    const int L = xcd_tile(a.nblk);
+    if (L >= g.nbx * g.nby) return;
+    const int bx = L % g.nbx, by = L / g.nbx;
+    const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
+    const int i0 = bx * TILE, j0 = by * TILE;
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    constexpr int NP = 1 - PAR;
+    // loads that do not depend on the scalars, all issued before the reduction
+    const CGMState S = a.st->s[PAR];
+    PointCoef<float> pc;
+    point_prefetch(g, a.c, a.mwu, a.mwv, i0 + lx, j0 + ly, has_u, has_v, pc);
+    const int ku = pc.ku, kv = pc.kv;
+    float r_u = 0.f, w_u = 0.f, p_u = 0.f, s_u = 0.f, x_u = 0.f;
+    float r_v = 0.f, w_v = 0.f, p_v = 0.f, s_v = 0.f, x_v = 0.f;
+    if (pc.oku) {
+        r_u = a.ru[ku];
+        if (!INIT) {
+            w_u = a.wu[ku];
+            p_u = a.pu[ku];
+            s_u = a.su[ku];
+            x_u = a.xu[ku];
+        }
+    }
+    if (pc.okv) {
+        r_v = a.rv[kv];
+        if (!INIT) {
+            w_v = a.wv[kv];
+            p_v = a.pv[kv];
+            s_v = a.sv[kv];
+            x_v = a.xv[kv];
+        }
+    }
+    StageRegs<float> tr, ts, tw;
+    optile_fetch(g, a.tru[PAR], a.trv[PAR], i0, j0, tr);
+    if (!INIT) {
+        optile_fetch(g, a.tsu[PAR], a.tsv[PAR], i0, j0, ts);
+        optile_fetch(g, a.twu[PAR], a.twv[PAR], i0, j0, tw);
+    } else {
+        ts.u0 = ts.v0 = ts.u1 = ts.v1 = 0.f;
+    }
+    CellCoef<float> c0, c1;
+    cell_prefetch(g, a.c, i0, j0, threadIdx.x, c0);
+    cell_prefetch(g, a.c, i0, j0, threadIdx.x + 256, c1);
+    if (!INIT) {
+        double gamma, delta;
+        lanes_sum2(a.Pg[PAR], a.Pd[PAR], g.npart, lds, gamma, delta);
+        if (S.done) {
+            if (L == 0 && threadIdx.x == 0) a.st->s[NP] = S;
+            return;
+        }
+        const double res = sqrt(gamma);
+        double tol = S.tol;
+        if (S.it == 0) {
+            const double abstol = a.tol_coarse * sqrt(lanes_sum(a.Pb, g.npart, lds));
+            tol = a.reltol * res;
+            if (abstol > tol) tol = abstol;
+        }
+        if (S.it >= a.maxiter || res <= tol) {
+            if (L == 0 && threadIdx.x == 0) {
+                CGMState n = S;
+                n.tol = tol;
+                n.done = 1;
+                a.st->s[NP] = n;
+                a.st->done = 1;
+                a.st->iters = S.it;
+            }
+            return;
+        }
+        const double beta = S.it == 0 ? 0.0 : gamma / S.gamma;
+        const double alpha = S.it == 0 ? gamma / delta : gamma / (delta - beta * gamma / S.alpha);
+        if (L == 0 && threadIdx.x == 0) {
+            CGMState n;
+            n.gamma = gamma;
+            n.alpha = alpha;
+            n.tol = tol;
+            n.it = S.it + 1;
+            n.done = 0;
+            a.st->s[NP] = n;
+        }
+        const float fa = (float)alpha, fb = (float)beta;
+        if (pc.oku) {
+            p_u = r_u + fb * p_u;
+            s_u = w_u + fb * s_u;
+            a.pu[ku] = p_u;
+            a.su[ku] = s_u;
+            a.xu[ku] = x_u + fa * p_u;
+            r_u = r_u - fa * s_u;
+            a.ru[ku] = r_u;
+        }
+        if (pc.okv) {
+            p_v = r_v + fb * p_v;
+            s_v = w_v + fb * s_v;
+            a.pv[kv] = p_v;
+            a.sv[kv] = s_v;
+            a.xv[kv] = x_v + fa * p_v;
+            r_v = r_v - fa * s_v;
+            a.rv[kv] = r_v;
+        }
+        ts.u0 = tw.u0 + fb * ts.u0;
+        ts.v0 = tw.v0 + fb * ts.v0;
+        ts.u1 = tw.u1 + fb * ts.u1;
+        ts.v1 = tw.v1 + fb * ts.v1;
+        tr.u0 = tr.u0 - fa * ts.u0;
+        tr.v0 = tr.v0 - fa * ts.v0;
+        tr.u1 = tr.u1 - fa * ts.u1;
+        tr.v1 = tr.v1 - fa * ts.v1;
+    }
+    optile_store(tr, s);
+    stage_write_own(g, tr, i0, j0, a.tru[NP], a.trv[NP]);
+    stage_write_own(g, ts, i0, j0, a.tsu[NP], a.tsv[NP]);
+    __syncthreads();
+    cell_compute(a.c, c0, threadIdx.x, s);
+    cell_compute(a.c, c1, threadIdx.x + 256, s);
+    __syncthreads();
+    tu[ly][lx] = (pc.oku && pc.ui) ? optile_u(a.c, s, pc, lx, ly) : 0.f;
+    tv[ly][lx] = (pc.okv && pc.vi) ? optile_v(a.c, s, pc, lx, ly) : 0.f;
+    __syncthreads();
+    haar_pm_tile2<float>(a.levels, 1, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
+    const float wn_u = pc.mwu ? tu[ly][lx] : 0.f, wn_v = pc.mwv ? tv[ly][lx] : 0.f;
+    if (pc.oku) a.wu[ku] = wn_u;
+    if (pc.okv) a.wv[kv] = wn_v;
+    tu[ly][lx] = wn_u;
+    tv[ly][lx] = wn_v;
+    __syncthreads();
+    haar_pm_tile2<float>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
+    if (pc.oku) a.twu[NP][ku] = pc.ui ? tu[ly][lx] : 0.f;
+    if (pc.okv) a.twv[NP][kv] = pc.vi ? tv[ly][lx] : 0.f;
+    double gu = (double)r_u * (double)r_u, gv = (double)r_v * (double)r_v;
+    double du = (double)wn_u * (double)r_u, dv = (double)wn_v * (double)r_v;
+    tree256x2(gu, gv, lds);
+    tree256x2(du, dv, lds);
+    if (threadIdx.x == 0) {
+        if (has_u) {
+            a.Pg[NP][bx + g.nbxu * by] = gu;
+            a.Pd[NP][bx + g.nbxu * by] = du;
+        }
+        if (has_v) {
+            const int o = g.nbxu * g.nbyu + bx + g.nbxv * by;
+            a.Pg[NP][o] = gv;
+            a.Pd[NP][o] = dv;
+        }
+    }
+}
+
+// T_r = P r and T_s = P s on the tile, into the set the next launch reads.
+template <int PAR>
+__global__ void __launch_bounds__(256) k_cgm_resync(CGMArgs a)
+{
+    __shared__ float tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
+    const GridDims &g = a.g;
+    const int L = xcd_tile(a.nblk);
+    if (L >= g.nbx * g.nby) return;
+    const int bx = L % g.nbx, by = L / g.nbx;
+    const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
+    const int i0 = bx * TILE, j0 = by * TILE;
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int i = i0 + lx, j = j0 + ly;
+    const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
+    const int ku = i + g.nxu * j, kv = i + g.nxv * j;
+    const bool mwu = oku && a.mwu[ku], mwv = okv && a.mwv[kv];
+    const bool iu = oku && a.ui[ku], iv = okv && a.vi[kv];
+    const float ru = mwu ? a.ru[ku] : 0.f, rv = mwv ? a.rv[kv] : 0.f;
+    const float su = mwu ? a.su[ku] : 0.f, sv = mwv ? a.sv[kv] : 0.f;
+    tu[ly][lx] = ru;
+    tv[ly][lx] = rv;
+    __syncthreads();
+    haar_pm_tile2<float>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
+    if (oku) a.tru[PAR][ku] = iu ? tu[ly][lx] : 0.f;
+    if (okv) a.trv[PAR][kv] = iv ? tv[ly][lx] : 0.f;
+    tu[ly][lx] = su;
+    tv[ly][lx] = sv;
+    __syncthreads();
+    haar_pm_tile2<float>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
+    if (oku) a.tsu[PAR][ku] = iu ? tu[ly][lx] : 0.f;
+    if (okv) a.tsv[PAR][kv] = iv ? tv[ly][lx] : 0.f;
 }
 
 // out = mask_out ? H(mask_in ? in : 0) : 0 on the u grid (comp 0) and v grid
