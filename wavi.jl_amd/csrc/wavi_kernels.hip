@@ -803,7 +803,7 @@ __device__ __forceinline__ void haar_pm_tile(int levels, int dir, T (*tile)[TILE
         const int m = min(nyw - (j0 + jb), B);
         const int r = ly - jb;
         tmp[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][r], c_haar[levels].neg[dir][m][r], &tile[jb][lx],
-                                     TILE + 1, B);
+                                     TILE + 1);
     }
     __syncthreads();
     if (i < nxw && j < nyw) {
