@@ -267,3 +267,38 @@ def test_iceberg_analytic(W, oracle):
     assert relerr_h < 1e-4
     assert relerr_u < 3e-4
     assert relerr_v < 3e-4
+
+
+def _butterfly(x, levels, transposed):
+    """The lane butterflies of the mixed-precision kernels (haar_line_P /
+    haar_line_R) on a zero-padded 16-point line: pairs (o, o + h), o a
+    multiple of 2h; P runs h = B/2 .. 1, R runs h = 1 .. B/2."""
+    v = np.zeros(16)
+    v[:len(x)] = x
+    B = 2 << levels
+    hs = [1 << k for k in range(levels + 1)]
+    for h in (hs if transposed else hs[::-1]):
+        y = v.copy()
+        for pos in range(16):
+            if pos % h:
+                continue
+            q = v[pos ^ h]
+            if transposed:
+                y[pos] = v[pos] - q if pos & h else v[pos] + q
+            else:
+                y[pos] = q + v[pos] if pos & h else v[pos] - q
+        v = y
+    return v[:len(x)]
+
+
+@pytest.mark.parametrize("levels", [0, 1, 2, 3])
+def test_butterfly_haar_matches_wavelet_matrix(oracle, levels):
+    """The butterfly form of P = idwt and R = idwt^T used by the FP32 coarse
+    CG equals wavelet_matrix's inverse and its transpose on every block
+    length, partial blocks at the grid end included (zero padding)."""
+    rng = np.random.default_rng(levels)
+    for n in range(1, 17):
+        Wi = oracle.wavelet_matrix(n, levels, False)
+        x = rng.integers(-8, 9, n).astype(float)  # small integers: sums are exact in any order
+        assert np.array_equal(_butterfly(x, levels, False), Wi @ x), n
+        assert np.array_equal(_butterfly(x, levels, True), Wi.T @ x), n

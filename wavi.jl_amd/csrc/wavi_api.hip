@@ -59,8 +59,14 @@ struct wavi_ctx {
     VSet<float> v32;
     // mixed-precision coarse CG: s, w, the parity-buffered images P r, P s,
     // P w and partials of (r, r) and (w, r)
+    // mixed coarse CG: one FP32 arena (enum CGMArray) and a mask-byte arena of
+    // stride ns; s, w, the rings and the packed coefficients are views into it
+    float *arena = nullptr;
+    uint8_t *marena = nullptr;
+    unsigned ns = 0;
     DevVecT<float> ms, mw, mtr[2], mts[2], mtw[2];
-    double *Pg[2] = {}, *Pdm[2] = {};
+    double2 *Pgd = nullptr;  // [2][tiles] (r, r) and (w, r) partials by parity
+    CGMPack pack{};
     CGMScalars *stm = nullptr;
     CGMScalars *h_stm = nullptr;  // pinned host copy
     double *Pr0 = nullptr, *Pr1 = nullptr, *Pd = nullptr, *Pb = nullptr, *Pn = nullptr;
@@ -466,16 +472,37 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
         CREATECHK(falloc(&w.heta_h, ctx->nh));
         CREATECHK(falloc(&w.heta_c, ctx->nc > 0 ? ctx->nc : 1));
         CREATECHK(falloc(&w.dimp, ctx->nh));
-        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cr, &w.cp, &w.cq, &w.ct, &w.xc,
-                                   &ctx->ms, &ctx->mw, &ctx->mtr[0], &ctx->mtr[1], &ctx->mts[0], &ctx->mts[1],
-                                   &ctx->mtw[0], &ctx->mtw[1]}) {
+        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cq, &w.ct}) {
             CREATECHK(falloc(&dv->u, ctx->nu));
             CREATECHK(falloc(&dv->v, ctx->nv));
         }
+        const unsigned ns = (unsigned)((std::max(ctx->nu, ctx->nv) + 63) / 64 * 64);
+        ctx->ns = ns;
+        CREATECHK(falloc(&ctx->arena, (size_t)CGM_NARRAY * ns));
+        CREATECHK(hipMalloc(&ctx->marena, 2 * (size_t)ns));
+        CREATECHK(hipMemset(ctx->marena, 0, 2 * (size_t)ns));
+        float *F = ctx->arena;
+        auto view = [&](int id) { return DevVecT<float>{F + (size_t)id * ns, F + (size_t)(id + 1) * ns}; };
+        w.cr = view(CGM_R);
+        w.cp = view(CGM_P);
+        w.xc = view(CGM_X);
+        ctx->ms = view(CGM_S);
+        ctx->mw = view(CGM_W);
         for (int p = 0; p < 2; ++p) {
-            CREATECHK(zalloc(&ctx->Pg[p], g.npart));
-            CREATECHK(zalloc(&ctx->Pdm[p], g.npart));
+            ctx->mtr[p] = view(CGM_RING + 6 * p);
+            ctx->mts[p] = view(CGM_RING + 6 * p + 2);
+            ctx->mtw[p] = view(CGM_RING + 6 * p + 4);
         }
+        ctx->pack.hu = F + (size_t)CGM_HU * ns;
+        ctx->pack.hv = F + (size_t)(CGM_HU + 1) * ns;
+        ctx->pack.bu = F + (size_t)CGM_BU * ns;
+        ctx->pack.bv = F + (size_t)(CGM_BU + 1) * ns;
+        ctx->pack.mu = ctx->marena;
+        ctx->pack.mv = ctx->marena + ns;
+        CREATECHK(hipMalloc(&ctx->Pgd, 2 * (size_t)g.nbx * g.nby * sizeof(double2)));
+        CREATECHK(hipMemset(ctx->Pgd, 0, 2 * (size_t)g.nbx * g.nby * sizeof(double2)));
+        ctx->pack.cell = reinterpret_cast<float2 *>(F + (size_t)CGM_CELL * ns);
+        ctx->pack.corner = F + (size_t)CGM_CORNER * ns;
         CREATECHK(hipMalloc(&ctx->stm, sizeof(CGMScalars)));
         CREATECHK(hipMemset(ctx->stm, 0, sizeof(CGMScalars)));
         CREATECHK(hipHostMalloc((void **)&ctx->h_stm, sizeof(CGMScalars), hipHostMallocDefault));
@@ -517,17 +544,13 @@ int wavi_ctx_destroy(wavi_ctx *ctx)
         VSet<float> &w = ctx->v32;
         for (float *p : {w.heta_h, w.heta_c, w.dimp})
             if (p) (void)hipFree(p);
-        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cr, &w.cp, &w.cq, &w.ct, &w.xc,
-                                   &ctx->ms, &ctx->mw, &ctx->mtr[0], &ctx->mtr[1], &ctx->mts[0], &ctx->mts[1],
-                                   &ctx->mtw[0], &ctx->mtw[1]}) {
+        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cq, &w.ct}) {
             if (dv->u) (void)hipFree(dv->u);
             if (dv->v) (void)hipFree(dv->v);
         }
-        for (int p = 0; p < 2; ++p) {
-            if (ctx->Pg[p]) (void)hipFree(ctx->Pg[p]);
-            if (ctx->Pdm[p]) (void)hipFree(ctx->Pdm[p]);
-        }
     }
+    for (void *p : {(void *)ctx->arena, (void *)ctx->marena, (void *)ctx->Pgd})
+        if (p) (void)hipFree(p);
     if (ctx->stm) (void)hipFree(ctx->stm);
     if (ctx->h_stm) (void)hipHostFree(ctx->h_stm);
     if (ctx->st) (void)hipFree(ctx->st);
@@ -928,8 +951,7 @@ static CGMArgs cgm_args(wavi_ctx *ctx)
     a.levels = ctx->S.levels;
     a.mwu = ctx->mwu;
     a.mwv = ctx->mwv;
-    a.ui = ctx->ui;
-    a.vi = ctx->vi;
+    a.k = ctx->pack;
     a.xu = w.xc.u;
     a.xv = w.xc.v;
     a.ru = w.cr.u;
@@ -943,14 +965,12 @@ static CGMArgs cgm_args(wavi_ctx *ctx)
     a.t0u = w.ct.u;
     a.t0v = w.ct.v;
     for (int p = 0; p < 2; ++p) {
-        a.tru[p] = ctx->mtr[p].u;
-        a.trv[p] = ctx->mtr[p].v;
-        a.tsu[p] = ctx->mts[p].u;
-        a.tsv[p] = ctx->mts[p].v;
-        a.twu[p] = ctx->mtw[p].u;
-        a.twv[p] = ctx->mtw[p].v;
-        a.Pg[p] = ctx->Pg[p];
-        a.Pd[p] = ctx->Pdm[p];
+        a.hru[p] = ctx->mtr[p].u;
+        a.hrv[p] = ctx->mtr[p].v;
+        a.hsu[p] = ctx->mts[p].u;
+        a.hsv[p] = ctx->mts[p].v;
+        a.hwu[p] = ctx->mtw[p].u;
+        a.hwv[p] = ctx->mtw[p].v;
     }
     a.Pb = ctx->Pb;
     a.st = ctx->stm;
@@ -961,20 +981,52 @@ static CGMArgs cgm_args(wavi_ctx *ctx)
     return a;
 }
 
+static CGMIterArgs cgm_iter_args(wavi_ctx *ctx)
+{
+    CGMIterArgs a;
+    a.F = ctx->arena;
+    a.M = ctx->marena;
+    a.Pgd = ctx->Pgd;
+    a.st = ctx->stm;
+    a.Pb = ctx->Pb;
+    a.g = ctx->g;
+    a.ns = ctx->ns;
+    a.rdx = (float)ctx->rdx;
+    a.rdy = (float)ctx->rdy;
+    a.tol_coarse = ctx->S.tol_coarse;
+    a.reltol = std::sqrt(2.220446049250313e-16);  // IterativeSolvers' default sqrt(eps)
+    a.maxiter = ctx->S.maxiter_coarse;
+    a.nblk = ctx->nblk;
+    return a;
+}
+
+template <int PAR, bool INIT>
+static void launch_cgm_iter(wavi_ctx *ctx, dim3 tb, const CGMIterArgs &a)
+{
+    switch (ctx->S.levels) {
+    case 0: hipLaunchKernelGGL((k_cgm_iter<PAR, INIT, 0>), tb, dim3(256), 0, ctx->stream, a); break;
+    case 1: hipLaunchKernelGGL((k_cgm_iter<PAR, INIT, 1>), tb, dim3(256), 0, ctx->stream, a); break;
+    case 2: hipLaunchKernelGGL((k_cgm_iter<PAR, INIT, 2>), tb, dim3(256), 0, ctx->stream, a); break;
+    default: hipLaunchKernelGGL((k_cgm_iter<PAR, INIT, 3>), tb, dim3(256), 0, ctx->stream, a); break;
+    }
+}
+
 // Mixed-precision coarse CG (Chronopoulos-Gear form, one launch per
-// iteration) on b_c held in v32.cr; solution in the warm start v32.xc.  The
-// images P r and P s are recomputed after every chunk of 32 launches.
+// iteration) on b_c held in v32.cr; solution in the warm start v32.xc.
 template <>
 int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
 {
     const GridDims &g = ctx->g;
     dim3 tg(g.nbx, g.nby), tb(ctx->nblk);
     CGMArgs a = cgm_args(ctx);
+    hipLaunchKernelGGL(k_cgm_pack, flat(std::max(ctx->nh, std::max(ctx->nu, ctx->nv))), dim3(256), 0, ctx->stream, g,
+                       a.c, ctx->mwu, ctx->mwv, ctx->pack);
     // P x0
     hipLaunchKernelGGL((k_tile_haar<float, float>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1,
                        a.xu, a.xv, ctx->mwu, ctx->mwv, w.ct.u, w.ct.v, ctx->ui, ctx->vi);
     hipLaunchKernelGGL(k_cgm_setup, tb, dim3(256), 0, ctx->stream, a);
-    hipLaunchKernelGGL((k_cgm_iter<0, true>), tb, dim3(256), 0, ctx->stream, a);
+    const CGMIterArgs ia = cgm_iter_args(ctx);
+    launch_cgm_iter<0, true>(ctx, tb, ia);
     HIPCHK(hipGetLastError());
     const int CHUNK = 32;
     int q = 1;  // launch index; launch q reads the set of parity q & 1
@@ -983,14 +1035,10 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
         for (int s = 0; s < CHUNK; ++s, ++q) {
             ProfScope ps(ctx, WAVI_KT_CG_ITER_MIXED);
             if (q & 1)
-                hipLaunchKernelGGL((k_cgm_iter<1, false>), tb, dim3(256), 0, ctx->stream, a);
+                launch_cgm_iter<1, false>(ctx, tb, ia);
             else
-                hipLaunchKernelGGL((k_cgm_iter<0, false>), tb, dim3(256), 0, ctx->stream, a);
+                launch_cgm_iter<0, false>(ctx, tb, ia);
         }
-        if (q & 1)
-            hipLaunchKernelGGL(k_cgm_resync<1>, tb, dim3(256), 0, ctx->stream, a);
-        else
-            hipLaunchKernelGGL(k_cgm_resync<0>, tb, dim3(256), 0, ctx->stream, a);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(ctx->h_stm, ctx->stm, sizeof(CGMScalars), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));

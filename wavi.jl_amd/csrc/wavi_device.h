@@ -188,6 +188,77 @@ __device__ __forceinline__ void lanes_sum2(const double *P, const double *Q, int
 }
 
 // ---------------------------------------------------------------------------
+// Lane exchange within the 16-lane rows of a wavefront by DPP (every lane of
+// the wavefront must be active).
+constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1,0,3,2]: lane ^ 1
+constexpr int DPP_XOR2 = 0x4E;         // quad_perm [2,3,0,1]: lane ^ 2
+constexpr int DPP_QUAD_BCAST0 = 0x00;  // quad_perm [0,0,0,0]: lane & ~3
+constexpr int DPP_MIRROR = 0x140;      // row_mirror: lane ^ 15
+constexpr int DPP_HALF_MIRROR = 0x141; // row_half_mirror: lane ^ 7
+constexpr int DPP_ROR8 = 0x128;        // row_ror 8: lane ^ 8
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Value of the lane at row position pos ^ H (H = 1, 2, 4, 8); for H = 4 it is
+// valid in the lanes whose position is a multiple of 4.
+template <int H>
+__device__ __forceinline__ float row_partner(float v)
+{
+    if constexpr (H == 1) return dpp_f<DPP_XOR1>(v);
+    else if constexpr (H == 2) return dpp_f<DPP_XOR2>(v);
+    else if constexpr (H == 4) return dpp_f<DPP_HALF_MIRROR>(dpp_f<DPP_QUAD_BCAST0>(v));
+    else return dpp_f<DPP_ROR8>(v);
+}
+
+// Sum over a 16-lane row, left in every lane of the row.  Each stage adds a
+// lane and its partner, so both get the same bits and the result does not
+// depend on the lane.
+__device__ __forceinline__ double row_sum16(double v)
+{
+    v = v + dpp_d<DPP_XOR1>(v);
+    v = v + dpp_d<DPP_XOR2>(v);
+    v = v + dpp_d<DPP_HALF_MIRROR>(v);
+    v = v + dpp_d<DPP_MIRROR>(v);
+    return v;
+}
+
+// Workgroup sums of a and b (256 threads): row sums by DPP, then the 16 row
+// sums through LDS (red holds 32 doubles and must not be written again
+// before the next barrier).  Every thread gets the same totals.
+__device__ __forceinline__ void wg_sum2(double &a, double &b, double *red)
+{
+    a = row_sum16(a);
+    b = row_sum16(b);
+    const int row = threadIdx.x >> 4;
+    if ((threadIdx.x & 15) == 0) {
+        red[row] = a;
+        red[16 + row] = b;
+    }
+    __syncthreads();
+    double x[4], y[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        x[k] = (red[4 * k] + red[4 * k + 1]) + (red[4 * k + 2] + red[4 * k + 3]);
+        y[k] = (red[16 + 4 * k] + red[16 + 4 * k + 1]) + (red[16 + 4 * k + 2] + red[16 + 4 * k + 3]);
+    }
+    a = (x[0] + x[1]) + (x[2] + x[3]);
+    b = (y[0] + y[1]) + (y[2] + y[3]);
+}
+
+// ---------------------------------------------------------------------------
 // The momentum operator op_fun! (src/utilities.jl:61-145), evaluated at one
 // u point or one v point from an input accessor.  Intermediates (r_xx, r_yy,
 // r_xy, extra) are recomputed by each output that needs them with the same

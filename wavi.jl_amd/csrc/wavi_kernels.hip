@@ -1324,28 +1324,28 @@ __global__ void __launch_bounds__(256) k_cg_update(CGArgs<T> a, int par)
 // ===========================================================================
 // Mixed-precision coarse CG: the Chronopoulos-Gear form of CG (one combined
 // reduction per iteration) on FP32 vectors with FP64 reductions and scalars,
-// one kernel per iteration.  In exact arithmetic it produces the iterates of
-// the CG above; it computes
+// one launch per iteration.  In exact arithmetic it produces the iterates of
+// the CG above; launch k computes
 //     p = r + beta p,  s = w + beta s,  x += alpha p,  r -= alpha s,
 //     w = A_c r,  gamma = (r, r),  delta = (w, r),
 //     beta = gamma / gamma_old,  alpha = gamma / (delta - beta gamma / alpha_old).
-// w = R A P r needs P r on a one-point halo around the tile.  The fine-grid
-// images T_r = P r, T_s = P s and T_w = P w are carried by the recurrences of
-// r and s (T_s = T_w + beta T_s, T_r = T_r - alpha T_s), so every halo value
-// is an elementwise function of buffers that the previous launch wrote.  T_w
-// is computed from w in every iteration; T_r and T_s are recomputed from r and
-// s every 32 iterations (k_cgm_resync) so that rounding in the recurrences
-// cannot accumulate.  The images, the partials and the scalars are
+// w = R A P r needs P r on the tile and on a one-point halo around it.  The
+// tile's own part is the block transform of its own r (the Haar blocks never
+// cross a tile).  The halo belongs to the neighbours' rings (their outermost
+// rows and columns), where every workgroup stores the exact images P r, P w
+// and P s of its own vectors at the end of each launch; the next launch forms
+// the halo of P r_new = P r - alpha (P w + beta P s) from them, one linear
+// step from exact values.  The rings, the partials and the scalars are
 // double-buffered: the launch of parity PAR reads set PAR and writes set
 // 1 - PAR, so no workgroup reads what another workgroup of the same launch
 // writes.
 
 struct CGMState {
-    double gamma;  // ||r||^2 at the start of the launch that wrote this state
-    double alpha;  // that launch's step length
-    double tol;    // stopping tolerance, fixed at iteration 0
-    int it;        // iterations done
-    int done;      // stop decided
+    double igamma;  // 1 / ||r||^2 at the start of the launch that wrote this state
+    double ialpha;  // 1 / that launch's step length
+    double tol2;    // square of the stopping tolerance, fixed at iteration 0
+    int it;         // iterations done
+    int done;       // stop decided
 };
 
 struct CGMScalars {
@@ -1354,52 +1354,67 @@ struct CGMScalars {
     int iters;      // iterations at the stop
 };
 
+// FP32 operator coefficients with the masks folded in (built by k_cgm_pack
+// once per V-cycle).
+struct CGMPack {
+    float2 *cell;       // per h cell: h*eta_av (0 off the h mask), dimp
+    float *corner;      // per h cell: corner h*eta_av (0 off the c grid / mask)
+    float *hu, *hv;     // thickness on u / v points, 0 off the u / v masks
+    float *bu, *bv;     // beta_eff on u / v points, 0 off the u / v masks
+    uint8_t *mu, *mv;   // bit 0: inner point, bit 1: wavelet mask
+};
+
 struct CGMArgs {
     GridDims g;
-    OpCoef<float> c;
+    OpCoef<float> c;         // setup only
+    CGMPack k;
     int levels;
-    const uint8_t *mwu, *mwv, *ui, *vi;
+    const uint8_t *mwu, *mwv;
     float *xu, *xv;          // solution (warm start on entry)
     float *ru, *rv;          // residual (right-hand side on entry to setup)
     float *pu, *pv;          // search direction p
     float *su, *sv;          // s = A_c p
     float *wu, *wv;          // w = A_c r
     const float *t0u, *t0v;  // P x0 (setup only)
-    float *tru[2], *trv[2];  // T_r = P r by parity
-    float *tsu[2], *tsv[2];  // T_s = P s
-    float *twu[2], *twv[2];  // T_w = P w
-    double *Pg[2], *Pd[2];   // partials of (r, r) and (w, r) by parity
-    double *Pb;              // partials of ||b||^2
+    float *hru[2], *hrv[2];  // rings of P r by parity (setup writes whole tiles of set 0)
+    float *hsu[2], *hsv[2];  // rings of P s
+    float *hwu[2], *hwv[2];  // rings of P w
+    double *Pb;              // partials of ||b||^2 (per u / v block)
     CGMScalars *st;
     double tol_coarse, reltol;
     int maxiter;
     int nblk;
 };
 
-// Write the tile's own entries of a staged 18 x 18 window (entries at window
-// coordinates 1..16) to a fine-grid u/v vector pair.
-__device__ __forceinline__ void stage_write_own(const GridDims &g, const StageRegs<float> &r, int i0, int j0,
-                                                float *U, float *V)
+__global__ void k_cgm_pack(GridDims g, OpCoef<float> c, const uint8_t *mwu, const uint8_t *mwv, CGMPack p)
 {
-    const int q0 = threadIdx.x, q1 = threadIdx.x + 256;
-    {
-        const int x = q0 % 18, y = q0 / 18, i = i0 - 1 + x, j = j0 - 1 + y;
-        if (x >= 1 && x <= 16 && y >= 1 && y <= 16) {
-            if (i < g.nxu && j < g.nyu) U[i + g.nxu * j] = r.u0;
-            if (i < g.nxv && j < g.nyv) V[i + g.nxv * j] = r.v0;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nh = g.nx * g.ny, nu = g.nxu * g.nyu, nv = g.nxv * g.nyv;
+    if (k < nh) {
+        const int a = k % g.nx, b = k / g.nx;
+        float hc = 0.f;
+        if (a < g.nx - 1 && b < g.ny - 1) {
+            const int kc = a + (g.nx - 1) * b;
+            hc = c.cm[kc] ? c.heta_c[kc] : 0.f;
         }
+        p.cell[k] = make_float2(c.hm[k] ? c.heta_h[k] : 0.f, c.dimp[k]);
+        p.corner[k] = hc;
     }
-    if (q1 < 324) {
-        const int x = q1 % 18, y = q1 / 18, i = i0 - 1 + x, j = j0 - 1 + y;
-        if (x >= 1 && x <= 16 && y >= 1 && y <= 16) {
-            if (i < g.nxu && j < g.nyu) U[i + g.nxu * j] = r.u1;
-            if (i < g.nxv && j < g.nyv) V[i + g.nxv * j] = r.v1;
-        }
+    if (k < nu) {
+        p.hu[k] = c.um[k] ? c.hu[k] : 0.f;
+        p.bu[k] = c.um[k] ? c.bu[k] : 0.f;
+        p.mu[k] = (uint8_t)((c.ui[k] ? 1 : 0) | (mwu[k] ? 2 : 0));
+    }
+    if (k < nv) {
+        p.hv[k] = c.vm[k] ? c.hv[k] : 0.f;
+        p.bv[k] = c.vm[k] ? c.bv[k] : 0.f;
+        p.mv[k] = (uint8_t)((c.vi[k] ? 1 : 0) | (mwv[k] ? 2 : 0));
     }
 }
 
 // setup (P x0 computed beforehand): r0 = b - R A P x0 (b arrives in r),
-// p = s = 0, T_r[0] = P r0, partials of ||b||^2; resets the scalars.
+// p = s = 0, P r0 into set 0 of the P r rings (whole tile), partials of
+// ||b||^2; resets the scalars.
 __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
 {
     __shared__ OpTile<float> s;
@@ -1410,9 +1425,9 @@ __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
     if (L >= g.nbx * g.nby) return;
     if (L == 0 && threadIdx.x == 0) {
         for (int p = 0; p < 2; ++p) {
-            a.st->s[p].gamma = 0.0;
-            a.st->s[p].alpha = 0.0;
-            a.st->s[p].tol = 0.0;
+            a.st->s[p].igamma = 0.0;
+            a.st->s[p].ialpha = 0.0;
+            a.st->s[p].tol2 = 0.0;
             a.st->s[p].it = 0;
             a.st->s[p].done = 0;
         }
@@ -1449,8 +1464,8 @@ __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
     tv[ly][lx] = pc.mwv ? rv : 0.f;
     __syncthreads();
     haar_pm_tile2<float>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
-    if (pc.oku) a.tru[0][pc.ku] = pc.ui ? tu[ly][lx] : 0.f;
-    if (pc.okv) a.trv[0][pc.kv] = pc.vi ? tv[ly][lx] : 0.f;
+    if (pc.oku) a.hru[0][pc.ku] = pc.ui ? tu[ly][lx] : 0.f;
+    if (pc.okv) a.hrv[0][pc.kv] = pc.vi ? tv[ly][lx] : 0.f;
     tree256x2(b2u, b2v, lds);
     if (threadIdx.x == 0) {
         if (has_u) a.Pb[bx + g.nbxu * by] = b2u;
@@ -1458,78 +1473,237 @@ __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
     }
 }
 
+// Block Haar transforms of 16-point lines held one point per lane (pos = the
+// point's position in the line, blocks of B = 2^(levels+1) points) as the
+// butterflies that compose wavelet_matrix (src/Wavelets/update_wavelets.jl:
+// 38-89).  A pair is (o, e = o + h) with o a multiple of 2h.
+//   P (inverse), h = B/2 .. 1:           y_o = x_o - x_e,  y_e = x_o + x_e
+//   R (transposed inverse), h = 1 .. B/2: y_o = x_o + x_e,  y_e = x_e - x_o
+// Points past the end of the grid must hold zero: a pair whose e lies past
+// the end then leaves x_o unchanged, as wavelet_matrix does for a partial
+// block, and the garbage left in those points never reaches a point in range.
+// One stage as v + sign * partner with sign in {-1, 0, +1}: the product is
+// exact, so the fused multiply-add rounds exactly as the addition would.
+template <int H>
+__device__ __forceinline__ float haar_stage_P(float v, int pos)
+{
+    const float sg = ((pos & (H - 1)) == 0) ? ((pos & H) ? 1.f : -1.f) : 0.f;
+    return __builtin_fmaf(sg, row_partner<H>(v), v);
+}
+
+template <int H>
+__device__ __forceinline__ float haar_stage_R(float v, int pos)
+{
+    const float sg = ((pos & (H - 1)) == 0) ? ((pos & H) ? -1.f : 1.f) : 0.f;
+    return __builtin_fmaf(sg, row_partner<H>(v), v);
+}
+
+template <int LEV>
+__device__ __forceinline__ float haar_line_P(float v, int pos)
+{
+    if constexpr (LEV >= 3) v = haar_stage_P<8>(v, pos);
+    if constexpr (LEV >= 2) v = haar_stage_P<4>(v, pos);
+    if constexpr (LEV >= 1) v = haar_stage_P<2>(v, pos);
+    return haar_stage_P<1>(v, pos);
+}
+
+template <int LEV>
+__device__ __forceinline__ float haar_line_R(float v, int pos)
+{
+    v = haar_stage_R<1>(v, pos);
+    if constexpr (LEV >= 1) v = haar_stage_R<2>(v, pos);
+    if constexpr (LEV >= 2) v = haar_stage_R<4>(v, pos);
+    if constexpr (LEV >= 3) v = haar_stage_R<8>(v, pos);
+    return v;
+}
+
+// Halo point h (0..67) of the 18 x 18 window: window coordinates.
+__device__ __forceinline__ void halo_xy(int h, int &x, int &y)
+{
+    if (h < 18) {
+        x = h;
+        y = 0;
+    } else if (h < 36) {
+        x = h - 18;
+        y = 17;
+    } else if (h < 52) {
+        x = 0;
+        y = h - 35;
+    } else {
+        x = 17;
+        y = h - 51;
+    }
+}
+
+// Arguments of the iteration kernel.  All FP32 vectors of the mixed coarse CG
+// live in one arena: array id A, component c (0 u, 1 v) starts at
+// F + (A + c) * ns.
+enum CGMArray {
+    CGM_R = 0,      // residual r
+    CGM_W = 2,      // w = A_c r
+    CGM_P = 4,      // search direction p
+    CGM_S = 6,      // s = A_c p
+    CGM_X = 8,      // solution (warm start on entry)
+    CGM_HU = 10,    // thickness on u / v points, 0 off the masks
+    CGM_BU = 12,    // beta_eff on u / v points, 0 off the masks
+    CGM_RING = 14,  // rings: CGM_RING + 6 parity + 2 kind (0 P r, 1 P s, 2 P w)
+    CGM_CELL = 26,  // per h cell: (h*eta_av, dimp) pairs (two arrays' room)
+    CGM_CORNER = 28, // per h cell: corner h*eta_av (0 off the c grid and mask)
+    CGM_NARRAY = 29
+};
+
+struct CGMIterArgs {
+    float *F;
+    const uint8_t *M;       // mask bits (bit 0 inner, bit 1 wavelet mask): u at M, v at M + ns
+    double2 *Pgd;           // [2][tiles]: per tile (r, r) and (w, r) partials by parity
+    CGMScalars *st;
+    const double *Pb;       // partials of ||b||^2 per u / v block
+    GridDims g;
+    unsigned ns;
+    float rdx, rdy;
+    double tol_coarse, reltol;
+    int maxiter, nblk;
+};
+
 // One iteration (INIT: the launch before the first iteration, which only
-// computes w0 = A_c r0, T_w = P w0 and the partials, and copies T_r[0] into
-// set 1 with T_s = 0).
-template <int PAR, bool INIT>
-__global__ void __launch_bounds__(256) k_cgm_iter(CGMArgs a)
+// computes w0 = A_c r0 and the rings of set 1: P r0, P s = 0, P w0, and the
+// partial (r0, r0), (w0, r0)).  All loads are issued up front from clamped,
+// always valid addresses, and masked afterwards, so that the memory round
+// trip is paid once.  Seven barriers: partial sums; block transforms of r
+// and s (x pass, transpose, y pass straight into the window); intermediates;
+// operator and R x pass; R y pass and P y pass; P x pass and the new
+// partials.
+template <int PAR, bool INIT, int LEV>
+__global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
 {
     static_assert(!INIT || PAR == 0, "the initial launch reads set 0");
-    __shared__ OpTile<float> s;
-    __shared__ float tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
-    __shared__ double lds[8];
+    struct Smem {
+        float U[18][19], V[18][19];      // window of P r (u and v points)
+        float HU[18][19], HV[18][19];    // thickness windows, 0 off the masks
+        float rxx[17][18], ryy[17][18], ext[17][18], rxy[17][18];
+        float T[4][TILE][TILE + 1];      // transposes of the block transforms
+        uint8_t mb[2][TILE][TILE + 1];   // mask bits of the tile
+        double red0[32], red1[32], red2[8];
+    };
+    __shared__ Smem sm;
     const GridDims &g = a.g;
     const int L = xcd_tile(a.nblk);
-    if (L >= g.nbx * g.nby) return;
+    const int ntile = g.nbx * g.nby;
+    if (L >= ntile) return;
+    const int tid = threadIdx.x;
     const int bx = L % g.nbx, by = L / g.nbx;
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
     const int i0 = bx * TILE, j0 = by * TILE;
-    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int lx = tid & 15, ly = tid >> 4;    // row-major position
+    const int cx = tid >> 4, cy = tid & 15;    // column-major position (transposes)
     constexpr int NP = 1 - PAR;
-    // loads that do not depend on the scalars, all issued before the reduction
+    const unsigned ns = a.ns;
+    float *F = a.F;
+    const float rdx = a.rdx, rdy = a.rdy;
+    // ---- loads, all issued before the first use
     const CGMState S = a.st->s[PAR];
-    PointCoef<float> pc;
-    point_prefetch(g, a.c, a.mwu, a.mwv, i0 + lx, j0 + ly, has_u, has_v, pc);
-    const int ku = pc.ku, kv = pc.kv;
-    float r_u = 0.f, w_u = 0.f, p_u = 0.f, s_u = 0.f, x_u = 0.f;
-    float r_v = 0.f, w_v = 0.f, p_v = 0.f, s_v = 0.f, x_v = 0.f;
-    if (pc.oku) {
-        r_u = a.ru[ku];
-        if (!INIT) {
-            w_u = a.wu[ku];
-            p_u = a.pu[ku];
-            s_u = a.su[ku];
-            x_u = a.xu[ku];
+    const int i = i0 + lx, j = j0 + ly;
+    const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
+    const unsigned ku = oku ? (unsigned)(i + g.nxu * j) : 0u;
+    const unsigned kv = okv ? (unsigned)(i + g.nxv * j) : 0u;
+    const uint8_t mbu0 = a.M[ku], mbv0 = a.M[ns + kv];
+    const float bu0 = F[CGM_BU * ns + ku], bv0 = F[(CGM_BU + 1) * ns + kv];
+    const float r_u0 = F[CGM_R * ns + ku], r_v0 = F[(CGM_R + 1) * ns + kv];
+    float w_u0 = 0.f, p_u0 = 0.f, s_u0 = 0.f, x_u0 = 0.f, w_v0 = 0.f, p_v0 = 0.f, s_v0 = 0.f, x_v0 = 0.f;
+    if (!INIT) {
+        w_u0 = F[CGM_W * ns + ku];
+        w_v0 = F[(CGM_W + 1) * ns + kv];
+        p_u0 = F[CGM_P * ns + ku];
+        p_v0 = F[(CGM_P + 1) * ns + kv];
+        s_u0 = F[CGM_S * ns + ku];
+        s_v0 = F[(CGM_S + 1) * ns + kv];
+        x_u0 = F[CGM_X * ns + ku];
+        x_v0 = F[(CGM_X + 1) * ns + kv];
+    }
+    // halo of the window: threads 0..67 the u grid, 68..135 the v grid
+    int hx, hy;
+    const int hcomp = tid >= 68 ? 1 : 0;
+    halo_xy(tid < 136 ? tid - 68 * hcomp : 0, hx, hy);
+    const int gi = i0 - 1 + hx, gj = j0 - 1 + hy;
+    const int hnx = hcomp ? g.nxv : g.nxu, hny = hcomp ? g.nyv : g.nyu;
+    const bool hin = tid < 136 && gi >= 0 && gi < hnx && gj >= 0 && gj < hny;
+    const unsigned hk = (CGM_RING + 6 * PAR + hcomp) * ns + (hin ? (unsigned)(gi + hnx * gj) : 0u);
+    const float hr0 = F[hk];
+    float hs0 = 0.f, hw0 = 0.f;
+    if (!INIT) {
+        hs0 = F[hk + 2 * ns];
+        hw0 = F[hk + 4 * ns];
+    }
+    // thickness windows, entries q = tid and tid + 256 of the 18 x 18 window
+    bool tin[4];
+    float th[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int q = tid + 256 * (e >> 1), comp = e & 1;
+        const int x = q % 18, y = q / 18, ii = i0 - 1 + x, jj = j0 - 1 + y;
+        const int nxw = comp ? g.nxv : g.nxu, nyw = comp ? g.nyv : g.nyu;
+        tin[e] = q < 324 && ii >= 0 && ii < nxw && jj >= 0 && jj < nyw;
+        th[e] = F[(CGM_HU + comp) * ns + (tin[e] ? (unsigned)(ii + nxw * jj) : 0u)];
+    }
+    // cell coefficients, cells q = tid and tid + 256 of the 17 x 17 window
+    bool cin[2];
+    float2 cc[2];
+    float ch[2];
+    const float2 *cell2 = reinterpret_cast<const float2 *>(F + CGM_CELL * ns);
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int q = tid + 256 * e;
+        const int x = q % 17, y = q / 17, ca = i0 - 1 + x, cb = j0 - 1 + y;
+        cin[e] = q < 289 && ca >= 0 && ca < g.nx && cb >= 0 && cb < g.ny;
+        const unsigned kc = cin[e] ? (unsigned)(ca + g.nx * cb) : 0u;
+        cc[e] = cell2[kc];
+        ch[e] = F[CGM_CORNER * ns + kc];
+    }
+    // partials of the previous launch (first 1024 tiles; larger grids loop below)
+    double2 pv[4];
+    if (!INIT) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int q = tid + 256 * e;
+            pv[e] = a.Pgd[PAR * ntile + (q < ntile ? q : 0)];
         }
     }
-    if (pc.okv) {
-        r_v = a.rv[kv];
-        if (!INIT) {
-            w_v = a.wv[kv];
-            p_v = a.pv[kv];
-            s_v = a.sv[kv];
-            x_v = a.xv[kv];
+    // ---- masks
+    const uint8_t mbu = oku ? mbu0 : 0, mbv = okv ? mbv0 : 0;
+    const bool inu = mbu & 1, inv = mbv & 1, mwu = mbu & 2, mwv = mbv & 2;
+    float r_u = oku ? r_u0 : 0.f, r_v = okv ? r_v0 : 0.f;
+    // ---- scalars from the previous launch's partials
+    float fa = 0.f, fb = 0.f;
+    if (!INIT) {
+        double gamma = 0.0, delta = 0.0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const bool ok = tid + 256 * e < ntile;
+            gamma = gamma + (ok ? pv[e].x : 0.0);
+            delta = delta + (ok ? pv[e].y : 0.0);
         }
-    }
-    StageRegs<float> tr, ts, tw;
-    optile_fetch(g, a.tru[PAR], a.trv[PAR], i0, j0, tr);
-    if (!INIT) {
-        optile_fetch(g, a.tsu[PAR], a.tsv[PAR], i0, j0, ts);
-        optile_fetch(g, a.twu[PAR], a.twv[PAR], i0, j0, tw);
-    } else {
-        ts.u0 = ts.v0 = ts.u1 = ts.v1 = 0.f;
-    }
-    CellCoef<float> c0, c1;
-    cell_prefetch(g, a.c, i0, j0, threadIdx.x, c0);
-    cell_prefetch(g, a.c, i0, j0, threadIdx.x + 256, c1);
-    if (!INIT) {
-        double gamma, delta;
-        lanes_sum2(a.Pg[PAR], a.Pd[PAR], g.npart, lds, gamma, delta);
+        for (int q = tid + 1024; q < ntile; q += 256) {
+            const double2 v = a.Pgd[PAR * ntile + q];
+            gamma = gamma + v.x;
+            delta = delta + v.y;
+        }
+        wg_sum2(gamma, delta, sm.red0);
         if (S.done) {
-            if (L == 0 && threadIdx.x == 0) a.st->s[NP] = S;
+            if (L == 0 && tid == 0) a.st->s[NP] = S;
             return;
         }
-        const double res = sqrt(gamma);
-        double tol = S.tol;
+        // stop when ||r|| <= tol, tested as ||r||^2 <= tol^2
+        double tol2 = S.tol2;
         if (S.it == 0) {
-            const double abstol = a.tol_coarse * sqrt(lanes_sum(a.Pb, g.npart, lds));
-            tol = a.reltol * res;
+            const double abstol = a.tol_coarse * sqrt(lanes_sum(a.Pb, g.npart, sm.red2));
+            double tol = a.reltol * sqrt(gamma);
             if (abstol > tol) tol = abstol;
+            tol2 = tol * tol;
         }
-        if (S.it >= a.maxiter || res <= tol) {
-            if (L == 0 && threadIdx.x == 0) {
+        if (S.it >= a.maxiter || gamma <= tol2) {
+            if (L == 0 && tid == 0) {
                 CGMState n = S;
-                n.tol = tol;
+                n.tol2 = tol2;
                 n.done = 1;
                 a.st->s[NP] = n;
                 a.st->done = 1;
@@ -1537,113 +1711,172 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMArgs a)
             }
             return;
         }
-        const double beta = S.it == 0 ? 0.0 : gamma / S.gamma;
-        const double alpha = S.it == 0 ? gamma / delta : gamma / (delta - beta * gamma / S.alpha);
-        if (L == 0 && threadIdx.x == 0) {
+        const double beta = S.it == 0 ? 0.0 : gamma * S.igamma;
+        const double alpha = gamma / (S.it == 0 ? delta : delta - beta * gamma * S.ialpha);
+        if (L == 0 && tid == 0) {
             CGMState n;
-            n.gamma = gamma;
-            n.alpha = alpha;
-            n.tol = tol;
+            n.igamma = 1.0 / gamma;
+            n.ialpha = 1.0 / alpha;
+            n.tol2 = tol2;
             n.it = S.it + 1;
             n.done = 0;
             a.st->s[NP] = n;
         }
-        const float fa = (float)alpha, fb = (float)beta;
-        if (pc.oku) {
-            p_u = r_u + fb * p_u;
-            s_u = w_u + fb * s_u;
-            a.pu[ku] = p_u;
-            a.su[ku] = s_u;
-            a.xu[ku] = x_u + fa * p_u;
+        fa = (float)alpha;
+        fb = (float)beta;
+    }
+    // ---- vector updates on the tile
+    float s_u = 0.f, s_v = 0.f;
+    if (!INIT) {
+        const float p_u = r_u + fb * p_u0, p_v = r_v + fb * p_v0;
+        s_u = w_u0 + fb * s_u0;
+        s_v = w_v0 + fb * s_v0;
+        if (!oku) s_u = 0.f;
+        if (!okv) s_v = 0.f;
+        if (oku) {
+            F[CGM_P * ns + ku] = p_u;
+            F[CGM_S * ns + ku] = s_u;
+            F[CGM_X * ns + ku] = x_u0 + fa * p_u;
             r_u = r_u - fa * s_u;
-            a.ru[ku] = r_u;
+            F[CGM_R * ns + ku] = r_u;
         }
-        if (pc.okv) {
-            p_v = r_v + fb * p_v;
-            s_v = w_v + fb * s_v;
-            a.pv[kv] = p_v;
-            a.sv[kv] = s_v;
-            a.xv[kv] = x_v + fa * p_v;
+        if (okv) {
+            F[(CGM_P + 1) * ns + kv] = p_v;
+            F[(CGM_S + 1) * ns + kv] = s_v;
+            F[(CGM_X + 1) * ns + kv] = x_v0 + fa * p_v;
             r_v = r_v - fa * s_v;
-            a.rv[kv] = r_v;
-        }
-        ts.u0 = tw.u0 + fb * ts.u0;
-        ts.v0 = tw.v0 + fb * ts.v0;
-        ts.u1 = tw.u1 + fb * ts.u1;
-        ts.v1 = tw.v1 + fb * ts.v1;
-        tr.u0 = tr.u0 - fa * ts.u0;
-        tr.v0 = tr.v0 - fa * ts.v0;
-        tr.u1 = tr.u1 - fa * ts.u1;
-        tr.v1 = tr.v1 - fa * ts.v1;
-    }
-    optile_store(tr, s);
-    stage_write_own(g, tr, i0, j0, a.tru[NP], a.trv[NP]);
-    stage_write_own(g, ts, i0, j0, a.tsu[NP], a.tsv[NP]);
-    __syncthreads();
-    cell_compute(a.c, c0, threadIdx.x, s);
-    cell_compute(a.c, c1, threadIdx.x + 256, s);
-    __syncthreads();
-    tu[ly][lx] = (pc.oku && pc.ui) ? optile_u(a.c, s, pc, lx, ly) : 0.f;
-    tv[ly][lx] = (pc.okv && pc.vi) ? optile_v(a.c, s, pc, lx, ly) : 0.f;
-    __syncthreads();
-    haar_pm_tile2<float>(a.levels, 1, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
-    const float wn_u = pc.mwu ? tu[ly][lx] : 0.f, wn_v = pc.mwv ? tv[ly][lx] : 0.f;
-    if (pc.oku) a.wu[ku] = wn_u;
-    if (pc.okv) a.wv[kv] = wn_v;
-    tu[ly][lx] = wn_u;
-    tv[ly][lx] = wn_v;
-    __syncthreads();
-    haar_pm_tile2<float>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
-    if (pc.oku) a.twu[NP][ku] = pc.ui ? tu[ly][lx] : 0.f;
-    if (pc.okv) a.twv[NP][kv] = pc.vi ? tv[ly][lx] : 0.f;
-    double gu = (double)r_u * (double)r_u, gv = (double)r_v * (double)r_v;
-    double du = (double)wn_u * (double)r_u, dv = (double)wn_v * (double)r_v;
-    tree256x2(gu, gv, lds);
-    tree256x2(du, dv, lds);
-    if (threadIdx.x == 0) {
-        if (has_u) {
-            a.Pg[NP][bx + g.nbxu * by] = gu;
-            a.Pd[NP][bx + g.nbxu * by] = du;
-        }
-        if (has_v) {
-            const int o = g.nbxu * g.nbyu + bx + g.nbxv * by;
-            a.Pg[NP][o] = gv;
-            a.Pd[NP][o] = dv;
+            F[(CGM_R + 1) * ns + kv] = r_v;
         }
     }
-}
-
-// T_r = P r and T_s = P s on the tile, into the set the next launch reads.
-template <int PAR>
-__global__ void __launch_bounds__(256) k_cgm_resync(CGMArgs a)
-{
-    __shared__ float tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
-    const GridDims &g = a.g;
-    const int L = xcd_tile(a.nblk);
-    if (L >= g.nbx * g.nby) return;
-    const int bx = L % g.nbx, by = L / g.nbx;
-    const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
-    const int i0 = bx * TILE, j0 = by * TILE;
-    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int i = i0 + lx, j = j0 + ly;
-    const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
-    const int ku = i + g.nxu * j, kv = i + g.nxv * j;
-    const bool mwu = oku && a.mwu[ku], mwv = okv && a.mwv[kv];
-    const bool iu = oku && a.ui[ku], iv = okv && a.vi[kv];
-    const float ru = mwu ? a.ru[ku] : 0.f, rv = mwv ? a.rv[kv] : 0.f;
-    const float su = mwu ? a.su[ku] : 0.f, sv = mwv ? a.sv[kv] : 0.f;
-    tu[ly][lx] = ru;
-    tv[ly][lx] = rv;
+    // ---- x pass of P r and P s; halo and thickness windows into LDS
+    sm.T[0][ly][lx] = haar_line_P<LEV>(mwu ? r_u : 0.f, lx);
+    sm.T[1][ly][lx] = haar_line_P<LEV>(mwv ? r_v : 0.f, lx);
+    if (!INIT) {
+        sm.T[2][ly][lx] = haar_line_P<LEV>(mwu ? s_u : 0.f, lx);
+        sm.T[3][ly][lx] = haar_line_P<LEV>(mwv ? s_v : 0.f, lx);
+    }
+    sm.mb[0][ly][lx] = mbu;
+    sm.mb[1][ly][lx] = mbv;
+    if (tid < 136) {
+        const float hr = hin ? hr0 : 0.f, hs = hin ? hs0 : 0.f, hw = hin ? hw0 : 0.f;
+        const float t = INIT ? hr : hr - fa * (hw + fb * hs);
+        if (!hcomp)
+            sm.U[hy][hx] = t;
+        else
+            sm.V[hy][hx] = t;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int q = tid + 256 * (e >> 1);
+        if (q < 324) {
+            const float v = tin[e] ? th[e] : 0.f;
+            if (e & 1)
+                sm.HV[q / 18][q % 18] = v;
+            else
+                sm.HU[q / 18][q % 18] = v;
+        }
+    }
     __syncthreads();
-    haar_pm_tile2<float>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
-    if (oku) a.tru[PAR][ku] = iu ? tu[ly][lx] : 0.f;
-    if (okv) a.trv[PAR][kv] = iv ? tv[ly][lx] : 0.f;
-    tu[ly][lx] = su;
-    tv[ly][lx] = sv;
+    // ---- y pass in the column-major lanes: P r into the window, rings of P r and P s
+    {
+        const uint8_t m0 = sm.mb[0][cy][cx], m1 = sm.mb[1][cy][cx];
+        const float y0 = haar_line_P<LEV>(sm.T[0][cy][cx], cy);
+        const float y1 = haar_line_P<LEV>(sm.T[1][cy][cx], cy);
+        const float pr_u = (m0 & 1) ? y0 : 0.f, pr_v = (m1 & 1) ? y1 : 0.f;
+        sm.U[cy + 1][cx + 1] = pr_u;
+        sm.V[cy + 1][cx + 1] = pr_v;
+        float ps_u = 0.f, ps_v = 0.f;
+        if (!INIT) {
+            const float y2 = haar_line_P<LEV>(sm.T[2][cy][cx], cy);
+            const float y3 = haar_line_P<LEV>(sm.T[3][cy][cx], cy);
+            ps_u = (m0 & 1) ? y2 : 0.f;
+            ps_v = (m1 & 1) ? y3 : 0.f;
+        }
+        if (cx == 0 || cx == 15 || cy == 0 || cy == 15) {
+            const int ic = i0 + cx, jc = j0 + cy;
+            const unsigned rb = (CGM_RING + 6 * NP) * ns;
+            if (has_u && ic < g.nxu && jc < g.nyu) {
+                const unsigned k = rb + (unsigned)(ic + g.nxu * jc);
+                F[k] = pr_u;
+                F[k + 2 * ns] = ps_u;
+            }
+            if (has_v && ic < g.nxv && jc < g.nyv) {
+                const unsigned k = rb + ns + (unsigned)(ic + g.nxv * jc);
+                F[k] = pr_v;
+                F[k + 2 * ns] = ps_v;
+            }
+        }
+    }
     __syncthreads();
-    haar_pm_tile2<float>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
-    if (oku) a.tsu[PAR][ku] = iu ? tu[ly][lx] : 0.f;
-    if (okv) a.tsv[PAR][kv] = iv ? tv[ly][lx] : 0.f;
+    // ---- intermediates on the 17 x 17 cells and corners
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int q = tid + 256 * e;
+        if (q < 289) {
+            const float4 c = cin[e] ? make_float4(cc[e].x, cc[e].y, ch[e], 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+            const int x = q % 17, y = q / 17;
+            const float u0 = sm.U[y][x], u1 = sm.U[y][x + 1], v0 = sm.V[y][x], v1 = sm.V[y + 1][x];
+            const float dudx = rdx * (u1 - u0), dvdy = rdy * (v1 - v0);
+            sm.rxx[y][x] = 2.f * c.x * (2.f * dudx + dvdy);
+            sm.ryy[y][x] = 2.f * c.x * (2.f * dvdy + dudx);
+            const float divq = rdx * (sm.HU[y][x + 1] * u1 - sm.HU[y][x] * u0) +
+                               rdy * (sm.HV[y + 1][x] * v1 - sm.HV[y][x] * v0);
+            sm.ext[y][x] = c.y * divq;
+            sm.rxy[y][x] = c.z * (rdy * (sm.U[y + 1][x + 1] - sm.U[y][x + 1]) +
+                                  rdx * (sm.V[y + 1][x + 1] - sm.V[y + 1][x]));
+        }
+    }
+    __syncthreads();
+    // ---- A at the tile's points (zero off the inner masks), R x pass
+    {
+        const int x = lx, y = ly;
+        float Au = 0.f, Av = 0.f;
+        if (inu) {
+            const float d_rxx = rdx * (sm.rxx[y + 1][x + 1] - sm.rxx[y + 1][x]);
+            const float d_rxy = rdy * (sm.rxy[y + 1][x] - sm.rxy[y][x]);
+            const float d_ext = rdx * (sm.ext[y + 1][x + 1] - sm.ext[y + 1][x]);
+            Au = ((d_rxx + d_rxy) - bu0 * sm.U[y + 1][x + 1]) - sm.HU[y + 1][x + 1] * d_ext;
+        }
+        if (inv) {
+            const float d_ryy = rdy * (sm.ryy[y + 1][x + 1] - sm.ryy[y][x + 1]);
+            const float d_rxy = rdx * (sm.rxy[y][x + 1] - sm.rxy[y][x]);
+            const float d_ext = rdy * (sm.ext[y + 1][x + 1] - sm.ext[y][x + 1]);
+            Av = ((d_ryy + d_rxy) - bv0 * sm.V[y + 1][x + 1]) - sm.HV[y + 1][x + 1] * d_ext;
+        }
+        sm.T[0][ly][lx] = haar_line_R<LEV>(Au, lx);
+        sm.T[1][ly][lx] = haar_line_R<LEV>(Av, lx);
+    }
+    __syncthreads();
+    // ---- R y pass, wavelet mask (w), P y pass
+    {
+        const float c0 = haar_line_R<LEV>(sm.T[0][cy][cx], cy);
+        const float c1 = haar_line_R<LEV>(sm.T[1][cy][cx], cy);
+        const float w0 = (sm.mb[0][cy][cx] & 2) ? c0 : 0.f;
+        const float w1 = (sm.mb[1][cy][cx] & 2) ? c1 : 0.f;
+        const float q0 = haar_line_P<LEV>(w0, cy), q1 = haar_line_P<LEV>(w1, cy);
+        sm.T[0][cy][cx] = w0;
+        sm.T[1][cy][cx] = w1;
+        sm.T[2][cy][cx] = q0;
+        sm.T[3][cy][cx] = q1;
+    }
+    __syncthreads();
+    // ---- P x pass, stores, partials of (r, r) and (w, r)
+    const float wn_u = sm.T[0][ly][lx], wn_v = sm.T[1][ly][lx];
+    const float tw_u = haar_line_P<LEV>(sm.T[2][ly][lx], lx), tw_v = haar_line_P<LEV>(sm.T[3][ly][lx], lx);
+    const bool ring = lx == 0 || lx == 15 || ly == 0 || ly == 15;
+    const unsigned wb = (CGM_RING + 6 * NP + 4) * ns;
+    if (oku) {
+        F[CGM_W * ns + ku] = wn_u;
+        if (ring) F[wb + ku] = inu ? tw_u : 0.f;
+    }
+    if (okv) {
+        F[(CGM_W + 1) * ns + kv] = wn_v;
+        if (ring) F[wb + ns + kv] = inv ? tw_v : 0.f;
+    }
+    double gg = (double)r_u * (double)r_u + (double)r_v * (double)r_v;
+    double dd = (double)wn_u * (double)r_u + (double)wn_v * (double)r_v;
+    wg_sum2(gg, dd, sm.red1);
+    if (tid == 0) a.Pgd[NP * ntile + L] = make_double2(gg, dd);
 }
 
 // out = mask_out ? H(mask_in ? in : 0) : 0 on the u grid (comp 0) and v grid
