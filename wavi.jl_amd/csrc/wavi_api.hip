@@ -59,14 +59,13 @@ struct wavi_ctx {
     VSet<float> v32;
     // mixed-precision coarse CG: s, w, the parity-buffered images P r, P s,
     // P w and partials of (r, r) and (w, r)
-    // mixed coarse CG: one FP32 arena (enum CGMArray) and a mask-byte arena of
-    // stride ns; s, w, the rings and the packed coefficients are views into it
+    // mixed coarse CG: one FP32 arena (enum CGMArray) and a mask-byte arena
+    // in the padded layout of pitch px and array stride ns
     float *arena = nullptr;
     uint8_t *marena = nullptr;
-    unsigned ns = 0;
-    DevVecT<float> ms, mw, mtr[2], mts[2], mtw[2];
-    double2 *Pgd = nullptr;  // [2][tiles] (r, r) and (w, r) partials by parity
-    CGMPack pack{};
+    unsigned px = 0, ns = 0;
+    float2 *Pgd = nullptr;  // [2][npad] (r, r) and (w, r) partials by parity
+    int npad = 0;
     CGMScalars *stm = nullptr;
     CGMScalars *h_stm = nullptr;  // pinned host copy
     double *Pr0 = nullptr, *Pr1 = nullptr, *Pd = nullptr, *Pb = nullptr, *Pn = nullptr;
@@ -472,37 +471,18 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
         CREATECHK(falloc(&w.heta_h, ctx->nh));
         CREATECHK(falloc(&w.heta_c, ctx->nc > 0 ? ctx->nc : 1));
         CREATECHK(falloc(&w.dimp, ctx->nh));
-        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cq, &w.ct}) {
+        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cr, &w.cp, &w.cq, &w.ct, &w.xc}) {
             CREATECHK(falloc(&dv->u, ctx->nu));
             CREATECHK(falloc(&dv->v, ctx->nv));
         }
-        const unsigned ns = (unsigned)((std::max(ctx->nu, ctx->nv) + 63) / 64 * 64);
-        ctx->ns = ns;
-        CREATECHK(falloc(&ctx->arena, (size_t)CGM_NARRAY * ns));
-        CREATECHK(hipMalloc(&ctx->marena, 2 * (size_t)ns));
-        CREATECHK(hipMemset(ctx->marena, 0, 2 * (size_t)ns));
-        float *F = ctx->arena;
-        auto view = [&](int id) { return DevVecT<float>{F + (size_t)id * ns, F + (size_t)(id + 1) * ns}; };
-        w.cr = view(CGM_R);
-        w.cp = view(CGM_P);
-        w.xc = view(CGM_X);
-        ctx->ms = view(CGM_S);
-        ctx->mw = view(CGM_W);
-        for (int p = 0; p < 2; ++p) {
-            ctx->mtr[p] = view(CGM_RING + 6 * p);
-            ctx->mts[p] = view(CGM_RING + 6 * p + 2);
-            ctx->mtw[p] = view(CGM_RING + 6 * p + 4);
-        }
-        ctx->pack.hu = F + (size_t)CGM_HU * ns;
-        ctx->pack.hv = F + (size_t)(CGM_HU + 1) * ns;
-        ctx->pack.bu = F + (size_t)CGM_BU * ns;
-        ctx->pack.bv = F + (size_t)(CGM_BU + 1) * ns;
-        ctx->pack.mu = ctx->marena;
-        ctx->pack.mv = ctx->marena + ns;
-        CREATECHK(hipMalloc(&ctx->Pgd, 2 * (size_t)g.nbx * g.nby * sizeof(double2)));
-        CREATECHK(hipMemset(ctx->Pgd, 0, 2 * (size_t)g.nbx * g.nby * sizeof(double2)));
-        ctx->pack.cell = reinterpret_cast<float2 *>(F + (size_t)CGM_CELL * ns);
-        ctx->pack.corner = F + (size_t)CGM_CORNER * ns;
+        ctx->px = (unsigned)((TILE * g.nbx + 2 + 31) / 32 * 32);
+        ctx->ns = ctx->px * (unsigned)(TILE * g.nby + 2);
+        CREATECHK(falloc(&ctx->arena, (size_t)CGM_NARRAY * ctx->ns));
+        CREATECHK(hipMalloc(&ctx->marena, 2 * (size_t)ctx->ns));
+        CREATECHK(hipMemset(ctx->marena, 0, 2 * (size_t)ctx->ns));
+        ctx->npad = (g.nbx * g.nby + 1023) / 1024 * 1024;
+        CREATECHK(hipMalloc(&ctx->Pgd, 2 * (size_t)ctx->npad * sizeof(float2)));
+        CREATECHK(hipMemset(ctx->Pgd, 0, 2 * (size_t)ctx->npad * sizeof(float2)));
         CREATECHK(hipMalloc(&ctx->stm, sizeof(CGMScalars)));
         CREATECHK(hipMemset(ctx->stm, 0, sizeof(CGMScalars)));
         CREATECHK(hipHostMalloc((void **)&ctx->h_stm, sizeof(CGMScalars), hipHostMallocDefault));
@@ -544,7 +524,7 @@ int wavi_ctx_destroy(wavi_ctx *ctx)
         VSet<float> &w = ctx->v32;
         for (float *p : {w.heta_h, w.heta_c, w.dimp})
             if (p) (void)hipFree(p);
-        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cq, &w.ct}) {
+        for (DevVecT<float> *dv : {&w.h, &w.beff, &w.e, &w.r, &w.d, &w.rho, &w.t, &w.cr, &w.cp, &w.cq, &w.ct, &w.xc}) {
             if (dv->u) (void)hipFree(dv->u);
             if (dv->v) (void)hipFree(dv->v);
         }
@@ -942,62 +922,14 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
     return WAVI_OK;
 }
 
-static CGMArgs cgm_args(wavi_ctx *ctx)
+static CGMLayout cgm_layout(const wavi_ctx *ctx)
 {
-    const VSet<float> &w = ctx->v32;
-    CGMArgs a;
-    a.g = ctx->g;
-    a.c = coef_of(ctx, w);
-    a.levels = ctx->S.levels;
-    a.mwu = ctx->mwu;
-    a.mwv = ctx->mwv;
-    a.k = ctx->pack;
-    a.xu = w.xc.u;
-    a.xv = w.xc.v;
-    a.ru = w.cr.u;
-    a.rv = w.cr.v;
-    a.pu = w.cp.u;
-    a.pv = w.cp.v;
-    a.su = ctx->ms.u;
-    a.sv = ctx->ms.v;
-    a.wu = ctx->mw.u;
-    a.wv = ctx->mw.v;
-    a.t0u = w.ct.u;
-    a.t0v = w.ct.v;
-    for (int p = 0; p < 2; ++p) {
-        a.hru[p] = ctx->mtr[p].u;
-        a.hrv[p] = ctx->mtr[p].v;
-        a.hsu[p] = ctx->mts[p].u;
-        a.hsv[p] = ctx->mts[p].v;
-        a.hwu[p] = ctx->mtw[p].u;
-        a.hwv[p] = ctx->mtw[p].v;
-    }
-    a.Pb = ctx->Pb;
-    a.st = ctx->stm;
-    a.tol_coarse = ctx->S.tol_coarse;
-    a.reltol = std::sqrt(2.220446049250313e-16);  // IterativeSolvers' default sqrt(eps)
-    a.maxiter = ctx->S.maxiter_coarse;
-    a.nblk = ctx->nblk;
-    return a;
-}
-
-static CGMIterArgs cgm_iter_args(wavi_ctx *ctx)
-{
-    CGMIterArgs a;
-    a.F = ctx->arena;
-    a.M = ctx->marena;
-    a.Pgd = ctx->Pgd;
-    a.st = ctx->stm;
-    a.Pb = ctx->Pb;
-    a.g = ctx->g;
-    a.ns = ctx->ns;
-    a.rdx = (float)ctx->rdx;
-    a.rdy = (float)ctx->rdy;
-    a.tol_coarse = ctx->S.tol_coarse;
-    a.reltol = std::sqrt(2.220446049250313e-16);  // IterativeSolvers' default sqrt(eps)
-    a.maxiter = ctx->S.maxiter_coarse;
-    a.nblk = ctx->nblk;
-    return a;
+    CGMLayout Lo;
+    Lo.F = ctx->arena;
+    Lo.M = ctx->marena;
+    Lo.px = ctx->px;
+    Lo.ns = ctx->ns;
+    return Lo;
 }
 
 template <int PAR, bool INIT>
@@ -1012,20 +944,51 @@ static void launch_cgm_iter(wavi_ctx *ctx, dim3 tb, const CGMIterArgs &a)
 }
 
 // Mixed-precision coarse CG (Chronopoulos-Gear form, one launch per
-// iteration) on b_c held in v32.cr; solution in the warm start v32.xc.
+// iteration) on b_c held in v32.cr; solution in the warm start v32.xc.  The
+// CG works in the padded arena: k_cgm_pack fills the coefficients and masks,
+// k_cgm_setup the vectors, k_cgm_unpack returns the solution.
 template <>
 int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
 {
     const GridDims &g = ctx->g;
     dim3 tg(g.nbx, g.nby), tb(ctx->nblk);
-    CGMArgs a = cgm_args(ctx);
+    const CGMLayout Lo = cgm_layout(ctx);
+    const OpCoef<float> c = coef_of(ctx, w);
     hipLaunchKernelGGL(k_cgm_pack, flat(std::max(ctx->nh, std::max(ctx->nu, ctx->nv))), dim3(256), 0, ctx->stream, g,
-                       a.c, ctx->mwu, ctx->mwv, ctx->pack);
+                       c, ctx->mwu, ctx->mwv, Lo);
     // P x0
     hipLaunchKernelGGL((k_tile_haar<float, float>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1,
-                       a.xu, a.xv, ctx->mwu, ctx->mwv, w.ct.u, w.ct.v, ctx->ui, ctx->vi);
-    hipLaunchKernelGGL(k_cgm_setup, tb, dim3(256), 0, ctx->stream, a);
-    const CGMIterArgs ia = cgm_iter_args(ctx);
+                       w.xc.u, w.xc.v, ctx->mwu, ctx->mwv, w.ct.u, w.ct.v, ctx->ui, ctx->vi);
+    CGMArgs sa;
+    sa.g = g;
+    sa.c = c;
+    sa.Lo = Lo;
+    sa.levels = ctx->S.levels;
+    sa.mwu = ctx->mwu;
+    sa.mwv = ctx->mwv;
+    sa.bu = w.cr.u;
+    sa.bv = w.cr.v;
+    sa.x0u = w.xc.u;
+    sa.x0v = w.xc.v;
+    sa.t0u = w.ct.u;
+    sa.t0v = w.ct.v;
+    sa.Pb = ctx->Pb;
+    sa.st = ctx->stm;
+    sa.nblk = ctx->nblk;
+    hipLaunchKernelGGL(k_cgm_setup, tb, dim3(256), 0, ctx->stream, sa);
+    CGMIterArgs ia;
+    ia.Lo = Lo;
+    ia.Pgd = ctx->Pgd;
+    ia.st = ctx->stm;
+    ia.Pb = ctx->Pb;
+    ia.g = g;
+    ia.npad = ctx->npad;
+    ia.rdx = (float)ctx->rdx;
+    ia.rdy = (float)ctx->rdy;
+    ia.tol_coarse = ctx->S.tol_coarse;
+    ia.reltol = std::sqrt(2.220446049250313e-16);  // IterativeSolvers' default sqrt(eps)
+    ia.maxiter = ctx->S.maxiter_coarse;
+    ia.nblk = ctx->nblk;
     launch_cgm_iter<0, true>(ctx, tb, ia);
     HIPCHK(hipGetLastError());
     const int CHUNK = 32;
@@ -1049,6 +1012,9 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
             return WAVI_ESTATE;
         }
     }
+    hipLaunchKernelGGL(k_cgm_unpack, flat(std::max(ctx->nu, ctx->nv)), dim3(256), 0, ctx->stream, g, Lo, w.xc.u,
+                       w.xc.v);
+    HIPCHK(hipGetLastError());
     *iters = ctx->h_stm->iters;
     return WAVI_OK;
 }

@@ -1323,9 +1323,10 @@ __global__ void __launch_bounds__(256) k_cg_update(CGArgs<T> a, int par)
 
 // ===========================================================================
 // Mixed-precision coarse CG: the Chronopoulos-Gear form of CG (one combined
-// reduction per iteration) on FP32 vectors with FP64 reductions and scalars,
-// one launch per iteration.  In exact arithmetic it produces the iterates of
-// the CG above; launch k computes
+// reduction per iteration) on FP32 vectors, with FP32 sums inside a
+// workgroup and FP64 sums across workgroups and FP64 scalars, one launch per
+// iteration.  In exact arithmetic it produces the iterates of the CG above;
+// launch k computes
 //     p = r + beta p,  s = w + beta s,  x += alpha p,  r -= alpha s,
 //     w = A_c r,  gamma = (r, r),  delta = (w, r),
 //     beta = gamma / gamma_old,  alpha = gamma / (delta - beta gamma / alpha_old).
@@ -1339,6 +1340,12 @@ __global__ void __launch_bounds__(256) k_cg_update(CGArgs<T> a, int par)
 // double-buffered: the launch of parity PAR reads set PAR and writes set
 // 1 - PAR, so no workgroup reads what another workgroup of the same launch
 // writes.
+//
+// Layout: every array of the CG lives in one FP32 arena in a zero-padded 2-D
+// layout shared by the u grid, the v grid and the h cells: point (i, j) at
+// (i + 1) + px (j + 1), with a zero border of one point and room for whole
+// tiles, so that windows, halos and tile stores need no bounds tests (what a
+// tile stores past the end of a grid is zero).
 
 struct CGMState {
     double igamma;  // 1 / ||r||^2 at the start of the launch that wrote this state
@@ -1354,42 +1361,36 @@ struct CGMScalars {
     int iters;      // iterations at the stop
 };
 
-// FP32 operator coefficients with the masks folded in (built by k_cgm_pack
-// once per V-cycle).
-struct CGMPack {
-    float2 *cell;       // per h cell: h*eta_av (0 off the h mask), dimp
-    float *corner;      // per h cell: corner h*eta_av (0 off the c grid / mask)
-    float *hu, *hv;     // thickness on u / v points, 0 off the u / v masks
-    float *bu, *bv;     // beta_eff on u / v points, 0 off the u / v masks
-    uint8_t *mu, *mv;   // bit 0: inner point, bit 1: wavelet mask
+// Arena arrays (component u; the v component is the next array).
+enum CGMArray {
+    CGM_R = 0,       // residual r
+    CGM_W = 2,       // w = A_c r
+    CGM_P = 4,       // search direction p
+    CGM_S = 6,       // s = A_c p
+    CGM_X = 8,       // solution
+    CGM_HU = 10,     // thickness on u / v points, 0 off the masks
+    CGM_BU = 12,     // beta_eff on u / v points, 0 off the masks
+    CGM_RING = 14,   // rings: CGM_RING + 6 parity + 2 kind (0 P r, 1 P s, 2 P w)
+    CGM_CELL = 26,   // per h cell: (h*eta_av on the h mask, dimp) pairs (two arrays' room)
+    CGM_CORNER = 28, // per h cell: corner h*eta_av (0 off the c grid and mask)
+    CGM_NARRAY = 29
 };
 
-struct CGMArgs {
-    GridDims g;
-    OpCoef<float> c;         // setup only
-    CGMPack k;
-    int levels;
-    const uint8_t *mwu, *mwv;
-    float *xu, *xv;          // solution (warm start on entry)
-    float *ru, *rv;          // residual (right-hand side on entry to setup)
-    float *pu, *pv;          // search direction p
-    float *su, *sv;          // s = A_c p
-    float *wu, *wv;          // w = A_c r
-    const float *t0u, *t0v;  // P x0 (setup only)
-    float *hru[2], *hrv[2];  // rings of P r by parity (setup writes whole tiles of set 0)
-    float *hsu[2], *hsv[2];  // rings of P s
-    float *hwu[2], *hwv[2];  // rings of P w
-    double *Pb;              // partials of ||b||^2 (per u / v block)
-    CGMScalars *st;
-    double tol_coarse, reltol;
-    int maxiter;
-    int nblk;
+struct CGMLayout {
+    float *F;       // FP32 arena, array A at F + A * ns
+    uint8_t *M;     // mask bits (bit 0 inner point, bit 1 wavelet mask): u at M, v at M + ns
+    unsigned px;    // row pitch
+    unsigned ns;    // array stride (px * rows)
+    __device__ __forceinline__ unsigned at(int i, int j) const { return (unsigned)(i + 1) + px * (unsigned)(j + 1); }
 };
 
-__global__ void k_cgm_pack(GridDims g, OpCoef<float> c, const uint8_t *mwu, const uint8_t *mwv, CGMPack p)
+// Coefficients and masks into the arena (once per V-cycle); 0 off the masks.
+__global__ void k_cgm_pack(GridDims g, OpCoef<float> c, const uint8_t *mwu, const uint8_t *mwv, CGMLayout Lo)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int nh = g.nx * g.ny, nu = g.nxu * g.nyu, nv = g.nxv * g.nyv;
+    float *F = Lo.F;
+    const unsigned ns = Lo.ns;
     if (k < nh) {
         const int a = k % g.nx, b = k / g.nx;
         float hc = 0.f;
@@ -1397,24 +1398,49 @@ __global__ void k_cgm_pack(GridDims g, OpCoef<float> c, const uint8_t *mwu, cons
             const int kc = a + (g.nx - 1) * b;
             hc = c.cm[kc] ? c.heta_c[kc] : 0.f;
         }
-        p.cell[k] = make_float2(c.hm[k] ? c.heta_h[k] : 0.f, c.dimp[k]);
-        p.corner[k] = hc;
+        const unsigned o = Lo.at(a, b);
+        reinterpret_cast<float2 *>(F + CGM_CELL * ns)[o] = make_float2(c.hm[k] ? c.heta_h[k] : 0.f, c.dimp[k]);
+        F[CGM_CORNER * ns + o] = hc;
     }
     if (k < nu) {
-        p.hu[k] = c.um[k] ? c.hu[k] : 0.f;
-        p.bu[k] = c.um[k] ? c.bu[k] : 0.f;
-        p.mu[k] = (uint8_t)((c.ui[k] ? 1 : 0) | (mwu[k] ? 2 : 0));
+        const unsigned o = Lo.at(k % g.nxu, k / g.nxu);
+        F[CGM_HU * ns + o] = c.um[k] ? c.hu[k] : 0.f;
+        F[CGM_BU * ns + o] = c.um[k] ? c.bu[k] : 0.f;
+        Lo.M[o] = (uint8_t)((c.ui[k] ? 1 : 0) | (mwu[k] ? 2 : 0));
     }
     if (k < nv) {
-        p.hv[k] = c.vm[k] ? c.hv[k] : 0.f;
-        p.bv[k] = c.vm[k] ? c.bv[k] : 0.f;
-        p.mv[k] = (uint8_t)((c.vi[k] ? 1 : 0) | (mwv[k] ? 2 : 0));
+        const unsigned o = Lo.at(k % g.nxv, k / g.nxv);
+        F[(CGM_HU + 1) * ns + o] = c.vm[k] ? c.hv[k] : 0.f;
+        F[(CGM_BU + 1) * ns + o] = c.vm[k] ? c.bv[k] : 0.f;
+        Lo.M[ns + o] = (uint8_t)((c.vi[k] ? 1 : 0) | (mwv[k] ? 2 : 0));
     }
 }
 
-// setup (P x0 computed beforehand): r0 = b - R A P x0 (b arrives in r),
-// p = s = 0, P r0 into set 0 of the P r rings (whole tile), partials of
-// ||b||^2; resets the scalars.
+// The solution back into the context's compact vectors.
+__global__ void k_cgm_unpack(GridDims g, CGMLayout Lo, float *xu, float *xv)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < g.nxu * g.nyu) xu[k] = Lo.F[CGM_X * Lo.ns + Lo.at(k % g.nxu, k / g.nxu)];
+    if (k < g.nxv * g.nyv) xv[k] = Lo.F[(CGM_X + 1) * Lo.ns + Lo.at(k % g.nxv, k / g.nxv)];
+}
+
+struct CGMArgs {
+    GridDims g;
+    OpCoef<float> c;         // compact coefficients (setup)
+    CGMLayout Lo;
+    int levels;
+    const uint8_t *mwu, *mwv;
+    const float *bu, *bv;    // right-hand side (compact)
+    const float *x0u, *x0v;  // warm start (compact)
+    const float *t0u, *t0v;  // P x0 (compact)
+    double *Pb;              // partials of ||b||^2 per u / v block
+    CGMScalars *st;
+    int nblk;
+};
+
+// setup (P x0 computed beforehand): r0 = b - R A P x0, p = s = 0, x = x0,
+// P r0 into the P r rings of set 0 (whole tile), partials of ||b||^2, in
+// the arena; resets the scalars.
 __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
 {
     __shared__ OpTile<float> s;
@@ -1438,25 +1464,30 @@ __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
     const int i0 = bx * TILE, j0 = by * TILE;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    float *F = a.Lo.F;
+    const unsigned ns = a.Lo.ns, o = a.Lo.at(i0 + lx, j0 + ly);
     PointCoef<float> pc;
     point_prefetch(g, a.c, a.mwu, a.mwv, i0 + lx, j0 + ly, has_u, has_v, pc);
-    const float bu = pc.oku ? a.ru[pc.ku] : 0.f, bv = pc.okv ? a.rv[pc.kv] : 0.f;
+    const float bu = pc.oku ? a.bu[pc.ku] : 0.f, bv = pc.okv ? a.bv[pc.kv] : 0.f;
+    const float xu = pc.oku ? a.x0u[pc.ku] : 0.f, xv = pc.okv ? a.x0v[pc.kv] : 0.f;
     optile_apply(g, a.c, a.t0u, a.t0v, i0, j0, pc, s, tu, tv);
     haar_pm_tile2<float>(a.levels, 1, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
     float ru = 0.f, rv = 0.f;
     double b2u = 0.0, b2v = 0.0;
     if (pc.oku) {
         ru = bu - (pc.mwu ? tu[ly][lx] : 0.f);
-        a.ru[pc.ku] = ru;
-        a.pu[pc.ku] = 0.f;
-        a.su[pc.ku] = 0.f;
+        F[CGM_R * ns + o] = ru;
+        F[CGM_P * ns + o] = 0.f;
+        F[CGM_S * ns + o] = 0.f;
+        F[CGM_X * ns + o] = xu;
         b2u = (double)bu * (double)bu;
     }
     if (pc.okv) {
         rv = bv - (pc.mwv ? tv[ly][lx] : 0.f);
-        a.rv[pc.kv] = rv;
-        a.pv[pc.kv] = 0.f;
-        a.sv[pc.kv] = 0.f;
+        F[(CGM_R + 1) * ns + o] = rv;
+        F[(CGM_P + 1) * ns + o] = 0.f;
+        F[(CGM_S + 1) * ns + o] = 0.f;
+        F[(CGM_X + 1) * ns + o] = xv;
         b2v = (double)bv * (double)bv;
     }
     // each thread rewrites only its own entry; the transform ended with a barrier
@@ -1464,8 +1495,8 @@ __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
     tv[ly][lx] = pc.mwv ? rv : 0.f;
     __syncthreads();
     haar_pm_tile2<float>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
-    if (pc.oku) a.hru[0][pc.ku] = pc.ui ? tu[ly][lx] : 0.f;
-    if (pc.okv) a.hrv[0][pc.kv] = pc.vi ? tv[ly][lx] : 0.f;
+    if (pc.oku) F[CGM_RING * ns + o] = pc.ui ? tu[ly][lx] : 0.f;
+    if (pc.okv) F[(CGM_RING + 1) * ns + o] = pc.vi ? tv[ly][lx] : 0.f;
     tree256x2(b2u, b2v, lds);
     if (threadIdx.x == 0) {
         if (has_u) a.Pb[bx + g.nbxu * by] = b2u;
@@ -1473,15 +1504,6 @@ __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
     }
 }
 
-// Block Haar transforms of 16-point lines held one point per lane (pos = the
-// point's position in the line, blocks of B = 2^(levels+1) points) as the
-// butterflies that compose wavelet_matrix (src/Wavelets/update_wavelets.jl:
-// 38-89).  A pair is (o, e = o + h) with o a multiple of 2h.
-//   P (inverse), h = B/2 .. 1:           y_o = x_o - x_e,  y_e = x_o + x_e
-//   R (transposed inverse), h = 1 .. B/2: y_o = x_o + x_e,  y_e = x_e - x_o
-// Points past the end of the grid must hold zero: a pair whose e lies past
-// the end then leaves x_o unchanged, as wavelet_matrix does for a partial
-// block, and the garbage left in those points never reaches a point in range.
 // One stage as v + sign * partner with sign in {-1, 0, +1}: the product is
 // exact, so the fused multiply-add rounds exactly as the addition would.
 template <int H>
@@ -1498,6 +1520,15 @@ __device__ __forceinline__ float haar_stage_R(float v, int pos)
     return __builtin_fmaf(sg, row_partner<H>(v), v);
 }
 
+// Block Haar transforms of 16-point lines held one point per lane (pos = the
+// point's position in the line, blocks of B = 2^(LEV+1) points) as the
+// butterflies that compose wavelet_matrix (src/Wavelets/update_wavelets.jl:
+// 38-89).  A pair is (o, e = o + h) with o a multiple of 2h.
+//   P (inverse), h = B/2 .. 1:           y_o = x_o - x_e,  y_e = x_o + x_e
+//   R (transposed inverse), h = 1 .. B/2: y_o = x_o + x_e,  y_e = x_e - x_o
+// Points past the end of the grid must hold zero: a pair whose e lies past
+// the end then leaves x_o unchanged, as wavelet_matrix does for a partial
+// block, and the garbage left in those points never reaches a point in range.
 template <int LEV>
 __device__ __forceinline__ float haar_line_P(float v, int pos)
 {
@@ -1535,31 +1566,23 @@ __device__ __forceinline__ void halo_xy(int h, int &x, int &y)
     }
 }
 
-// Arguments of the iteration kernel.  All FP32 vectors of the mixed coarse CG
-// live in one arena: array id A, component c (0 u, 1 v) starts at
-// F + (A + c) * ns.
-enum CGMArray {
-    CGM_R = 0,      // residual r
-    CGM_W = 2,      // w = A_c r
-    CGM_P = 4,      // search direction p
-    CGM_S = 6,      // s = A_c p
-    CGM_X = 8,      // solution (warm start on entry)
-    CGM_HU = 10,    // thickness on u / v points, 0 off the masks
-    CGM_BU = 12,    // beta_eff on u / v points, 0 off the masks
-    CGM_RING = 14,  // rings: CGM_RING + 6 parity + 2 kind (0 P r, 1 P s, 2 P w)
-    CGM_CELL = 26,  // per h cell: (h*eta_av, dimp) pairs (two arrays' room)
-    CGM_CORNER = 28, // per h cell: corner h*eta_av (0 off the c grid and mask)
-    CGM_NARRAY = 29
-};
+// Sum over a 16-lane row in FP32, left in every lane of the row.
+__device__ __forceinline__ float row_sum16f(float v)
+{
+    v = v + dpp_f<DPP_XOR1>(v);
+    v = v + dpp_f<DPP_XOR2>(v);
+    v = v + dpp_f<DPP_HALF_MIRROR>(v);
+    v = v + dpp_f<DPP_MIRROR>(v);
+    return v;
+}
 
 struct CGMIterArgs {
-    float *F;
-    const uint8_t *M;       // mask bits (bit 0 inner, bit 1 wavelet mask): u at M, v at M + ns
-    double2 *Pgd;           // [2][tiles]: per tile (r, r) and (w, r) partials by parity
+    CGMLayout Lo;
+    float2 *Pgd;            // [2][npad]: per tile (r, r) and (w, r) partials by parity (0 past the tiles)
     CGMScalars *st;
     const double *Pb;       // partials of ||b||^2 per u / v block
     GridDims g;
-    unsigned ns;
+    int npad;               // partials per parity, a multiple of 1024
     float rdx, rdy;
     double tol_coarse, reltol;
     int maxiter, nblk;
@@ -1567,12 +1590,11 @@ struct CGMIterArgs {
 
 // One iteration (INIT: the launch before the first iteration, which only
 // computes w0 = A_c r0 and the rings of set 1: P r0, P s = 0, P w0, and the
-// partial (r0, r0), (w0, r0)).  All loads are issued up front from clamped,
-// always valid addresses, and masked afterwards, so that the memory round
-// trip is paid once.  Seven barriers: partial sums; block transforms of r
-// and s (x pass, transpose, y pass straight into the window); intermediates;
-// operator and R x pass; R y pass and P y pass; P x pass and the new
-// partials.
+// partial (r0, r0), (w0, r0)).  All loads are issued up front, unconditional
+// in the padded layout, so that the memory round trip is paid once.  Seven
+// barriers: partial sums; block transforms of r and s (x pass, transpose,
+// y pass straight into the window); intermediates; operator and R x pass;
+// R y pass and P y pass; P x pass and the new partials.
 template <int PAR, bool INIT, int LEV>
 __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
 {
@@ -1583,7 +1605,9 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
         float rxx[17][18], ryy[17][18], ext[17][18], rxy[17][18];
         float T[4][TILE][TILE + 1];      // transposes of the block transforms
         uint8_t mb[2][TILE][TILE + 1];   // mask bits of the tile
-        double red0[32], red1[32], red2[8];
+        double red0[32];
+        float red1[32];
+        double red2[8];
     };
     __shared__ Smem sm;
     const GridDims &g = a.g;
@@ -1592,100 +1616,83 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     if (L >= ntile) return;
     const int tid = threadIdx.x;
     const int bx = L % g.nbx, by = L / g.nbx;
-    const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
     const int i0 = bx * TILE, j0 = by * TILE;
     const int lx = tid & 15, ly = tid >> 4;    // row-major position
     const int cx = tid >> 4, cy = tid & 15;    // column-major position (transposes)
     constexpr int NP = 1 - PAR;
-    const unsigned ns = a.ns;
-    float *F = a.F;
+    const unsigned px = a.Lo.px, ns = a.Lo.ns;
+    float *F = a.Lo.F;
+    const uint8_t *M = a.Lo.M;
     const float rdx = a.rdx, rdy = a.rdy;
+    // Array A is the uniform base F + A ns; a point is a 32-bit byte offset
+    // into it (the scalar-base addressing of global loads and stores).
+    auto arr = [&](int A) { return F + (size_t)A * ns; };
+    auto el = [](float *base, unsigned boff) -> float & { return *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + boff); };
+    // window entry (x, y) is arena offset wb + x + px y
+    const unsigned wb = (unsigned)i0 + px * (unsigned)j0;
+    const unsigned o = wb + (unsigned)(lx + 1) + px * (unsigned)(ly + 1);   // own point
+    const unsigned ob = 4u * o;
     // ---- loads, all issued before the first use
     const CGMState S = a.st->s[PAR];
-    const int i = i0 + lx, j = j0 + ly;
-    const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
-    const unsigned ku = oku ? (unsigned)(i + g.nxu * j) : 0u;
-    const unsigned kv = okv ? (unsigned)(i + g.nxv * j) : 0u;
-    const uint8_t mbu0 = a.M[ku], mbv0 = a.M[ns + kv];
-    const float bu0 = F[CGM_BU * ns + ku], bv0 = F[(CGM_BU + 1) * ns + kv];
-    const float r_u0 = F[CGM_R * ns + ku], r_v0 = F[(CGM_R + 1) * ns + kv];
+    const uint8_t mbu = M[o], mbv = (M + ns)[o];
+    const float bu0 = el(arr(CGM_BU), ob), bv0 = el(arr(CGM_BU + 1), ob);
+    float r_u = el(arr(CGM_R), ob), r_v = el(arr(CGM_R + 1), ob);
     float w_u0 = 0.f, p_u0 = 0.f, s_u0 = 0.f, x_u0 = 0.f, w_v0 = 0.f, p_v0 = 0.f, s_v0 = 0.f, x_v0 = 0.f;
     if (!INIT) {
-        w_u0 = F[CGM_W * ns + ku];
-        w_v0 = F[(CGM_W + 1) * ns + kv];
-        p_u0 = F[CGM_P * ns + ku];
-        p_v0 = F[(CGM_P + 1) * ns + kv];
-        s_u0 = F[CGM_S * ns + ku];
-        s_v0 = F[(CGM_S + 1) * ns + kv];
-        x_u0 = F[CGM_X * ns + ku];
-        x_v0 = F[(CGM_X + 1) * ns + kv];
+        w_u0 = el(arr(CGM_W), ob);
+        w_v0 = el(arr(CGM_W + 1), ob);
+        p_u0 = el(arr(CGM_P), ob);
+        p_v0 = el(arr(CGM_P + 1), ob);
+        s_u0 = el(arr(CGM_S), ob);
+        s_v0 = el(arr(CGM_S + 1), ob);
+        x_u0 = el(arr(CGM_X), ob);
+        x_v0 = el(arr(CGM_X + 1), ob);
     }
     // halo of the window: threads 0..67 the u grid, 68..135 the v grid
     int hx, hy;
     const int hcomp = tid >= 68 ? 1 : 0;
     halo_xy(tid < 136 ? tid - 68 * hcomp : 0, hx, hy);
-    const int gi = i0 - 1 + hx, gj = j0 - 1 + hy;
-    const int hnx = hcomp ? g.nxv : g.nxu, hny = hcomp ? g.nyv : g.nyu;
-    const bool hin = tid < 136 && gi >= 0 && gi < hnx && gj >= 0 && gj < hny;
-    const unsigned hk = (CGM_RING + 6 * PAR + hcomp) * ns + (hin ? (unsigned)(gi + hnx * gj) : 0u);
-    const float hr0 = F[hk];
-    float hs0 = 0.f, hw0 = 0.f;
+    const unsigned hb = 4u * (hcomp * ns + wb + (unsigned)hx + px * (unsigned)hy);
+    const float hr = el(arr(CGM_RING + 6 * PAR), hb);
+    float hs = 0.f, hw = 0.f;
     if (!INIT) {
-        hs0 = F[hk + 2 * ns];
-        hw0 = F[hk + 4 * ns];
+        hs = el(arr(CGM_RING + 6 * PAR + 2), hb);
+        hw = el(arr(CGM_RING + 6 * PAR + 4), hb);
     }
-    // thickness windows, entries q = tid and tid + 256 of the 18 x 18 window
-    bool tin[4];
-    float th[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int q = tid + 256 * (e >> 1), comp = e & 1;
-        const int x = q % 18, y = q / 18, ii = i0 - 1 + x, jj = j0 - 1 + y;
-        const int nxw = comp ? g.nxv : g.nxu, nyw = comp ? g.nyv : g.nyu;
-        tin[e] = q < 324 && ii >= 0 && ii < nxw && jj >= 0 && jj < nyw;
-        th[e] = F[(CGM_HU + comp) * ns + (tin[e] ? (unsigned)(ii + nxw * jj) : 0u)];
-    }
-    // cell coefficients, cells q = tid and tid + 256 of the 17 x 17 window
-    bool cin[2];
-    float2 cc[2];
-    float ch[2];
-    const float2 *cell2 = reinterpret_cast<const float2 *>(F + CGM_CELL * ns);
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-        const int q = tid + 256 * e;
-        const int x = q % 17, y = q / 17, ca = i0 - 1 + x, cb = j0 - 1 + y;
-        cin[e] = q < 289 && ca >= 0 && ca < g.nx && cb >= 0 && cb < g.ny;
-        const unsigned kc = cin[e] ? (unsigned)(ca + g.nx * cb) : 0u;
-        cc[e] = cell2[kc];
-        ch[e] = F[CGM_CORNER * ns + kc];
-    }
-    // partials of the previous launch (first 1024 tiles; larger grids loop below)
-    double2 pv[4];
+    // thickness windows, entries q = tid and tid + 256 (< 324) of the 18 x 18 window
+    const int q1 = tid + 256 < 324 ? tid + 256 : tid;
+    const unsigned t0 = 4u * (wb + (unsigned)(tid % 18) + px * (unsigned)(tid / 18));
+    const unsigned t1 = 4u * (wb + (unsigned)(q1 % 18) + px * (unsigned)(q1 / 18));
+    const float thu0 = el(arr(CGM_HU), t0), thv0 = el(arr(CGM_HU + 1), t0);
+    const float thu1 = el(arr(CGM_HU), t1), thv1 = el(arr(CGM_HU + 1), t1);
+    // cell coefficients, cells q = tid and tid + 256 (< 289) of the 17 x 17 window
+    const int c1q = tid + 256 < 289 ? tid + 256 : tid;
+    const unsigned c0o = wb + (unsigned)(tid % 17) + px * (unsigned)(tid / 17);
+    const unsigned c1o = wb + (unsigned)(c1q % 17) + px * (unsigned)(c1q / 17);
+    const char *cell2 = reinterpret_cast<const char *>(arr(CGM_CELL));
+    const float2 cc0 = *reinterpret_cast<const float2 *>(cell2 + 8u * c0o);
+    const float2 cc1 = *reinterpret_cast<const float2 *>(cell2 + 8u * c1o);
+    const float ch0 = el(arr(CGM_CORNER), 4u * c0o), ch1 = el(arr(CGM_CORNER), 4u * c1o);
+    // partials of the previous launch
+    float2 pv[4];
     if (!INIT) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int q = tid + 256 * e;
-            pv[e] = a.Pgd[PAR * ntile + (q < ntile ? q : 0)];
-        }
+        for (int e = 0; e < 4; ++e) pv[e] = a.Pgd[PAR * a.npad + tid + 256 * e];
     }
-    // ---- masks
-    const uint8_t mbu = oku ? mbu0 : 0, mbv = okv ? mbv0 : 0;
     const bool inu = mbu & 1, inv = mbv & 1, mwu = mbu & 2, mwv = mbv & 2;
-    float r_u = oku ? r_u0 : 0.f, r_v = okv ? r_v0 : 0.f;
-    // ---- scalars from the previous launch's partials
+    // ---- scalars from the previous launch's partials (FP64 across workgroups)
     float fa = 0.f, fb = 0.f;
     if (!INIT) {
         double gamma = 0.0, delta = 0.0;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const bool ok = tid + 256 * e < ntile;
-            gamma = gamma + (ok ? pv[e].x : 0.0);
-            delta = delta + (ok ? pv[e].y : 0.0);
+            gamma = gamma + (double)pv[e].x;
+            delta = delta + (double)pv[e].y;
         }
-        for (int q = tid + 1024; q < ntile; q += 256) {
-            const double2 v = a.Pgd[PAR * ntile + q];
-            gamma = gamma + v.x;
-            delta = delta + v.y;
+        for (int q = tid + 1024; q < a.npad; q += 256) {
+            const float2 v = a.Pgd[PAR * a.npad + q];
+            gamma = gamma + (double)v.x;
+            delta = delta + (double)v.y;
         }
         wg_sum2(gamma, delta, sm.red0);
         if (S.done) {
@@ -1725,28 +1732,22 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
         fa = (float)alpha;
         fb = (float)beta;
     }
-    // ---- vector updates on the tile
+    // ---- vector updates on the tile (zero past the ends of the grids)
     float s_u = 0.f, s_v = 0.f;
     if (!INIT) {
         const float p_u = r_u + fb * p_u0, p_v = r_v + fb * p_v0;
         s_u = w_u0 + fb * s_u0;
         s_v = w_v0 + fb * s_v0;
-        if (!oku) s_u = 0.f;
-        if (!okv) s_v = 0.f;
-        if (oku) {
-            F[CGM_P * ns + ku] = p_u;
-            F[CGM_S * ns + ku] = s_u;
-            F[CGM_X * ns + ku] = x_u0 + fa * p_u;
-            r_u = r_u - fa * s_u;
-            F[CGM_R * ns + ku] = r_u;
-        }
-        if (okv) {
-            F[(CGM_P + 1) * ns + kv] = p_v;
-            F[(CGM_S + 1) * ns + kv] = s_v;
-            F[(CGM_X + 1) * ns + kv] = x_v0 + fa * p_v;
-            r_v = r_v - fa * s_v;
-            F[(CGM_R + 1) * ns + kv] = r_v;
-        }
+        r_u = r_u - fa * s_u;
+        r_v = r_v - fa * s_v;
+        el(arr(CGM_P), ob) = p_u;
+        el(arr(CGM_P + 1), ob) = p_v;
+        el(arr(CGM_S), ob) = s_u;
+        el(arr(CGM_S + 1), ob) = s_v;
+        el(arr(CGM_X), ob) = x_u0 + fa * p_u;
+        el(arr(CGM_X + 1), ob) = x_v0 + fa * p_v;
+        el(arr(CGM_R), ob) = r_u;
+        el(arr(CGM_R + 1), ob) = r_v;
     }
     // ---- x pass of P r and P s; halo and thickness windows into LDS
     sm.T[0][ly][lx] = haar_line_P<LEV>(mwu ? r_u : 0.f, lx);
@@ -1758,23 +1759,17 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     sm.mb[0][ly][lx] = mbu;
     sm.mb[1][ly][lx] = mbv;
     if (tid < 136) {
-        const float hr = hin ? hr0 : 0.f, hs = hin ? hs0 : 0.f, hw = hin ? hw0 : 0.f;
         const float t = INIT ? hr : hr - fa * (hw + fb * hs);
         if (!hcomp)
             sm.U[hy][hx] = t;
         else
             sm.V[hy][hx] = t;
     }
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int q = tid + 256 * (e >> 1);
-        if (q < 324) {
-            const float v = tin[e] ? th[e] : 0.f;
-            if (e & 1)
-                sm.HV[q / 18][q % 18] = v;
-            else
-                sm.HU[q / 18][q % 18] = v;
-        }
+    sm.HU[tid / 18][tid % 18] = thu0;
+    sm.HV[tid / 18][tid % 18] = thv0;
+    if (tid + 256 < 324) {
+        sm.HU[q1 / 18][q1 % 18] = thu1;
+        sm.HV[q1 / 18][q1 % 18] = thv1;
     }
     __syncthreads();
     // ---- y pass in the column-major lanes: P r into the window, rings of P r and P s
@@ -1793,18 +1788,11 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
             ps_v = (m1 & 1) ? y3 : 0.f;
         }
         if (cx == 0 || cx == 15 || cy == 0 || cy == 15) {
-            const int ic = i0 + cx, jc = j0 + cy;
-            const unsigned rb = (CGM_RING + 6 * NP) * ns;
-            if (has_u && ic < g.nxu && jc < g.nyu) {
-                const unsigned k = rb + (unsigned)(ic + g.nxu * jc);
-                F[k] = pr_u;
-                F[k + 2 * ns] = ps_u;
-            }
-            if (has_v && ic < g.nxv && jc < g.nyv) {
-                const unsigned k = rb + ns + (unsigned)(ic + g.nxv * jc);
-                F[k] = pr_v;
-                F[k + 2 * ns] = ps_v;
-            }
+            const unsigned kb = 4u * (wb + (unsigned)(cx + 1) + px * (unsigned)(cy + 1));
+            el(arr(CGM_RING + 6 * NP), kb) = pr_u;
+            el(arr(CGM_RING + 6 * NP + 1), kb) = pr_v;
+            el(arr(CGM_RING + 6 * NP + 2), kb) = ps_u;
+            el(arr(CGM_RING + 6 * NP + 3), kb) = ps_v;
         }
     }
     __syncthreads();
@@ -1813,38 +1801,33 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     for (int e = 0; e < 2; ++e) {
         const int q = tid + 256 * e;
         if (q < 289) {
-            const float4 c = cin[e] ? make_float4(cc[e].x, cc[e].y, ch[e], 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float hh = e ? cc1.x : cc0.x, dimp = e ? cc1.y : cc0.y, hc = e ? ch1 : ch0;
             const int x = q % 17, y = q / 17;
             const float u0 = sm.U[y][x], u1 = sm.U[y][x + 1], v0 = sm.V[y][x], v1 = sm.V[y + 1][x];
             const float dudx = rdx * (u1 - u0), dvdy = rdy * (v1 - v0);
-            sm.rxx[y][x] = 2.f * c.x * (2.f * dudx + dvdy);
-            sm.ryy[y][x] = 2.f * c.x * (2.f * dvdy + dudx);
+            sm.rxx[y][x] = 2.f * hh * (2.f * dudx + dvdy);
+            sm.ryy[y][x] = 2.f * hh * (2.f * dvdy + dudx);
             const float divq = rdx * (sm.HU[y][x + 1] * u1 - sm.HU[y][x] * u0) +
                                rdy * (sm.HV[y + 1][x] * v1 - sm.HV[y][x] * v0);
-            sm.ext[y][x] = c.y * divq;
-            sm.rxy[y][x] = c.z * (rdy * (sm.U[y + 1][x + 1] - sm.U[y][x + 1]) +
-                                  rdx * (sm.V[y + 1][x + 1] - sm.V[y + 1][x]));
+            sm.ext[y][x] = dimp * divq;
+            sm.rxy[y][x] = hc * (rdy * (sm.U[y + 1][x + 1] - sm.U[y][x + 1]) +
+                                 rdx * (sm.V[y + 1][x + 1] - sm.V[y + 1][x]));
         }
     }
     __syncthreads();
     // ---- A at the tile's points (zero off the inner masks), R x pass
     {
         const int x = lx, y = ly;
-        float Au = 0.f, Av = 0.f;
-        if (inu) {
-            const float d_rxx = rdx * (sm.rxx[y + 1][x + 1] - sm.rxx[y + 1][x]);
-            const float d_rxy = rdy * (sm.rxy[y + 1][x] - sm.rxy[y][x]);
-            const float d_ext = rdx * (sm.ext[y + 1][x + 1] - sm.ext[y + 1][x]);
-            Au = ((d_rxx + d_rxy) - bu0 * sm.U[y + 1][x + 1]) - sm.HU[y + 1][x + 1] * d_ext;
-        }
-        if (inv) {
-            const float d_ryy = rdy * (sm.ryy[y + 1][x + 1] - sm.ryy[y][x + 1]);
-            const float d_rxy = rdx * (sm.rxy[y][x + 1] - sm.rxy[y][x]);
-            const float d_ext = rdy * (sm.ext[y + 1][x + 1] - sm.ext[y][x + 1]);
-            Av = ((d_ryy + d_rxy) - bv0 * sm.V[y + 1][x + 1]) - sm.HV[y + 1][x + 1] * d_ext;
-        }
-        sm.T[0][ly][lx] = haar_line_R<LEV>(Au, lx);
-        sm.T[1][ly][lx] = haar_line_R<LEV>(Av, lx);
+        const float d_rxx = rdx * (sm.rxx[y + 1][x + 1] - sm.rxx[y + 1][x]);
+        const float d_rxy_u = rdy * (sm.rxy[y + 1][x] - sm.rxy[y][x]);
+        const float d_ext_u = rdx * (sm.ext[y + 1][x + 1] - sm.ext[y + 1][x]);
+        const float Au = ((d_rxx + d_rxy_u) - bu0 * sm.U[y + 1][x + 1]) - sm.HU[y + 1][x + 1] * d_ext_u;
+        const float d_ryy = rdy * (sm.ryy[y + 1][x + 1] - sm.ryy[y][x + 1]);
+        const float d_rxy_v = rdx * (sm.rxy[y][x + 1] - sm.rxy[y][x]);
+        const float d_ext_v = rdy * (sm.ext[y + 1][x + 1] - sm.ext[y][x + 1]);
+        const float Av = ((d_ryy + d_rxy_v) - bv0 * sm.V[y + 1][x + 1]) - sm.HV[y + 1][x + 1] * d_ext_v;
+        sm.T[0][ly][lx] = haar_line_R<LEV>(inu ? Au : 0.f, lx);
+        sm.T[1][ly][lx] = haar_line_R<LEV>(inv ? Av : 0.f, lx);
     }
     __syncthreads();
     // ---- R y pass, wavelet mask (w), P y pass
@@ -1853,30 +1836,40 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
         const float c1 = haar_line_R<LEV>(sm.T[1][cy][cx], cy);
         const float w0 = (sm.mb[0][cy][cx] & 2) ? c0 : 0.f;
         const float w1 = (sm.mb[1][cy][cx] & 2) ? c1 : 0.f;
-        const float q0 = haar_line_P<LEV>(w0, cy), q1 = haar_line_P<LEV>(w1, cy);
+        const float q0 = haar_line_P<LEV>(w0, cy), q1v = haar_line_P<LEV>(w1, cy);
         sm.T[0][cy][cx] = w0;
         sm.T[1][cy][cx] = w1;
         sm.T[2][cy][cx] = q0;
-        sm.T[3][cy][cx] = q1;
+        sm.T[3][cy][cx] = q1v;
     }
     __syncthreads();
     // ---- P x pass, stores, partials of (r, r) and (w, r)
     const float wn_u = sm.T[0][ly][lx], wn_v = sm.T[1][ly][lx];
     const float tw_u = haar_line_P<LEV>(sm.T[2][ly][lx], lx), tw_v = haar_line_P<LEV>(sm.T[3][ly][lx], lx);
-    const bool ring = lx == 0 || lx == 15 || ly == 0 || ly == 15;
-    const unsigned wb = (CGM_RING + 6 * NP + 4) * ns;
-    if (oku) {
-        F[CGM_W * ns + ku] = wn_u;
-        if (ring) F[wb + ku] = inu ? tw_u : 0.f;
+    el(arr(CGM_W), ob) = wn_u;
+    el(arr(CGM_W + 1), ob) = wn_v;
+    if (lx == 0 || lx == 15 || ly == 0 || ly == 15) {
+        el(arr(CGM_RING + 6 * NP + 4), ob) = inu ? tw_u : 0.f;
+        el(arr(CGM_RING + 6 * NP + 5), ob) = inv ? tw_v : 0.f;
     }
-    if (okv) {
-        F[(CGM_W + 1) * ns + kv] = wn_v;
-        if (ring) F[wb + ns + kv] = inv ? tw_v : 0.f;
+    float gg = r_u * r_u + r_v * r_v;
+    float dd = wn_u * r_u + wn_v * r_v;
+    gg = row_sum16f(gg);
+    dd = row_sum16f(dd);
+    if (lx == 0) {
+        sm.red1[ly] = gg;
+        sm.red1[16 + ly] = dd;
     }
-    double gg = (double)r_u * (double)r_u + (double)r_v * (double)r_v;
-    double dd = (double)wn_u * (double)r_u + (double)wn_v * (double)r_v;
-    wg_sum2(gg, dd, sm.red1);
-    if (tid == 0) a.Pgd[NP * ntile + L] = make_double2(gg, dd);
+    __syncthreads();
+    if (tid == 0) {
+        float x[4], y[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            x[k] = (sm.red1[4 * k] + sm.red1[4 * k + 1]) + (sm.red1[4 * k + 2] + sm.red1[4 * k + 3]);
+            y[k] = (sm.red1[16 + 4 * k] + sm.red1[16 + 4 * k + 1]) + (sm.red1[16 + 4 * k + 2] + sm.red1[16 + 4 * k + 3]);
+        }
+        a.Pgd[NP * a.npad + L] = make_float2((x[0] + x[1]) + (x[2] + x[3]), (y[0] + y[1]) + (y[2] + y[3]));
+    }
 }
 
 // out = mask_out ? H(mask_in ? in : 0) : 0 on the u grid (comp 0) and v grid
