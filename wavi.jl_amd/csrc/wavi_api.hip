@@ -67,7 +67,9 @@ struct wavi_ctx {
     float2 *Pgd = nullptr;  // [2][npad] (r, r) and (w, r) partials by parity
     int npad = 0;
     CGMScalars *stm = nullptr;
-    CGMScalars *h_stm = nullptr;  // pinned host copy
+    CGMHostFlag *h_flag = nullptr;  // pinned, host-mapped stop report of the mixed coarse CG
+    CGMHostFlag *d_flag = nullptr;  // its device address
+    hipEvent_t cg_ev[3] = {};       // chunk completion events of the mixed coarse CG
     double *Pr0 = nullptr, *Pr1 = nullptr, *Pd = nullptr, *Pb = nullptr, *Pn = nullptr;
     double *quad = nullptr;  // 4 * nh pos_fraction quadrants
     double *norms = nullptr; // device: 2 doubles
@@ -461,6 +463,9 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
     CREATECHK(hipMemset(ctx->st, 0, sizeof(CGScalars)));
     CREATECHK(hipHostMalloc((void **)&ctx->h_st, sizeof(CGScalars), hipHostMallocDefault));
     ctx->nblk = ((g.nbx * g.nby + 7) / 8) * 8;
+    CREATECHK(hipHostMalloc((void **)&ctx->h_flag, sizeof(CGMHostFlag), hipHostMallocMapped | hipHostMallocCoherent));
+    CREATECHK(hipHostGetDevicePointer((void **)&ctx->d_flag, ctx->h_flag, 0));
+    for (hipEvent_t &e : ctx->cg_ev) CREATECHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     if (precision_mode == WAVI_PRECISION_MIXED) {
         VSet<float> &w = ctx->v32;
         auto falloc = [&](float **p, size_t n) -> hipError_t {
@@ -485,7 +490,6 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
         CREATECHK(hipMemset(ctx->Pgd, 0, 2 * (size_t)ctx->npad * sizeof(float2)));
         CREATECHK(hipMalloc(&ctx->stm, sizeof(CGMScalars)));
         CREATECHK(hipMemset(ctx->stm, 0, sizeof(CGMScalars)));
-        CREATECHK(hipHostMalloc((void **)&ctx->h_stm, sizeof(CGMScalars), hipHostMallocDefault));
     }
     {
         HaarMasks hm_tab[4];
@@ -532,7 +536,9 @@ int wavi_ctx_destroy(wavi_ctx *ctx)
     for (void *p : {(void *)ctx->arena, (void *)ctx->marena, (void *)ctx->Pgd})
         if (p) (void)hipFree(p);
     if (ctx->stm) (void)hipFree(ctx->stm);
-    if (ctx->h_stm) (void)hipHostFree(ctx->h_stm);
+    if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
+    for (hipEvent_t e : ctx->cg_ev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->st) (void)hipFree(ctx->st);
     if (ctx->tab) (void)hipFree(ctx->tab);
     if (ctx->h_norms) (void)hipHostFree(ctx->h_norms);
@@ -870,6 +876,7 @@ static CGArgs<T> cg_args(wavi_ctx *ctx, const VSet<T> &w)
     a.Pd = ctx->Pd;
     a.Pb = ctx->Pb;
     a.st = ctx->st;
+    a.hflag = ctx->d_flag;
     a.tol_coarse = ctx->S.tol_coarse;
     a.reltol = std::sqrt(2.220446049250313e-16);  // IterativeSolvers' default sqrt(eps)
     a.maxiter = ctx->S.maxiter_coarse;
@@ -887,13 +894,21 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
     // t = P x0
     hipLaunchKernelGGL((k_tile_haar<T, T>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1, a.xu, a.xv,
                        ctx->mwu, ctx->mwv, a.tu, a.tv, ctx->ui, ctx->vi);
+    ctx->h_flag->done = 0;
+    ctx->h_flag->iters = 0;
     hipLaunchKernelGGL(k_cg_setup<T>, tb, dim3(256), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
+    // Triplet q performs iteration q unless the CG has stopped; triplet
+    // maxiter always stops, so no more are needed.  Chunks of 32 triplets are
+    // queued while at most two chunks are in flight; the host reads the stop
+    // report between chunks and queues no more after a stop.
+    volatile CGMHostFlag *hf = ctx->h_flag;
     const int CHUNK = 32;
-    int q = 0;
-    const int qmax = ctx->S.maxiter_coarse + 1;
-    while (true) {
-        for (int s = 0; s < CHUNK; ++s, ++q) {
+    const int nq = ctx->S.maxiter_coarse + 1;
+    int q = 0, k = 0;
+    while (q < nq) {
+        const int n = std::min(CHUNK, nq - q);
+        for (int s = 0; s < n; ++s, ++q) {
             const int par = q & 1;
             {
                 ProfScope ps(ctx, WAVI_KT_CG_DIR);
@@ -909,16 +924,20 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
             }
         }
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(ctx->h_st, ctx->st, sizeof(CGScalars), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        prof_collect(ctx);
-        if (ctx->h_st->done) break;
-        if (q > qmax + CHUNK) {
-            ctx->err = "coarse CG did not terminate";
-            return WAVI_ESTATE;
+        HIPCHK(hipEventRecord(ctx->cg_ev[k % 3], ctx->stream));
+        if (k >= 2) {
+            HIPCHK(hipEventSynchronize(ctx->cg_ev[(k - 2) % 3]));
+            if (hf->done) break;
         }
+        ++k;
     }
-    *iters = ctx->h_st->iters;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    if (!hf->done) {
+        ctx->err = "coarse CG did not terminate";
+        return WAVI_ESTATE;
+    }
+    *iters = hf->iters;
     return WAVI_OK;
 }
 
@@ -975,11 +994,14 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     sa.Pb = ctx->Pb;
     sa.st = ctx->stm;
     sa.nblk = ctx->nblk;
+    ctx->h_flag->done = 0;
+    ctx->h_flag->iters = 0;
     hipLaunchKernelGGL(k_cgm_setup, tb, dim3(256), 0, ctx->stream, sa);
     CGMIterArgs ia;
     ia.Lo = Lo;
     ia.Pgd = ctx->Pgd;
     ia.st = ctx->stm;
+    ia.hflag = ctx->d_flag;
     ia.Pb = ctx->Pb;
     ia.g = g;
     ia.npad = ctx->npad;
@@ -991,11 +1013,17 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     ia.nblk = ctx->nblk;
     launch_cgm_iter<0, true>(ctx, tb, ia);
     HIPCHK(hipGetLastError());
+    // Launch q (q >= 1) performs iteration q - 1 unless the CG has stopped;
+    // launch maxiter + 1 always stops, so no more are needed.  Chunks of 32
+    // launches are queued while at most two chunks are in flight; the host
+    // reads the stop report between chunks and queues no more after a stop.
+    volatile CGMHostFlag *hf = ctx->h_flag;
     const int CHUNK = 32;
-    int q = 1;  // launch index; launch q reads the set of parity q & 1
-    const int qmax = ctx->S.maxiter_coarse + 2;
-    while (true) {
-        for (int s = 0; s < CHUNK; ++s, ++q) {
+    const int qmax = ctx->S.maxiter_coarse + 1;
+    int q = 1, k = 0;
+    while (q <= qmax) {
+        const int n = std::min(CHUNK, qmax - q + 1);
+        for (int s = 0; s < n; ++s, ++q) {
             ProfScope ps(ctx, WAVI_KT_CG_ITER_MIXED);
             if (q & 1)
                 launch_cgm_iter<1, false>(ctx, tb, ia);
@@ -1003,19 +1031,23 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
                 launch_cgm_iter<0, false>(ctx, tb, ia);
         }
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(ctx->h_stm, ctx->stm, sizeof(CGMScalars), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        prof_collect(ctx);
-        if (ctx->h_stm->done) break;
-        if (q > qmax + CHUNK) {
-            ctx->err = "coarse CG did not terminate";
-            return WAVI_ESTATE;
+        HIPCHK(hipEventRecord(ctx->cg_ev[k % 3], ctx->stream));
+        if (k >= 2) {
+            HIPCHK(hipEventSynchronize(ctx->cg_ev[(k - 2) % 3]));
+            if (hf->done) break;
         }
+        ++k;
     }
     hipLaunchKernelGGL(k_cgm_unpack, flat(std::max(ctx->nu, ctx->nv)), dim3(256), 0, ctx->stream, g, Lo, w.xc.u,
                        w.xc.v);
     HIPCHK(hipGetLastError());
-    *iters = ctx->h_stm->iters;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    if (!hf->done) {
+        ctx->err = "coarse CG did not terminate";
+        return WAVI_ESTATE;
+    }
+    *iters = hf->iters;
     return WAVI_OK;
 }
 

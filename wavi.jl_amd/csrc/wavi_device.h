@@ -235,27 +235,54 @@ __device__ __forceinline__ double row_sum16(double v)
     return v;
 }
 
-// Workgroup sums of a and b (256 threads): row sums by DPP, then the 16 row
-// sums through LDS (red holds 32 doubles and must not be written again
+// DPP with a row mask: rows not in ROWMASK read 0.
+constexpr int DPP_ROW_BCAST15 = 0x142; // lane 15 of each row to the next row
+constexpr int DPP_ROW_BCAST31 = 0x143; // lane 31 to rows 2 and 3
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_rows_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xF, false));
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_rows_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Wavefront total from row sums (every lane of a row holding its row's sum):
+// rows 1 and 3 add rows 0 and 2, then rows 2 and 3 add rows 0 + 1; lane 63
+// ends with (r3 + r2) + (r1 + r0).
+__device__ __forceinline__ double wave_total_rows(double v)
+{
+    v = v + dpp_rows_d<DPP_ROW_BCAST15, 0xA>(v);
+    return v + dpp_rows_d<DPP_ROW_BCAST31, 0xC>(v);
+}
+
+__device__ __forceinline__ float wave_total_rows_f(float v)
+{
+    v = v + dpp_rows_f<DPP_ROW_BCAST15, 0xA>(v);
+    return v + dpp_rows_f<DPP_ROW_BCAST31, 0xC>(v);
+}
+
+// Workgroup sums of a and b (256 threads): wavefront totals by DPP, the four
+// totals through LDS (red holds 8 doubles and must not be written again
 // before the next barrier).  Every thread gets the same totals.
 __device__ __forceinline__ void wg_sum2(double &a, double &b, double *red)
 {
-    a = row_sum16(a);
-    b = row_sum16(b);
-    const int row = threadIdx.x >> 4;
-    if ((threadIdx.x & 15) == 0) {
-        red[row] = a;
-        red[16 + row] = b;
+    a = wave_total_rows(row_sum16(a));
+    b = wave_total_rows(row_sum16(b));
+    if ((threadIdx.x & 63) == 63) {
+        red[threadIdx.x >> 6] = a;
+        red[4 + (threadIdx.x >> 6)] = b;
     }
     __syncthreads();
-    double x[4], y[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        x[k] = (red[4 * k] + red[4 * k + 1]) + (red[4 * k + 2] + red[4 * k + 3]);
-        y[k] = (red[16 + 4 * k] + red[16 + 4 * k + 1]) + (red[16 + 4 * k + 2] + red[16 + 4 * k + 3]);
-    }
-    a = (x[0] + x[1]) + (x[2] + x[3]);
-    b = (y[0] + y[1]) + (y[2] + y[3]);
+    a = (red[0] + red[1]) + (red[2] + red[3]);
+    b = (red[4] + red[5]) + (red[6] + red[7]);
 }
 
 // ---------------------------------------------------------------------------

@@ -1075,6 +1075,14 @@ __device__ __forceinline__ void optile_apply(const GridDims &g, const OpCoef<T> 
 // parity and every workgroup writes the same value; a kernel never reads a
 // scalar that another workgroup of the same kernel writes.
 
+// Stop report of a coarse CG in host-mapped pinned memory: written once, by
+// the launch that stops, so the host learns of the stop without a copy on
+// the stream.
+struct CGMHostFlag {
+    int done;
+    int iters;
+};
+
 struct CGScalars {
     double res[2];    // residual norm by iteration parity (written by k_cg_dir)
     double tol;       // max(reltol ||r0||, abstol)
@@ -1096,6 +1104,7 @@ struct CGArgs {
     T *tu, *tv;       // P u on the fine grid
     double *Pr, *Pd, *Pb;
     CGScalars *st;
+    CGMHostFlag *hflag;   // host-mapped stop report
     double tol_coarse, reltol;
     int maxiter;
     int nblk;         // launched workgroups, a multiple of 8
@@ -1219,6 +1228,10 @@ __global__ void __launch_bounds__(256) k_cg_dir(CGArgs<T> a, int par)
             a.st->done = 1;
             a.st->iters = it;
             a.st->it[1 - par] = it;
+            if (L == 0) {
+                a.hflag->iters = it;
+                a.hflag->done = 1;
+            }
         }
     }
     if (done) return;
@@ -1580,6 +1593,7 @@ struct CGMIterArgs {
     CGMLayout Lo;
     float2 *Pgd;            // [2][npad]: per tile (r, r) and (w, r) partials by parity (0 past the tiles)
     CGMScalars *st;
+    CGMHostFlag *hflag;     // host-mapped stop report
     const double *Pb;       // partials of ||b||^2 per u / v block
     GridDims g;
     int npad;               // partials per parity, a multiple of 1024
@@ -1605,8 +1619,8 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
         float rxx[17][18], ryy[17][18], ext[17][18], rxy[17][18];
         float T[4][TILE][TILE + 1];      // transposes of the block transforms
         uint8_t mb[2][TILE][TILE + 1];   // mask bits of the tile
-        double red0[32];
-        float red1[32];
+        double red0[8];
+        float red1[8];
         double red2[8];
     };
     __shared__ Smem sm;
@@ -1715,6 +1729,8 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
                 a.st->s[NP] = n;
                 a.st->done = 1;
                 a.st->iters = S.it;
+                a.hflag->iters = S.it;
+                a.hflag->done = 1;
             }
             return;
         }
@@ -1854,22 +1870,16 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     }
     float gg = r_u * r_u + r_v * r_v;
     float dd = wn_u * r_u + wn_v * r_v;
-    gg = row_sum16f(gg);
-    dd = row_sum16f(dd);
-    if (lx == 0) {
-        sm.red1[ly] = gg;
-        sm.red1[16 + ly] = dd;
+    gg = wave_total_rows_f(row_sum16f(gg));
+    dd = wave_total_rows_f(row_sum16f(dd));
+    if ((tid & 63) == 63) {
+        sm.red1[tid >> 6] = gg;
+        sm.red1[4 + (tid >> 6)] = dd;
     }
     __syncthreads();
-    if (tid == 0) {
-        float x[4], y[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            x[k] = (sm.red1[4 * k] + sm.red1[4 * k + 1]) + (sm.red1[4 * k + 2] + sm.red1[4 * k + 3]);
-            y[k] = (sm.red1[16 + 4 * k] + sm.red1[16 + 4 * k + 1]) + (sm.red1[16 + 4 * k + 2] + sm.red1[16 + 4 * k + 3]);
-        }
-        a.Pgd[NP * a.npad + L] = make_float2((x[0] + x[1]) + (x[2] + x[3]), (y[0] + y[1]) + (y[2] + y[3]));
-    }
+    if (tid == 0)
+        a.Pgd[NP * a.npad + L] = make_float2((sm.red1[0] + sm.red1[1]) + (sm.red1[2] + sm.red1[3]),
+                                             (sm.red1[4] + sm.red1[5]) + (sm.red1[6] + sm.red1[7]));
 }
 
 // out = mask_out ? H(mask_in ? in : 0) : 0 on the u grid (comp 0) and v grid
