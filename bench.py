@@ -1,13 +1,16 @@
 """Benchmark: velocity solves per second of WAVI's update_velocities! on one
 MI355X at 1280 x 160 x 8 (the channel single-solve workload: 20 Picard
-iterations to 1e-8, coarse CG to 1e-8, 3 Haar levels), with the inputs
-resident in HBM.
+iterations to 1e-8, coarse CG to 1e-8 with at most 1000 iterations, 3 Haar
+levels), with the inputs resident in HBM.
 
 A step is one update_velocities! from a fixed reset state (velocities,
 viscosity, drag, quadratures, warm start and wavelet masks restored on the
-device before every solve, outside the timed interval).  Prints one JSON
-line.  --precision mixed runs the FP32 V-cycle inside FP64 refinement and
-also reports its velocity error against an FP64 solve of the same state.
+device before every solve, outside the timed interval).  The headline value
+is the mixed-precision solve (FP32 V-cycle with the single-launch coarse CG,
+FP64 residual refinement); the FP64 solve of the same state is timed beside
+it and gives the speedup and the velocity error of the mixed solve.  Prints
+one JSON line.  --precision fp64 or mixed times one of them alone (for
+profiling runs).
 """
 import argparse
 import ctypes
@@ -21,6 +24,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
+METRIC = "velocity solves/s at 1280x160x8, mixed vs FP64 GPU @1e-6 rel err"
 
 
 def _package():
@@ -39,25 +43,29 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def sizes(nx, ny):
+    return (nx + 1) * ny, nx * (ny + 1), nx * ny, (nx - 1) * (ny - 1)
+
+
 def cg_apply_bytes(nx, ny, w=8):
-    """Algorithmic HBM bytes of one k_cg_apply launch (A then R on a tile):
-    t_u, t_v in; c_u, c_v out; u_u, u_v in for dot(u, c); beta_eff on u, v;
-    thickness on u, v; h*eta_av on h; dimp on h; h*eta_av on c (w bytes
-    each); masks (one byte each): inner u, v; wavelet u, v; u, v masks; h and
-    c masks."""
-    nu, nv, nh, nc = (nx + 1) * ny, nx * (ny + 1), nx * ny, (nx - 1) * (ny - 1)
+    """Algorithmic HBM bytes of one k_cg_apply launch (FP64 coarse CG, A then
+    R on a tile): t_u, t_v in; c_u, c_v out; u_u, u_v in for dot(u, c);
+    beta_eff on u, v; thickness on u, v; h*eta_av on h; dimp on h; h*eta_av
+    on c (w bytes each); masks (one byte each): inner u, v; wavelet u, v;
+    u, v masks; h and c masks."""
+    nu, nv, nh, nc = sizes(nx, ny)
     return w * (5 * (nu + nv) + 2 * nh + nc) + 3 * (nu + nv) + nh + nc
 
 
 def cgm_iter_bytes(nx, ny, w=4):
     """Algorithmic HBM bytes of one k_cgm_iter launch (one whole iteration of
-    the mixed-precision coarse CG): r, w, p, s, x on the tile and the images
-    P r, P s, P w in; p, s, x, r, w and the three images out; beta_eff and
-    thickness on u and v, h*eta_av on h and c, dimp on h (w bytes each);
-    masks (one byte each): inner u, v; wavelet u, v; u, v masks; h and c
-    masks."""
-    nu, nv, nh, nc = (nx + 1) * ny, nx * (ny + 1), nx * ny, (nx - 1) * (ny - 1)
-    return w * (18 * (nu + nv) + 2 * nh + nc) + 3 * (nu + nv) + nh + nc
+    the mixed-precision coarse CG): r, w, p, s, x in and out (10 (N_u + N_v)
+    values); beta_eff and thickness on u and v; h*eta_av and dimp on h;
+    h*eta_av on c; the ring images P r, P s, P w (a 60-point ring written
+    and a 68-point halo read per 16 x 16 tile and kind: 3 * 128 / 256 values
+    per point); mask bits (one byte per u and v point)."""
+    nu, nv, nh, nc = sizes(nx, ny)
+    return int(w * (10 * (nu + nv) + 2 * (nu + nv) + 1.5 * (nu + nv) + 2 * nh + nc) + (nu + nv))
 
 
 DOMINANT = {  # precision -> (kernel, timing id, bytes per launch)
@@ -109,8 +117,8 @@ def setup(W, nx, ny, precision, picard=None, maxiter_coarse=None):
     return m, spec, lib
 
 
-def solve(spec, lib):
-    rep = W_lib.SolveReport()
+def solve(W, spec, lib):
+    rep = W._lib.SolveReport()
     rc = lib.wavi_update_velocities(spec.ctx, ctypes.byref(rep))
     if rc != 0:
         raise RuntimeError("wavi_update_velocities failed: %s" % lib.wavi_last_error(spec.ctx))
@@ -118,14 +126,48 @@ def solve(spec, lib):
 
 
 def velocities(W, m, spec):
-    """u and v of the model's device state."""
+    """u and v of the model's device state, concatenated."""
     lib = spec._lib
     nx, ny = m.grid.nx, m.grid.ny
     u = np.empty((nx + 1) * ny)
     v = np.empty(nx * (ny + 1))
     assert lib.wavi_get_field(spec.ctx, W._lib.FIELD["U_U"], W._lib.dptr(u), u.size) == 0
     assert lib.wavi_get_field(spec.ctx, W._lib.FIELD["V_V"], W._lib.dptr(v), v.size) == 0
-    return u, v
+    return np.concatenate([u, v])
+
+
+def timed_solves(W, nx, ny, precision, steps, warmup, picard=None, maxiter_coarse=None):
+    """Solve from the reset state warmup + steps times; time the last steps.
+    Returns (times, report, velocities, roofline of the dominant kernel)."""
+    log("[bench] setup %dx%dx8 precision=%s" % (nx, ny, precision))
+    m, spec, lib = setup(W, nx, ny, precision, picard, maxiter_coarse)
+    for k in range(warmup):
+        assert lib.wavi_state_restore(spec.ctx) == 0
+        t0 = time.perf_counter()
+        rep = solve(W, spec, lib)
+        log("[bench] %s warmup %d: %.3f s, picard %d, cg %d, rel %.3e" % (
+            precision, k, time.perf_counter() - t0, rep["picard_iters"], rep["coarse_cg_total"], rep["rel_resid"]))
+    kernel, kid, nbytes = DOMINANT[precision]
+    lib.wavi_profile_enable(spec.ctx, 1)
+    times = []
+    for k in range(steps):
+        assert lib.wavi_state_restore(spec.ctx) == 0
+        t0 = time.perf_counter()
+        rep = solve(W, spec, lib)
+        times.append(time.perf_counter() - t0)
+        log("[bench] %s step %d: %.3f s, picard %d, cg %d, rel %.3e" % (
+            precision, k, times[-1], rep["picard_iters"], rep["coarse_cg_total"], rep["rel_resid"]))
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_int64(0)
+    lib.wavi_profile_read(spec.ctx, W._lib.KT[kid], ctypes.byref(ms), ctypes.byref(n))
+    lib.wavi_profile_enable(spec.ctx, 0)
+    avg_s = ms.value / 1e3 / max(n.value, 1)
+    B = nbytes(nx, ny)
+    achieved = B / avg_s / 1e9
+    roof = {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(nx, ny, precision),
+            "bytes_per_launch": B, "avg_launch_us": avg_s * 1e6, "launches": n.value}
+    return times, rep, velocities(W, m, spec), roof
 
 
 def cpu_baseline(W, nx, ny, gpu_report):
@@ -134,8 +176,8 @@ def cpu_baseline(W, nx, ny, gpu_report):
     coarse CG capped at 60 and at 120 iterations.  The two timings separate
     the per-Picard work (inner update, RHS, residual, diagonal probes, 42
     smoothing sweeps, transfers) from the per-CG-iteration cost, and the full
-    solve is projected with the GPU run's own Picard and CG counts (which the
-    oracle reproduces exactly)."""
+    solve is projected with the FP64 GPU run's own Picard and CG counts (which
+    the oracle reproduces exactly)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     times = {}
@@ -153,8 +195,8 @@ def cpu_baseline(W, nx, ny, gpu_report):
     total = gpu_report["picard_iters"] * per_picard + gpu_report["coarse_cg_total"] * per_cg
     return {
         "value": 1.0 / total, "unit": "solves/s", "cores": 1, "kind": "port",
-        "sample": "oracle, first Picard iteration at %dx%dx8 with the coarse CG capped at 60 and 120 "
-                  "iterations (%.1f s + %.1f s); per-Picard %.2f s and per-CG-iteration %.4f s "
+        "sample": "oracle (C restatement, FP64), first Picard iteration at %dx%dx8 with the coarse CG capped at "
+                  "60 and 120 iterations (%.1f s + %.1f s); per-Picard %.2f s and per-CG-iteration %.4f s "
                   "projected to %d Picard / %d CG iterations = %.0f s per solve"
                   % (nx, ny, t1, t2, per_picard, per_cg, gpu_report["picard_iters"],
                      gpu_report["coarse_cg_total"], total),
@@ -162,12 +204,11 @@ def cpu_baseline(W, nx, ny, gpu_report):
 
 
 def main():
-    global W_lib
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--nx", type=int, default=1280)
     ap.add_argument("--ny", type=int, default=160)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -183,40 +224,38 @@ def main():
               "--gpus %d is not supported (multi-GPU sharding is a design note only)" % args.gpus)
         sys.exit(2)
     W = _package()
-    W_lib = W._lib
     nx, ny = args.nx, args.ny
-    log("[bench] setup %dx%dx8 precision=%s" % (nx, ny, args.precision))
-    m, spec, lib = setup(W, nx, ny, args.precision, args.picard, args.maxiter_coarse)
     if args.calibration_copy_mib:
         calibration_copy(args.calibration_copy_mib)
-    for k in range(args.warmup):
-        assert lib.wavi_state_restore(spec.ctx) == 0
-        t0 = time.perf_counter()
-        rep = solve(spec, lib)
-        log("[bench] warmup %d: %.3f s, picard %d, cg %d, rel %.3e" % (
-            k, time.perf_counter() - t0, rep["picard_iters"], rep["coarse_cg_total"], rep["rel_resid"]))
-    lib.wavi_profile_enable(spec.ctx, 1)
-    times = []
-    for k in range(args.steps):
-        assert lib.wavi_state_restore(spec.ctx) == 0
-        t0 = time.perf_counter()
-        rep = solve(spec, lib)
-        dt = time.perf_counter() - t0
-        times.append(dt)
-        log("[bench] step %d: %.3f s, picard %d, cg %d, rel %.3e" % (
-            k, dt, rep["picard_iters"], rep["coarse_cg_total"], rep["rel_resid"]))
-    ms = ctypes.c_double(0.0)
-    n = ctypes.c_int64(0)
-    kernel, kid, nbytes = DOMINANT[args.precision]
-    lib.wavi_profile_read(spec.ctx, W._lib.KT[kid], ctypes.byref(ms), ctypes.byref(n))
-    lib.wavi_profile_enable(spec.ctx, 0)
-    avg_s = ms.value / 1e3 / max(n.value, 1)
-    B = nbytes(nx, ny)
-    achieved = B / avg_s / 1e9
+    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    res = {}
+    for p in precs:
+        res[p] = timed_solves(W, nx, ny, p, args.steps, args.warmup, args.picard, args.maxiter_coarse)
+    head = "mixed" if "mixed" in res else "fp64"
+    times, rep, vel, roof = res[head]
     total = float(sum(times))
     value = args.steps / total
+    cfg = {"workload": "channel single velocity solve %dx%d, nsigma 8, 500 m" % (nx, ny),
+           "grid": [nx, ny, 8], "precision": head,
+           "picard_iters": rep["picard_iters"], "coarse_cg_total": rep["coarse_cg_total"],
+           "rel_resid": rep["rel_resid"]}
+    if head == "mixed":
+        cfg["precision"] = "mixed: FP32 V-cycle (operator, smoother, coarse CG), FP64 residual refinement"
+        cfg["refinements"] = rep["refinements"]
+    if "fp64" in res and "mixed" in res:
+        t64, rep64, vel64, roof64 = res["fp64"]
+        v64 = args.steps / float(sum(t64))
+        err = vel - vel64
+        cfg["fp64_solves_per_s"] = v64
+        cfg["mixed_over_fp64"] = value / v64
+        cfg["error_vs_fp64"] = {
+            "rel_l2": float(np.linalg.norm(err) / np.linalg.norm(vel64)),
+            "max_abs": float(np.max(np.abs(err))),
+            "max_rel_to_max": float(np.max(np.abs(err)) / np.max(np.abs(vel64))),
+        }
+        cfg["fp64_roofline"] = roof64
     out = {
-        "metric": "velocity solves/s at 1280x160x8 (update_velocities!, 20 Picard iterations, coarse CG tol 1e-8)",
+        "metric": METRIC,
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
@@ -226,34 +265,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic (seeded channel bed with Gaussian random-field roughness, analytic thickness)",
-        "config": {"workload": "channel single velocity solve %dx%d, nsigma 8, 500 m" % (nx, ny),
-                   "grid": [nx, ny, 8], "precision": args.precision,
-                   "picard_iters": rep["picard_iters"], "coarse_cg_total": rep["coarse_cg_total"],
-                   "rel_resid": rep["rel_resid"]},
-        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(nx, ny, args.precision),
-                     "bytes_per_launch": B, "avg_launch_us": avg_s * 1e6, "launches": n.value},
+        "config": cfg,
+        "roofline": roof,
     }
-    if args.precision == "mixed":
-        log("[bench] FP64 solve of the same state for the error of the mixed solve")
-        mu, mv = velocities(W, m, spec)
-        m64, spec64, lib64 = setup(W, nx, ny, "fp64", args.picard, args.maxiter_coarse)
-        solve(spec64, lib64)
-        ru, rv = velocities(W, m64, spec64)
-        ref = np.concatenate([ru.ravel(), rv.ravel()])
-        got = np.concatenate([mu.ravel(), mv.ravel()])
-        out["config"]["error_vs_fp64"] = {
-            "rel_l2": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
-            "max_abs": float(np.max(np.abs(got - ref))),
-            "max_rel_to_max": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))),
-        }
-        out["config"]["refinements"] = rep["refinements"]
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle sample)")
-        out["cpu_baseline"] = cpu_baseline(W, nx, ny, rep)
+        out["cpu_baseline"] = cpu_baseline(W, nx, ny, res["fp64"][1] if "fp64" in res else rep)
     print(json.dumps(out), flush=True)
 
 
