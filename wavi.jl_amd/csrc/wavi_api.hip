@@ -65,6 +65,7 @@ struct wavi_ctx {
     uint8_t *marena = nullptr;
     unsigned px = 0, ns = 0;
     float2 *Pgd = nullptr;  // [2][npad] (r, r) and (w, r) partials by parity
+    float *rings = nullptr; // ring images [2][tiles + 1][3][2][64]
     int npad = 0;
     CGMScalars *stm = nullptr;
     CGMHostFlag *h_flag = nullptr;  // pinned, host-mapped stop report of the mixed coarse CG
@@ -486,6 +487,7 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
         CREATECHK(hipMalloc(&ctx->marena, 2 * (size_t)ctx->ns));
         CREATECHK(hipMemset(ctx->marena, 0, 2 * (size_t)ctx->ns));
         ctx->npad = (g.nbx * g.nby + 1023) / 1024 * 1024;
+        CREATECHK(falloc(&ctx->rings, 2 * (size_t)(g.nbx * g.nby + 1) * 6 * 64));
         CREATECHK(hipMalloc(&ctx->Pgd, 2 * (size_t)ctx->npad * sizeof(float2)));
         CREATECHK(hipMemset(ctx->Pgd, 0, 2 * (size_t)ctx->npad * sizeof(float2)));
         CREATECHK(hipMalloc(&ctx->stm, sizeof(CGMScalars)));
@@ -533,7 +535,7 @@ int wavi_ctx_destroy(wavi_ctx *ctx)
             if (dv->v) (void)hipFree(dv->v);
         }
     }
-    for (void *p : {(void *)ctx->arena, (void *)ctx->marena, (void *)ctx->Pgd})
+    for (void *p : {(void *)ctx->arena, (void *)ctx->marena, (void *)ctx->Pgd, (void *)ctx->rings})
         if (p) (void)hipFree(p);
     if (ctx->stm) (void)hipFree(ctx->stm);
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
@@ -991,6 +993,7 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     sa.x0v = w.xc.v;
     sa.t0u = w.ct.u;
     sa.t0v = w.ct.v;
+    sa.R = ctx->rings;
     sa.Pb = ctx->Pb;
     sa.st = ctx->stm;
     sa.nblk = ctx->nblk;
@@ -999,6 +1002,7 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     hipLaunchKernelGGL(k_cgm_setup, tb, dim3(256), 0, ctx->stream, sa);
     CGMIterArgs ia;
     ia.Lo = Lo;
+    ia.R = ctx->rings;
     ia.Pgd = ctx->Pgd;
     ia.st = ctx->stm;
     ia.hflag = ctx->d_flag;

@@ -1383,11 +1383,30 @@ enum CGMArray {
     CGM_X = 8,       // solution
     CGM_HU = 10,     // thickness on u / v points, 0 off the masks
     CGM_BU = 12,     // beta_eff on u / v points, 0 off the masks
-    CGM_RING = 14,   // rings: CGM_RING + 6 parity + 2 kind (0 P r, 1 P s, 2 P w)
-    CGM_CELL = 26,   // per h cell: (h*eta_av on the h mask, dimp) pairs (two arrays' room)
-    CGM_CORNER = 28, // per h cell: corner h*eta_av (0 off the c grid and mask)
-    CGM_NARRAY = 29
+    CGM_CELL = 14,   // per h cell: (h*eta_av on the h mask, dimp) pairs (two arrays' room)
+    CGM_CORNER = 16, // per h cell: corner h*eta_av (0 off the c grid and mask)
+    CGM_NARRAY = 17
 };
+
+// Ring images, compact per tile: for each parity, tile (ntile + 1 tiles, the
+// last one all zero and never written, standing in for tiles past the edge),
+// kind (0 P r, 1 P s, 2 P w) and component, 64 slots: the tile's bottom row
+// (y = 0, slots 0-15 by x), top row (y = 15, 16-31), left column (x = 0,
+// 32-47 by y) and right column (x = 15, 48-63).  A side of a neighbour's halo
+// is then 16 consecutive floats.
+__device__ __forceinline__ unsigned ring_at(int par, int ntile, int tile, int kind, int comp)
+{
+    return ((unsigned)(par * (ntile + 1) + tile) * 6u + (unsigned)(2 * kind + comp)) * 64u;
+}
+
+// Store v at the ring slots of tile position (x, y) (a corner has two).
+__device__ __forceinline__ void ring_put(float *R, unsigned base, int x, int y, float v)
+{
+    if (y == 0) R[base + x] = v;
+    if (y == 15) R[base + 16 + x] = v;
+    if (x == 0) R[base + 32 + y] = v;
+    if (x == 15) R[base + 48 + y] = v;
+}
 
 struct CGMLayout {
     float *F;       // FP32 arena, array A at F + A * ns
@@ -1446,14 +1465,15 @@ struct CGMArgs {
     const float *bu, *bv;    // right-hand side (compact)
     const float *x0u, *x0v;  // warm start (compact)
     const float *t0u, *t0v;  // P x0 (compact)
+    float *R;                // ring images
     double *Pb;              // partials of ||b||^2 per u / v block
     CGMScalars *st;
     int nblk;
 };
 
-// setup (P x0 computed beforehand): r0 = b - R A P x0, p = s = 0, x = x0,
-// P r0 into the P r rings of set 0 (whole tile), partials of ||b||^2, in
-// the arena; resets the scalars.
+// setup (P x0 computed beforehand): r0 = b - R A P x0, p = s = 0, x = x0 in
+// the arena, P r0 into the P r rings of set 0, partials of ||b||^2; resets
+// the scalars.
 __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
 {
     __shared__ OpTile<float> s;
@@ -1508,8 +1528,8 @@ __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
     tv[ly][lx] = pc.mwv ? rv : 0.f;
     __syncthreads();
     haar_pm_tile2<float>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
-    if (pc.oku) F[CGM_RING * ns + o] = pc.ui ? tu[ly][lx] : 0.f;
-    if (pc.okv) F[(CGM_RING + 1) * ns + o] = pc.vi ? tv[ly][lx] : 0.f;
+    if (pc.oku) ring_put(a.R, ring_at(0, g.nbx * g.nby, L, 0, 0), lx, ly, pc.ui ? tu[ly][lx] : 0.f);
+    if (pc.okv) ring_put(a.R, ring_at(0, g.nbx * g.nby, L, 0, 1), lx, ly, pc.vi ? tv[ly][lx] : 0.f);
     tree256x2(b2u, b2v, lds);
     if (threadIdx.x == 0) {
         if (has_u) a.Pb[bx + g.nbxu * by] = b2u;
@@ -1591,6 +1611,7 @@ __device__ __forceinline__ float row_sum16f(float v)
 
 struct CGMIterArgs {
     CGMLayout Lo;
+    float *R;               // ring images
     float2 *Pgd;            // [2][npad]: per tile (r, r) and (w, r) partials by parity (0 past the tiles)
     CGMScalars *st;
     CGMHostFlag *hflag;     // host-mapped stop report
@@ -1662,16 +1683,25 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
         x_u0 = el(arr(CGM_X), ob);
         x_v0 = el(arr(CGM_X + 1), ob);
     }
-    // halo of the window: threads 0..67 the u grid, 68..135 the v grid
+    // halo of the window from the neighbours' rings: threads 0..67 the u
+    // grid, 68..135 the v grid
     int hx, hy;
     const int hcomp = tid >= 68 ? 1 : 0;
-    halo_xy(tid < 136 ? tid - 68 * hcomp : 0, hx, hy);
-    const unsigned hb = 4u * (hcomp * ns + wb + (unsigned)hx + px * (unsigned)hy);
-    const float hr = el(arr(CGM_RING + 6 * PAR), hb);
+    const int hh = tid < 136 ? tid - 68 * hcomp : 0;
+    halo_xy(hh, hx, hy);
+    // neighbour tile (dx, dy) and the slot in its ring
+    const int hdx = hx == 0 ? -1 : (hx == 17 ? 1 : 0), hdy = hy == 0 ? -1 : (hy == 17 ? 1 : 0);
+    const int hslot = hdy == -1 ? 16 + (hdx == -1 ? 15 : (hdx == 1 ? 0 : hx - 1))
+                    : hdy == 1 ? (hdx == -1 ? 15 : (hdx == 1 ? 0 : hx - 1))
+                    : hdx == -1 ? 48 + (hy - 1) : 32 + (hy - 1);
+    const int nbxx = bx + hdx, nbyy = by + hdy;
+    const int ntl = (nbxx >= 0 && nbxx < g.nbx && nbyy >= 0 && nbyy < g.nby) ? nbxx + g.nbx * nbyy : ntile;
+    const float *Rr = a.R;
+    const float hr = Rr[ring_at(PAR, ntile, ntl, 0, hcomp) + hslot];
     float hs = 0.f, hw = 0.f;
     if (!INIT) {
-        hs = el(arr(CGM_RING + 6 * PAR + 2), hb);
-        hw = el(arr(CGM_RING + 6 * PAR + 4), hb);
+        hs = Rr[ring_at(PAR, ntile, ntl, 1, hcomp) + hslot];
+        hw = Rr[ring_at(PAR, ntile, ntl, 2, hcomp) + hslot];
     }
     // thickness windows, entries q = tid and tid + 256 (< 324) of the 18 x 18 window
     const int q1 = tid + 256 < 324 ? tid + 256 : tid;
@@ -1804,11 +1834,10 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
             ps_v = (m1 & 1) ? y3 : 0.f;
         }
         if (cx == 0 || cx == 15 || cy == 0 || cy == 15) {
-            const unsigned kb = 4u * (wb + (unsigned)(cx + 1) + px * (unsigned)(cy + 1));
-            el(arr(CGM_RING + 6 * NP), kb) = pr_u;
-            el(arr(CGM_RING + 6 * NP + 1), kb) = pr_v;
-            el(arr(CGM_RING + 6 * NP + 2), kb) = ps_u;
-            el(arr(CGM_RING + 6 * NP + 3), kb) = ps_v;
+            ring_put(a.R, ring_at(NP, ntile, L, 0, 0), cx, cy, pr_u);
+            ring_put(a.R, ring_at(NP, ntile, L, 0, 1), cx, cy, pr_v);
+            ring_put(a.R, ring_at(NP, ntile, L, 1, 0), cx, cy, ps_u);
+            ring_put(a.R, ring_at(NP, ntile, L, 1, 1), cx, cy, ps_v);
         }
     }
     __syncthreads();
@@ -1865,8 +1894,8 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     el(arr(CGM_W), ob) = wn_u;
     el(arr(CGM_W + 1), ob) = wn_v;
     if (lx == 0 || lx == 15 || ly == 0 || ly == 15) {
-        el(arr(CGM_RING + 6 * NP + 4), ob) = inu ? tw_u : 0.f;
-        el(arr(CGM_RING + 6 * NP + 5), ob) = inv ? tw_v : 0.f;
+        ring_put(a.R, ring_at(NP, ntile, L, 2, 0), lx, ly, inu ? tw_u : 0.f);
+        ring_put(a.R, ring_at(NP, ntile, L, 2, 1), lx, ly, inv ? tw_v : 0.f);
     }
     float gg = r_u * r_u + r_v * r_v;
     float dd = wn_u * r_u + wn_v * r_v;
