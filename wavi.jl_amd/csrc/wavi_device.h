@@ -188,6 +188,14 @@ __device__ __forceinline__ void lanes_sum2(const double *P, const double *Q, int
 }
 
 // ---------------------------------------------------------------------------
+// Non-temporal store (streamed past the caches).
+template <typename T>
+__device__ __forceinline__ void st_nt(T *p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
+// ---------------------------------------------------------------------------
 // Lane exchange within the 16-lane rows of a wavefront by DPP (every lane of
 // the wavefront must be active).
 constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1,0,3,2]: lane ^ 1

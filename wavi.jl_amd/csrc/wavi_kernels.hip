@@ -1786,14 +1786,14 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
         s_v = w_v0 + fb * s_v0;
         r_u = r_u - fa * s_u;
         r_v = r_v - fa * s_v;
-        el(arr(CGM_P), ob) = p_u;
-        el(arr(CGM_P + 1), ob) = p_v;
-        el(arr(CGM_S), ob) = s_u;
-        el(arr(CGM_S + 1), ob) = s_v;
-        el(arr(CGM_X), ob) = x_u0 + fa * p_u;
-        el(arr(CGM_X + 1), ob) = x_v0 + fa * p_v;
-        el(arr(CGM_R), ob) = r_u;
-        el(arr(CGM_R + 1), ob) = r_v;
+        st_nt(&el(arr(CGM_P), ob), p_u);
+        st_nt(&el(arr(CGM_P + 1), ob), p_v);
+        st_nt(&el(arr(CGM_S), ob), s_u);
+        st_nt(&el(arr(CGM_S + 1), ob), s_v);
+        st_nt(&el(arr(CGM_X), ob), x_u0 + fa * p_u);
+        st_nt(&el(arr(CGM_X + 1), ob), x_v0 + fa * p_v);
+        st_nt(&el(arr(CGM_R), ob), r_u);
+        st_nt(&el(arr(CGM_R + 1), ob), r_v);
     }
     // ---- x pass of P r and P s; halo and thickness windows into LDS
     sm.T[0][ly][lx] = haar_line_P<LEV>(mwu ? r_u : 0.f, lx);
@@ -1891,8 +1891,8 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     // ---- P x pass, stores, partials of (r, r) and (w, r)
     const float wn_u = sm.T[0][ly][lx], wn_v = sm.T[1][ly][lx];
     const float tw_u = haar_line_P<LEV>(sm.T[2][ly][lx], lx), tw_v = haar_line_P<LEV>(sm.T[3][ly][lx], lx);
-    el(arr(CGM_W), ob) = wn_u;
-    el(arr(CGM_W + 1), ob) = wn_v;
+    st_nt(&el(arr(CGM_W), ob), wn_u);
+    st_nt(&el(arr(CGM_W + 1), ob), wn_v);
     if (lx == 0 || lx == 15 || ly == 0 || ly == 15) {
         ring_put(a.R, ring_at(NP, ntile, L, 2, 0), lx, ly, inu ? tw_u : 0.f);
         ring_put(a.R, ring_at(NP, ntile, L, 2, 1), lx, ly, inv ? tw_v : 0.f);
