@@ -1362,7 +1362,8 @@ __global__ void __launch_bounds__(256) k_cg_update(CGArgs<T> a, int par)
 
 struct CGMState {
     double igamma;  // 1 / ||r||^2 at the start of the launch that wrote this state
-    double ialpha;  // 1 / that launch's step length
+    double alpha;   // that launch's step length
+    double ialpha;  // 1 / alpha
     double tol2;    // square of the stopping tolerance, fixed at iteration 0
     int it;         // iterations done
     int done;       // stop decided
@@ -1407,6 +1408,22 @@ __device__ __forceinline__ void ring_put(float *R, unsigned base, int x, int y, 
     if (x == 0) R[base + 32 + y] = v;
     if (x == 15) R[base + 48 + y] = v;
 }
+
+// The ring slots of tile position (x, y): a row slot and a column slot, -1
+// where there is none.
+struct RingSlots {
+    int row, col;
+    __device__ __forceinline__ RingSlots(int x, int y)
+    {
+        row = y == 0 ? x : (y == 15 ? 16 + x : -1);
+        col = x == 0 ? 32 + y : (x == 15 ? 48 + y : -1);
+    }
+    __device__ __forceinline__ void put(float *R, unsigned base, float v) const
+    {
+        if (row >= 0) R[base + (unsigned)row] = v;
+        if (col >= 0) R[base + (unsigned)col] = v;
+    }
+};
 
 struct CGMLayout {
     float *F;       // FP32 arena, array A at F + A * ns
@@ -1485,6 +1502,7 @@ __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
     if (L == 0 && threadIdx.x == 0) {
         for (int p = 0; p < 2; ++p) {
             a.st->s[p].igamma = 0.0;
+            a.st->s[p].alpha = 0.0;
             a.st->s[p].ialpha = 0.0;
             a.st->s[p].tol2 = 0.0;
             a.st->s[p].it = 0;
@@ -1655,6 +1673,12 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     const int lx = tid & 15, ly = tid >> 4;    // row-major position
     const int cx = tid >> 4, cy = tid & 15;    // column-major position (transposes)
     constexpr int NP = 1 - PAR;
+    // Launch q >= 1 performs iteration k = q - 1, so PAR == 0 iff k is odd.
+    // x is advanced every other iteration: an even iteration leaves it, the
+    // odd one after it applies both steps, x_{k+1} = x_{k-1} +
+    // (alpha_{k-1} + alpha_k beta_k) p_{k-1} + alpha_k r_k, and a stop after
+    // an even iteration applies the pending alpha_{k-1} p_{k-1}.
+    constexpr bool XUPD = PAR == 0;
     const unsigned px = a.Lo.px, ns = a.Lo.ns;
     float *F = a.Lo.F;
     const uint8_t *M = a.Lo.M;
@@ -1680,8 +1704,10 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
         p_v0 = el(arr(CGM_P + 1), ob);
         s_u0 = el(arr(CGM_S), ob);
         s_v0 = el(arr(CGM_S + 1), ob);
-        x_u0 = el(arr(CGM_X), ob);
-        x_v0 = el(arr(CGM_X + 1), ob);
+        if (XUPD) {
+            x_u0 = el(arr(CGM_X), ob);
+            x_v0 = el(arr(CGM_X + 1), ob);
+        }
     }
     // halo of the window from the neighbours' rings: threads 0..67 the u
     // grid, 68..135 the v grid
@@ -1725,7 +1751,7 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     }
     const bool inu = mbu & 1, inv = mbv & 1, mwu = mbu & 2, mwv = mbv & 2;
     // ---- scalars from the previous launch's partials (FP64 across workgroups)
-    float fa = 0.f, fb = 0.f;
+    float fa = 0.f, fb = 0.f, fxp = 0.f;
     if (!INIT) {
         double gamma = 0.0, delta = 0.0;
 #pragma unroll
@@ -1762,6 +1788,11 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
                 a.hflag->iters = S.it;
                 a.hflag->done = 1;
             }
+            if (XUPD && S.it > 0) {
+                const float fp = (float)S.alpha;
+                el(arr(CGM_X), ob) = x_u0 + fp * p_u0;
+                el(arr(CGM_X + 1), ob) = x_v0 + fp * p_v0;
+            }
             return;
         }
         const double beta = S.it == 0 ? 0.0 : gamma * S.igamma;
@@ -1769,6 +1800,7 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
         if (L == 0 && tid == 0) {
             CGMState n;
             n.igamma = 1.0 / gamma;
+            n.alpha = alpha;
             n.ialpha = 1.0 / alpha;
             n.tol2 = tol2;
             n.it = S.it + 1;
@@ -1777,11 +1809,16 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
         }
         fa = (float)alpha;
         fb = (float)beta;
+        fxp = XUPD ? (float)(S.alpha + alpha * beta) : 0.f;
     }
     // ---- vector updates on the tile (zero past the ends of the grids)
     float s_u = 0.f, s_v = 0.f;
     if (!INIT) {
         const float p_u = r_u + fb * p_u0, p_v = r_v + fb * p_v0;
+        if (XUPD) {
+            st_nt(&el(arr(CGM_X), ob), (x_u0 + fxp * p_u0) + fa * r_u);
+            st_nt(&el(arr(CGM_X + 1), ob), (x_v0 + fxp * p_v0) + fa * r_v);
+        }
         s_u = w_u0 + fb * s_u0;
         s_v = w_v0 + fb * s_v0;
         r_u = r_u - fa * s_u;
@@ -1790,8 +1827,6 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
         st_nt(&el(arr(CGM_P + 1), ob), p_v);
         st_nt(&el(arr(CGM_S), ob), s_u);
         st_nt(&el(arr(CGM_S + 1), ob), s_v);
-        st_nt(&el(arr(CGM_X), ob), x_u0 + fa * p_u);
-        st_nt(&el(arr(CGM_X + 1), ob), x_v0 + fa * p_v);
         st_nt(&el(arr(CGM_R), ob), r_u);
         st_nt(&el(arr(CGM_R + 1), ob), r_v);
     }
@@ -1833,12 +1868,12 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
             ps_u = (m0 & 1) ? y2 : 0.f;
             ps_v = (m1 & 1) ? y3 : 0.f;
         }
-        if (cx == 0 || cx == 15 || cy == 0 || cy == 15) {
-            ring_put(a.R, ring_at(NP, ntile, L, 0, 0), cx, cy, pr_u);
-            ring_put(a.R, ring_at(NP, ntile, L, 0, 1), cx, cy, pr_v);
-            ring_put(a.R, ring_at(NP, ntile, L, 1, 0), cx, cy, ps_u);
-            ring_put(a.R, ring_at(NP, ntile, L, 1, 1), cx, cy, ps_v);
-        }
+        const RingSlots rs(cx, cy);
+        const unsigned rb = ring_at(NP, ntile, L, 0, 0);
+        rs.put(a.R, rb, pr_u);
+        rs.put(a.R, rb + 64, pr_v);
+        rs.put(a.R, rb + 128, ps_u);
+        rs.put(a.R, rb + 192, ps_v);
     }
     __syncthreads();
     // ---- intermediates on the 17 x 17 cells and corners
@@ -1893,9 +1928,11 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     const float tw_u = haar_line_P<LEV>(sm.T[2][ly][lx], lx), tw_v = haar_line_P<LEV>(sm.T[3][ly][lx], lx);
     st_nt(&el(arr(CGM_W), ob), wn_u);
     st_nt(&el(arr(CGM_W + 1), ob), wn_v);
-    if (lx == 0 || lx == 15 || ly == 0 || ly == 15) {
-        ring_put(a.R, ring_at(NP, ntile, L, 2, 0), lx, ly, inu ? tw_u : 0.f);
-        ring_put(a.R, ring_at(NP, ntile, L, 2, 1), lx, ly, inv ? tw_v : 0.f);
+    {
+        const RingSlots rs(lx, ly);
+        const unsigned rb = ring_at(NP, ntile, L, 2, 0);
+        rs.put(a.R, rb, inu ? tw_u : 0.f);
+        rs.put(a.R, rb + 64, inv ? tw_v : 0.f);
     }
     float gg = r_u * r_u + r_v * r_v;
     float dd = wn_u * r_u + wn_v * r_v;
