@@ -42,7 +42,10 @@ extern "C" {
 
 /* Precision of the solve (argument of wavi_ctx_create). */
 #define WAVI_PRECISION_FP64 0   /* plain FP64: the reference scheme, bit-reproducible */
-#define WAVI_PRECISION_MIXED 1  /* FP32 V-cycle inside FP64 residual refinement */
+#define WAVI_PRECISION_MIXED 1  /* FP32 V-cycle (smoother, Haar transfers and a one-launch-per-
+                                  * iteration coarse CG with FP64 cross-workgroup sums) inside
+                                  * FP64 residual refinement: every Picard iteration forms the
+                                  * FP64 residual b - A(u)u and applies the FP32 correction */
 
 /* Grid: the reference's Grid(;...) keyword arguments (src/Grid.jl:66-82).
  * Masks are 0/1 bytes.  u_iszero/v_iszero remove points from the u/v masks
