@@ -59,31 +59,43 @@ def cg_apply_bytes(nx, ny, w=8):
 
 def cgm_iter_bytes(nx, ny, w=4):
     """Algorithmic HBM bytes of one k_cgm_iter launch (one whole iteration of
-    the mixed-precision coarse CG): r, w, p, s, x in and out (10 (N_u + N_v)
-    values); beta_eff and thickness on u and v; h*eta_av and dimp on h;
-    h*eta_av on c; the ring images P r, P s, P w (a 60-point ring written
-    and a 68-point halo read per 16 x 16 tile and kind: 3 * 128 / 256 values
-    per point); mask bits (one byte per u and v point)."""
+    the mixed-precision coarse CG), averaged over the two alternating
+    launch parities: r, w, p, s in and out (8 (N_u + N_v) values) and x in
+    and out every other launch (1 (N_u + N_v) on average); beta_eff and
+    thickness on u and v; h*eta_av and dimp on h; h*eta_av on c; the ring
+    images P r, P s, P w (a 60-point ring written and a 68-point halo read
+    per 16 x 16 tile and kind: 3 * 128 / 256 values per point); mask bits
+    (one byte per u and v point)."""
     nu, nv, nh, nc = sizes(nx, ny)
-    return int(w * (10 * (nu + nv) + 2 * (nu + nv) + 1.5 * (nu + nv) + 2 * nh + nc) + (nu + nv))
+    return int(w * (9 * (nu + nv) + 2 * (nu + nv) + 1.5 * (nu + nv) + 2 * nh + nc) + (nu + nv))
 
 
-DOMINANT = {  # precision -> (kernel, timing id, bytes per launch)
-    "fp64": ("k_cg_apply", "cg_apply", lambda nx, ny: cg_apply_bytes(nx, ny, 8)),
-    "mixed": ("k_cgm_iter", "cg_iter_mixed", lambda nx, ny: cgm_iter_bytes(nx, ny, 4)),
+def cg_iteration_fp64_bytes(nx, ny, w=8):
+    """Algorithmic HBM bytes of one FP64 coarse-CG iteration (k_cg_dir +
+    k_cg_apply + k_cg_update).  k_cg_dir: r, u in, u, t out, wavelet and
+    inner masks; k_cg_apply as above; k_cg_update: x, u, r, c in, x, r out."""
+    nu, nv, nh, nc = sizes(nx, ny)
+    return 4 * w * (nu + nv) + 2 * (nu + nv) + cg_apply_bytes(nx, ny, w) + 6 * w * (nu + nv)
+
+
+DOMINANT = {  # precision -> (what is timed, timing id, profile name, bytes per launch)
+    "fp64": ("FP64 CG iteration (k_cg_dir + k_cg_apply + k_cg_update)", "cg_iter_fp64", "cg_iteration",
+             lambda nx, ny: cg_iteration_fp64_bytes(nx, ny, 8)),
+    "mixed": ("k_cgm_iter", "cg_iter_mixed", "k_cgm_iter", lambda nx, ny: cgm_iter_bytes(nx, ny, 4)),
 }
 
 
 def measured_traffic(nx, ny, precision):
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary for this workload, scaled by the
-    copy-kernel calibration recorded there; None when no summary matches."""
-    kernel = DOMINANT[precision][0]
-    path = os.path.join(ROOT, "profiles", "%s_%dx%d_%s_traffic.md" % (kernel, nx, ny, precision))
+    """HBM bytes per launch (per iteration for FP64) of the dominant kernel
+    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary for this
+    workload, scaled by the copy-kernel calibration recorded there; None when
+    no summary matches."""
+    name = DOMINANT[precision][2]
+    path = os.path.join(ROOT, "profiles", "%s_%dx%d_%s_traffic.md" % (name, nx, ny, precision))
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch %s:" % kernel):
+        if line.startswith("traffic_bytes_per_launch %s:" % name):
             return int(line.split(":")[1])
     return None
 
@@ -147,7 +159,7 @@ def timed_solves(W, nx, ny, precision, steps, warmup, picard=None, maxiter_coars
         rep = solve(W, spec, lib)
         log("[bench] %s warmup %d: %.3f s, picard %d, cg %d, rel %.3e" % (
             precision, k, time.perf_counter() - t0, rep["picard_iters"], rep["coarse_cg_total"], rep["rel_resid"]))
-    kernel, kid, nbytes = DOMINANT[precision]
+    kernel, kid, _, nbytes = DOMINANT[precision]
     lib.wavi_profile_enable(spec.ctx, 1)
     times = []
     for k in range(steps):
@@ -161,13 +173,77 @@ def timed_solves(W, nx, ny, precision, steps, warmup, picard=None, maxiter_coars
     n = ctypes.c_int64(0)
     lib.wavi_profile_read(spec.ctx, W._lib.KT[kid], ctypes.byref(ms), ctypes.byref(n))
     lib.wavi_profile_enable(spec.ctx, 0)
-    avg_s = ms.value / 1e3 / max(n.value, 1)
+    # no complete chunk of 32 iterations (a capped profiling run): no timing
+    avg_s = ms.value / 1e3 / n.value if n.value > 0 else None
     B = nbytes(nx, ny)
-    achieved = B / avg_s / 1e9
+    achieved = B / avg_s / 1e9 if avg_s else None
     roof = {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(nx, ny, precision),
-            "bytes_per_launch": B, "avg_launch_us": avg_s * 1e6, "launches": n.value}
+            "frac": achieved / HBM_PEAK_GBS if achieved else None,
+            "traffic": measured_traffic(nx, ny, precision),
+            "bytes_per_launch": B, "avg_launch_us": avg_s * 1e6 if avg_s else None, "launches": n.value,
+            "timing": "HIP events on the solve's stream around chunks of 32 back-to-back CG iterations "
+                      "(chunks that ran past the CG stop excluded)"}
     return times, rep, velocities(W, m, spec), roof
+
+
+def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed"):
+    """The seeded ensemble (channel solves of mixed sizes, nσ, Glen A and
+    basal-drag fields) solved from the reset state with one
+    wavi_update_velocities_batch call per step; also timed member after
+    member with one thread, for the concurrency gain.  Returns the JSON line."""
+    specs = W.workloads.ensemble_specs(n)
+    log("[bench] ensemble: %d members, setup" % n)
+    members = []
+    for sp in specs:
+        m = W.workloads.ensemble_member(sp, precision)
+        W.update_preconditioner(m)
+        W.push_fields(m)
+        spec = m.parallel_spec
+        assert spec._lib.wavi_update_geometry(spec.ctx) == 0
+        assert spec._lib.wavi_state_save(spec.ctx) == 0
+        members.append(m)
+    lib = members[0].parallel_spec._lib
+    ctxs = (ctypes.c_void_p * n)(*[m.parallel_spec.ctx for m in members])
+    reps = (W._lib.SolveReport * n)()
+    codes = (ctypes.c_int * n)()
+
+    def run(threads):
+        for m in members:
+            assert lib.wavi_state_restore(m.parallel_spec.ctx) == 0
+        t0 = time.perf_counter()
+        rc = lib.wavi_update_velocities_batch(ctxs, n, threads, reps, codes)
+        dt = time.perf_counter() - t0
+        if rc != 0:
+            bad = [k for k in range(n) if codes[k] != 0]
+            raise RuntimeError("batch failed (%d) at members %s: %s"
+                               % (rc, bad, lib.wavi_last_error(ctxs[bad[0]]) if bad else ""))
+        return dt
+
+    for k in range(warmup):
+        log("[bench] ensemble warmup %d: %.3f s" % (k, run(nthreads)))
+    times = []
+    for k in range(steps):
+        times.append(run(nthreads))
+        log("[bench] ensemble step %d: %.3f s (%d threads)" % (k, times[-1], nthreads))
+    serial = run(1)
+    log("[bench] ensemble one member at a time: %.3f s" % serial)
+    picard = [reps[k].picard_iters for k in range(n)]
+    total = float(sum(times))
+    unknowns = sum(2 * sp["nx"] * sp["ny"] + sp["nx"] + sp["ny"] for sp in specs)
+    return {
+        "metric": "ensemble members solved per second (forward velocity solves)",
+        "value": n * steps / total, "unit": "members/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * total / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if precision == "mixed" else "f64",
+        "data": "synthetic (seeded ensemble of channel setups)",
+        "config": {"workload": "ensemble of %d channel solves, grids from %s, nsigma 4 or 8, "
+                               "A log-uniform in [1e-17, 4e-17], C = 1e4 exp(GRF std 0.3)"
+                               % (n, ["%dx%d" % s for s in W.workloads.ENSEMBLE_SIZES]),
+                   "precision": precision, "host_threads": nthreads, "unknowns_total": unknowns,
+                   "serial_members_per_s": n / serial, "batched_over_serial": serial / (total / steps),
+                   "picard_iters_min_max": [min(picard), max(picard)],
+                   "all_converged": all(reps[k].converged for k in range(n))},
+    }
 
 
 def cpu_baseline(W, nx, ny, gpu_report):
@@ -216,6 +292,9 @@ def main():
                     help="profiling runs only: cap the Picard iterations")
     ap.add_argument("--maxiter-coarse", type=int, default=None,
                     help="profiling runs only: cap the coarse CG iterations")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="time the seeded ensemble of this many members instead of the single solve")
+    ap.add_argument("--threads", type=int, default=8, help="host threads of the ensemble batch")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="profiling runs only: issue one device-to-device copy of this size first")
     args = ap.parse_args()
@@ -224,6 +303,11 @@ def main():
               "--gpus %d is not supported (multi-GPU sharding is a design note only)" % args.gpus)
         sys.exit(2)
     W = _package()
+    if args.ensemble:
+        prec = "fp64" if args.precision == "fp64" else "mixed"
+        print(json.dumps(ensemble_run(W, args.ensemble, args.threads, args.steps, args.warmup, prec)),
+              flush=True)
+        return
     nx, ny = args.nx, args.ny
     if args.calibration_copy_mib:
         calibration_copy(args.calibration_copy_mib)

@@ -167,6 +167,15 @@ int wavi_state_restore(wavi_ctx *ctx);
 /* update_velocities!(model) (src/Models/update_velocities.jl:11-29):
  * Picard loop of inner_update! + precondition! on device-resident fields. */
 int wavi_update_velocities(wavi_ctx *ctx, wavi_solve_report *report);
+/* update_velocities! for each of n contexts (an ensemble of independent
+ * models, of any grid sizes and precisions), run by nthreads host threads;
+ * every context has its own stream, so the GPU overlaps the members'
+ * kernels.  reports and codes (each NULL or n entries) receive each
+ * member's report and return code (wavi_last_error of that member's context
+ * holds its message).  Returns 0, or the first nonzero member code in member
+ * order.  No reference counterpart: the reference solves one model per call. */
+int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,
+                                 wavi_solve_report *reports, int *codes);
 
 /* inner_update!(model) (update_velocities.jl:32-45). */
 int wavi_inner_update(wavi_ctx *ctx);
@@ -206,21 +215,27 @@ int wavi_vcycle(wavi_ctx *ctx, const double *ru, const double *rv,
 int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out);
 
 /* Kernel timing with HIP events on the context's stream, for measurement.
- * kernel: WAVI_KT_CG_APPLY (coarse CG: A then R on one tile, dot partials),
- * WAVI_KT_CG_DIR (coarse CG: direction update and prolongation),
- * WAVI_KT_CG_UPDATE (coarse CG: axpys and norm partials), WAVI_KT_COLOUR
- * (one smoother colour sweep), WAVI_KT_CG_ITER_MIXED (mixed-precision coarse
- * CG: one whole iteration per launch).  enable=1 starts (and clears) the
- * accumulation; every 16th launch of each kernel is timed, and
- * wavi_profile_read returns the total milliseconds and the number of timed
- * launches.
+ * Sampled single launches (events around every 16th launch, so each time
+ * includes the cost of the two event records): WAVI_KT_CG_APPLY (FP64
+ * coarse CG: A then R on one tile, dot partials), WAVI_KT_CG_DIR (direction
+ * update and prolongation), WAVI_KT_CG_UPDATE (axpys and norm partials),
+ * WAVI_KT_COLOUR (one smoother colour sweep).  Bursts (events around each
+ * chunk of 32 back-to-back iterations; only chunks in which every launch
+ * still iterates are counted, and the count is of launches):
+ * WAVI_KT_CG_ITER_MIXED (mixed-precision coarse CG, one whole iteration per
+ * launch of k_cgm_iter) and WAVI_KT_CG_ITER_FP64 (FP64 coarse CG, one
+ * iteration = k_cg_dir + k_cg_apply + k_cg_update).  FP64 chunks alternate
+ * between burst timing and the sampled single launches.
+ * enable=1 starts (and clears) the accumulation; wavi_profile_read returns
+ * the total milliseconds and the number of timed launches (iterations).
  * No reference counterpart. */
 #define WAVI_KT_CG_APPLY 0
 #define WAVI_KT_CG_DIR 1
 #define WAVI_KT_CG_UPDATE 2
 #define WAVI_KT_COLOUR 3
 #define WAVI_KT_CG_ITER_MIXED 4
-#define WAVI_KT_COUNT 5
+#define WAVI_KT_CG_ITER_FP64 5
+#define WAVI_KT_COUNT 6
 int wavi_profile_enable(wavi_ctx *ctx, int enable);
 int wavi_profile_read(wavi_ctx *ctx, int kernel, double *total_ms, int64_t *launches);
 

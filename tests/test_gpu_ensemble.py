@@ -1,0 +1,59 @@
+"""Batched update_velocities over an ensemble of independent models
+(wavi_update_velocities_batch): every member's velocities and report must be
+bit-identical to solving that member alone with wavi_update_velocities, for
+both precisions, whatever the number of host threads driving the batch."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _step(name):
+    print("[gpu] %s" % name, flush=True)
+    sys.stdout.flush()
+
+
+def _small_specs(W, n, seed):
+    specs = [sp for sp in W.workloads.ensemble_specs(64, seed) if sp["nx"] <= 320]
+    return specs[:n]
+
+
+def _members(W, specs, precision):
+    return [W.workloads.ensemble_member(sp, precision, maxiter_picard=6) for sp in specs]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_matches_individual_solves(W, precision):
+    specs = _small_specs(W, 6, seed=5)
+    assert len({(s["nx"], s["nσ"]) for s in specs}) > 1  # a ragged batch
+    _step("ensemble %s: %d individual solves" % (precision, len(specs)))
+    alone = _members(W, specs, precision)
+    for m in alone:
+        W.update_velocities(m)
+    _step("ensemble %s: one batched call, 4 threads" % precision)
+    batch = _members(W, specs, precision)
+    W.update_velocities_batch(batch, nthreads=4)
+    for a, b in zip(alone, batch):
+        assert a.last_report["picard_iters"] == b.last_report["picard_iters"]
+        assert a.last_report["coarse_cg_iters"] == b.last_report["coarse_cg_iters"]
+        np.testing.assert_array_equal(a.fields.gu.u, b.fields.gu.u)
+        np.testing.assert_array_equal(a.fields.gv.v, b.fields.gv.v)
+        np.testing.assert_array_equal(a.fields.gh.av_speed, b.fields.gh.av_speed)
+
+
+def test_batch_edge_cases(W):
+    _step("ensemble: empty batch, one member, more threads than members")
+    assert W.update_velocities_batch([], nthreads=4) == []
+    specs = _small_specs(W, 2, seed=7)
+    alone = _members(W, specs, "mixed")
+    for m in alone:
+        W.update_velocities(m)
+    one = _members(W, specs[:1], "mixed")
+    W.update_velocities_batch(one, nthreads=16)
+    np.testing.assert_array_equal(alone[0].fields.gu.u, one[0].fields.gu.u)
+    two = _members(W, specs, "mixed")
+    W.update_velocities_batch(two, nthreads=1)
+    for a, b in zip(alone, two):
+        np.testing.assert_array_equal(a.fields.gv.v, b.fields.gv.v)

@@ -21,7 +21,7 @@ PRECISION = {"fp64": 0, "mixed": 1}
 
 REPORT_MAX_PICARD = 64
 
-KT = {"cg_apply": 0, "cg_dir": 1, "cg_update": 2, "colour": 3, "cg_iter_mixed": 4}
+KT = {"cg_apply": 0, "cg_dir": 1, "cg_update": 2, "colour": 3, "cg_iter_mixed": 4, "cg_iter_fp64": 5}
 
 # enum wavi_field, in header order
 FIELD_NAMES = [
@@ -112,6 +112,9 @@ SIGNATURES = {
     "wavi_state_save": (ctypes.c_int, [ctypes.c_void_p]),
     "wavi_state_restore": (ctypes.c_int, [ctypes.c_void_p]),
     "wavi_update_velocities": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveReport)]),
+    "wavi_update_velocities_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
+                                                    ctypes.POINTER(SolveReport),
+                                                    ctypes.POINTER(ctypes.c_int)]),
     "wavi_inner_update": (ctypes.c_int, [ctypes.c_void_p]),
     "wavi_update_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveReport)]),
     "wavi_update_geometry": (ctypes.c_int, [ctypes.c_void_p]),

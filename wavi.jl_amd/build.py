@@ -18,7 +18,7 @@ FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-fvisibility=hidden", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    "-pthread", "-fvisibility=hidden", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
 

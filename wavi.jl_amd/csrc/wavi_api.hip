@@ -2,10 +2,13 @@
 // device-resident fields and the orchestration of update_velocities!.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/wavi_hip.h"
@@ -90,6 +93,11 @@ struct wavi_ctx {
     double prof_ms[WAVI_KT_COUNT] = {};
     int64_t prof_n[WAVI_KT_COUNT] = {};
     int64_t prof_seen[WAVI_KT_COUNT] = {};  // launches seen; every 16th is timed
+    std::vector<hipEvent_t> burst_ev;        // event pairs around CG chunks
+    struct Burst {
+        int ev, last, n;  // first event index, index of the chunk's last launch, launches
+    };
+    std::vector<Burst> bursts;
     std::string err;
 };
 
@@ -98,9 +106,9 @@ struct ProfScope {
     wavi_ctx *ctx;
     int kid;
     int idx = -1;
-    ProfScope(wavi_ctx *c, int k) : ctx(c), kid(k)
+    ProfScope(wavi_ctx *c, int k, bool on = true) : ctx(c), kid(k)
     {
-        if (!ctx->prof) return;
+        if (!ctx->prof || !on) return;
         if ((ctx->prof_seen[kid]++ & 15) != 0) return;
         size_t need = (ctx->ev_pending.size() + 1) * 2;
         while (ctx->ev_pool.size() < need) {
@@ -131,6 +139,45 @@ static void prof_collect(wavi_ctx *ctx)
         }
     }
     ctx->ev_pending.clear();
+}
+
+// Burst timing of a chunk of back-to-back CG launches: burst_begin records
+// the opening event (returns its index, or -1 when not profiling or out of
+// events), burst_end the closing one.
+static int burst_begin(wavi_ctx *ctx)
+{
+    if (!ctx->prof) return -1;
+    const size_t need = (ctx->bursts.size() + 1) * 2;
+    while (ctx->burst_ev.size() < need) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        ctx->burst_ev.push_back(e);
+    }
+    const int idx = (int)ctx->bursts.size() * 2;
+    if (hipEventRecord(ctx->burst_ev[idx], ctx->stream) != hipSuccess) return -1;
+    return idx;
+}
+
+static void burst_end(wavi_ctx *ctx, int idx, int last, int n)
+{
+    if (idx < 0) return;
+    if (hipEventRecord(ctx->burst_ev[idx + 1], ctx->stream) != hipSuccess) return;
+    ctx->bursts.push_back({idx, last, n});
+}
+
+// After the stream synchronisation that ends a coarse CG: fold the bursts
+// whose launches all iterated (last <= last_active) into kernel id kid.
+static void burst_collect(wavi_ctx *ctx, int kid, int last_active)
+{
+    for (const wavi_ctx::Burst &b : ctx->bursts) {
+        if (b.last > last_active) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->burst_ev[b.ev], ctx->burst_ev[b.ev + 1]) == hipSuccess) {
+            ctx->prof_ms[kid] += ms;
+            ctx->prof_n[kid] += b.n;
+        }
+    }
+    ctx->bursts.clear();
 }
 
 #define HIPCHK(expr)                                                                        \
@@ -546,6 +593,7 @@ int wavi_ctx_destroy(wavi_ctx *ctx)
     if (ctx->h_norms) (void)hipHostFree(ctx->h_norms);
     if (ctx->h_st) (void)hipHostFree(ctx->h_st);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->burst_ev) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return WAVI_OK;
@@ -910,21 +958,25 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
     int q = 0, k = 0;
     while (q < nq) {
         const int n = std::min(CHUNK, nq - q);
+        // even chunks: one burst over the chunk; odd chunks: sampled launches
+        const bool sampled = (k & 1) != 0;
+        const int bev = sampled ? -1 : burst_begin(ctx);
         for (int s = 0; s < n; ++s, ++q) {
             const int par = q & 1;
             {
-                ProfScope ps(ctx, WAVI_KT_CG_DIR);
+                ProfScope ps(ctx, WAVI_KT_CG_DIR, sampled);
                 hipLaunchKernelGGL(k_cg_dir<T>, tb, dim3(256), 0, ctx->stream, a, par);
             }
             {
-                ProfScope ps(ctx, WAVI_KT_CG_APPLY);
+                ProfScope ps(ctx, WAVI_KT_CG_APPLY, sampled);
                 hipLaunchKernelGGL(k_cg_apply<T>, tb, dim3(256), 0, ctx->stream, a);
             }
             {
-                ProfScope ps(ctx, WAVI_KT_CG_UPDATE);
+                ProfScope ps(ctx, WAVI_KT_CG_UPDATE, sampled);
                 hipLaunchKernelGGL(k_cg_update<T>, tb, dim3(256), 0, ctx->stream, a, par);
             }
         }
+        burst_end(ctx, bev, q - 1, n);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ctx->cg_ev[k % 3], ctx->stream));
         if (k >= 2) {
@@ -935,6 +987,8 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     prof_collect(ctx);
+    // triplet q iterates when q < iters
+    burst_collect(ctx, WAVI_KT_CG_ITER_FP64, hf->done ? hf->iters - 1 : -1);
     if (!hf->done) {
         ctx->err = "coarse CG did not terminate";
         return WAVI_ESTATE;
@@ -1027,13 +1081,14 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     int q = 1, k = 0;
     while (q <= qmax) {
         const int n = std::min(CHUNK, qmax - q + 1);
+        const int bev = burst_begin(ctx);
         for (int s = 0; s < n; ++s, ++q) {
-            ProfScope ps(ctx, WAVI_KT_CG_ITER_MIXED);
             if (q & 1)
                 launch_cgm_iter<1, false>(ctx, tb, ia);
             else
                 launch_cgm_iter<0, false>(ctx, tb, ia);
         }
+        burst_end(ctx, bev, q - 1, n);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ctx->cg_ev[k % 3], ctx->stream));
         if (k >= 2) {
@@ -1047,6 +1102,8 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     prof_collect(ctx);
+    // launch q iterates when q <= iters
+    burst_collect(ctx, WAVI_KT_CG_ITER_MIXED, hf->done ? hf->iters : -1);
     if (!hf->done) {
         ctx->err = "coarse CG did not terminate";
         return WAVI_ESTATE;
@@ -1271,6 +1328,37 @@ int wavi_update_velocities(wavi_ctx *ctx, wavi_solve_report *report)
     return WAVI_OK;
 }
 
+int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,
+                                 wavi_solve_report *reports, int *codes)
+{
+    if (n < 0 || nthreads < 1 || (n > 0 && !ctxs)) return WAVI_EARG;
+    for (int k = 0; k < n; ++k)
+        if (!ctxs[k]) return WAVI_EARG;
+    std::vector<int> rcs(n, WAVI_OK);
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        for (int k = next.fetch_add(1); k < n; k = next.fetch_add(1)) {
+            wavi_ctx *ctx = ctxs[k];
+            if (hipSetDevice(ctx->device) != hipSuccess) {
+                ctx->err = "hipSetDevice failed";
+                rcs[k] = WAVI_EHIP;
+                continue;
+            }
+            rcs[k] = wavi_update_velocities(ctx, reports ? &reports[k] : nullptr);
+        }
+    };
+    const int nt = std::min(nthreads, std::max(n, 1));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (std::thread &t : pool) t.join();
+    if (codes)
+        for (int k = 0; k < n; ++k) codes[k] = rcs[k];
+    for (int k = 0; k < n; ++k)
+        if (rcs[k] != WAVI_OK) return rcs[k];
+    return WAVI_OK;
+}
+
 int wavi_inner_update(wavi_ctx *ctx)
 {
     if (!ctx) return WAVI_EARG;
@@ -1421,6 +1509,7 @@ int wavi_profile_enable(wavi_ctx *ctx, int enable)
     if (!ctx) return WAVI_EARG;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     prof_collect(ctx);
+    ctx->bursts.clear();
     ctx->prof = enable != 0;
     if (enable)
         for (int k = 0; k < WAVI_KT_COUNT; ++k) {

@@ -374,6 +374,32 @@ def update_velocities(model, verbose=False):
     return model
 
 
+def update_velocities_batch(models, nthreads=8, verbose=False):
+    """update_velocities! on every model of an ensemble in one native call:
+    host threads drive the members concurrently, each on its own stream, so
+    small members share the GPU.  Equivalent, member by member, to calling
+    update_velocities on each model in turn."""
+    models = list(models)
+    for m in models:
+        update_preconditioner(m)
+        push_fields(m)
+    if not models:
+        return models
+    lib = models[0].parallel_spec._lib
+    ctxs = (ctypes.c_void_p * len(models))(*[m.parallel_spec.ctx for m in models])
+    reps = (_lib.SolveReport * len(models))()
+    codes = (ctypes.c_int * len(models))()
+    rc = lib.wavi_update_velocities_batch(ctxs, len(models), int(nthreads), reps, codes)
+    if rc != _lib.WAVI_OK:
+        for m, code in zip(models, codes):
+            _check(lib, m.parallel_spec.ctx, code)
+        _check(lib, None, rc)
+    for m, rep in zip(models, reps):
+        pull_fields(m)
+        _report_solve(m, rep, verbose)
+    return models
+
+
 def inner_update(model):
     """inner_update!(model) (update_velocities.jl:32-45)."""
     update_preconditioner(model)

@@ -9,6 +9,10 @@ Every workload is a WAVI model setup:
   channel_solve        a single velocity solve on an nx x ny grid (1280x160 at
                        500 m or 2560x320 at 250 m), nσ=8, 20 Picard iterations
                        to tol 1e-8, tol_coarse 1e-8, 3 Haar levels, dt 1.0
+  ensemble_members     64 channel solves of seeded sizes and parameters
+                       (grid from {160x20, 320x40, 640x80, 1280x160},
+                       nσ from {4, 8}, A log-uniform in [1e-17, 4e-17],
+                       log C = log 1e4 + a seeded random field of std 0.3)
 
 Channel bed: the MISMIP+ along-flow sixth-order polynomial (from -150 m down
 to the -720 m floor, with its retrograde section), plus side-wall ridges of
@@ -106,3 +110,39 @@ def channel_solve(nx=1280, ny=160, precision="fp64", seed=2, nσ=8, maxiter_pica
     ic = InitialConditions(initial_thickness=H)
     return Model(grid=grid, bed_elevation=bed, params=params, solver_params=solver_params,
                  initial_conditions=ic, parallel_spec=HIPSpec(precision))
+
+
+ENSEMBLE_SIZES = ((160, 20), (320, 40), (640, 80), (1280, 160))
+
+
+def ensemble_specs(n=64, seed=5):
+    """Seeded member list of the ensemble: dicts with nx, ny, nσ, A (Pa^-3 a^-1)
+    and the seed of the member's log C field and roughness."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(n):
+        nx, ny = ENSEMBLE_SIZES[rng.integers(len(ENSEMBLE_SIZES))]
+        nσ = (4, 8)[rng.integers(2)]
+        A = float(np.exp(rng.uniform(np.log(1e-17), np.log(4e-17))))
+        specs.append({"nx": int(nx), "ny": int(ny), "nσ": int(nσ), "A": A,
+                      "seed": int(seed * 1000 + k)})
+    return specs
+
+
+def ensemble_member(spec, precision="mixed", maxiter_picard=20):
+    """One ensemble member: channel_solve at the member's size and nσ, with
+    Glen A scaled to spec["A"] (A is proportional to glen_a_ref at a fixed
+    temperature) and C = 1e4 exp(f), f a Gaussian random field of std 0.3 and
+    correlation length 10 km."""
+    nx, ny = spec["nx"], spec["ny"]
+    Lx, Ly = 640000.0, 80000.0
+    glen_a_ref = Params().glen_a_ref * spec["A"] / 2e-17
+    f = gaussian_random_field(nx, ny, Lx / nx, Ly / ny, 0.3, 10000.0, spec["seed"])
+    return channel_solve(nx=nx, ny=ny, precision=precision, seed=spec["seed"], nσ=spec["nσ"],
+                         maxiter_picard=maxiter_picard, weertman_c=np.asfortranarray(1e4 * np.exp(f)),
+                         glen_a_ref=glen_a_ref)
+
+
+def ensemble_members(n=64, seed=5, precision="mixed", maxiter_picard=20):
+    """The n members of the seeded ensemble, as Models."""
+    return [ensemble_member(sp, precision, maxiter_picard) for sp in ensemble_specs(n, seed)]
