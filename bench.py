@@ -246,6 +246,95 @@ def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed"):
     }
 
 
+def adjoint_run(W, nx, ny, steps, warmup, ndir=16, eps=1e-6, n_picard=10, tol=1e-13, maxit=200):
+    """The basal-drag inversion gradient: observations from an FP64 forward
+    at the true log C (log 1e4 + GRF(0.5, 10 km)) times (1 + 0.01 N(0, 1));
+    dJ/dlog C at the prior log C = log 1e4 timed in FP64 and mixed precision,
+    and checked against central differences (eps) along ndir seeded
+    directions with each forward.  Returns the JSON line."""
+    lib = W._lib.load()
+    lt = W.workloads.inversion_logc_true(nx, ny, seed=4)
+    log("[bench] adjoint: observations from the FP64 forward at the true log C")
+    m = W.workloads.inversion_model(lt, "fp64", nσ=8, n_picard=n_picard)
+    W.update_geometry(m)
+    W.misfit_gradient(m, n_picard, np.zeros((nx + 1, ny)), np.zeros((nx, ny + 1)), tol, maxit, gradient=False)
+    uo, vo = W.workloads.inversion_observations(m.fields.gu.u, m.fields.gv.v, seed=4)
+    uo = np.ascontiguousarray(uo.ravel(order="F"))
+    vo = np.ascontiguousarray(vo.ravel(order="F"))
+    del m
+    l0 = np.log(1e4) * np.ones((nx, ny))
+    rng = np.random.default_rng(16)
+    dirs = [rng.standard_normal((nx, ny)) for _ in range(ndir)]
+    out, grads, fds = {}, {}, {}
+    for prec in ("fp64", "mixed"):
+        m = W.workloads.inversion_model(l0, prec, nσ=8, n_picard=n_picard)
+        W.update_geometry(m)
+        spec = m.parallel_spec
+        assert lib.wavi_state_save(spec.ctx) == 0
+        J = ctypes.c_double(0.0)
+        g = np.zeros(nx * ny)
+        rep = W._lib.AdjointReport()
+
+        def run(logc, grad):
+            assert lib.wavi_state_restore(spec.ctx) == 0
+            c = np.ascontiguousarray(np.exp(logc).ravel(order="F"))
+            assert lib.wavi_set_field(spec.ctx, W._lib.FIELD["P_WEERTMAN_C"], W._lib.dptr(c), c.size) == 0
+            assert lib.wavi_update_geometry(spec.ctx) == 0
+            t0 = time.perf_counter()
+            rc = lib.wavi_misfit_gradient(spec.ctx, n_picard, tol, maxit, W._lib.dptr(uo), W._lib.dptr(vo),
+                                          ctypes.byref(J), W._lib.dptr(g) if grad else None, ctypes.byref(rep))
+            dt = time.perf_counter() - t0
+            if rc != 0:
+                raise RuntimeError("wavi_misfit_gradient: %s" % lib.wavi_last_error(spec.ctx))
+            return J.value, dt
+
+        for k in range(warmup):
+            log("[bench] adjoint %s warmup %d: %.3f s" % (prec, k, run(l0, True)[1]))
+        times = []
+        for k in range(steps):
+            times.append(run(l0, True)[1])
+            r = rep.as_dict()
+            log("[bench] adjoint %s step %d: %.3f s (V-cycles %d forward, %d adjoint; max rel %.1e / %.1e)"
+                % (prec, k, times[-1], r["fwd_vcycles"], r["adj_vcycles"], r["max_fwd_rel"], r["max_adj_rel"]))
+        J0, r0 = J.value, rep.as_dict()
+        grads[prec] = g.reshape((nx, ny), order="F").copy()
+        _, tf = run(l0, False)
+        fds[prec] = {}
+        for e in (eps, 10 * eps):
+            fds[prec][e] = [(run(l0 + e * d, False)[0] - run(l0 - e * d, False)[0]) / (2 * e) for d in dirs]
+        out[prec] = {"ms": 1e3 * float(np.median(times)), "forward_ms": 1e3 * tf, "J": J0,
+                     "grad_norm": float(np.linalg.norm(grads[prec])), "report": r0}
+        del m
+
+    def fd_err(gprec, fprec, e):
+        errs = [abs(fd - float(np.sum(grads[gprec] * d))) / abs(float(np.sum(grads[gprec] * d)))
+                for fd, d in zip(fds[fprec][e], dirs)]
+        return {"max_rel": max(errs), "median_rel": float(np.median(errs))}
+
+    for prec in ("fp64", "mixed"):
+        out[prec]["fd_eps_%.0e" % eps] = fd_err(prec, prec, eps)
+        out[prec]["fd_eps_%.0e" % (10 * eps)] = fd_err(prec, prec, 10 * eps)
+        log("[bench] adjoint %s FD check (own forward): eps %.0e max rel %.2e, eps %.0e max rel %.2e"
+            % (prec, eps, out[prec]["fd_eps_%.0e" % eps]["max_rel"], 10 * eps,
+               out[prec]["fd_eps_%.0e" % (10 * eps)]["max_rel"]))
+    out["mixed"]["gradient_vs_fp64_forward_fd_eps_%.0e" % eps] = fd_err("mixed", "fp64", eps)
+    out["mixed"]["gradient_rel_diff_to_fp64"] = float(np.linalg.norm(grads["mixed"] - grads["fp64"]) /
+                                                      np.linalg.norm(grads["fp64"]))
+    cfg = {"workload": "basal-drag inversion gradient dJ/dlogC at %dx%d, nsigma 8, %d Picard iterations with "
+                       "linear solves to %.0e (flexible CG, V-cycle preconditioner)" % (nx, ny, n_picard, tol),
+           "fd_check": "central differences along %d seeded N(0,1) directions with each precision's own forward, "
+                       "at eps %.0e and %.0e; the mixed gradient also against the FP64 forward's differences"
+                       % (ndir, eps, 10 * eps),
+           "fp64": out["fp64"], "mixed": out["mixed"],
+           "mixed_over_fp64": out["fp64"]["ms"] / out["mixed"]["ms"]}
+    return {
+        "metric": "adjoint ms: one dJ/dlogC (forward + reverse sweep) at %dx%dx8" % (nx, ny),
+        "value": out["mixed"]["ms"], "unit": "ms", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": out["mixed"]["ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic (seeded true log C field and observation noise)", "config": cfg,
+    }
+
+
 def cpu_baseline(W, nx, ny, gpu_report):
     """The oracle (a single-threaded C restatement of the reference algorithm)
     on a bounded sample of the same solve: the first Picard iteration with the
@@ -295,6 +384,9 @@ def main():
     ap.add_argument("--ensemble", type=int, default=0,
                     help="time the seeded ensemble of this many members instead of the single solve")
     ap.add_argument("--threads", type=int, default=8, help="host threads of the ensemble batch")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time the basal-drag gradient (FP64 and mixed) and check it by finite differences")
+    ap.add_argument("--fd-directions", type=int, default=16)
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="profiling runs only: issue one device-to-device copy of this size first")
     args = ap.parse_args()
@@ -303,6 +395,9 @@ def main():
               "--gpus %d is not supported (multi-GPU sharding is a design note only)" % args.gpus)
         sys.exit(2)
     W = _package()
+    if args.adjoint:
+        print(json.dumps(adjoint_run(W, args.nx, args.ny, args.steps, args.warmup, args.fd_directions)), flush=True)
+        return
     if args.ensemble:
         prec = "fp64" if args.precision == "fp64" else "mixed"
         print(json.dumps(ensemble_run(W, args.ensemble, args.threads, args.steps, args.warmup, prec)),

@@ -194,6 +194,37 @@ int wavi_run_steps(wavi_ctx *ctx, int nsteps, wavi_solve_report *last_report);
 /* update_wavelets!(model) (src/Wavelets/update_wavelets.jl:6-32). */
 int wavi_update_wavelets(wavi_ctx *ctx);
 
+/* ---- adjoint basal-drag gradient (no reference counterpart: WAVI.jl has
+ * no adjoint; this extends its solve for inversions) ----
+ * The map.  From the context's current state (velocities, viscosity,
+ * drag, quadrature f2, as update_velocities! would start from them):
+ * n_picard Picard iterations, each inner_update! (update_velocities.jl:32-45)
+ * followed by the linear system A u = b of that iteration solved to
+ * ||b - A u|| <= tol_linear ||b|| by flexible CG in FP64 preconditioned
+ * with the V-cycle of update_velocities! in the context's precision (at
+ * most maxit_linear V-cycles per solve); then
+ * J = 1/2 sum over inner u and v points of (u - u_obs)^2.
+ * grad_logc (NULL: the forward and J only) receives dJ/d log C on the h
+ * grid (nx*ny, column-major), C the per-cell Weertman coefficient
+ * P_WEERTMAN_C, through H_WEERTMAN_C = C * grounded fraction: call
+ * wavi_update_geometry after setting C.  The gradient is exact for the map
+ * (reverse sweep over the checkpointed Picard states, with the adjoint
+ * systems A_k lambda = ubar solved like the forward ones: A_k is symmetric).
+ * u_obs: (nx+1)*ny, v_obs: nx*(ny+1).  The context is left in the state of
+ * the forward's last Picard iteration.  Fixed velocities (u_isfixed,
+ * v_isfixed) are rejected with WAVI_EARG. */
+typedef struct wavi_adjoint_report {
+    int32_t n_picard;
+    int32_t fwd_vcycles;   /* V-cycles of all forward linear solves */
+    int32_t adj_vcycles;   /* V-cycles of all adjoint linear solves */
+    int32_t unconverged;   /* linear solves stopped by maxit_linear */
+    double max_fwd_rel;    /* largest final ||b - A u|| / ||b|| of a forward solve */
+    double max_adj_rel;    /* the same for the adjoint solves */
+} wavi_adjoint_report;
+int wavi_misfit_gradient(wavi_ctx *ctx, int n_picard, double tol_linear, int maxit_linear,
+                         const double *u_obs, const double *v_obs, double *J, double *grad_logc,
+                         wavi_adjoint_report *report);
+
 /* ---- test hooks: single pieces of the solve on host arrays ---- */
 /* op_fun! (src/utilities.jl:61-145) on full-grid inputs xu ((nx+1)*ny) and
  * xv (nx*(ny+1)); the output holds the operator at inner points and zero

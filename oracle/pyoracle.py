@@ -55,6 +55,8 @@ def load():
         "om_canonical_dot": (ctypes.c_double, [vp, _dp, _dp, _dp, _dp]),
         "om_det_pow": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
         "om_pos_fraction": (None, [ctypes.c_int, ctypes.c_int, _dp, _u8p, _dp, _dp, _dp]),
+        "om_misfit_gradient": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                              ctypes.POINTER(ctypes.c_double), _dp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -212,6 +214,19 @@ class OracleModel:
 
     def update_geometry(self):
         self._lib.om_update_geometry(self.h)
+
+    def misfit_gradient(self, n_picard, tol_linear, maxit_linear, u_obs, v_obs, gradient=True):
+        """J and (gradient=True) dJ/dlog C of the adjoint map; see
+        wavi_misfit_gradient.  Returns (J, grad (nx, ny) or None, report)."""
+        uo, vo = self._u(u_obs), self._u(v_obs)
+        J = ctypes.c_double(0.0)
+        g = np.zeros(self.nx * self.ny) if gradient else None
+        rep = self._pkg._lib.AdjointReport()
+        rc = self._lib.om_misfit_gradient(self.h, n_picard, tol_linear, maxit_linear, _d(uo), _d(vo),
+                                          ctypes.byref(J), _d(g) if gradient else None, ctypes.addressof(rep))
+        if rc != 0:
+            raise ValueError("om_misfit_gradient: code %d" % rc)
+        return J.value, (g.reshape((self.nx, self.ny), order="F") if gradient else None), rep.as_dict()
 
     def update_thickness(self, dt):
         self._lib.om_update_thickness(self.h, dt)

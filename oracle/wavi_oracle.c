@@ -1270,3 +1270,333 @@ EXPORT void om_run_steps(om_model *m, int nsteps, wavi_solve_report *rep)
         om_update_thickness(m, m->P.dt);
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Adjoint basal-drag gradient (no reference counterpart: WAVI.jl has no
+ * adjoint).  The map and its gradient are specified beside
+ * wavi_misfit_gradient in include/wavi_hip.h.  The forward steps are
+ * inner_update! (update_velocities.jl:32-45, restated above) and linear
+ * solves by the V-cycle of update_preconditioners.jl:54-83 iterated on the
+ * residual; the reverse sweep is the chain rule of those steps, written
+ * out by hand.  Sums here follow no fixed order: the GPU gradient is
+ * compared with this one to a tolerance, and both with central finite
+ * differences of the map.                                             */
+
+/* x = A^{-1} b by V-cycles on the residual, from x = 0.  Each V-cycle's
+ * coarse CG starts from zero (the Picard loop's warm start would be the
+ * previous, much larger correction). */
+static int solve_linear(om_model *m, const double *bu, const double *bv, double *xu, double *xv,
+                        double tol, int maxit, int *vcycles, double *rel_out)
+{
+    int nu = m->nu, nv = m->nv;
+    double *ru = (double *)malloc(sizeof(double) * nu), *rv = (double *)malloc(sizeof(double) * nv);
+    double *eu = (double *)malloc(sizeof(double) * nu), *ev = (double *)malloc(sizeof(double) * nv);
+    for (int k = 0; k < nu; ++k) xu[k] = 0.0;
+    for (int k = 0; k < nv; ++k) xv[k] = 0.0;
+    double bn = canonical_norm(m, bu, bv), rel = 0.0;
+    int it = 0;
+    for (;;) {
+        resid(m, xu, xv, bu, bv, ru, rv);
+        rel = bn > 0.0 ? canonical_norm(m, ru, rv) / bn : 0.0;
+        if (rel <= tol || it >= maxit) break;
+        memset(m->f[WAVI_WU_CORRECTION_COARSE], 0, sizeof(double) * nu);
+        memset(m->f[WAVI_WV_CORRECTION_COARSE], 0, sizeof(double) * nv);
+        om_vcycle(m, ru, rv, eu, ev);
+        for (int k = 0; k < nu; ++k) if (m->u_inner[k]) xu[k] = xu[k] + eu[k];
+        for (int k = 0; k < nv; ++k) if (m->v_inner[k]) xv[k] = xv[k] + ev[k];
+        ++it;
+    }
+    free(ru); free(rv); free(eu); free(ev);
+    *vcycles += it;
+    *rel_out = rel;
+    return rel <= tol;
+}
+
+typedef struct {
+    double *u, *v, *eta, *beta, *f2;
+} om_ckpt;
+
+static void ckpt_alloc(const om_model *m, om_ckpt *c)
+{
+    c->u = (double *)malloc(sizeof(double) * m->nu);
+    c->v = (double *)malloc(sizeof(double) * m->nv);
+    c->eta = (double *)malloc(sizeof(double) * m->nh * m->ns);
+    c->beta = (double *)malloc(sizeof(double) * m->nh);
+    c->f2 = (double *)malloc(sizeof(double) * m->nh);
+}
+
+static void ckpt_free(om_ckpt *c)
+{
+    free(c->u); free(c->v); free(c->eta); free(c->beta); free(c->f2);
+}
+
+static void ckpt_save(const om_model *m, om_ckpt *c)
+{
+    memcpy(c->u, m->f[WAVI_U_U], sizeof(double) * m->nu);
+    memcpy(c->v, m->f[WAVI_V_V], sizeof(double) * m->nv);
+    memcpy(c->eta, m->f[WAVI_S_ETA], sizeof(double) * m->nh * m->ns);
+    memcpy(c->beta, m->f[WAVI_H_BETA], sizeof(double) * m->nh);
+    memcpy(c->f2, m->f[WAVI_H_QUAD_F2], sizeof(double) * m->nh);
+}
+
+static void ckpt_load(om_model *m, const om_ckpt *c)
+{
+    memcpy(m->f[WAVI_U_U], c->u, sizeof(double) * m->nu);
+    memcpy(m->f[WAVI_V_V], c->v, sizeof(double) * m->nv);
+    memcpy(m->f[WAVI_S_ETA], c->eta, sizeof(double) * m->nh * m->ns);
+    memcpy(m->f[WAVI_H_BETA], c->beta, sizeof(double) * m->nh);
+    memcpy(m->f[WAVI_H_QUAD_F2], c->f2, sizeof(double) * m->nh);
+}
+
+/* Reverse of Picard iteration k.  Fields on entry: inner_update! of
+ * iteration k recomputed from checkpoint k-1 (ck_old: u_{k-1} and the
+ * eta, beta, f2 it started from).  uk, vk: u_k; lu, lv: lambda_k =
+ * A_k^{-1} ubar_k.  etab, betab, f2b: in, the adjoint of the eta, beta, f2
+ * leaving iteration k; out, of those entering it.  ubp, vbp: out, the
+ * adjoint of u_{k-1} at inner points.  wcb: += dJ/d(weertman_c on h). */
+static void picard_reverse(om_model *m, const om_ckpt *ck_old, const double *uk, const double *vk,
+                           const double *lu, const double *lv, double *etab, double *betab, double *f2b,
+                           double *ubp, double *vbp, double *wcb)
+{
+    const int nx = m->nx, ny = m->ny, ns = m->ns, nh = m->nh, nc = m->nc;
+    const double rdx = m->rdx, rdy = m->rdy;
+    const double *up = ck_old->u, *vp = ck_old->v;
+    const double *h = m->f[WAVI_H_H];
+    const double *sr = m->f[WAVI_H_SHELF_STRAIN_RATE], *avs = m->f[WAVI_H_AV_SPEED];
+    const double *bs = m->f[WAVI_H_BED_SPEED], *beta = m->f[WAVI_H_BETA], *tau = m->f[WAVI_H_TAU_BED];
+    const double *f2 = m->f[WAVI_H_QUAD_F2], *eta = m->f[WAVI_S_ETA], *glen_b = m->f[WAVI_S_GLEN_B];
+    const double *gfh = m->f[WAVI_H_GROUNDED_FRACTION];
+    const double *gfu = m->f[WAVI_U_GROUNDED_FRACTION], *gfv = m->f[WAVI_V_GROUNDED_FRACTION];
+    double *hhb = (double *)calloc(nh, sizeof(double));
+    double *hcb = (double *)calloc(nc > 0 ? nc : 1, sizeof(double));
+    double *scb = (double *)calloc(nc > 0 ? nc : 1, sizeof(double));
+    double *sc = (double *)calloc(nc > 0 ? nc : 1, sizeof(double));
+    double *bub = (double *)calloc(m->nu, sizeof(double)), *bvb = (double *)calloc(m->nv, sizeof(double));
+    double *cub = (double *)calloc(nh, sizeof(double)), *cvb = (double *)calloc(nh, sizeof(double));
+    double *uxb = (double *)calloc(nh, sizeof(double)), *vyb = (double *)calloc(nh, sizeof(double));
+    double *shb = (double *)calloc(nh, sizeof(double));
+    /* 1. coefficient adjoints cbar = -d(lambda^T A(c) u_k)/dc of op_fun!:
+     *    h eta on h (extensional stresses), h eta on c (shear), beta_eff on
+     *    u and v (drag) */
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            int k = HI(i, j);
+            if (!m->h_mask[k]) continue;
+            double dudx = rdx * uk[UI(i + 1, j)] - rdx * uk[UI(i, j)];
+            double dvdy = rdy * vk[VI(i, j + 1)] - rdy * vk[VI(i, j)];
+            double dlx = rdx * lu[UI(i + 1, j)] - rdx * lu[UI(i, j)];
+            double dly = rdy * lv[VI(i, j + 1)] - rdy * lv[VI(i, j)];
+            hhb[k] = 2.0 * (2.0 * dudx + dvdy) * dlx + 2.0 * (2.0 * dvdy + dudx) * dly;
+        }
+    for (int j = 0; j < ny - 1; ++j)
+        for (int i = 0; i < nx - 1; ++i) {
+            int k = CI(i, j);
+            if (!m->c_mask[k]) continue;
+            double s = (rdy * uk[UI(i + 1, j + 1)] - rdy * uk[UI(i + 1, j)]) +
+                       (rdx * vk[VI(i + 1, j + 1)] - rdx * vk[VI(i, j + 1)]);
+            double l = (rdy * lu[UI(i + 1, j + 1)] - rdy * lu[UI(i + 1, j)]) +
+                       (rdx * lv[VI(i + 1, j + 1)] - rdx * lv[VI(i, j + 1)]);
+            hcb[k] = s * l;
+        }
+    for (int k = 0; k < m->nu; ++k) bub[k] = m->u_mask[k] ? lu[k] * uk[k] : 0.0;
+    for (int k = 0; k < m->nv; ++k) bvb[k] = m->v_mask[k] ? lv[k] * vk[k] : 0.0;
+    /* shear strain of u_{k-1} on c (update_shelf_strain_rate!) */
+    for (int j = 0; j < ny - 1; ++j)
+        for (int i = 0; i < nx - 1; ++i) {
+            double u1 = m->u_mask[UI(i + 1, j + 1)] ? up[UI(i + 1, j + 1)] : 0.0;
+            double u0 = m->u_mask[UI(i + 1, j)] ? up[UI(i + 1, j)] : 0.0;
+            double v1 = m->v_mask[VI(i + 1, j + 1)] ? vp[VI(i + 1, j + 1)] : 0.0;
+            double v0 = m->v_mask[VI(i, j + 1)] ? vp[VI(i, j + 1)] : 0.0;
+            sc[CI(i, j)] = m->c_mask[CI(i, j)] ? (rdy * u1 - rdy * u0) + (rdx * v1 - rdx * v0) : 0.0;
+        }
+    /* 2. per h cell, in reverse order of inner_update! */
+    const double gexp = 1.0 / m->P.glen_n - 1.0, wexp = 1.0 / m->P.weertman_m - 1.0;
+    const double greg2 = m->P.glen_reg_strain_rate * m->P.glen_reg_strain_rate;
+    const double wreg2 = m->P.weertman_reg_speed * m->P.weertman_reg_speed;
+    const int npass = m->S.n_iter_viscosity;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            int k = HI(i, j);
+            /* update_rheological_operators!: h eta_av on h and (cent) on c */
+            double hqb = m->h_mask[k] ? hhb[k] : 0.0;
+            for (int b = j - 1; b <= j; ++b)
+                for (int a = i - 1; a <= i; ++a)
+                    if (a >= 0 && a < nx - 1 && b >= 0 && b < ny - 1 && m->c_mask[CI(a, b)])
+                        hqb += 0.25 * hcb[CI(a, b)];
+            double etaavb = h[k] * hqb;
+            /* update_betaeff_on_uv_grids! */
+            double beffb = 0.0;
+            if (m->h_mask[k]) {
+                for (int a = i; a <= i + 1; ++a) {
+                    int ku = UI(a, j);
+                    if (!m->u_mask[ku]) continue;
+                    double den = mask_avg_u(m, a, j), ip = avgm_h_to_u(m, gfh, a, j) / den;
+                    beffb += bub[ku] * (0.5 / den) * (ip > 0.0 ? gfu[ku] / ip : 1.0);
+                }
+                for (int b = j; b <= j + 1; ++b) {
+                    int kv = VI(i, b);
+                    if (!m->v_mask[kv]) continue;
+                    double den = mask_avg_v(m, i, b), ip = avgm_h_to_v(m, gfh, i, b) / den;
+                    beffb += bvb[kv] * (0.5 / den) * (ip > 0.0 ? gfv[kv] / ip : 1.0);
+                }
+            }
+            /* update_betaeff!: beta_eff = beta / (1 + f2 beta) */
+            double D = 1.0 + f2[k] * beta[k];
+            double bb = betab[k] + beffb / (D * D);
+            double fb = f2b[k] - beffb * (beta[k] * beta[k]) / (D * D);
+            double taub = 0.0, srb = 0.0;
+            if (m->h_mask[k]) {
+                for (int s = 0; s < ns; ++s) {
+                    int k3 = k + nh * s;
+                    double z = m->zeta[s];
+                    /* update_av_viscosity!, update_quadrature_falpha! */
+                    double eb = etab[k3] + m->qw[s] * etaavb -
+                                fb * ((m->qw[s] * h[k]) * (z * z)) / (eta[k3] * eta[k3]);
+                    /* inner_update_viscosity!: npass fixed-point passes */
+                    double e[9];
+                    e[0] = ck_old->eta[k3];
+                    for (int p = 0; p < npass; ++p) {
+                        double q = (tau[k] * z) / e[p];
+                        double arg = sqrt((sr[k] * sr[k] + 0.25 * (q * q)) + greg2);
+                        e[p + 1] = (0.5 * glen_b[k3]) * om_det_pow(arg, gexp);
+                    }
+                    for (int p = npass - 1; p >= 0; --p) {
+                        double q = (tau[k] * z) / e[p];
+                        double arg = sqrt((sr[k] * sr[k] + 0.25 * (q * q)) + greg2);
+                        double g = eb * e[p + 1] * gexp / arg;
+                        srb += g * sr[k] / arg;
+                        double qb = g * 0.25 * q / arg;
+                        taub += qb * z / e[p];
+                        eb = -qb * q / e[p];
+                    }
+                    etab[k3] = eb;
+                }
+            }
+            /* update_basal_drag!: tau = beta bs; update_beta!: beta =
+             * wc (bs^2 + reg^2)^(wexp/2) */
+            double bsb = taub * beta[k];
+            bb += taub * bs[k];
+            double S = bs[k] * bs[k] + wreg2;
+            wcb[k] += bb * om_det_pow(sqrt(S), wexp);
+            bsb += bb * beta[k] * wexp * bs[k] / S;
+            /* update_bed_speed!: bs = avs / (1 + f2_old beta_old) */
+            double D0 = 1.0 + ck_old->f2[k] * ck_old->beta[k];
+            double avsb = bsb / D0;
+            f2b[k] = -bsb * avs[k] * ck_old->beta[k] / (D0 * D0);
+            betab[k] = -bsb * avs[k] * ck_old->f2[k] / (D0 * D0);
+            if (!m->h_mask[k]) continue;
+            /* update_av_speed! of u_{k-1} */
+            double u0 = m->u_mask[UI(i, j)] ? up[UI(i, j)] : 0.0, u1 = m->u_mask[UI(i + 1, j)] ? up[UI(i + 1, j)] : 0.0;
+            double v0 = m->v_mask[VI(i, j)] ? vp[VI(i, j)] : 0.0, v1 = m->v_mask[VI(i, j + 1)] ? vp[VI(i, j + 1)] : 0.0;
+            double cu = 0.5 * u0 + 0.5 * u1, cv = 0.5 * v0 + 0.5 * v1;
+            if (avs[k] > 0.0) {
+                cub[k] = avsb * cu / avs[k];
+                cvb[k] = avsb * cv / avs[k];
+            }
+            /* update_shelf_strain_rate! of u_{k-1} */
+            if (sr[k] > 0.0) {
+                double ux = rdx * u1 - rdx * u0, vy = rdy * v1 - rdy * v0, sh = cent_c_to_h(m, sc, i, j);
+                uxb[k] = srb * (2.0 * ux + vy) / (2.0 * sr[k]);
+                vyb[k] = srb * (2.0 * vy + ux) / (2.0 * sr[k]);
+                shb[k] = srb * 0.25 * sh / sr[k];
+            }
+        }
+    /* 3. shear on c: sh = cent(sc) on h */
+    for (int j = 0; j < ny - 1; ++j)
+        for (int i = 0; i < nx - 1; ++i) {
+            int k = CI(i, j);
+            if (!m->c_mask[k]) continue;
+            scb[k] = 0.25 * (shb[HI(i, j)] + shb[HI(i + 1, j)] + shb[HI(i, j + 1)] + shb[HI(i + 1, j + 1)]);
+        }
+    /* 4. adjoint of u_{k-1} at inner points */
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i <= nx; ++i) {
+            int k = UI(i, j);
+            double a = 0.0;
+            if (m->u_inner[k]) {
+                if (i >= 1) a += 0.5 * cub[HI(i - 1, j)] + rdx * uxb[HI(i - 1, j)];
+                if (i <= nx - 1) a += 0.5 * cub[HI(i, j)] - rdx * uxb[HI(i, j)];
+                if (i >= 1 && i <= nx - 1) {
+                    if (j >= 1) a += rdy * scb[CI(i - 1, j - 1)];
+                    if (j <= ny - 2) a -= rdy * scb[CI(i - 1, j)];
+                }
+            }
+            ubp[k] = a;
+        }
+    for (int j = 0; j <= ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            int k = VI(i, j);
+            double a = 0.0;
+            if (m->v_inner[k]) {
+                if (j >= 1) a += 0.5 * cvb[HI(i, j - 1)] + rdy * vyb[HI(i, j - 1)];
+                if (j <= ny - 1) a += 0.5 * cvb[HI(i, j)] - rdy * vyb[HI(i, j)];
+                if (j >= 1 && j <= ny - 1) {
+                    if (i >= 1) a += rdx * scb[CI(i - 1, j - 1)];
+                    if (i <= nx - 2) a -= rdx * scb[CI(i, j - 1)];
+                }
+            }
+            vbp[k] = a;
+        }
+    free(hhb); free(hcb); free(scb); free(sc); free(bub); free(bvb);
+    free(cub); free(cvb); free(uxb); free(vyb); free(shb);
+}
+
+EXPORT int om_misfit_gradient(om_model *m, int npic, double tol, int maxit, const double *uobs,
+                              const double *vobs, double *J, double *grad, wavi_adjoint_report *rep)
+{
+    const int nu = m->nu, nv = m->nv, nh = m->nh;
+    for (int k = 0; k < nu; ++k) if (m->u_isfixed[k]) return WAVI_EARG;
+    for (int k = 0; k < nv; ++k) if (m->v_isfixed[k]) return WAVI_EARG;
+    if (npic < 1 || m->S.n_iter_viscosity > 8) return WAVI_EARG;
+    wavi_adjoint_report lrep;
+    if (!rep) rep = &lrep;
+    memset(rep, 0, sizeof(*rep));
+    rep->n_picard = npic;
+    om_ckpt *ck = (om_ckpt *)malloc(sizeof(om_ckpt) * (npic + 1));
+    for (int k = 0; k <= npic; ++k) ckpt_alloc(m, &ck[k]);
+    double *bu = (double *)malloc(sizeof(double) * nu), *bv = (double *)malloc(sizeof(double) * nv);
+    double *xu = (double *)malloc(sizeof(double) * nu), *xv = (double *)malloc(sizeof(double) * nv);
+    /* forward */
+    for (int k = 0; k < npic; ++k) {
+        ckpt_save(m, &ck[k]);
+        om_inner_update(m);
+        om_rhs(m, bu, bv);
+        double rel;
+        if (!solve_linear(m, bu, bv, xu, xv, tol, maxit, &rep->fwd_vcycles, &rel)) rep->unconverged++;
+        if (rel > rep->max_fwd_rel) rep->max_fwd_rel = rel;
+        for (int q = 0; q < nu; ++q) if (m->u_inner[q]) m->f[WAVI_U_U][q] = xu[q];
+        for (int q = 0; q < nv; ++q) if (m->v_inner[q]) m->f[WAVI_V_V][q] = xv[q];
+    }
+    ckpt_save(m, &ck[npic]);
+    double js = 0.0;
+    for (int q = 0; q < nu; ++q) if (m->u_inner[q]) { double d = m->f[WAVI_U_U][q] - uobs[q]; js += d * d; }
+    for (int q = 0; q < nv; ++q) if (m->v_inner[q]) { double d = m->f[WAVI_V_V][q] - vobs[q]; js += d * d; }
+    *J = 0.5 * js;
+    if (grad) {
+        double *ubar = (double *)calloc(nu, sizeof(double)), *vbar = (double *)calloc(nv, sizeof(double));
+        double *etab = (double *)calloc((size_t)nh * m->ns, sizeof(double));
+        double *betab = (double *)calloc(nh, sizeof(double)), *f2b = (double *)calloc(nh, sizeof(double));
+        double *wcb = (double *)calloc(nh, sizeof(double));
+        for (int q = 0; q < nu; ++q) ubar[q] = m->u_inner[q] ? ck[npic].u[q] - uobs[q] : 0.0;
+        for (int q = 0; q < nv; ++q) vbar[q] = m->v_inner[q] ? ck[npic].v[q] - vobs[q] : 0.0;
+        for (int k = npic; k >= 1; --k) {
+            ckpt_load(m, &ck[k - 1]);
+            om_inner_update(m);
+            double rel;
+            if (!solve_linear(m, ubar, vbar, xu, xv, tol, maxit, &rep->adj_vcycles, &rel)) rep->unconverged++;
+            if (rel > rep->max_adj_rel) rep->max_adj_rel = rel;
+            picard_reverse(m, &ck[k - 1], ck[k].u, ck[k].v, xu, xv, etab, betab, f2b, ubar, vbar, wcb);
+        }
+        const double *wc = m->f[WAVI_H_WEERTMAN_C];
+        for (int q = 0; q < nh; ++q) grad[q] = wc[q] * wcb[q];
+        /* leave the state of the last forward iteration */
+        ckpt_load(m, &ck[npic - 1]);
+        om_inner_update(m);
+        memcpy(m->f[WAVI_U_U], ck[npic].u, sizeof(double) * nu);
+        memcpy(m->f[WAVI_V_V], ck[npic].v, sizeof(double) * nv);
+        free(ubar); free(vbar); free(etab); free(betab); free(f2b); free(wcb);
+    }
+    for (int k = 0; k <= npic; ++k) ckpt_free(&ck[k]);
+    free(ck); free(bu); free(bv); free(xu); free(xv);
+    return 0;
+}

@@ -1,0 +1,79 @@
+"""Adjoint basal-drag gradient on the GPU (wavi_misfit_gradient): the
+gradient against the oracle's reverse sweep, and against central finite
+differences of the map itself in both precisions."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N_PICARD = 3
+TOL_LIN = 1e-13
+
+
+def _step(name):
+    print("[gpu] %s" % name, flush=True)
+    sys.stdout.flush()
+
+
+def _model(W, logc, precision, nsig=4):
+    m = W.workloads.inversion_model(logc, precision, nσ=nsig, n_picard=N_PICARD)
+    W.update_geometry(m)
+    return m
+
+
+def _setup(W, nx, ny):
+    lt = W.workloads.inversion_logc_true(nx, ny, seed=4)
+    m = _model(W, lt, "fp64")
+    zu, zv = np.zeros((nx + 1, ny)), np.zeros((nx, ny + 1))
+    W.misfit_gradient(m, N_PICARD, zu, zv, TOL_LIN, gradient=False)
+    uo, vo = W.workloads.inversion_observations(m.fields.gu.u, m.fields.gv.v, seed=4)
+    return np.log(1e4) * np.ones((nx, ny)), uo, vo
+
+
+def test_gradient_matches_oracle(W, oracle):
+    nx, ny = 64, 16
+    _step("adjoint vs oracle %dx%d" % (nx, ny))
+    l0, uo, vo = _setup(W, nx, ny)
+    m = _model(W, l0, "fp64")
+    J, g, rep = W.misfit_gradient(m, N_PICARD, uo, vo, TOL_LIN)
+    assert rep["unconverged"] == 0
+    o = oracle.OracleModel(W.workloads.inversion_model(l0, "fp64", nσ=4, n_picard=N_PICARD))
+    o.update_geometry()
+    Jo, go, repo = o.misfit_gradient(N_PICARD, TOL_LIN, 60, uo, vo)
+    assert repo["unconverged"] == 0
+    assert abs(J - Jo) <= 1e-10 * abs(Jo)
+    assert np.linalg.norm(g - go) <= 1e-8 * np.linalg.norm(go)
+    # the context holds the forward's last state: the same velocities as the oracle's
+    uf = o.field("U_U")
+    assert np.max(np.abs(m.fields.gu.u - uf)) <= 1e-9 * np.max(np.abs(uf))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gradient_matches_finite_differences(W, precision):
+    nx, ny = 96, 24
+    _step("adjoint FD check %s %dx%d" % (precision, nx, ny))
+    l0, uo, vo = _setup(W, nx, ny)
+    J0, g, rep = W.misfit_gradient(_model(W, l0, precision), N_PICARD, uo, vo, TOL_LIN)
+    assert rep["unconverged"] == 0 and rep["max_fwd_rel"] <= TOL_LIN and rep["max_adj_rel"] <= TOL_LIN
+    rng = np.random.default_rng(11)
+    eps = 1e-6
+    for trial in range(3):
+        d = rng.standard_normal((nx, ny))
+        Jp = W.misfit_gradient(_model(W, l0 + eps * d, precision), N_PICARD, uo, vo, TOL_LIN, gradient=False)[0]
+        Jm = W.misfit_gradient(_model(W, l0 - eps * d, precision), N_PICARD, uo, vo, TOL_LIN, gradient=False)[0]
+        fd = (Jp - Jm) / (2 * eps)
+        gd = float(np.sum(g * d))
+        print("  direction %d: fd %.12e adjoint %.12e rel %.2e" % (trial, fd, gd, abs(fd - gd) / abs(gd)))
+        assert abs(fd - gd) <= 1e-5 * abs(gd)
+
+
+def test_fixed_velocities_rejected(W):
+    _step("adjoint rejects fixed velocities")
+    nx, ny = 32, 8
+    m = W.workloads.channel_solve(nx=nx, ny=ny, nσ=4, maxiter_picard=2)
+    m.grid.u_isfixed[3, 3] = True
+    W.update_geometry(m)
+    with pytest.raises(W.ArgumentError):
+        W.misfit_gradient(m, 2, np.zeros((nx + 1, ny)), np.zeros((nx, ny + 1)))

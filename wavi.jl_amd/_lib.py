@@ -97,6 +97,16 @@ class SolveReport(ctypes.Structure):
         }
 
 
+class AdjointReport(ctypes.Structure):
+    _fields_ = [
+        ("n_picard", ctypes.c_int32), ("fwd_vcycles", ctypes.c_int32), ("adj_vcycles", ctypes.c_int32),
+        ("unconverged", ctypes.c_int32), ("max_fwd_rel", ctypes.c_double), ("max_adj_rel", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # exported symbols and their ctypes signatures (kept in step with the header)
 SIGNATURES = {
     "wavi_ctx_create": (ctypes.c_int, [ctypes.POINTER(GridDesc), ctypes.POINTER(PhysParams),
@@ -121,6 +131,8 @@ SIGNATURES = {
     "wavi_update_thickness": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "wavi_run_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(SolveReport)]),
     "wavi_update_wavelets": (ctypes.c_int, [ctypes.c_void_p]),
+    "wavi_misfit_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                            ctypes.POINTER(ctypes.c_double), _dp, ctypes.POINTER(AdjointReport)]),
     "wavi_op_apply": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.c_int]),
     "wavi_rhs": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "wavi_op_diag": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),

@@ -49,6 +49,7 @@ struct wavi_ctx {
     // masks (uint8, device)
     uint8_t *hm = nullptr, *hfix = nullptr, *um = nullptr, *vm = nullptr, *cm = nullptr;
     uint8_t *ui = nullptr, *vi = nullptr, *ufix = nullptr, *vfix = nullptr, *mwu = nullptr, *mwv = nullptr;
+    bool has_fixed = false;     // any u_isfixed / v_isfixed point
     // fields (double, device); mask fields use mwu/mwv instead
     double *f[WAVI_NFIELDS] = {};
     // sigma-level constants
@@ -452,6 +453,8 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
         if (e != hipSuccess) return e;
         return hipMemcpy(*p, src.data(), src.size(), hipMemcpyHostToDevice);
     };
+    for (uint8_t f : ufix) ctx->has_fixed = ctx->has_fixed || f;
+    for (uint8_t f : vfix) ctx->has_fixed = ctx->has_fixed || f;
     CREATECHK(alloc8(&ctx->hm, hm));
     CREATECHK(alloc8(&ctx->hfix, hfix));
     CREATECHK(alloc8(&ctx->um, um));
@@ -1241,6 +1244,289 @@ static int do_update_velocities(wavi_ctx *ctx, wavi_solve_report *rep)
     return WAVI_OK;
 }
 
+// Work vectors of do_linear_solve: direction p, q = A p, and dot partials.
+struct LinScratch {
+    DevVec p, q;
+    double *part = nullptr;  // 2 * npart
+};
+
+// Canonical dots of [u; v] pairs (b == nullptr: sums of squares), summed on
+// the device and returned to the host.
+static int dots(wavi_ctx *ctx, const LinScratch &s, int count, const double *const *au, const double *const *av,
+                const double *const *bu, const double *const *bv, double *out)
+{
+    const GridDims &g = ctx->g;
+    dim3 tg(g.nbx, g.nby);
+    for (int c = 0; c < count; ++c)
+        hipLaunchKernelGGL(k_tile_dot<double>, tg, dim3(256), 0, ctx->stream, g, au[c], av[c], bu[c], bv[c],
+                           s.part + (size_t)c * g.npart);
+    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(256), 0, ctx->stream, s.part, g.npart, count, ctx->norms);
+    HIPCHK(hipMemcpyAsync(ctx->h_norms, ctx->norms, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < count; ++c) out[c] = ctx->h_norms[c];
+    return WAVI_OK;
+}
+
+// x = A^{-1} b at inner points (b zero elsewhere) by flexible CG in FP64
+// (Polak-Ribiere direction updates, so that the preconditioner may vary)
+// preconditioned by the V-cycle of update_velocities! in the context's
+// precision, from x = 0, until ||b - A x|| <= tol ||b|| (recomputed from
+// x at the end) or maxit V-cycles.  Each V-cycle starts its coarse CG from
+// zero: the warm start of the Picard loop (the previous correction) would
+// be orders of magnitude larger than the shrinking residual's correction,
+// and cancelling it costs the digits the solve is after.
+static int do_linear_solve(wavi_ctx *ctx, const DevVec &b, const DevVec &x, const LinScratch &s, double tol,
+                           int maxit, int *vcycles, double *rel_out, bool *converged)
+{
+    const GridDims &g = ctx->g;
+    const int nu = ctx->nu, nv = ctx->nv;
+    dim3 tg(g.nbx, g.nby);
+    auto vcycle = [&](int *cg) -> int {
+        hipLaunchKernelGGL(k_fill<double>, flat(nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_WU_CORRECTION_COARSE],
+                           0.0, nu);
+        hipLaunchKernelGGL(k_fill<double>, flat(nv), dim3(256), 0, ctx->stream, ctx->f[WAVI_WV_CORRECTION_COARSE],
+                           0.0, nv);
+        return do_vcycle(ctx, cg);
+    };
+    hipLaunchKernelGGL(k_fill<double>, flat(nu), dim3(256), 0, ctx->stream, x.u, 0.0, nu);
+    hipLaunchKernelGGL(k_fill<double>, flat(nv), dim3(256), 0, ctx->stream, x.v, 0.0, nv);
+    auto true_rel = [&](double *rel) -> int {
+        hipLaunchKernelGGL(k_tile_resid<double>, tg, dim3(256), 0, ctx->stream, g, coef64(ctx), x.u, x.v, b.u, b.v,
+                           ctx->r.u, ctx->r.v, ctx->Pr0, ctx->Pb);
+        hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, ctx->stream, ctx->Pr0, ctx->Pb, g.npart,
+                           ctx->norms);
+        HIPCHK(hipMemcpyAsync(ctx->h_norms, ctx->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *rel = ctx->h_norms[1] > 0.0 ? ctx->h_norms[0] / ctx->h_norms[1] : 0.0;
+        return WAVI_OK;
+    };
+    // FCG from the true residual of x; restarted (at most 3 times) when the
+    // recursively updated residual met the tolerance but the true one does not
+    double rel = 0.0, bn = 0.0;
+    int it = 0;
+    for (int restart = 0; restart < 4; ++restart) {
+        CHK(true_rel(&rel));
+        if (restart == 0) bn = ctx->h_norms[1];
+        if (rel <= tol || it >= maxit) break;
+        int cg = 0;
+        CHK(vcycle(&cg));  // z = M r into ctx->e
+        ++it;
+        hipLaunchKernelGGL(k_fcg_dir, flat(nu + nv), dim3(256), 0, ctx->stream, nu, nv, 0.0, ctx->ui, ctx->vi,
+                           ctx->e.u, ctx->e.v, s.p.u, s.p.v);
+        double rz;
+        {
+            const double *au[1] = {ctx->r.u}, *av[1] = {ctx->r.v}, *bu[1] = {ctx->e.u}, *bv[1] = {ctx->e.v};
+            CHK(dots(ctx, s, 1, au, av, bu, bv, &rz));
+        }
+        for (;;) {
+            hipLaunchKernelGGL(k_op<double>, flat(nu + nv), dim3(256), 0, ctx->stream, g, coef64(ctx), s.p.u, s.p.v,
+                               s.q.u, s.q.v);
+            double pq;
+            {
+                const double *au[1] = {s.p.u}, *av[1] = {s.p.v}, *bu[1] = {s.q.u}, *bv[1] = {s.q.v};
+                CHK(dots(ctx, s, 1, au, av, bu, bv, &pq));
+            }
+            const double alpha = rz / pq;
+            hipLaunchKernelGGL(k_fcg_step, flat(nu + nv), dim3(256), 0, ctx->stream, nu, nv, alpha, x.u, x.v,
+                               ctx->r.u, ctx->r.v, s.p.u, s.p.v, s.q.u, s.q.v);
+            double rr;
+            {
+                const double *au[1] = {ctx->r.u}, *av[1] = {ctx->r.v}, *bu[1] = {nullptr}, *bv[1] = {nullptr};
+                CHK(dots(ctx, s, 1, au, av, bu, bv, &rr));
+            }
+            if (sqrt(rr) <= tol * bn || it >= maxit) break;
+            CHK(vcycle(&cg));
+            ++it;
+            double zz[2];
+            {
+                const double *au[2] = {ctx->e.u, ctx->e.u}, *av[2] = {ctx->e.v, ctx->e.v};
+                const double *bu[2] = {s.q.u, ctx->r.u}, *bv[2] = {s.q.v, ctx->r.v};
+                CHK(dots(ctx, s, 2, au, av, bu, bv, zz));
+            }
+            const double beta = -alpha * zz[0] / rz;
+            rz = zz[1];
+            hipLaunchKernelGGL(k_fcg_dir, flat(nu + nv), dim3(256), 0, ctx->stream, nu, nv, beta, ctx->ui, ctx->vi,
+                               ctx->e.u, ctx->e.v, s.p.u, s.p.v);
+            HIPCHK(hipGetLastError());
+        }
+        if (restart == 3) CHK(true_rel(&rel));
+    }
+    *vcycles += it;
+    *rel_out = rel;
+    *converged = rel <= tol;
+    return WAVI_OK;
+}
+
+// Device storage of the adjoint: checkpoints of (u, v, eta, beta, f2) for
+// npic + 1 states, the state adjoints and the per-cell work arrays.
+struct AdjStore {
+    double *base = nullptr;
+    std::vector<double *> u, v, eta, beta, f2;
+    double *ubar = nullptr, *vbar = nullptr, *lu = nullptr, *lv = nullptr;
+    double *etab = nullptr, *betab = nullptr, *f2b = nullptr, *wcb = nullptr, *grad = nullptr;
+    double *cub = nullptr, *cvb = nullptr, *uxb = nullptr, *vyb = nullptr, *shb = nullptr;
+    LinScratch ls;
+    ~AdjStore()
+    {
+        if (base) (void)hipFree(base);
+    }
+};
+
+static int adj_alloc(wavi_ctx *ctx, int npic, AdjStore &A)
+{
+    const size_t nu = ctx->nu, nv = ctx->nv, nh = ctx->nh, ns3 = (size_t)ctx->ns3;
+    const size_t per = nu + nv + ns3 + 2 * nh;
+    const size_t npart = ctx->g.npart;
+    const size_t total = per * (npic + 1) + 4 * (nu + nv) + ns3 + 4 * nh + 5 * nh + 2 * npart;
+    HIPCHK(hipMalloc(&A.base, total * sizeof(double)));
+    double *q = A.base;
+    auto take = [&](size_t n) {
+        double *r = q;
+        q += n;
+        return r;
+    };
+    for (int k = 0; k <= npic; ++k) {
+        A.u.push_back(take(nu));
+        A.v.push_back(take(nv));
+        A.eta.push_back(take(ns3));
+        A.beta.push_back(take(nh));
+        A.f2.push_back(take(nh));
+    }
+    A.ubar = take(nu);
+    A.vbar = take(nv);
+    A.lu = take(nu);
+    A.lv = take(nv);
+    A.etab = take(ns3);
+    A.betab = take(nh);
+    A.f2b = take(nh);
+    A.wcb = take(nh);
+    A.grad = take(nh);
+    A.cub = take(nh);
+    A.cvb = take(nh);
+    A.uxb = take(nh);
+    A.vyb = take(nh);
+    A.shb = take(nh);
+    A.ls.p.u = take(nu);
+    A.ls.p.v = take(nv);
+    A.ls.q.u = take(nu);
+    A.ls.q.v = take(nv);
+    A.ls.part = take(2 * npart);
+    return WAVI_OK;
+}
+
+static int adj_ckpt(wavi_ctx *ctx, AdjStore &A, int k, bool save)
+{
+    const size_t nh = ctx->nh, ns3 = (size_t)ctx->ns3;
+    struct {
+        double *field, *slot;
+        size_t n;
+    } items[5] = {{ctx->f[WAVI_U_U], A.u[k], (size_t)ctx->nu},
+                  {ctx->f[WAVI_V_V], A.v[k], (size_t)ctx->nv},
+                  {ctx->f[WAVI_S_ETA], A.eta[k], ns3},
+                  {ctx->f[WAVI_H_BETA], A.beta[k], nh},
+                  {ctx->f[WAVI_H_QUAD_F2], A.f2[k], nh}};
+    for (auto &it : items)
+        HIPCHK(hipMemcpyAsync(save ? it.slot : it.field, save ? it.field : it.slot, it.n * sizeof(double),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    return WAVI_OK;
+}
+
+static int do_misfit_gradient(wavi_ctx *ctx, int npic, double tol, int maxit, const double *uobs,
+                              const double *vobs, double *J, double *grad, wavi_adjoint_report *rep)
+{
+    const int nu = ctx->nu, nv = ctx->nv, nh = ctx->nh;
+    memset(rep, 0, sizeof(*rep));
+    rep->n_picard = npic;
+    AdjStore A;
+    CHK(adj_alloc(ctx, npic, A));
+    std::vector<uint8_t> ui(nu), vi(nv);
+    HIPCHK(hipMemcpy(ui.data(), ctx->ui, nu, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(vi.data(), ctx->vi, nv, hipMemcpyDeviceToHost));
+    // forward: npic Picard iterations with converged linear solves
+    for (int k = 0; k < npic; ++k) {
+        CHK(adj_ckpt(ctx, A, k, true));
+        CHK(do_inner_update(ctx));
+        CHK(do_rhs(ctx));
+        double rel = 0.0;
+        bool ok = false;
+        CHK(do_linear_solve(ctx, ctx->b, ctx->x, A.ls, tol, maxit, &rep->fwd_vcycles, &rel, &ok));
+        if (!ok) rep->unconverged++;
+        rep->max_fwd_rel = std::max(rep->max_fwd_rel, rel);
+        hipLaunchKernelGGL(k_copy_masked<double>, flat(nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_U_U], ctx->x.u,
+                           ctx->ui, nu);
+        hipLaunchKernelGGL(k_copy_masked<double>, flat(nv), dim3(256), 0, ctx->stream, ctx->f[WAVI_V_V], ctx->x.v,
+                           ctx->vi, nv);
+        HIPCHK(hipGetLastError());
+    }
+    CHK(adj_ckpt(ctx, A, npic, true));
+    // J and ubar on the host
+    std::vector<double> U(nu), V(nv);
+    HIPCHK(hipMemcpyAsync(U.data(), A.u[npic], nu * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(V.data(), A.v[npic], nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double js = 0.0;
+    for (int q = 0; q < nu; ++q) {
+        U[q] = ui[q] ? U[q] - uobs[q] : 0.0;
+        js += U[q] * U[q];
+    }
+    for (int q = 0; q < nv; ++q) {
+        V[q] = vi[q] ? V[q] - vobs[q] : 0.0;
+        js += V[q] * V[q];
+    }
+    *J = 0.5 * js;
+    if (!grad) return WAVI_OK;
+    // reverse sweep
+    HIPCHK(hipMemcpyAsync(A.ubar, U.data(), nu * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(A.vbar, V.data(), nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const size_t ns3 = (size_t)ctx->ns3;
+    hipLaunchKernelGGL(k_fill<double>, flat((int)ns3), dim3(256), 0, ctx->stream, A.etab, 0.0, (int)ns3);
+    hipLaunchKernelGGL(k_fill<double>, flat(3 * nh), dim3(256), 0, ctx->stream, A.betab, 0.0, 3 * nh);
+    HIPCHK(hipGetLastError());
+    const DevVec bar{A.ubar, A.vbar}, lam{A.lu, A.lv};
+    for (int k = npic; k >= 1; --k) {
+        CHK(adj_ckpt(ctx, A, k - 1, false));
+        CHK(do_inner_update(ctx));
+        double rel = 0.0;
+        bool ok = false;
+        CHK(do_linear_solve(ctx, bar, lam, A.ls, tol, maxit, &rep->adj_vcycles, &rel, &ok));
+        if (!ok) rep->unconverged++;
+        rep->max_adj_rel = std::max(rep->max_adj_rel, rel);
+        AdjArgs a;
+        a.p = inner_args(ctx);
+        a.uk = A.u[k];
+        a.vk = A.v[k];
+        a.lu = A.lu;
+        a.lv = A.lv;
+        a.eta_old = A.eta[k - 1];
+        a.beta_old = A.beta[k - 1];
+        a.f2_old = A.f2[k - 1];
+        a.etab = A.etab;
+        a.betab = A.betab;
+        a.f2b = A.f2b;
+        a.wcb = A.wcb;
+        a.cub = A.cub;
+        a.cvb = A.cvb;
+        a.uxb = A.uxb;
+        a.vyb = A.vyb;
+        a.shb = A.shb;
+        a.ubar = A.ubar;
+        a.vbar = A.vbar;
+        hipLaunchKernelGGL(k_adj_h, flat(nh), dim3(256), 0, ctx->stream, ctx->g, a);
+        hipLaunchKernelGGL(k_adj_uv, flat(nu + nv), dim3(256), 0, ctx->stream, ctx->g, a);
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_adj_grad, flat(nh), dim3(256), 0, ctx->stream, nh, ctx->f[WAVI_H_WEERTMAN_C], A.wcb,
+                       A.grad);
+    HIPCHK(hipMemcpyAsync(grad, A.grad, nh * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    // leave the state of the last forward iteration
+    CHK(adj_ckpt(ctx, A, npic - 1, false));
+    CHK(do_inner_update(ctx));
+    HIPCHK(hipMemcpyAsync(ctx->f[WAVI_U_U], A.u[npic], nu * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->f[WAVI_V_V], A.v[npic], nv * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAVI_OK;
+}
+
 static GeomArgs geom_args(wavi_ctx *ctx)
 {
     GeomArgs a;
@@ -1326,6 +1612,24 @@ int wavi_update_velocities(wavi_ctx *ctx, wavi_solve_report *report)
     CHK(do_update_velocities(ctx, report));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return WAVI_OK;
+}
+
+int wavi_misfit_gradient(wavi_ctx *ctx, int n_picard, double tol_linear, int maxit_linear, const double *u_obs,
+                         const double *v_obs, double *J, double *grad_logc, wavi_adjoint_report *report)
+{
+    if (!ctx) return WAVI_EARG;
+    if (n_picard < 1 || maxit_linear < 0 || !(tol_linear >= 0.0) || !u_obs || !v_obs || !J) {
+        ctx->err = "wavi_misfit_gradient: n_picard >= 1, maxit_linear >= 0, tol_linear >= 0, u_obs, v_obs and J "
+                   "are required";
+        return WAVI_EARG;
+    }
+    if (ctx->has_fixed) {
+        ctx->err = "wavi_misfit_gradient: fixed velocities (u_isfixed / v_isfixed) are not supported";
+        return WAVI_EARG;
+    }
+    wavi_adjoint_report local;
+    return do_misfit_gradient(ctx, n_picard, tol_linear, maxit_linear, u_obs, v_obs, J, grad_logc,
+                              report ? report : &local);
 }
 
 int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,

@@ -118,6 +118,45 @@ __global__ void k_finish_norms(const double *P0, const double *P1, int np, doubl
     }
 }
 
+// out[c] = sum of partial array c (P + c * np), c < count (no square root)
+__global__ void k_finish_sums(const double *P, int np, int count, double *out)
+{
+    __shared__ double lds[256];
+    for (int c = 0; c < count; ++c) {
+        double a = lanes_sum(P + (size_t)c * np, np, lds);
+        if (threadIdx.x == 0) out[c] = a;
+        __syncthreads();
+    }
+}
+
+// flexible CG: x += alpha p, r -= alpha q on the u then the v grid
+__global__ void k_fcg_step(int nu, int nv, double alpha, double *xu, double *xv, double *ru, double *rv,
+                           const double *pu, const double *pv, const double *qu, const double *qv)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nu) {
+        xu[t] = xu[t] + alpha * pu[t];
+        ru[t] = ru[t] - alpha * qu[t];
+    } else if (t < nu + nv) {
+        t -= nu;
+        xv[t] = xv[t] + alpha * pv[t];
+        rv[t] = rv[t] - alpha * qv[t];
+    }
+}
+
+// flexible CG: p = z + beta p at inner points, 0 elsewhere
+__global__ void k_fcg_dir(int nu, int nv, double beta, const uint8_t *ui, const uint8_t *vi, const double *zu,
+                          const double *zv, double *pu, double *pv)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nu) {
+        pu[t] = ui[t] ? zu[t] + beta * pu[t] : 0.0;
+    } else if (t < nu + nv) {
+        t -= nu;
+        pv[t] = vi[t] ? zv[t] + beta * pv[t] : 0.0;
+    }
+}
+
 // ===========================================================================
 // update_state! steps before the solve (src/Models/update_state.jl:51-147).
 
@@ -498,6 +537,217 @@ __global__ void k_inner_uvc(GridDims g, InnerArgs p)
         }
         p.heta_c[tc] = out;
     }
+}
+
+// ===========================================================================
+// Reverse of one Picard iteration, for the adjoint basal-drag gradient (no
+// reference counterpart).  Iteration k is inner_update! from the state
+// (u_{k-1}, eta, beta, f2 of checkpoint k-1), then u_k = A_k^{-1} b.  With
+// lambda_k = A_k^{-1} ubar_k, the coefficient adjoints are
+// cbar = -d(lambda^T A(c) u_k)/dc, and the chain rule runs back through
+// update_rheological_operators!, update_βeff_on_uv_grids!, update_βeff!,
+// the quadratures, the viscosity passes, update_basal_drag!, update_β!,
+// update_bed_speed!, update_av_speed! and update_shelf_strain_rate!
+// (update_velocities.jl:202-383).  k_adj_h does the per-cell part, k_adj_uv
+// the gathers onto u_{k-1} and v_{k-1}.
+
+struct AdjArgs {
+    InnerArgs p;                               // iteration k recomputed; p.U, p.V = u_{k-1}
+    const double *uk, *vk;                     // u_k
+    const double *lu, *lv;                     // lambda_k
+    const double *eta_old, *beta_old, *f2_old; // checkpoint k-1
+    double *etab, *betab, *f2b;                // state adjoints: leaving k on entry, entering k on exit
+    double *wcb;                               // += dJ/d(weertman_c on h)
+    double *cub, *cvb, *uxb, *vyb, *shb;       // per h cell, for k_adj_uv
+    double *ubar, *vbar;                       // out: adjoint of u_{k-1} at inner points
+};
+
+// (du/dy + dv/dx) of (U, V) at corner (a, b), unmasked as op_fun! reads it
+__device__ __forceinline__ double corner_rate(const GridDims &g, const double *U, const double *V, double rdx,
+                                              double rdy, int a, int b)
+{
+    const int nxu = g.nxu, nx = g.nx;
+    return (rdy * U[(a + 1) + nxu * (b + 1)] - rdy * U[(a + 1) + nxu * b]) +
+           (rdx * V[(a + 1) + nx * (b + 1)] - rdx * V[a + nx * (b + 1)]);
+}
+
+__global__ void k_adj_h(GridDims g, AdjArgs a)
+{
+    const int nh = g.nx * g.ny;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nh) return;
+    const InnerArgs &p = a.p;
+    const int nx = g.nx, ny = g.ny, nxu = g.nxu, nxc = g.nx - 1, nyc = g.ny - 1;
+    const int i = k % nx, j = k / nx;
+    const bool hm = p.hm[k];
+    const double rdx = p.rdx, rdy = p.rdy;
+    // update_rheological_operators!: h eta_av on h, and cent onto c
+    double hqb = 0.0;
+    if (hm) {
+        const double dudx = rdx * a.uk[(i + 1) + nxu * j] - rdx * a.uk[i + nxu * j];
+        const double dvdy = rdy * a.vk[i + nx * (j + 1)] - rdy * a.vk[i + nx * j];
+        const double dlx = rdx * a.lu[(i + 1) + nxu * j] - rdx * a.lu[i + nxu * j];
+        const double dly = rdy * a.lv[i + nx * (j + 1)] - rdy * a.lv[i + nx * j];
+        hqb = 2.0 * (2.0 * dudx + dvdy) * dlx + 2.0 * (2.0 * dvdy + dudx) * dly;
+    }
+    for (int b = j - 1; b <= j; ++b)
+        for (int c = i - 1; c <= i; ++c)
+            if (c >= 0 && c < nxc && b >= 0 && b < nyc && p.cm[c + nxc * b])
+                hqb += 0.25 * (corner_rate(g, a.uk, a.vk, rdx, rdy, c, b) *
+                               corner_rate(g, a.lu, a.lv, rdx, rdy, c, b));
+    const double etaavb = p.h[k] * hqb;
+    // update_βeff_on_uv_grids!
+    double beffb = 0.0;
+    if (hm) {
+        for (int c = i; c <= i + 1; ++c) {
+            const int ku = c + nxu * j;
+            if (!p.um[ku]) continue;
+            double den = 0.0, ng = 0.0;
+            if (c >= 1) {
+                den = den + 0.5 * (double)p.hm[(c - 1) + nx * j];
+                ng = ng + 0.5 * (p.hm[(c - 1) + nx * j] ? p.gfh[(c - 1) + nx * j] : 0.0);
+            }
+            if (c <= nx - 1) {
+                den = den + 0.5 * (double)p.hm[c + nx * j];
+                ng = ng + 0.5 * (p.hm[c + nx * j] ? p.gfh[c + nx * j] : 0.0);
+            }
+            const double ip = ng / den;
+            beffb += (a.lu[ku] * a.uk[ku]) * (0.5 / den) * (ip > 0.0 ? p.gfu[ku] / ip : 1.0);
+        }
+        for (int b = j; b <= j + 1; ++b) {
+            const int kv = i + nx * b;
+            if (!p.vm[kv]) continue;
+            double den = 0.0, ng = 0.0;
+            if (b >= 1) {
+                den = den + 0.5 * (double)p.hm[i + nx * (b - 1)];
+                ng = ng + 0.5 * (p.hm[i + nx * (b - 1)] ? p.gfh[i + nx * (b - 1)] : 0.0);
+            }
+            if (b <= ny - 1) {
+                den = den + 0.5 * (double)p.hm[i + nx * b];
+                ng = ng + 0.5 * (p.hm[i + nx * b] ? p.gfh[i + nx * b] : 0.0);
+            }
+            const double ip = ng / den;
+            beffb += (a.lv[kv] * a.vk[kv]) * (0.5 / den) * (ip > 0.0 ? p.gfv[kv] / ip : 1.0);
+        }
+    }
+    // update_βeff!: beta_eff = beta / (1 + f2 beta)
+    const double beta = p.beta[k], f2 = p.f2[k];
+    const double D = 1.0 + f2 * beta;
+    double bb = a.betab[k] + beffb / (D * D);
+    const double fb = a.f2b[k] - beffb * (beta * beta) / (D * D);
+    const double sr = p.sr[k], tau = p.tau[k], bs = p.bs[k];
+    double taub = 0.0, srb = 0.0;
+    if (hm) {
+        for (int s = 0; s < p.ns; ++s) {
+            const int k3 = k + nh * s;
+            const double z = p.zeta[s], w = p.qw[s], eta = p.eta[k3], gb = p.glen_b[k3];
+            // update_av_viscosity!, update_quadrature_falpha!
+            double eb = a.etab[k3] + w * etaavb - fb * ((w * p.h[k]) * (z * z)) / (eta * eta);
+            // inner_update_viscosity!: the passes from eta_old, then back
+            double e[9];
+            e[0] = a.eta_old[k3];
+            for (int q = 0; q < p.niter_visc; ++q) {
+                const double qq = (tau * z) / e[q];
+                const double arg = sqrt((sr * sr + 0.25 * (qq * qq)) + p.greg2);
+                e[q + 1] = (0.5 * gb) * det_pow(arg, p.gexp);
+            }
+            for (int q = p.niter_visc - 1; q >= 0; --q) {
+                const double qq = (tau * z) / e[q];
+                const double arg = sqrt((sr * sr + 0.25 * (qq * qq)) + p.greg2);
+                const double gg = eb * e[q + 1] * p.gexp / arg;
+                srb += gg * sr / arg;
+                const double qb = gg * 0.25 * qq / arg;
+                taub += qb * z / e[q];
+                eb = -qb * qq / e[q];
+            }
+            a.etab[k3] = eb;
+        }
+    }
+    // update_basal_drag!: tau = beta bs; update_β!: beta = wc (bs^2 + reg^2)^(wexp/2)
+    double bsb = taub * beta;
+    bb += taub * bs;
+    const double S = bs * bs + p.wreg2;
+    a.wcb[k] += bb * det_pow(sqrt(S), p.wexp);
+    bsb += bb * beta * p.wexp * bs / S;
+    // update_bed_speed!: bs = avs / (1 + f2_old beta_old)
+    const double D0 = 1.0 + a.f2_old[k] * a.beta_old[k];
+    const double avsb = bsb / D0, avs = p.avs[k];
+    a.f2b[k] = -bsb * avs * a.beta_old[k] / (D0 * D0);
+    a.betab[k] = -bsb * avs * a.f2_old[k] / (D0 * D0);
+    // update_av_speed! and update_shelf_strain_rate! of u_{k-1}
+    double cub = 0.0, cvb = 0.0, uxb = 0.0, vyb = 0.0, shb = 0.0;
+    if (hm) {
+        const double u0 = p.um[i + nxu * j] ? p.U[i + nxu * j] : 0.0;
+        const double u1 = p.um[(i + 1) + nxu * j] ? p.U[(i + 1) + nxu * j] : 0.0;
+        const double v0 = p.vm[i + nx * j] ? p.V[i + nx * j] : 0.0;
+        const double v1 = p.vm[i + nx * (j + 1)] ? p.V[i + nx * (j + 1)] : 0.0;
+        if (avs > 0.0) {
+            cub = avsb * (0.5 * u0 + 0.5 * u1) / avs;
+            cvb = avsb * (0.5 * v0 + 0.5 * v1) / avs;
+        }
+        if (sr > 0.0) {
+            const double ux = rdx * u1 - rdx * u0, vy = rdy * v1 - rdy * v0;
+            const double sh = cent_c_to_h(g, p, i, j);
+            uxb = srb * (2.0 * ux + vy) / (2.0 * sr);
+            vyb = srb * (2.0 * vy + ux) / (2.0 * sr);
+            shb = srb * 0.25 * sh / sr;
+        }
+    }
+    a.cub[k] = cub;
+    a.cvb[k] = cvb;
+    a.uxb[k] = uxb;
+    a.vyb[k] = vyb;
+    a.shb[k] = shb;
+}
+
+// adjoint of the corner shear at c (a, b): cent (c -> h) transposed
+__device__ __forceinline__ double adj_corner(const GridDims &g, const AdjArgs &a, int c, int b)
+{
+    const int nx = g.nx;
+    if (!a.p.cm[c + (nx - 1) * b]) return 0.0;
+    return 0.25 * (a.shb[c + nx * b] + a.shb[(c + 1) + nx * b] + a.shb[c + nx * (b + 1)] +
+                   a.shb[(c + 1) + nx * (b + 1)]);
+}
+
+__global__ void k_adj_uv(GridDims g, AdjArgs a)
+{
+    const int nu = g.nxu * g.nyu, nv = g.nxv * g.nyv;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = g.nx, ny = g.ny;
+    const double rdx = a.p.rdx, rdy = a.p.rdy;
+    if (t < nu) {
+        const int i = t % g.nxu, j = t / g.nxu;
+        double s = 0.0;
+        if (a.p.um[t]) {
+            if (i >= 1) s += 0.5 * a.cub[(i - 1) + nx * j] + rdx * a.uxb[(i - 1) + nx * j];
+            if (i <= nx - 1) s += 0.5 * a.cub[i + nx * j] - rdx * a.uxb[i + nx * j];
+            if (i >= 1 && i <= nx - 1) {
+                if (j >= 1) s += rdy * adj_corner(g, a, i - 1, j - 1);
+                if (j <= ny - 2) s -= rdy * adj_corner(g, a, i - 1, j);
+            }
+        }
+        a.ubar[t] = s;
+    } else if (t < nu + nv) {
+        const int tv = t - nu;
+        const int i = tv % g.nxv, j = tv / g.nxv;
+        double s = 0.0;
+        if (a.p.vm[tv]) {
+            if (j >= 1) s += 0.5 * a.cvb[i + nx * (j - 1)] + rdy * a.vyb[i + nx * (j - 1)];
+            if (j <= ny - 1) s += 0.5 * a.cvb[i + nx * j] - rdy * a.vyb[i + nx * j];
+            if (j >= 1 && j <= ny - 1) {
+                if (i >= 1) s += rdx * adj_corner(g, a, i - 1, j - 1);
+                if (i <= nx - 2) s -= rdx * adj_corner(g, a, i, j - 1);
+            }
+        }
+        a.vbar[tv] = s;
+    }
+}
+
+// grad = wc * wcb (dJ/d log C through weertman_c = C * grounded fraction)
+__global__ void k_adj_grad(int nh, const double *wc, const double *wcb, double *grad)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nh) grad[k] = wc[k] * wcb[k];
 }
 
 // ===========================================================================

@@ -400,6 +400,45 @@ def update_velocities_batch(models, nthreads=8, verbose=False):
     return models
 
 
+def update_geometry(model):
+    """The update_state! steps that precede the velocity solve (surface,
+    geometry on u/v, height above floatation, grounded fractions,
+    weertman_c = C * grounded fraction, dsdh; update_state.jl:8-15)."""
+    update_preconditioner(model)
+    spec = model.parallel_spec
+    push_fields(model)
+    _check(spec._lib, spec.ctx, spec._lib.wavi_update_geometry(spec.ctx))
+    pull_fields(model)
+    return model
+
+
+def misfit_gradient(model, n_picard, u_obs, v_obs, tol_linear=1e-13, maxit_linear=60, gradient=True):
+    """Adjoint basal-drag gradient (no reference counterpart; see
+    wavi_misfit_gradient in include/wavi_hip.h): J = 1/2 ||u - u_obs||^2 over
+    inner u and v points after n_picard Picard iterations with converged
+    linear solves, and dJ/d log C on the h grid.  The model's fields must
+    already hold the state after update_geometry (as update_velocities
+    expects).  Returns (J, grad (nx, ny) or None, report dict); the model is
+    left in the forward's last Picard state."""
+    update_preconditioner(model)
+    spec = model.parallel_spec
+    lib = spec._lib
+    push_fields(model)
+    nx, ny = model.grid.nx, model.grid.ny
+    uo = np.ascontiguousarray(np.asarray(u_obs, dtype=np.float64).ravel(order="F"))
+    vo = np.ascontiguousarray(np.asarray(v_obs, dtype=np.float64).ravel(order="F"))
+    if uo.size != (nx + 1) * ny or vo.size != nx * (ny + 1):
+        raise DimensionMismatch("u_obs must have (nx+1)*ny values and v_obs nx*(ny+1)")
+    J = ctypes.c_double(0.0)
+    g = np.zeros(nx * ny) if gradient else None
+    rep = _lib.AdjointReport()
+    _check(lib, spec.ctx, lib.wavi_misfit_gradient(spec.ctx, int(n_picard), float(tol_linear), int(maxit_linear),
+                                                   _lib.dptr(uo), _lib.dptr(vo), ctypes.byref(J),
+                                                   _lib.dptr(g) if gradient else None, ctypes.byref(rep)))
+    pull_fields(model)
+    return J.value, (g.reshape((nx, ny), order="F") if gradient else None), rep.as_dict()
+
+
 def inner_update(model):
     """inner_update!(model) (update_velocities.jl:32-45)."""
     update_preconditioner(model)

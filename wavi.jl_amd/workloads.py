@@ -9,6 +9,11 @@ Every workload is a WAVI model setup:
   channel_solve        a single velocity solve on an nx x ny grid (1280x160 at
                        500 m or 2560x320 at 250 m), nσ=8, 20 Picard iterations
                        to tol 1e-8, tol_coarse 1e-8, 3 Haar levels, dt 1.0
+  inversion_*          the basal-drag inversion: true log C = log 1e4 + a
+                       seeded Gaussian random field (std 0.5, correlation
+                       length 10 km), prior log C = log 1e4, observations =
+                       the FP64 velocities of the true field times
+                       (1 + 0.01 N(0, 1)), 10 Picard iterations
   ensemble_members     64 channel solves of seeded sizes and parameters
                        (grid from {160x20, 320x40, 640x80, 1280x160},
                        nσ from {4, 8}, A log-uniform in [1e-17, 4e-17],
@@ -146,3 +151,23 @@ def ensemble_member(spec, precision="mixed", maxiter_picard=20):
 def ensemble_members(n=64, seed=5, precision="mixed", maxiter_picard=20):
     """The n members of the seeded ensemble, as Models."""
     return [ensemble_member(sp, precision, maxiter_picard) for sp in ensemble_specs(n, seed)]
+
+
+def inversion_logc_true(nx=1280, ny=160, seed=4):
+    """True log C of the inversion workload: log 1e4 + GRF(std 0.5, 10 km)."""
+    Lx, Ly = 640000.0, 80000.0
+    return np.log(1e4) + gaussian_random_field(nx, ny, Lx / nx, Ly / ny, 0.5, 10000.0, seed)
+
+
+def inversion_model(logc, precision="fp64", nσ=8, n_picard=10, seed=2):
+    """Channel model of the inversion workload with C = exp(logc)."""
+    nx, ny = logc.shape
+    return channel_solve(nx=nx, ny=ny, precision=precision, seed=seed, nσ=nσ, maxiter_picard=n_picard,
+                         weertman_c=np.asfortranarray(np.exp(logc)))
+
+
+def inversion_observations(u, v, seed=4):
+    """Synthetic observations: velocities times (1 + 0.01 N(0, 1))."""
+    rng = np.random.default_rng(seed + 7)
+    return (np.asfortranarray(u * (1.0 + 0.01 * rng.standard_normal(u.shape))),
+            np.asfortranarray(v * (1.0 + 0.01 * rng.standard_normal(v.shape))))
