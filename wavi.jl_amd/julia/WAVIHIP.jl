@@ -74,6 +74,15 @@ struct SolveReport
     rel_resid_hist::NTuple{64,Float64}
 end
 
+struct AdjointReport
+    n_picard::Int32
+    fwd_vcycles::Int32
+    adj_vcycles::Int32
+    unconverged::Int32
+    max_fwd_rel::Float64
+    max_adj_rel::Float64
+end
+
 const PRECISION = Dict(:fp64 => Int32(0), :mixed => Int32(1))
 
 """
@@ -220,6 +229,54 @@ function run_steps!(simulation, nsteps::Integer)
     return simulation
 end
 
-export HIPSpec, run_steps!
+"""
+    update_velocities_batch!(models; nthreads = 8)
+
+`update_velocities!` on every model of an ensemble in one native call (host
+threads, one stream per model).  No counterpart in WAVI.jl.
+"""
+function update_velocities_batch!(models::AbstractVector; nthreads::Integer = 8)
+    for m in models
+        WAVI.update_preconditioner!(m)
+        push_fields!(m)
+    end
+    ctxs = Ptr{Cvoid}[m.parallel_spec.ctx for m in models]
+    reps = Vector{SolveReport}(undef, length(models))
+    codes = zeros(Cint, length(models))
+    ccall((:wavi_update_velocities_batch, LIB), Cint,
+          (Ptr{Ptr{Cvoid}}, Cint, Cint, Ptr{SolveReport}, Ptr{Cint}),
+          ctxs, length(models), nthreads, reps, codes)
+    for (m, c) in zip(models, codes)
+        check(m.parallel_spec.ctx, c)
+        pull_fields!(m)
+    end
+    return models
+end
+
+"""
+    misfit_gradient(model, n_picard, u_obs, v_obs; tol_linear = 1e-13, maxit_linear = 60)
+
+J = 1/2 ||u - u_obs||^2 after `n_picard` Picard iterations with converged
+linear solves, and dJ/d log C (an nx x ny array) by the adjoint sweep.  The
+model must hold the state after `update_state!`'s geometry steps.  No
+counterpart in WAVI.jl.
+"""
+function misfit_gradient(model, n_picard::Integer, u_obs::Matrix{Float64}, v_obs::Matrix{Float64};
+                         tol_linear::Float64 = 1e-13, maxit_linear::Integer = 60)
+    WAVI.update_preconditioner!(model)
+    push_fields!(model)
+    J = Ref{Cdouble}(0.0)
+    grad = zeros(Float64, model.grid.nx, model.grid.ny)
+    rep = Ref{AdjointReport}()
+    check(model.parallel_spec.ctx,
+          ccall((:wavi_misfit_gradient, LIB), Cint,
+                (Ptr{Cvoid}, Cint, Cdouble, Cint, Ptr{Float64}, Ptr{Float64}, Ref{Cdouble}, Ptr{Float64},
+                 Ref{AdjointReport}),
+                model.parallel_spec.ctx, n_picard, tol_linear, maxit_linear, u_obs, v_obs, J, grad, rep))
+    pull_fields!(model)
+    return J[], grad, rep[]
+end
+
+export HIPSpec, run_steps!, update_velocities_batch!, misfit_gradient
 
 end # module
