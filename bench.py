@@ -186,16 +186,20 @@ def timed_solves(W, nx, ny, precision, steps, warmup, picard=None, maxiter_coars
     return times, rep, velocities(W, m, spec), roof
 
 
-def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed"):
+def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False, n_picard=10):
     """The seeded ensemble (channel solves of mixed sizes, nσ, Glen A and
-    basal-drag fields) solved from the reset state with one
-    wavi_update_velocities_batch call per step; also timed member after
-    member with one thread, for the concurrency gain.  Returns the JSON line."""
+    basal-drag fields) from the reset state, one batched call per step:
+    forward solves (wavi_update_velocities_batch, 20 Picard iterations), or
+    with adjoint=True forward + adjoint gradients (wavi_misfit_gradient_batch,
+    n_picard Picard iterations with converged solves; each member's
+    observations are its own forward velocities times (1 + 0.01 N(0, 1))).
+    Also timed member after member with one thread, for the concurrency
+    gain.  Returns the JSON line."""
     specs = W.workloads.ensemble_specs(n)
     log("[bench] ensemble: %d members, setup" % n)
     members = []
     for sp in specs:
-        m = W.workloads.ensemble_member(sp, precision)
+        m = W.workloads.ensemble_member(sp, precision, n_picard if adjoint else 20)
         W.update_preconditioner(m)
         W.push_fields(m)
         spec = m.parallel_spec
@@ -204,14 +208,39 @@ def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed"):
         members.append(m)
     lib = members[0].parallel_spec._lib
     ctxs = (ctypes.c_void_p * n)(*[m.parallel_spec.ctx for m in members])
-    reps = (W._lib.SolveReport * n)()
     codes = (ctypes.c_int * n)()
+    if adjoint:
+        obs = []
+        for sp, m in zip(specs, members):
+            nx, ny = sp["nx"], sp["ny"]
+            z = np.zeros((nx + 1) * ny + nx * (ny + 1))
+            J = ctypes.c_double(0.0)
+            ctx = m.parallel_spec.ctx
+            assert lib.wavi_misfit_gradient(ctx, n_picard, 1e-13, 200, W._lib.dptr(z), W._lib.dptr(z),
+                                            ctypes.byref(J), None, None) == 0
+            u, v = np.empty((nx + 1) * ny), np.empty(nx * (ny + 1))
+            assert lib.wavi_get_field(ctx, W._lib.FIELD["U_U"], W._lib.dptr(u), u.size) == 0
+            assert lib.wavi_get_field(ctx, W._lib.FIELD["V_V"], W._lib.dptr(v), v.size) == 0
+            uo, vo = W.workloads.inversion_observations(u, v, seed=sp["seed"])
+            obs.append((np.ascontiguousarray(uo), np.ascontiguousarray(vo)))
+        up = (W._lib._dp * n)(*[W._lib.dptr(a) for a, _ in obs])
+        vp = (W._lib._dp * n)(*[W._lib.dptr(b) for _, b in obs])
+        Js = np.zeros(n)
+        grads = [np.zeros(sp["nx"] * sp["ny"]) for sp in specs]
+        gp = (W._lib._dp * n)(*[W._lib.dptr(g) for g in grads])
+        reps = (W._lib.AdjointReport * n)()
+    else:
+        reps = (W._lib.SolveReport * n)()
 
     def run(threads):
         for m in members:
             assert lib.wavi_state_restore(m.parallel_spec.ctx) == 0
         t0 = time.perf_counter()
-        rc = lib.wavi_update_velocities_batch(ctxs, n, threads, reps, codes)
+        if adjoint:
+            rc = lib.wavi_misfit_gradient_batch(ctxs, n, threads, n_picard, 1e-13, 200, up, vp, W._lib.dptr(Js),
+                                                gp, reps, codes)
+        else:
+            rc = lib.wavi_update_velocities_batch(ctxs, n, threads, reps, codes)
         dt = time.perf_counter() - t0
         if rc != 0:
             bad = [k for k in range(n) if codes[k] != 0]
@@ -227,22 +256,29 @@ def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed"):
         log("[bench] ensemble step %d: %.3f s (%d threads)" % (k, times[-1], nthreads))
     serial = run(1)
     log("[bench] ensemble one member at a time: %.3f s" % serial)
-    picard = [reps[k].picard_iters for k in range(n)]
     total = float(sum(times))
     unknowns = sum(2 * sp["nx"] * sp["ny"] + sp["nx"] + sp["ny"] for sp in specs)
+    cfg = {"workload": "ensemble of %d channel %s, grids from %s, nsigma 4 or 8, A log-uniform in [1e-17, 4e-17], "
+                       "C = 1e4 exp(GRF std 0.3)"
+                       % (n, "forward + adjoint gradients (%d Picard iterations, solves to 1e-13)" % n_picard
+                          if adjoint else "solves (20 Picard iterations)",
+                          ["%dx%d" % s for s in W.workloads.ENSEMBLE_SIZES]),
+           "precision": precision, "host_threads": nthreads, "unknowns_total": unknowns,
+           "serial_members_per_s": n / serial, "batched_over_serial": serial / (total / steps)}
+    if adjoint:
+        cfg["unconverged_solves"] = int(sum(reps[k].unconverged for k in range(n)))
+        cfg["max_rel_residual"] = float(max(max(reps[k].max_fwd_rel, reps[k].max_adj_rel) for k in range(n)))
+    else:
+        picard = [reps[k].picard_iters for k in range(n)]
+        cfg["picard_iters_min_max"] = [min(picard), max(picard)]
+        cfg["all_converged"] = all(reps[k].converged for k in range(n))
     return {
-        "metric": "ensemble members solved per second (forward velocity solves)",
+        "metric": "ensemble members per second (%s)" % ("forward + adjoint gradients" if adjoint
+                                                        else "forward velocity solves"),
         "value": n * steps / total, "unit": "members/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": 1e3 * total / steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64" if precision == "mixed" else "f64",
-        "data": "synthetic (seeded ensemble of channel setups)",
-        "config": {"workload": "ensemble of %d channel solves, grids from %s, nsigma 4 or 8, "
-                               "A log-uniform in [1e-17, 4e-17], C = 1e4 exp(GRF std 0.3)"
-                               % (n, ["%dx%d" % s for s in W.workloads.ENSEMBLE_SIZES]),
-                   "precision": precision, "host_threads": nthreads, "unknowns_total": unknowns,
-                   "serial_members_per_s": n / serial, "batched_over_serial": serial / (total / steps),
-                   "picard_iters_min_max": [min(picard), max(picard)],
-                   "all_converged": all(reps[k].converged for k in range(n))},
+        "data": "synthetic (seeded ensemble of channel setups)", "config": cfg,
     }
 
 
@@ -384,6 +420,8 @@ def main():
     ap.add_argument("--ensemble", type=int, default=0,
                     help="time the seeded ensemble of this many members instead of the single solve")
     ap.add_argument("--threads", type=int, default=8, help="host threads of the ensemble batch")
+    ap.add_argument("--with-adjoint", action="store_true",
+                    help="with --ensemble: forward + adjoint gradient per member")
     ap.add_argument("--adjoint", action="store_true",
                     help="time the basal-drag gradient (FP64 and mixed) and check it by finite differences")
     ap.add_argument("--fd-directions", type=int, default=16)
@@ -400,8 +438,8 @@ def main():
         return
     if args.ensemble:
         prec = "fp64" if args.precision == "fp64" else "mixed"
-        print(json.dumps(ensemble_run(W, args.ensemble, args.threads, args.steps, args.warmup, prec)),
-              flush=True)
+        print(json.dumps(ensemble_run(W, args.ensemble, args.threads, args.steps, args.warmup, prec,
+                                      args.with_adjoint)), flush=True)
         return
     nx, ny = args.nx, args.ny
     if args.calibration_copy_mib:

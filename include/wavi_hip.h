@@ -224,6 +224,16 @@ typedef struct wavi_adjoint_report {
 int wavi_misfit_gradient(wavi_ctx *ctx, int n_picard, double tol_linear, int maxit_linear,
                          const double *u_obs, const double *v_obs, double *J, double *grad_logc,
                          wavi_adjoint_report *report);
+/* wavi_misfit_gradient for each of n contexts (an ensemble), driven by
+ * nthreads host threads with one stream per context, like
+ * wavi_update_velocities_batch.  u_obs[k], v_obs[k]: member k's
+ * observations; J (n entries) receives the misfits; grad_logc (NULL, or n
+ * pointers of which any may be NULL) the gradients; reports and codes (each
+ * NULL or n entries) the members' reports and return codes.  Returns 0 or
+ * the first nonzero member code.  No reference counterpart. */
+int wavi_misfit_gradient_batch(wavi_ctx **ctxs, int n, int nthreads, int n_picard, double tol_linear,
+                               int maxit_linear, const double *const *u_obs, const double *const *v_obs,
+                               double *J, double *const *grad_logc, wavi_adjoint_report *reports, int *codes);
 
 /* ---- test hooks: single pieces of the solve on host arrays ---- */
 /* op_fun! (src/utilities.jl:61-145) on full-grid inputs xu ((nx+1)*ny) and

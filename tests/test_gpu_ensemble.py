@@ -57,3 +57,31 @@ def test_batch_edge_cases(W):
     W.update_velocities_batch(two, nthreads=1)
     for a, b in zip(alone, two):
         np.testing.assert_array_equal(a.fields.gv.v, b.fields.gv.v)
+
+
+def test_gradient_batch_matches_individual_calls(W):
+    specs = _small_specs(W, 4, seed=9)
+    _step("ensemble gradients: %d individual calls" % len(specs))
+    obs = []
+    for sp in specs:
+        m = W.workloads.ensemble_member(sp, "mixed", maxiter_picard=3)
+        W.update_geometry(m)
+        nx, ny = sp["nx"], sp["ny"]
+        W.misfit_gradient(m, 3, np.zeros((nx + 1, ny)), np.zeros((nx, ny + 1)), gradient=False)
+        obs.append(W.workloads.inversion_observations(m.fields.gu.u, m.fields.gv.v, seed=sp["seed"]))
+
+    def members():
+        out = []
+        for sp in specs:
+            m = W.workloads.ensemble_member(sp, "mixed", maxiter_picard=3)
+            W.update_geometry(m)
+            out.append(m)
+        return out
+
+    alone = [W.misfit_gradient(m, 3, uo, vo) for m, (uo, vo) in zip(members(), obs)]
+    _step("ensemble gradients: one batched call, 3 threads")
+    batch = W.misfit_gradient_batch(members(), 3, obs, nthreads=3)
+    for (Ja, ga, ra), (Jb, gb, rb) in zip(alone, batch):
+        assert Ja == Jb
+        np.testing.assert_array_equal(ga, gb)
+        assert ra == rb

@@ -10,17 +10,18 @@ from .core import (ArgumentError, DimensionMismatch, Grid, InitialConditions, Pa
                    get_u_mask, get_v_mask, glen_b, height_above_floatation, mismip_plus_bed,
                    orientations2bc, volume_above_floatation)
 from .model import (BasicParallelSpec, Fields, HIPSpec, Model, get_op_diag, get_rhs, haar,
-                    inner_update, misfit_gradient, op_apply, update_geometry, pull_fields, push_fields, update_preconditioner,
-                    update_state, update_velocities, update_velocities_batch, update_wavelets,
-                    vcycle)
+                    inner_update, misfit_gradient, misfit_gradient_batch, op_apply, pull_fields,
+                    push_fields, update_geometry, update_preconditioner, update_state,
+                    update_velocities, update_velocities_batch, update_wavelets, vcycle)
 from .simulation import Clock, Simulation, run_simulation, timestep, update_thickness
 from . import _lib, workloads
 
 __all__ = [
     "ArgumentError", "DimensionMismatch", "Grid", "InitialConditions", "Params", "SolverParams",
     "TimesteppingParams", "UniformMeltRate", "Model", "HIPSpec", "BasicParallelSpec", "Fields",
-    "Simulation", "Clock", "update_velocities", "update_velocities_batch", "update_state", "inner_update",
-    "update_wavelets", "update_preconditioner", "misfit_gradient", "update_geometry", "timestep", "run_simulation",
+    "Simulation", "Clock", "update_velocities", "update_velocities_batch", "update_state",
+    "inner_update", "update_wavelets", "update_preconditioner", "update_geometry",
+    "misfit_gradient", "misfit_gradient_batch", "timestep", "run_simulation",
     "update_thickness", "op_apply", "get_rhs", "get_op_diag", "vcycle", "haar",
     "push_fields", "pull_fields", "glen_b", "mismip_plus_bed", "orientations2bc",
     "get_u_mask", "get_v_mask", "get_c_mask", "clip", "height_above_floatation",

@@ -122,6 +122,11 @@ SIGNATURES = {
     "wavi_state_save": (ctypes.c_int, [ctypes.c_void_p]),
     "wavi_state_restore": (ctypes.c_int, [ctypes.c_void_p]),
     "wavi_update_velocities": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveReport)]),
+    "wavi_misfit_gradient_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                  ctypes.POINTER(_dp), ctypes.POINTER(_dp), _dp,
+                                                  ctypes.POINTER(_dp), ctypes.POINTER(AdjointReport),
+                                                  ctypes.POINTER(ctypes.c_int)]),
     "wavi_update_velocities_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
                                                     ctypes.POINTER(SolveReport),
                                                     ctypes.POINTER(ctypes.c_int)]),

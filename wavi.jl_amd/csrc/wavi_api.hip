@@ -322,6 +322,38 @@ static void build_haar_masks(HaarMasks tabs[4])
     }
 }
 
+// Run fn(ctx, k) for members k < n on a pool of nthreads host threads (the
+// calling thread included), each first making the member's device current.
+// codes (NULL or n entries) receives every member's return code; the
+// result is the first nonzero one in member order.
+template <typename F>
+static int run_members(wavi_ctx **ctxs, int n, int nthreads, int *codes, F &&fn)
+{
+    std::vector<int> rcs(n, WAVI_OK);
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        for (int k = next.fetch_add(1); k < n; k = next.fetch_add(1)) {
+            wavi_ctx *ctx = ctxs[k];
+            if (hipSetDevice(ctx->device) != hipSuccess) {
+                ctx->err = "hipSetDevice failed";
+                rcs[k] = WAVI_EHIP;
+                continue;
+            }
+            rcs[k] = fn(ctx, k);
+        }
+    };
+    const int nt = std::min(nthreads, std::max(n, 1));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (std::thread &t : pool) t.join();
+    if (codes)
+        for (int k = 0; k < n; ++k) codes[k] = rcs[k];
+    for (int k = 0; k < n; ++k)
+        if (rcs[k] != WAVI_OK) return rcs[k];
+    return WAVI_OK;
+}
+
 // ---------------------------------------------------------------------------
 extern "C" {
 #pragma GCC visibility push(default)
@@ -1638,29 +1670,22 @@ int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,
     if (n < 0 || nthreads < 1 || (n > 0 && !ctxs)) return WAVI_EARG;
     for (int k = 0; k < n; ++k)
         if (!ctxs[k]) return WAVI_EARG;
-    std::vector<int> rcs(n, WAVI_OK);
-    std::atomic<int> next{0};
-    auto worker = [&]() {
-        for (int k = next.fetch_add(1); k < n; k = next.fetch_add(1)) {
-            wavi_ctx *ctx = ctxs[k];
-            if (hipSetDevice(ctx->device) != hipSuccess) {
-                ctx->err = "hipSetDevice failed";
-                rcs[k] = WAVI_EHIP;
-                continue;
-            }
-            rcs[k] = wavi_update_velocities(ctx, reports ? &reports[k] : nullptr);
-        }
-    };
-    const int nt = std::min(nthreads, std::max(n, 1));
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
-    worker();
-    for (std::thread &t : pool) t.join();
-    if (codes)
-        for (int k = 0; k < n; ++k) codes[k] = rcs[k];
+    return run_members(ctxs, n, nthreads, codes, [&](wavi_ctx *ctx, int k) {
+        return wavi_update_velocities(ctx, reports ? &reports[k] : nullptr);
+    });
+}
+
+int wavi_misfit_gradient_batch(wavi_ctx **ctxs, int n, int nthreads, int n_picard, double tol_linear,
+                               int maxit_linear, const double *const *u_obs, const double *const *v_obs, double *J,
+                               double *const *grad_logc, wavi_adjoint_report *reports, int *codes)
+{
+    if (n < 0 || nthreads < 1 || (n > 0 && (!ctxs || !u_obs || !v_obs || !J))) return WAVI_EARG;
     for (int k = 0; k < n; ++k)
-        if (rcs[k] != WAVI_OK) return rcs[k];
-    return WAVI_OK;
+        if (!ctxs[k]) return WAVI_EARG;
+    return run_members(ctxs, n, nthreads, codes, [&](wavi_ctx *ctx, int k) {
+        return wavi_misfit_gradient(ctx, n_picard, tol_linear, maxit_linear, u_obs[k], v_obs[k], &J[k],
+                                    grad_logc ? grad_logc[k] : nullptr, reports ? &reports[k] : nullptr);
+    });
 }
 
 int wavi_inner_update(wavi_ctx *ctx)

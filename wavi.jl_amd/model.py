@@ -439,6 +439,50 @@ def misfit_gradient(model, n_picard, u_obs, v_obs, tol_linear=1e-13, maxit_linea
     return J.value, (g.reshape((nx, ny), order="F") if gradient else None), rep.as_dict()
 
 
+def misfit_gradient_batch(models, n_picard, observations, tol_linear=1e-13, maxit_linear=60, nthreads=8,
+                          gradient=True):
+    """misfit_gradient on every model of an ensemble in one native call
+    (host threads, one stream per model).  observations: one (u_obs, v_obs)
+    pair per model.  Returns a list of (J, grad or None, report dict)."""
+    models = list(models)
+    n = len(models)
+    if n == 0:
+        return []
+    obs = []
+    for m, (uo, vo) in zip(models, observations):
+        update_preconditioner(m)
+        push_fields(m)
+        nx, ny = m.grid.nx, m.grid.ny
+        a = np.ascontiguousarray(np.asarray(uo, dtype=np.float64).ravel(order="F"))
+        b = np.ascontiguousarray(np.asarray(vo, dtype=np.float64).ravel(order="F"))
+        if a.size != (nx + 1) * ny or b.size != nx * (ny + 1):
+            raise DimensionMismatch("u_obs must have (nx+1)*ny values and v_obs nx*(ny+1)")
+        obs.append((a, b))
+    if len(obs) != n:
+        raise ArgumentError("one (u_obs, v_obs) pair per model is required")
+    lib = models[0].parallel_spec._lib
+    ctxs = (ctypes.c_void_p * n)(*[m.parallel_spec.ctx for m in models])
+    up = (_lib._dp * n)(*[_lib.dptr(a) for a, _ in obs])
+    vp = (_lib._dp * n)(*[_lib.dptr(b) for _, b in obs])
+    J = np.zeros(n)
+    grads = [np.zeros(m.grid.nx * m.grid.ny) for m in models] if gradient else None
+    gp = (_lib._dp * n)(*[_lib.dptr(g) for g in grads]) if gradient else None
+    reps = (_lib.AdjointReport * n)()
+    codes = (ctypes.c_int * n)()
+    rc = lib.wavi_misfit_gradient_batch(ctxs, n, int(nthreads), int(n_picard), float(tol_linear), int(maxit_linear),
+                                        up, vp, _lib.dptr(J), gp, reps, codes)
+    if rc != _lib.WAVI_OK:
+        for m, code in zip(models, codes):
+            _check(lib, m.parallel_spec.ctx, code)
+        _check(lib, None, rc)
+    out = []
+    for k, m in enumerate(models):
+        pull_fields(m)
+        g = grads[k].reshape((m.grid.nx, m.grid.ny), order="F") if gradient else None
+        out.append((float(J[k]), g, reps[k].as_dict()))
+    return out
+
+
 def inner_update(model):
     """inner_update!(model) (update_velocities.jl:32-45)."""
     update_preconditioner(model)
