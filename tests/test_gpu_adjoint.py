@@ -77,3 +77,23 @@ def test_fixed_velocities_rejected(W):
     W.update_geometry(m)
     with pytest.raises(W.ArgumentError):
         W.misfit_gradient(m, 2, np.zeros((nx + 1, ny)), np.zeros((nx, ny + 1)))
+
+
+def test_torch_autograd_binding(W):
+    """MisfitLogC.apply(logc, problem).backward() gives the adjoint gradient,
+    and the problem re-evaluates from its fixed starting state."""
+    import importlib
+    torch = pytest.importorskip("torch")
+    autodiff = importlib.import_module("wavi_jl_amd.autodiff")
+    nx, ny = 64, 16
+    _step("adjoint through torch.autograd %dx%d" % (nx, ny))
+    l0, uo, vo = _setup(W, nx, ny)
+    problem = autodiff.MisfitProblem(_model(W, l0, "mixed"), N_PICARD, uo, vo, TOL_LIN)
+    logc = torch.tensor(l0, dtype=torch.float64, requires_grad=True)
+    J = autodiff.MisfitLogC.apply(logc, problem)
+    J.backward()
+    Jr, gr, _ = W.misfit_gradient(_model(W, l0, "mixed"), N_PICARD, uo, vo, TOL_LIN)
+    assert float(J.detach()) == Jr
+    np.testing.assert_array_equal(logc.grad.numpy(), gr)
+    J2 = autodiff.MisfitLogC.apply(logc.detach(), problem)
+    assert float(J2) == Jr

@@ -14,6 +14,7 @@ been run: the build image has no Julia.
 module WAVIHIP
 
 using WAVI
+using ChainRulesCore
 import WAVI: update_velocities!, update_state!, AbstractParallelSpec, AbstractModel
 
 const LIB = get(ENV, "WAVI_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libwavi_hip.so"))
@@ -277,6 +278,58 @@ function misfit_gradient(model, n_picard::Integer, u_obs::Matrix{Float64}, v_obs
     return J[], grad, rep[]
 end
 
-export HIPSpec, run_steps!, update_velocities_batch!, misfit_gradient
+"""
+    MisfitProblem(model, n_picard, u_obs, v_obs)
+
+The misfit J(log C) of `misfit_gradient` from a fixed starting state (the
+model's fields when the problem is made): `misfit(problem, logC)` is a pure
+function of `logC`, and its `rrule` returns the adjoint gradient computed in
+the same library call.
+"""
+struct MisfitProblem{Mo}
+    model::Mo
+    n_picard::Int
+    u_obs::Matrix{Float64}
+    v_obs::Matrix{Float64}
+    start::Vector{Any}   # copies of fieldlist(model) when the problem was made
+end
+
+function MisfitProblem(model, n_picard::Integer, u_obs::Matrix{Float64}, v_obs::Matrix{Float64})
+    return MisfitProblem(model, Int(n_picard), u_obs, v_obs, Any[copy(a) for a in fieldlist(model)])
+end
+
+"""
+    update_geometry!(model)
+
+The steps of `update_state!` before the solve (`src/Models/update_state.jl:8-15`),
+including `weertman_c = C * grounded fraction`, on the device.
+"""
+function update_geometry!(model)
+    WAVI.update_preconditioner!(model)
+    push_fields!(model)
+    check(model.parallel_spec.ctx, ccall((:wavi_update_geometry, LIB), Cint, (Ptr{Cvoid},), model.parallel_spec.ctx))
+    pull_fields!(model)
+    return model
+end
+
+function evaluate(problem::MisfitProblem, logC::AbstractMatrix{Float64})
+    model = problem.model
+    for (a, a0) in zip(fieldlist(model), problem.start)
+        a .= a0
+    end
+    model.params.weertman_c .= exp.(logC)
+    update_geometry!(model)
+    return misfit_gradient(model, problem.n_picard, problem.u_obs, problem.v_obs)
+end
+
+misfit(problem::MisfitProblem, logC::AbstractMatrix{Float64}) = evaluate(problem, logC)[1]
+
+function ChainRulesCore.rrule(::typeof(misfit), problem::MisfitProblem, logC::AbstractMatrix{Float64})
+    J, grad, _ = evaluate(problem, logC)
+    misfit_pullback(Jbar) = (NoTangent(), NoTangent(), Jbar .* grad)
+    return J, misfit_pullback
+end
+
+export HIPSpec, run_steps!, update_velocities_batch!, update_geometry!, misfit_gradient, MisfitProblem, misfit
 
 end # module
