@@ -1849,6 +1849,69 @@ __device__ __forceinline__ float haar_line_R(float v, int pos)
     return v;
 }
 
+// The same transforms on N lines at once, each stage one v_fmac_f32 with a
+// DPP source per line (v += sign * partner(v)) instead of a DPP move and an
+// FMA.  Every block starts with two wait states: a DPP read of a register
+// that the previous VALU instruction wrote needs them, and the compiler
+// does not see the VALU writes inside these blocks.  The H = 4 partner is
+// row_half_mirror of quad_perm [0,0,0,0], two DPP steps.
+#define WAVI_DPP_TAIL " row_mask:0xf bank_mask:0xf bound_ctrl:1"
+
+template <int H>
+__device__ __forceinline__ void haar_stage_dpp(float &v0, float &v1, float sg)
+{
+    if constexpr (H == 1)
+        asm("s_nop 1\n\tv_fmac_f32_dpp %0, %0, %2 quad_perm:[1,0,3,2]" WAVI_DPP_TAIL
+            "\n\tv_fmac_f32_dpp %1, %1, %2 quad_perm:[1,0,3,2]" WAVI_DPP_TAIL
+            : "+v"(v0), "+v"(v1) : "v"(sg));
+    else if constexpr (H == 2)
+        asm("s_nop 1\n\tv_fmac_f32_dpp %0, %0, %2 quad_perm:[2,3,0,1]" WAVI_DPP_TAIL
+            "\n\tv_fmac_f32_dpp %1, %1, %2 quad_perm:[2,3,0,1]" WAVI_DPP_TAIL
+            : "+v"(v0), "+v"(v1) : "v"(sg));
+    else if constexpr (H == 8)
+        asm("s_nop 1\n\tv_fmac_f32_dpp %0, %0, %2 row_ror:8" WAVI_DPP_TAIL
+            "\n\tv_fmac_f32_dpp %1, %1, %2 row_ror:8" WAVI_DPP_TAIL
+            : "+v"(v0), "+v"(v1) : "v"(sg));
+    else {
+        float t0, t1;
+        asm("s_nop 1\n\tv_mov_b32_dpp %2, %0 quad_perm:[0,0,0,0]" WAVI_DPP_TAIL
+            "\n\tv_mov_b32_dpp %3, %1 quad_perm:[0,0,0,0]" WAVI_DPP_TAIL
+            "\n\ts_nop 1\n\tv_fmac_f32_dpp %0, %2, %4 row_half_mirror" WAVI_DPP_TAIL
+            "\n\tv_fmac_f32_dpp %1, %3, %4 row_half_mirror" WAVI_DPP_TAIL
+            : "+v"(v0), "+v"(v1), "=&v"(t0), "=&v"(t1) : "v"(sg));
+    }
+}
+
+template <int H>
+__device__ __forceinline__ float haar_sign_P(int pos)
+{
+    return ((pos & (H - 1)) == 0) ? ((pos & H) ? 1.f : -1.f) : 0.f;
+}
+
+template <int H>
+__device__ __forceinline__ float haar_sign_R(int pos)
+{
+    return ((pos & (H - 1)) == 0) ? ((pos & H) ? -1.f : 1.f) : 0.f;
+}
+
+template <int LEV>
+__device__ __forceinline__ void haar_lines_P(float &v0, float &v1, int pos)
+{
+    if constexpr (LEV >= 3) haar_stage_dpp<8>(v0, v1, haar_sign_P<8>(pos));
+    if constexpr (LEV >= 2) haar_stage_dpp<4>(v0, v1, haar_sign_P<4>(pos));
+    if constexpr (LEV >= 1) haar_stage_dpp<2>(v0, v1, haar_sign_P<2>(pos));
+    haar_stage_dpp<1>(v0, v1, haar_sign_P<1>(pos));
+}
+
+template <int LEV>
+__device__ __forceinline__ void haar_lines_R(float &v0, float &v1, int pos)
+{
+    haar_stage_dpp<1>(v0, v1, haar_sign_R<1>(pos));
+    if constexpr (LEV >= 1) haar_stage_dpp<2>(v0, v1, haar_sign_R<2>(pos));
+    if constexpr (LEV >= 2) haar_stage_dpp<4>(v0, v1, haar_sign_R<4>(pos));
+    if constexpr (LEV >= 3) haar_stage_dpp<8>(v0, v1, haar_sign_R<8>(pos));
+}
+
 // Halo point h (0..67) of the 18 x 18 window: window coordinates.
 __device__ __forceinline__ void halo_xy(int h, int &x, int &y)
 {
@@ -2081,11 +2144,17 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
         st_nt(&el(arr(CGM_R + 1), ob), r_v);
     }
     // ---- x pass of P r and P s; halo and thickness windows into LDS
-    sm.T[0][ly][lx] = haar_line_P<LEV>(mwu ? r_u : 0.f, lx);
-    sm.T[1][ly][lx] = haar_line_P<LEV>(mwv ? r_v : 0.f, lx);
-    if (!INIT) {
-        sm.T[2][ly][lx] = haar_line_P<LEV>(mwu ? s_u : 0.f, lx);
-        sm.T[3][ly][lx] = haar_line_P<LEV>(mwv ? s_v : 0.f, lx);
+    {
+        float a0 = mwu ? r_u : 0.f, a1 = mwv ? r_v : 0.f;
+        haar_lines_P<LEV>(a0, a1, lx);
+        sm.T[0][ly][lx] = a0;
+        sm.T[1][ly][lx] = a1;
+        if (!INIT) {
+            float a2 = mwu ? s_u : 0.f, a3 = mwv ? s_v : 0.f;
+            haar_lines_P<LEV>(a2, a3, lx);
+            sm.T[2][ly][lx] = a2;
+            sm.T[3][ly][lx] = a3;
+        }
     }
     sm.mb[0][ly][lx] = mbu;
     sm.mb[1][ly][lx] = mbv;
@@ -2106,15 +2175,15 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     // ---- y pass in the column-major lanes: P r into the window, rings of P r and P s
     {
         const uint8_t m0 = sm.mb[0][cy][cx], m1 = sm.mb[1][cy][cx];
-        const float y0 = haar_line_P<LEV>(sm.T[0][cy][cx], cy);
-        const float y1 = haar_line_P<LEV>(sm.T[1][cy][cx], cy);
+        float y0 = sm.T[0][cy][cx], y1 = sm.T[1][cy][cx];
+        haar_lines_P<LEV>(y0, y1, cy);
         const float pr_u = (m0 & 1) ? y0 : 0.f, pr_v = (m1 & 1) ? y1 : 0.f;
         sm.U[cy + 1][cx + 1] = pr_u;
         sm.V[cy + 1][cx + 1] = pr_v;
         float ps_u = 0.f, ps_v = 0.f;
         if (!INIT) {
-            const float y2 = haar_line_P<LEV>(sm.T[2][cy][cx], cy);
-            const float y3 = haar_line_P<LEV>(sm.T[3][cy][cx], cy);
+            float y2 = sm.T[2][cy][cx], y3 = sm.T[3][cy][cx];
+            haar_lines_P<LEV>(y2, y3, cy);
             ps_u = (m0 & 1) ? y2 : 0.f;
             ps_v = (m1 & 1) ? y3 : 0.f;
         }
@@ -2156,17 +2225,20 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
         const float d_rxy_v = rdx * (sm.rxy[y][x + 1] - sm.rxy[y][x]);
         const float d_ext_v = rdy * (sm.ext[y + 1][x + 1] - sm.ext[y][x + 1]);
         const float Av = ((d_ryy + d_rxy_v) - bv0 * sm.V[y + 1][x + 1]) - sm.HV[y + 1][x + 1] * d_ext_v;
-        sm.T[0][ly][lx] = haar_line_R<LEV>(inu ? Au : 0.f, lx);
-        sm.T[1][ly][lx] = haar_line_R<LEV>(inv ? Av : 0.f, lx);
+        float a0 = inu ? Au : 0.f, a1 = inv ? Av : 0.f;
+        haar_lines_R<LEV>(a0, a1, lx);
+        sm.T[0][ly][lx] = a0;
+        sm.T[1][ly][lx] = a1;
     }
     __syncthreads();
     // ---- R y pass, wavelet mask (w), P y pass
     {
-        const float c0 = haar_line_R<LEV>(sm.T[0][cy][cx], cy);
-        const float c1 = haar_line_R<LEV>(sm.T[1][cy][cx], cy);
+        float c0 = sm.T[0][cy][cx], c1 = sm.T[1][cy][cx];
+        haar_lines_R<LEV>(c0, c1, cy);
         const float w0 = (sm.mb[0][cy][cx] & 2) ? c0 : 0.f;
         const float w1 = (sm.mb[1][cy][cx] & 2) ? c1 : 0.f;
-        const float q0 = haar_line_P<LEV>(w0, cy), q1v = haar_line_P<LEV>(w1, cy);
+        float q0 = w0, q1v = w1;
+        haar_lines_P<LEV>(q0, q1v, cy);
         sm.T[0][cy][cx] = w0;
         sm.T[1][cy][cx] = w1;
         sm.T[2][cy][cx] = q0;
@@ -2175,7 +2247,8 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     __syncthreads();
     // ---- P x pass, stores, partials of (r, r) and (w, r)
     const float wn_u = sm.T[0][ly][lx], wn_v = sm.T[1][ly][lx];
-    const float tw_u = haar_line_P<LEV>(sm.T[2][ly][lx], lx), tw_v = haar_line_P<LEV>(sm.T[3][ly][lx], lx);
+    float tw_u = sm.T[2][ly][lx], tw_v = sm.T[3][ly][lx];
+    haar_lines_P<LEV>(tw_u, tw_v, lx);
     st_nt(&el(arr(CGM_W), ob), wn_u);
     st_nt(&el(arr(CGM_W + 1), ob), wn_v);
     {
