@@ -88,7 +88,7 @@ typedef struct wavi_solver_params {
     int32_t maxiter_picard;
     double tol_coarse;
     int32_t maxiter_coarse;
-    int32_t levels;               /* Haar levels: blocks of 2^(levels+1), 0..4 */
+    int32_t levels;               /* Haar levels: blocks of 2^(levels+1), 0..3 */
     double wavelet_threshold;
     int32_t nsmooth;
     double smoother_omega;

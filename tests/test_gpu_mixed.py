@@ -98,3 +98,82 @@ def test_mixed_solve_is_deterministic(W):
         runs.append((_velocities(W, m, spec), rep.as_dict()["coarse_cg_iters"]))
     assert np.array_equal(runs[0][0], runs[1][0])
     assert runs[0][1] == runs[1][1]
+
+
+def _solve_pair(W, make):
+    """Velocities of the FP64 and the mixed solve of the setup make(precision)."""
+    vel, reps = {}, {}
+    for prec in ("fp64", "mixed"):
+        m = make(prec)
+        W.update_geometry(m)
+        W.update_velocities(m)
+        reps[prec] = m.last_report
+        vel[prec] = np.concatenate([m.fields.gu.u.ravel(), m.fields.gv.v.ravel()])
+    return vel, reps
+
+
+@pytest.mark.parametrize("levels", [0, 1, 2])
+def test_mixed_other_haar_levels(W, levels):
+    """The mixed coarse CG's butterflies for blocks of 2, 4 and 8."""
+    _step("mixed levels %d" % levels)
+
+    def make(prec):
+        m = channel(W, nx=96, ny=24, precision=prec, maxiter_picard=20, seed=3, nσ=4)
+        m.solver_params.levels = levels
+        return m
+
+    vel, reps = _solve_pair(W, make)
+    err = rel_l2(vel["mixed"], vel["fp64"])
+    print("[gpu] levels %d: rel L2 %.3e" % (levels, err), flush=True)
+    assert err <= 1e-6
+
+
+# The next two setups are compared after the Picard loop has converged to
+# 1e-9: before that the FP32 coarse CG, which cannot reach tol_coarse = 1e-8
+# and stops at maxiter, gives other (equally valid) corrections than the
+# FP64 one, and the unconverged iterates differ by up to 2e-4 (MISMIP+ at 20
+# iterations).
+def test_mixed_iceberg_masks_and_fixed_velocities(W):
+    """An irregular ice mask (a circular shelf) with non-homogeneous Dirichlet
+    points on the centre lines: the mixed solve's masks and fixed values.
+
+    The floating iceberg has no basal drag and moves almost as a solid body,
+    so its discrete problem is nearly singular: two FP64 solves with
+    different preconditioners (3 or 2 Haar levels, or twice the smoothing)
+    both reach a relative residual of 8e-13 at solutions 4-6% apart.  The
+    mixed solve reaches the same residual 5e-6 from the FP64 solve with the
+    same preconditioner; the bar is 1e-5, with the residual checked too."""
+    from helpers import iceberg
+    _step("mixed iceberg")
+
+    def make(prec):
+        m, _, _ = iceberg(W, precision=prec, nx=40, ny=40)
+        m.solver_params.maxiter_picard = 100
+        m.solver_params.tol_picard = 1e-9
+        return m
+
+    vel, reps = _solve_pair(W, make)
+    err = rel_l2(vel["mixed"], vel["fp64"])
+    print("[gpu] iceberg: rel L2 %.3e, picard %d / %d" % (err, reps["fp64"]["picard_iters"],
+                                                         reps["mixed"]["picard_iters"]), flush=True)
+    assert reps["fp64"]["converged"] and reps["mixed"]["converged"]
+    assert reps["mixed"]["rel_resid"] < 1e-9
+    assert err <= 1e-5
+
+
+def test_mixed_mismip_partial_tiles(W):
+    """MISMIP+ at 80 x 10: the grid is narrower than one 16 x 16 tile."""
+    _step("mixed mismip+ 80x10")
+
+    def make(prec):
+        m = W.workloads.mismip_plus_fixture(precision=prec)
+        m.solver_params.maxiter_picard = 100
+        m.solver_params.tol_picard = 1e-9
+        return m
+
+    vel, reps = _solve_pair(W, make)
+    err = rel_l2(vel["mixed"], vel["fp64"])
+    print("[gpu] mismip+: rel L2 %.3e, picard %d / %d" % (err, reps["fp64"]["picard_iters"],
+                                                         reps["mixed"]["picard_iters"]), flush=True)
+    assert reps["fp64"]["converged"] and reps["mixed"]["converged"]
+    assert err <= 1e-6

@@ -211,3 +211,21 @@ def test_channel_transient_200_steps(W, oracle):
     o.run_steps(200)
     assert rel_l2(m.fields.gh.h, o.field("H_H")) <= 1e-9
     _assert_fields_equal(W, m, o)
+
+
+@pytest.mark.parametrize("levels", [0, 1, 2])
+def test_update_velocities_other_haar_levels(W, oracle, levels):
+    """Haar levels 0..2 (blocks of 2, 4, 8): bit-equal to the oracle."""
+    _step("update_velocities levels %d" % levels)
+    m = channel(W, nx=40, ny=24, maxiter_picard=2)
+    m.solver_params.levels = levels
+    o = oracle.OracleModel(m)
+    spec = _gpu_model(W, m)
+    lib = spec._lib
+    assert lib.wavi_update_geometry(spec.ctx) == 0
+    o.update_geometry()
+    rep = W._lib.SolveReport()
+    assert lib.wavi_update_velocities(spec.ctx, ctypes.byref(rep)) == 0
+    orep = o.update_velocities()
+    assert rep.as_dict()["coarse_cg_iters"] == orep["coarse_cg_iters"]
+    _assert_fields_equal(W, m, o)
