@@ -176,6 +176,11 @@ int wavi_update_velocities(wavi_ctx *ctx, wavi_solve_report *report);
  * order.  No reference counterpart: the reference solves one model per call. */
 int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,
                                  wavi_solve_report *reports, int *codes);
+/* update_state! (update_state.jl:7-21) on each of n contexts, like
+ * wavi_update_velocities_batch: the subdomain solves of the restricted
+ * additive Schwarz preconditioner (src/SharedMemorySpec.jl:96-133 runs
+ * update_state! on every subdomain model, one task each). */
+int wavi_update_state_batch(wavi_ctx **ctxs, int n, int nthreads, wavi_solve_report *reports, int *codes);
 
 /* inner_update!(model) (update_velocities.jl:32-45). */
 int wavi_inner_update(wavi_ctx *ctx);
@@ -187,6 +192,9 @@ int wavi_update_state(wavi_ctx *ctx, wavi_solve_report *report);
  * on u/v, height above floatation, grounded fractions, accumulation, melt,
  * weertman_c, dsdh (update_state.jl:8-15). */
 int wavi_update_geometry(wavi_ctx *ctx);
+/* The update_state! steps after the solve: velocities on the h grid, dh/dt
+ * (update_state.jl:16-20, 164-189) and update_wavelets! (:21). */
+int wavi_after_solve(wavi_ctx *ctx);
 /* update_thickness!(simulation) (src/Simulations/run_simulation.jl:24-42). */
 int wavi_update_thickness(wavi_ctx *ctx, double dt);
 /* nsteps of timestep!: update_state! then update_thickness! (run_simulation.jl:6-17). */

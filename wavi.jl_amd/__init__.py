@@ -5,7 +5,7 @@ InitialConditions, Model, Simulation, update_velocities!, update_state!,
 timestep!, run_simulation!) over libwavi_hip.so, whose HIP kernels for gfx950
 do all of the solve's arithmetic.  Julia names ending in "!" drop the "!".
 """
-from .core import (ArgumentError, DimensionMismatch, Grid, InitialConditions, Params,
+from .core import (ArgumentError, BoundsError, DimensionMismatch, Grid, InitialConditions, Params,
                    SolverParams, TimesteppingParams, UniformMeltRate, clip, get_c_mask,
                    get_u_mask, get_v_mask, glen_b, height_above_floatation, mismip_plus_bed,
                    orientations2bc, volume_above_floatation)
@@ -14,6 +14,8 @@ from .model import (BasicParallelSpec, Fields, HIPSpec, Model, get_op_diag, get_
                     push_fields, update_geometry, update_preconditioner, update_state,
                     update_velocities, update_velocities_batch, update_wavelets, vcycle)
 from .simulation import Clock, Simulation, run_simulation, timestep, update_thickness
+from .schwarz import (SharedMemorySpec, schwarz_model, schwarz_partition_of_unity, schwarz_prolong_velocities,
+                      schwarz_restrict_velocities)
 from . import _lib, workloads
 
 __all__ = [
@@ -25,5 +27,6 @@ __all__ = [
     "update_thickness", "op_apply", "get_rhs", "get_op_diag", "vcycle", "haar",
     "push_fields", "pull_fields", "glen_b", "mismip_plus_bed", "orientations2bc",
     "get_u_mask", "get_v_mask", "get_c_mask", "clip", "height_above_floatation",
-    "volume_above_floatation", "workloads",
+    "volume_above_floatation", "workloads", "BoundsError", "SharedMemorySpec", "schwarz_model",
+    "schwarz_partition_of_unity", "schwarz_prolong_velocities", "schwarz_restrict_velocities",
 ]

@@ -127,6 +127,9 @@ SIGNATURES = {
                                                   ctypes.POINTER(_dp), ctypes.POINTER(_dp), _dp,
                                                   ctypes.POINTER(_dp), ctypes.POINTER(AdjointReport),
                                                   ctypes.POINTER(ctypes.c_int)]),
+    "wavi_update_state_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(SolveReport), ctypes.POINTER(ctypes.c_int)]),
+    "wavi_after_solve": (ctypes.c_int, [ctypes.c_void_p]),
     "wavi_update_velocities_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
                                                     ctypes.POINTER(SolveReport),
                                                     ctypes.POINTER(ctypes.c_int)]),

@@ -18,6 +18,10 @@ class DimensionMismatch(ValueError):
     """Raised where WAVI.jl throws DimensionMismatch."""
 
 
+class BoundsError(IndexError):
+    """Raised where WAVI.jl throws BoundsError."""
+
+
 def _bool2d(a, shape, name):
     try:
         a = np.asarray(a)

@@ -1638,10 +1638,8 @@ static int do_update_wavelets(wavi_ctx *ctx)
     return WAVI_OK;
 }
 
-static int do_update_state(wavi_ctx *ctx, wavi_solve_report *rep)
+static int do_after_solve(wavi_ctx *ctx)
 {
-    CHK(do_update_geometry(ctx));
-    CHK(do_update_velocities(ctx, rep));
     hipLaunchKernelGGL(k_post_solve, flat(ctx->nh), dim3(256), 0, ctx->stream, ctx->g, ctx->hm, ctx->um, ctx->vm,
                        ctx->f[WAVI_U_U], ctx->f[WAVI_V_V], ctx->f[WAVI_H_BETA], ctx->f[WAVI_H_QUAD_F1],
                        ctx->f[WAVI_H_QUAD_F2], ctx->f[WAVI_U_H], ctx->f[WAVI_V_H], ctx->f[WAVI_H_ACCUMULATION],
@@ -1651,6 +1649,13 @@ static int do_update_state(wavi_ctx *ctx, wavi_solve_report *rep)
     HIPCHK(hipGetLastError());
     CHK(do_update_wavelets(ctx));
     return WAVI_OK;
+}
+
+static int do_update_state(wavi_ctx *ctx, wavi_solve_report *rep)
+{
+    CHK(do_update_geometry(ctx));
+    CHK(do_update_velocities(ctx, rep));
+    return do_after_solve(ctx);
 }
 
 static int do_update_thickness(wavi_ctx *ctx, double dt)
@@ -1702,6 +1707,16 @@ int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,
     });
 }
 
+int wavi_update_state_batch(wavi_ctx **ctxs, int n, int nthreads, wavi_solve_report *reports, int *codes)
+{
+    if (n < 0 || nthreads < 1 || (n > 0 && !ctxs)) return WAVI_EARG;
+    for (int k = 0; k < n; ++k)
+        if (!ctxs[k]) return WAVI_EARG;
+    return run_members(ctxs, n, nthreads, codes, [&](wavi_ctx *ctx, int k) {
+        return wavi_update_state(ctx, reports ? &reports[k] : nullptr);
+    });
+}
+
 int wavi_misfit_gradient_batch(wavi_ctx **ctxs, int n, int nthreads, int n_picard, double tol_linear,
                                int maxit_linear, const double *const *u_obs, const double *const *v_obs, double *J,
                                double *const *grad_logc, wavi_adjoint_report *reports, int *codes)
@@ -1735,6 +1750,14 @@ int wavi_update_state(wavi_ctx *ctx, wavi_solve_report *report)
 {
     if (!ctx) return WAVI_EARG;
     CHK(do_update_state(ctx, report));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAVI_OK;
+}
+
+int wavi_after_solve(wavi_ctx *ctx)
+{
+    if (!ctx) return WAVI_EARG;
+    CHK(do_after_solve(ctx));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return WAVI_OK;
 }

@@ -360,10 +360,18 @@ def _report_solve(model, rep, verbose):
               % (rep.rel_resid, rep.picard_iters))
 
 
+def _schwarz(model):
+    return getattr(model.parallel_spec, "schwarz_model_array", None) is not None
+
+
 def update_velocities(model, verbose=False):
     """update_velocities!(model) (src/Models/update_velocities.jl:11-29) on
     the GPU: Picard iteration of inner_update! and one V-cycle per
-    iteration."""
+    iteration (with a SharedMemorySpec, the restricted additive Schwarz
+    preconditioner instead of the V-cycle)."""
+    if _schwarz(model):
+        from .schwarz import update_velocities_schwarz
+        return update_velocities_schwarz(model, verbose)
     update_preconditioner(model)
     spec = model.parallel_spec
     push_fields(model)
@@ -495,6 +503,9 @@ def inner_update(model):
 
 def update_state(model, clock=None, verbose=False):
     """update_state!(model[, clock]) (src/Models/update_state.jl:7-42)."""
+    if _schwarz(model):
+        from .schwarz import update_state_schwarz
+        return update_state_schwarz(model, verbose)
     update_preconditioner(model)
     spec = model.parallel_spec
     push_fields(model)

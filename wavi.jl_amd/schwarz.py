@@ -1,0 +1,266 @@
+"""Restricted additive Schwarz (RAS) velocity solve: SharedMemorySpec
+(src/SharedMemorySpec.jl).
+
+The full-domain model keeps one device context (SharedMemorySpec is a
+HIPSpec) for its inner update, right-hand side, residual, geometry and the
+steps after the solve.  Every subdomain is a Model of its own with its own
+device context; update_state! of all subdomains runs in one batched call
+(wavi_update_state_batch: host threads, one stream per subdomain), as the
+reference spawns one task per subdomain.  Restriction, damping and the
+partition-of-unity prolongation are host array operations, as in the
+reference.  Subdomain contexts persist across solves while the subdomain
+grid is unchanged.
+"""
+import copy
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from .core import ArgumentError, BoundsError, Grid, InitialConditions
+from .model import HIPSpec, Model, _check, _report_solve, op_apply, pull_fields, push_fields, update_preconditioner
+from .model import get_rhs as _get_rhs
+
+
+class SharedMemorySpec(HIPSpec):
+    """SharedMemorySpec(; ngridsx, ngridsy, overlap, niterations, damping)
+    (SharedMemorySpec.jl:6-13).  precision applies to the full domain's and
+    the subdomains' device solves; nthreads host threads drive the
+    subdomains."""
+
+    def __init__(self, ngridsx=1, ngridsy=1, overlap=1, niterations=0, damping=0.0, precision="fp64",
+                 nthreads=8):
+        super().__init__(precision)
+        self.ngridsx, self.ngridsy = int(ngridsx), int(ngridsy)
+        self.overlap, self.niterations = int(overlap), int(niterations)
+        self.damping = float(damping)
+        self.nthreads = int(nthreads)
+        self.schwarz_model_array = [[None] * self.ngridsy for _ in range(self.ngridsx)]
+
+    def close(self):
+        for row in getattr(self, "schwarz_model_array", []):
+            for mg in row:
+                if mg is not None:
+                    mg.parallel_spec.close()
+        super().close()
+
+
+def _bounds(model, igrid, jgrid, ngridsx, ngridsy, overlap):
+    """Index ranges (0-based, stop exclusive) of subdomain (igrid, jgrid)
+    (1-based, as in the reference) with its overlap (SharedMemorySpec.jl:183-199)."""
+    nx, ny = model.grid.nx, model.grid.ny
+    if nx % ngridsx != 0:
+        raise ArgumentError("Model domain is not an integer number of subdomains in x-direction")
+    if ny % ngridsy != 0:
+        raise ArgumentError("Model domain is not an integer number of subdomains in y-direction")
+    if not 1 <= igrid <= ngridsx:
+        raise BoundsError("igrid %d out of 1..%d" % (igrid, ngridsx))
+    if not 1 <= jgrid <= ngridsy:
+        raise BoundsError("jgrid %d out of 1..%d" % (jgrid, ngridsy))
+    nxd, nyd = nx // ngridsx, ny // ngridsy
+    i0 = max((igrid - 1) * nxd + 1 - overlap, 1)
+    i1 = min(igrid * nxd + overlap, nx)
+    j0 = max((jgrid - 1) * nyd + 1 - overlap, 1)
+    j1 = min(jgrid * nyd + overlap, ny)
+    return i0 - 1, i1, j0 - 1, j1
+
+
+def schwarz_model(model, igrid=1, jgrid=1, ngridsx=1, ngridsy=1, overlap=1, parallel_spec=None):
+    """schwarzModel (SharedMemorySpec.jl:181-287): the subdomain model, its
+    halo velocities fixed, its fields taken from the full model."""
+    g = model.grid
+    f = model.fields
+    i0, i1, j0, j1 = _bounds(model, igrid, jgrid, ngridsx, ngridsy, overlap)
+    nxg, nyg = i1 - i0, j1 - j0
+    u_isfixed = g.u_isfixed[i0:i1 + 1, j0:j1].copy()
+    v_isfixed = g.v_isfixed[i0:i1, j0:j1 + 1].copy()
+    # halo points become fixed-velocity points
+    if igrid != 1:
+        u_isfixed[0, :] = True
+        v_isfixed[0, :] = True
+    if igrid != ngridsx:
+        u_isfixed[nxg, :] = True
+        v_isfixed[nxg - 1, :] = True
+    if jgrid != 1:
+        v_isfixed[:, 0] = True
+        u_isfixed[:, 0] = True
+    if jgrid != ngridsy:
+        v_isfixed[:, nyg] = True
+        u_isfixed[:, nyg - 1] = True
+    grid_g = Grid(nx=nxg, ny=nyg, dx=g.dx, dy=g.dy, nσ=g.nσ, x0=g.x0 + i0 * g.dx, y0=g.y0 + j0 * g.dy,
+                  h_mask=g.h_mask[i0:i1, j0:j1], h_isfixed=g.h_isfixed[i0:i1, j0:j1],
+                  u_iszero=g.u_iszero[i0:i1 + 1, j0:j1], v_iszero=g.v_iszero[i0:i1, j0:j1 + 1],
+                  u_isfixed=u_isfixed, v_isfixed=v_isfixed, quadrature_weights=g.quadrature_weights, σ=g.σ)
+    p = model.params
+    params_g = copy.copy(p)
+    params_g.weertman_c = np.asfortranarray(p.weertman_c[i0:i1, j0:j1])
+    params_g.accumulation_rate = np.asfortranarray(p.accumulation_rate[i0:i1, j0:j1])
+    params_g.glen_a_ref = np.asfortranarray(np.broadcast_to(p.glen_a_ref, (g.nx, g.ny))[i0:i1, j0:j1])
+    ic = InitialConditions(
+        initial_thickness=f.gh.h[i0:i1, j0:j1],
+        initial_grounded_fraction=f.gh.grounded_fraction[i0:i1, j0:j1],
+        initial_u_veloc=f.gu.u[i0:i1 + 1, j0:j1],
+        initial_v_veloc=f.gv.v[i0:i1, j0:j1 + 1],
+        initial_viscosity=f.g3d.η[i0:i1, j0:j1, :],
+        initial_temperature=f.g3d.θ[i0:i1, j0:j1, :],
+        initial_damage=f.g3d.Φ[i0:i1, j0:j1, :])
+    return Model(grid=grid_g, bed_elevation=f.gh.b[i0:i1, j0:j1], params=params_g,
+                 solver_params=model.solver_params, initial_conditions=ic, melt_rate=model.melt_rate,
+                 parallel_spec=parallel_spec if parallel_spec is not None else HIPSpec(model.parallel_spec.precision))
+
+
+def schwarz_restrict_velocities(model_g, model, igrid=1, jgrid=1, ngridsx=1, ngridsy=1, overlap=1):
+    """schwarzRestrictVelocities! (SharedMemorySpec.jl:306-331)."""
+    i0, i1, j0, j1 = _bounds(model, igrid, jgrid, ngridsx, ngridsy, overlap)
+    model_g.fields.gu.u[:, :] = model.fields.gu.u[i0:i1 + 1, j0:j1]
+    model_g.fields.gv.v[:, :] = model.fields.gv.v[i0:i1, j0:j1 + 1]
+    return model_g
+
+
+def schwarz_partition_of_unity(m, n, leavei1, leaveim, leavej1, leavejn, overlapi, overlapj):
+    """schwarzPartitionOfUnity (SharedMemorySpec.jl:395-407): an m x n array
+    ramping from 1 inside to 0 at the edges not marked to be left at 1."""
+    if not m > ((not leavei1) and overlapi) + ((not leaveim) and overlapi):
+        raise ArgumentError("subdomain too small for the overlap in i")
+    if not n > ((not leavej1) and overlapj) + ((not leavejn) and overlapj):
+        raise ArgumentError("subdomain too small for the overlap in j")
+    i = np.arange(1, m + 1, dtype=float)[:, None]
+    j = np.arange(1, n + 1, dtype=float)[None, :]
+    inf = math.inf
+    with np.errstate(divide="ignore", invalid="ignore"):
+        pi = np.minimum(1.0, np.minimum(inf if leavei1 else (i - 1) / overlapi, inf if leaveim else (m - i) / overlapi))
+        pj = np.minimum(1.0, np.minimum(inf if leavej1 else (j - 1) / overlapj, inf if leavejn else (n - j) / overlapj))
+    return np.asfortranarray(pi * pj * np.ones((m, n)))
+
+
+def schwarz_prolong_velocities(model, model_g, igrid=1, jgrid=1, ngridsx=1, ngridsy=1, overlap=1, damping=0.0):
+    """schwarzProlongVelocities! (SharedMemorySpec.jl:351-382)."""
+    i0, i1, j0, j1 = _bounds(model, igrid, jgrid, ngridsx, ngridsy, overlap)
+    nxg, nyg = i1 - i0, j1 - j0
+    upou = schwarz_partition_of_unity(nxg + 1, nyg, igrid == 1, igrid == ngridsx, jgrid == 1, jgrid == ngridsy,
+                                      2 * overlap, 2 * overlap - 1)
+    vpou = schwarz_partition_of_unity(nxg, nyg + 1, igrid == 1, igrid == ngridsx, jgrid == 1, jgrid == ngridsy,
+                                      2 * overlap - 1, 2 * overlap)
+    model.fields.gu.u[i0:i1 + 1, j0:j1] += (1.0 - damping) * upou * model_g.fields.gu.u
+    model.fields.gv.v[i0:i1, j0:j1 + 1] += (1.0 - damping) * vpou * model_g.fields.gv.v
+    return model
+
+
+def update_preconditioner_schwarz(model):
+    """update_preconditioner!(model, ::SharedMemorySpec) (SharedMemorySpec.jl:51-68):
+    fresh subdomain models from the full model's fields, on persistent device
+    contexts."""
+    spec = model.parallel_spec
+    update_preconditioner(model)  # the full domain's context
+    for ig in range(spec.ngridsx):
+        for jg in range(spec.ngridsy):
+            old = spec.schwarz_model_array[ig][jg]
+            keep = None
+            if old is not None:
+                keep = old.parallel_spec
+            mg = schwarz_model(model, ig + 1, jg + 1, spec.ngridsx, spec.ngridsy, spec.overlap,
+                               parallel_spec=HIPSpec(spec.precision) if keep is None else keep)
+            if old is not None and not _same_grid(old.grid, mg.grid):
+                keep.close()
+                mg.parallel_spec = HIPSpec(spec.precision)
+            spec.schwarz_model_array[ig][jg] = mg
+            update_preconditioner(mg)
+            push_fields(mg)
+    return model
+
+
+def _same_grid(a, b):
+    return (a.nx == b.nx and a.ny == b.ny and a.nσ == b.nσ and a.dx == b.dx and a.dy == b.dy and
+            all(np.array_equal(getattr(a, k), getattr(b, k))
+                for k in ("h_mask", "h_isfixed", "u_iszero", "v_iszero", "u_isfixed", "v_isfixed",
+                          "quadrature_weights", "σ")))
+
+
+def _inner_vectors(model, au, av):
+    f = model.fields
+    return np.concatenate([au[f.gu.mask_inner], av[f.gv.mask_inner]])
+
+
+def precondition_schwarz(model):
+    """precondition!(model, ::SharedMemorySpec) (SharedMemorySpec.jl:76-153):
+    the residual of the start guess, then niterations of restrict, subdomain
+    update_state!, damping and prolongation.  Returns (converged, rel_resid)."""
+    spec = model.parallel_spec
+    f = model.fields
+    xu = np.where(f.gu.mask_inner, f.gu.u, 0.0)
+    xv = np.where(f.gv.mask_inner, f.gv.v, 0.0)
+    bu, bv = _get_rhs(model)
+    au, av = op_apply(model, xu, xv, sampled=True)
+    ru, rv = bu - au, bv - av
+    # set_residial! (update_velocities.jl:428-433)
+    f.gu.residual[f.gu.mask_inner] = ru[f.gu.mask_inner]
+    f.gv.residual[f.gv.mask_inner] = rv[f.gv.mask_inner]
+    rel = float(np.linalg.norm(_inner_vectors(model, ru, rv)) / np.linalg.norm(_inner_vectors(model, bu, bv)))
+    converged = rel < model.solver_params.tol_picard
+    if not converged:
+        members = [(ig, jg, spec.schwarz_model_array[ig][jg])
+                   for ig in range(spec.ngridsx) for jg in range(spec.ngridsy)]
+        lib = spec._lib
+        for _ in range(spec.niterations):
+            for ig, jg, mg in members:
+                schwarz_restrict_velocities(mg, model, ig + 1, jg + 1, spec.ngridsx, spec.ngridsy, spec.overlap)
+                for fid, arr in (("U_U", mg.fields.gu.u), ("V_V", mg.fields.gv.v)):
+                    buf = np.ascontiguousarray(arr.ravel(order="F"))
+                    _check(lib, mg.parallel_spec.ctx,
+                           lib.wavi_set_field(mg.parallel_spec.ctx, _lib.FIELD[fid], _lib.dptr(buf), buf.size))
+            n = len(members)
+            ctxs = (ctypes.c_void_p * n)(*[mg.parallel_spec.ctx for _, _, mg in members])
+            codes = (ctypes.c_int * n)()
+            rc = lib.wavi_update_state_batch(ctxs, n, spec.nthreads, None, codes)
+            if rc != 0:
+                for (_, _, mg), code in zip(members, codes):
+                    _check(lib, mg.parallel_spec.ctx, code)
+                _check(lib, None, rc)
+            for _, _, mg in members:
+                for fid, arr in (("U_U", mg.fields.gu.u), ("V_V", mg.fields.gv.v)):
+                    buf = np.empty(arr.size)
+                    _check(lib, mg.parallel_spec.ctx,
+                           lib.wavi_get_field(mg.parallel_spec.ctx, _lib.FIELD[fid], _lib.dptr(buf), buf.size))
+                    arr[:, :] = buf.reshape(arr.shape, order="F")
+            f.gu.u[:, :] = spec.damping * f.gu.u
+            f.gv.v[:, :] = spec.damping * f.gv.v
+            for ig, jg, mg in members:
+                schwarz_prolong_velocities(model, mg, ig + 1, jg + 1, spec.ngridsx, spec.ngridsy, spec.overlap,
+                                           spec.damping)
+    return converged, rel
+
+
+def update_velocities_schwarz(model, verbose=False):
+    """update_velocities! (update_velocities.jl:11-29) with the RAS
+    preconditioner: the Picard loop of inner_update! (on the full domain's
+    device context) and precondition!(model, ::SharedMemorySpec)."""
+    update_preconditioner_schwarz(model)
+    spec = model.parallel_spec
+    converged, it, rel = False, 0, math.inf
+    while not converged and it < model.solver_params.maxiter_picard:
+        it += 1
+        push_fields(model)
+        _check(spec._lib, spec.ctx, spec._lib.wavi_inner_update(spec.ctx))
+        pull_fields(model)
+        converged, rel = precondition_schwarz(model)
+    rep = _lib.SolveReport()
+    rep.picard_iters, rep.converged, rep.rel_resid = it, int(converged), rel
+    _report_solve(model, rep, verbose)
+    return model
+
+
+def update_state_schwarz(model, verbose=False):
+    """update_state! (update_state.jl:7-21) of a model whose parallel_spec is
+    a SharedMemorySpec: geometry and the steps after the solve on the full
+    domain's context, the velocity solve by RAS."""
+    update_preconditioner(model)
+    spec = model.parallel_spec
+    push_fields(model)
+    _check(spec._lib, spec.ctx, spec._lib.wavi_update_geometry(spec.ctx))
+    pull_fields(model)
+    update_velocities_schwarz(model, verbose)
+    push_fields(model)
+    _check(spec._lib, spec.ctx, spec._lib.wavi_after_solve(spec.ctx))
+    pull_fields(model)
+    return model

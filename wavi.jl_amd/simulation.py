@@ -10,7 +10,7 @@ from dataclasses import dataclass
 
 from . import _lib
 from .core import ArgumentError
-from .model import _check, _report_solve, pull_fields, push_fields, update_preconditioner
+from .model import _check, _report_solve, _schwarz, pull_fields, push_fields, update_preconditioner, update_state
 
 
 @dataclass
@@ -46,6 +46,14 @@ def update_clock(simulation):
 
 def _run_steps(simulation, nsteps, verbose=False):
     model = simulation.model
+    if _schwarz(model):
+        # restricted additive Schwarz: the steps are driven from the host
+        for _ in range(nsteps):
+            update_state(model, verbose=verbose)
+            if simulation.timestepping_params.step_thickness:
+                update_thickness(simulation)
+            update_clock(simulation)
+        return simulation
     update_preconditioner(model)
     spec = model.parallel_spec
     push_fields(model)
