@@ -55,3 +55,13 @@ def test_schwarz_model_halos_and_bounds(W):
         W.schwarz_model(m, 17, 1, ngridsx=16, ngridsy=2)
     with pytest.raises(W.ArgumentError):
         W.schwarz_model(m, 1, 1, ngridsx=3, ngridsy=2)
+
+
+def test_timestepping_checkpoint_steps_and_output_params(W, tmp_path):
+    tp = W.TimesteppingParams(dt=0.5, n_iter_total=40, chkpt_freq=2.0, pchkpt_freq=10.0, chkpt_path=str(tmp_path))
+    assert (tp.n_iter_chkpt, tp.n_iter_pchkpt) == (4, 20)
+    assert tp.chkpt_path.endswith("/")
+    with pytest.warns(UserWarning):
+        assert W.TimesteppingParams(dt=1.0, n_iter_total=2, chkpt_path=str(tmp_path / "missing")).chkpt_path == "./"
+    with pytest.raises(W.ArgumentError):
+        W.OutputParams(outputs={"h": None})

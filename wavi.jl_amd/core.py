@@ -214,20 +214,52 @@ def compute_iterations_and_end_time(end_time, n_iter_total, dt):
 
 
 class TimesteppingParams:
-    """TimesteppingParams(; niter0, dt, end_time, n_iter_total, ...)
-    (src/TimesteppingParams.jl:43-74); checkpointing is not part of this
-    package, so the checkpoint frequencies are accepted but unused."""
+    """TimesteppingParams(; niter0, dt, end_time, n_iter_total, chkpt_freq,
+    pchkpt_freq, chkpt_path, step_thickness) (src/TimesteppingParams.jl:43-74).
+    Checkpoints are written every n_iter_chkpt steps (alternating files A and
+    B) and every n_iter_pchkpt steps (permanent) by run_simulation."""
 
     def __init__(self, niter0=0, dt=1.0, end_time=None, n_iter_total=None, chkpt_freq=math.inf,
                  pchkpt_freq=math.inf, chkpt_path="./", step_thickness=True):
+        import os
+        import warnings
         self.niter0 = niter0
         self.dt = float(dt)
         self.end_time, self.n_iter_total = compute_iterations_and_end_time(end_time, n_iter_total, dt)
         self.t0 = niter0 * dt if niter0 > 0 else 0.0
         self.chkpt_freq = chkpt_freq
         self.pchkpt_freq = pchkpt_freq
+        self.n_iter_chkpt = math.inf if chkpt_freq == math.inf else int(round(chkpt_freq / dt))
+        self.n_iter_pchkpt = math.inf if pchkpt_freq == math.inf else int(round(pchkpt_freq / dt))
+        if not chkpt_path.endswith("/"):
+            chkpt_path = chkpt_path + "/"
+        if not os.path.isdir(chkpt_path):
+            warnings.warn("Did not find output path %s. Any outputs will go to the working directory %s"
+                          % (chkpt_path, os.getcwd()))
+            chkpt_path = "./"
         self.chkpt_path = chkpt_path
         self.step_thickness = step_thickness
+
+
+class OutputParams:
+    """OutputParams(; outputs, output_freq, output_format, prefix, output_path,
+    dump_vel, zip_format, output_start) (src/OutputParams/OutputParams.jl:2-64).
+    Of the outputs, this package writes the velocity dump (dump_vel) only;
+    field outputs (a non-empty outputs) are rejected."""
+
+    def __init__(self, outputs=(), output_freq=math.inf, output_format="jld2", prefix="outfile",
+                 output_path="./", dump_vel=False, zip_format="none", output_start=False):
+        if len(outputs) > 0:
+            raise ArgumentError("field outputs are not supported; only dump_vel")
+        self.outputs = outputs
+        self.output_freq = output_freq
+        self.output_format = output_format
+        self.prefix = prefix
+        self.output_path = output_path
+        self.dump_vel = bool(dump_vel)
+        self.zip_format = zip_format
+        self.output_start = output_start
+        self.n_iter_out = math.inf
 
 
 def glen_b(temperature, damage, glen_a_ref, glen_n, glen_a_activation_energy,

@@ -1,15 +1,24 @@
 """Simulation driver: Simulation, timestep!, update_thickness!, run_simulation!.
 
-Mirrors src/Simulations/Simulation.jl:24-59 and run_simulation.jl:6-100.
-The whole time loop runs on the device (wavi_run_steps): fields are copied to
-the device once before the loop and back once after it.  Checkpoint, output
-and pickup files are not part of this package.
+Mirrors src/Simulations/Simulation.jl:24-86 and run_simulation.jl:6-125.
+The time loop runs on the device (wavi_run_steps) between the steps at
+which run_simulation writes something: fields travel to the device once per
+stretch and back at its end.  Checkpoints (temporary A/B and permanent,
+with pickup from a permanent one) hold every field of the model and the
+clock in numpy .npz files under the reference's names (ChkptA, ChkptB,
+PChkpt_<10-digit step>), since no JLD2 writer is available here; the
+velocity dump at the final step (dump_vel) is the reference's big-endian
+<prefix>_U.bin / <prefix>_V.bin.
 """
 import ctypes
+import math
+import os
 from dataclasses import dataclass
 
+import numpy as np
+
 from . import _lib
-from .core import ArgumentError
+from .core import ArgumentError, OutputParams
 from .model import _check, _report_solve, _schwarz, pull_fields, push_fields, update_preconditioner, update_state
 
 
@@ -27,14 +36,62 @@ class Simulation:
                  pickup_output_update_flag=False):
         if timestepping_params is None:
             raise ArgumentError("You must specify a timestepping parameters")
-        if timestepping_params.niter0 > 0:
-            raise ArgumentError("pickup from checkpoints is not supported")
         self.model = model
         self.timestepping_params = timestepping_params
-        self.output_params = output_params
+        self.output_params = OutputParams() if output_params is None else output_params
         self.clock = Clock(0, 0.0)
         # set_dt_in_model! (Simulation.jl:56-59)
         model.params.dt = timestepping_params.dt
+        pickup(self, pickup_output_update_flag)
+
+
+CHECKPOINT_EXT = ".npz"
+
+
+def save_checkpoint(simulation, fname):
+    """Every field of the model (the device state of a solve) and the clock."""
+    from .model import FIELD_MAP
+    m = simulation.model
+    arrays = {fid: np.asarray(m._array(c, a), dtype=np.float64) for c, a, fid in FIELD_MAP}
+    arrays["clock"] = np.array([simulation.clock.n_iter, simulation.clock.time], dtype=np.float64)
+    arrays["dt"] = np.array([m.params.dt])
+    np.savez(fname, **arrays)
+
+
+def load_checkpoint(simulation, fname):
+    """Restore the fields and the clock written by save_checkpoint into the
+    simulation's model (built with the same grid and parameters)."""
+    from .model import FIELD_MAP
+    m = simulation.model
+    with np.load(fname, allow_pickle=False) as d:
+        for c, a, fid in FIELD_MAP:
+            m._set_array(c, a, d[fid])
+        n_iter, time = d["clock"]
+    simulation.clock = Clock(int(n_iter), float(time))
+    return simulation
+
+
+def pickup(simulation, pickup_output_update_flag=False):
+    """pickup! (Simulation.jl:68-86): with niter0 > 0, resume from
+    PChkpt_<niter0, 10 digits> in the working directory."""
+    niter0 = simulation.timestepping_params.niter0
+    if niter0 > 0:
+        fname = "PChkpt_%010d%s" % (niter0, CHECKPOINT_EXT)
+        print("detected niter0 > 0 (niter0 = %d). Looking for pickup..." % niter0)
+        if not os.path.exists(fname):
+            raise RuntimeError("Pickup error, terminating run: %s not found" % fname)
+        load_checkpoint(simulation, fname)
+        print("Pickup successful")
+    return simulation
+
+
+def write_vel(simulation):
+    """write_vel (run_simulation.jl:107-125): u without its last x row and v
+    without its last y column, big-endian Float64, column-major."""
+    m = simulation.model
+    prefix = simulation.output_params.prefix
+    np.asfortranarray(m.fields.gu.u[:-1, :]).astype(">f8").ravel(order="F").tofile(prefix + "_U.bin")
+    np.asfortranarray(m.fields.gv.v[:, :-1]).astype(">f8").ravel(order="F").tofile(prefix + "_V.bin")
 
 
 def update_clock(simulation):
@@ -87,9 +144,29 @@ def update_thickness(simulation):
 
 
 def run_simulation(simulation, verbose=False):
-    """run_simulation!(simulation) (run_simulation.jl:61-100) without
-    checkpoints or outputs."""
-    n = simulation.timestepping_params.n_iter_total - simulation.clock.n_iter
-    if n > 0:
-        _run_steps(simulation, n, verbose)
+    """run_simulation!(simulation) (run_simulation.jl:61-100): the steps run
+    on the device in stretches that end at the checkpoint steps and the final
+    step, where the checkpoints and the velocity dump are written."""
+    tp = simulation.timestepping_params
+    events = [tp.n_iter_chkpt, tp.n_iter_pchkpt]
+    chkpt_tag = "A"
+    i = simulation.clock.n_iter
+    while i < tp.n_iter_total:
+        # the next step at which something is written (or the last step)
+        nxt = tp.n_iter_total
+        for e in events:
+            if e != math.inf:
+                nxt = min(nxt, (i // e + 1) * e)
+        _run_steps(simulation, nxt - i, verbose)
+        i = nxt
+        if tp.n_iter_chkpt != math.inf and i % tp.n_iter_chkpt == 0:
+            save_checkpoint(simulation, "%sChkpt%s%s" % (tp.chkpt_path, chkpt_tag, CHECKPOINT_EXT))
+            chkpt_tag = "B" if chkpt_tag == "A" else "A"
+            print("making temporary checkpoint at timestep number %d" % simulation.clock.n_iter)
+        if tp.n_iter_pchkpt != math.inf and i % tp.n_iter_pchkpt == 0:
+            save_checkpoint(simulation, "%sPChkpt_%010d%s" % (tp.chkpt_path, simulation.clock.n_iter,
+                                                              CHECKPOINT_EXT))
+            print("making permanent checkpoint at timestep number %d" % simulation.clock.n_iter)
+        if i == tp.n_iter_total and simulation.output_params.dump_vel:
+            write_vel(simulation)
     return simulation
