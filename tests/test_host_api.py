@@ -147,3 +147,22 @@ def test_bench_rejects_multi_gpu():
     import sys
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stdout
+
+
+def test_batch_and_gradient_argument_checks_without_gpu(W):
+    """The batch and adjoint entry points reject invalid arguments before any
+    device call."""
+    if not os.path.exists(W._lib.LIB_PATH):
+        pytest.skip("libwavi_hip.so not built")
+    lib = W._lib.load()
+    EARG = W._lib.WAVI_EARG
+    null_ctxs = (ctypes.c_void_p * 2)(None, None)
+    assert lib.wavi_update_velocities_batch(None, -1, 1, None, None) == EARG
+    assert lib.wavi_update_velocities_batch(null_ctxs, 2, 0, None, None) == EARG
+    assert lib.wavi_update_velocities_batch(null_ctxs, 2, 4, None, None) == EARG  # NULL members
+    assert lib.wavi_update_velocities_batch(None, 0, 4, None, None) == W._lib.WAVI_OK  # empty batch
+    assert lib.wavi_update_state_batch(null_ctxs, 2, 4, None, None) == EARG
+    assert lib.wavi_misfit_gradient_batch(null_ctxs, 2, 4, 3, 1e-13, 60, None, None, None, None, None, None) == EARG
+    J = ctypes.c_double()
+    assert lib.wavi_misfit_gradient(None, 3, 1e-13, 60, None, None, ctypes.byref(J), None, None) == EARG
+    assert lib.wavi_after_solve(None) == EARG
