@@ -138,6 +138,12 @@ def test_create_errors_without_gpu(W):
     assert lib.wavi_ctx_create(ctypes.byref(desc), ctypes.byref(phys), ctypes.byref(solver), 7,
                                ctypes.byref(out)) == W._lib.WAVI_EARG
     assert lib.wavi_ctx_create(None, ctypes.byref(phys), ctypes.byref(solver), 0, ctypes.byref(out)) == W._lib.WAVI_EARG
+    # 16400 x 16400 padded to whole tiles exceeds 2^28 points per array; the
+    # check runs before any array is read
+    desc.nx, desc.ny = 16400, 16400
+    assert lib.wavi_ctx_create(ctypes.byref(desc), ctypes.byref(phys), ctypes.byref(solver), 0,
+                               ctypes.byref(out)) == W._lib.WAVI_EDIM
+    assert b"grid too large" in lib.wavi_last_error(None)
     with pytest.raises(W.ArgumentError):
         W.HIPSpec("bf16")
 

@@ -84,7 +84,6 @@ struct wavi_ctx {
     CGMHostFlag *d_flag = nullptr;  // its device address
     hipEvent_t cg_ev[3] = {};       // chunk completion events of the mixed coarse CG
     hipGraphExec_t cgm_chunk = nullptr;  // 32 k_cgm_iter launches (parities 1, 0, ...) as one graph
-    hipGraphExec_t cg_chunk = nullptr;   // 32 FP64 CG iterations (3 launches each) as one graph
     double *Pr0 = nullptr, *Pr1 = nullptr, *Pd = nullptr, *Pb = nullptr, *Pn = nullptr;
     double *quad = nullptr;  // 4 * nh pos_fraction quadrants
     double *norms = nullptr; // device: 2 doubles
@@ -417,6 +416,17 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
         g_create_error = "evolve_shelves = false is not supported";
         return WAVI_EARG;
     }
+    {
+        // The mixed coarse CG addresses its arena with 32-bit byte offsets
+        // and 24-bit row products: (nx + 1) and (ny + 1) padded to whole
+        // 16-point tiles must give fewer than 2^28 points per array.
+        const size_t tx = ((size_t)grid->nx + 1 + TILE - 1) / TILE, ty = ((size_t)grid->ny + 1 + TILE - 1) / TILE;
+        const size_t pitch = (TILE * tx + 2 + 31) / 32 * 32;
+        if (pitch * (TILE * ty + 2) >= ((size_t)1 << 28)) {
+            g_create_error = "grid too large: at most 2^28 padded points per array";
+            return WAVI_EDIM;
+        }
+    }
     wavi_ctx *ctx = new wavi_ctx();
     ctx->P = *phys;
     ctx->S = *solver;
@@ -642,7 +652,6 @@ int wavi_ctx_destroy(wavi_ctx *ctx)
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->burst_ev) (void)hipEventDestroy(e);
     if (ctx->cgm_chunk) (void)hipGraphExecDestroy(ctx->cgm_chunk);
-    if (ctx->cg_chunk) (void)hipGraphExecDestroy(ctx->cg_chunk);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return WAVI_OK;
@@ -1020,39 +1029,19 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
         // when profiling
         const bool sampled = ctx->prof && (k & 1) != 0;
         const int bev = sampled ? -1 : burst_begin(ctx);
-        if (n == CHUNK && !sampled && use_cg_graph()) {
-            // a full chunk starts at an even q (CHUNK is even): the graph's parities
-            if (!ctx->cg_chunk) {
-                std::lock_guard<std::mutex> lock(g_graph_mu);
-                hipGraph_t gr = nullptr;
-                HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
-                for (int s = 0; s < CHUNK; ++s) {
-                    hipLaunchKernelGGL(k_cg_dir<T>, tb, dim3(256), 0, ctx->stream, a, s & 1);
-                    hipLaunchKernelGGL(k_cg_apply<T>, tb, dim3(256), 0, ctx->stream, a);
-                    hipLaunchKernelGGL(k_cg_update<T>, tb, dim3(256), 0, ctx->stream, a, s & 1);
-                }
-                HIPCHK(hipStreamEndCapture(ctx->stream, &gr));
-                const hipError_t e = hipGraphInstantiate(&ctx->cg_chunk, gr, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(gr);
-                HIPCHK(e);
+        for (int s = 0; s < n; ++s, ++q) {
+            const int par = q & 1;
+            {
+                ProfScope ps(ctx, WAVI_KT_CG_DIR, sampled);
+                hipLaunchKernelGGL(k_cg_dir<T>, tb, dim3(256), 0, ctx->stream, a, par);
             }
-            HIPCHK(hipGraphLaunch(ctx->cg_chunk, ctx->stream));
-            q += CHUNK;
-        } else {
-            for (int s = 0; s < n; ++s, ++q) {
-                const int par = q & 1;
-                {
-                    ProfScope ps(ctx, WAVI_KT_CG_DIR, sampled);
-                    hipLaunchKernelGGL(k_cg_dir<T>, tb, dim3(256), 0, ctx->stream, a, par);
-                }
-                {
-                    ProfScope ps(ctx, WAVI_KT_CG_APPLY, sampled);
-                    hipLaunchKernelGGL(k_cg_apply<T>, tb, dim3(256), 0, ctx->stream, a);
-                }
-                {
-                    ProfScope ps(ctx, WAVI_KT_CG_UPDATE, sampled);
-                    hipLaunchKernelGGL(k_cg_update<T>, tb, dim3(256), 0, ctx->stream, a, par);
-                }
+            {
+                ProfScope ps(ctx, WAVI_KT_CG_APPLY, sampled);
+                hipLaunchKernelGGL(k_cg_apply<T>, tb, dim3(256), 0, ctx->stream, a);
+            }
+            {
+                ProfScope ps(ctx, WAVI_KT_CG_UPDATE, sampled);
+                hipLaunchKernelGGL(k_cg_update<T>, tb, dim3(256), 0, ctx->stream, a, par);
             }
         }
         burst_end(ctx, bev, q - 1, n);

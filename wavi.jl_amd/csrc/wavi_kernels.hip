@@ -2001,13 +2001,22 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     auto arr = [&](int A) { return F + (size_t)A * ns; };
     auto el = [](float *base, unsigned boff) -> float & { return *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + boff); };
     // window entry (x, y) is arena offset wb + x + px y
-    const unsigned wb = (unsigned)i0 + px * (unsigned)j0;
-    const unsigned o = wb + (unsigned)(lx + 1) + px * (unsigned)(ly + 1);   // own point
+    // (24-bit products: a 32-bit multiply-add is emitted as a 64-bit one whose
+    // unused high addend can be a register with a load in flight)
+    const unsigned wb = (unsigned)i0 + __umul24(px, (unsigned)j0);
+    const unsigned o = wb + (unsigned)(lx + 1) + __umul24(px, (unsigned)(ly + 1));   // own point
     const unsigned ob = 4u * o;
-    // ---- loads, all issued before the first use
+    // ---- loads, all issued before the first use, in the order they are
+    // needed: the wait for a load also waits for every load issued before
+    // it, so the partials come first and the coefficient windows last
     const CGMState S = a.st->s[PAR];
+    // partials of the previous launch
+    float2 pv[4];
+    if (!INIT) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) pv[e] = a.Pgd[PAR * a.npad + tid + 256 * e];
+    }
     const uint8_t mbu = M[o], mbv = (M + ns)[o];
-    const float bu0 = el(arr(CGM_BU), ob), bv0 = el(arr(CGM_BU + 1), ob);
     float r_u = el(arr(CGM_R), ob), r_v = el(arr(CGM_R + 1), ob);
     float w_u0 = 0.f, p_u0 = 0.f, s_u0 = 0.f, x_u0 = 0.f, w_v0 = 0.f, p_v0 = 0.f, s_v0 = 0.f, x_v0 = 0.f;
     if (!INIT) {
@@ -2044,24 +2053,19 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     }
     // thickness windows, entries q = tid and tid + 256 (< 324) of the 18 x 18 window
     const int q1 = tid + 256 < 324 ? tid + 256 : tid;
-    const unsigned t0 = 4u * (wb + (unsigned)(tid % 18) + px * (unsigned)(tid / 18));
-    const unsigned t1 = 4u * (wb + (unsigned)(q1 % 18) + px * (unsigned)(q1 / 18));
+    const unsigned t0 = 4u * (wb + (unsigned)(tid % 18) + __umul24(px, (unsigned)(tid / 18)));
+    const unsigned t1 = 4u * (wb + (unsigned)(q1 % 18) + __umul24(px, (unsigned)(q1 / 18)));
     const float thu0 = el(arr(CGM_HU), t0), thv0 = el(arr(CGM_HU + 1), t0);
     const float thu1 = el(arr(CGM_HU), t1), thv1 = el(arr(CGM_HU + 1), t1);
     // cell coefficients, cells q = tid and tid + 256 (< 289) of the 17 x 17 window
     const int c1q = tid + 256 < 289 ? tid + 256 : tid;
-    const unsigned c0o = wb + (unsigned)(tid % 17) + px * (unsigned)(tid / 17);
-    const unsigned c1o = wb + (unsigned)(c1q % 17) + px * (unsigned)(c1q / 17);
+    const unsigned c0o = wb + (unsigned)(tid % 17) + __umul24(px, (unsigned)(tid / 17));
+    const unsigned c1o = wb + (unsigned)(c1q % 17) + __umul24(px, (unsigned)(c1q / 17));
     const char *cell2 = reinterpret_cast<const char *>(arr(CGM_CELL));
     const float2 cc0 = *reinterpret_cast<const float2 *>(cell2 + 8u * c0o);
     const float2 cc1 = *reinterpret_cast<const float2 *>(cell2 + 8u * c1o);
     const float ch0 = el(arr(CGM_CORNER), 4u * c0o), ch1 = el(arr(CGM_CORNER), 4u * c1o);
-    // partials of the previous launch
-    float2 pv[4];
-    if (!INIT) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) pv[e] = a.Pgd[PAR * a.npad + tid + 256 * e];
-    }
+    const float bu0 = el(arr(CGM_BU), ob), bv0 = el(arr(CGM_BU + 1), ob);
     const bool inu = mbu & 1, inv = mbv & 1, mwu = mbu & 2, mwv = mbv & 2;
     // ---- scalars from the previous launch's partials (FP64 across workgroups)
     float fa = 0.f, fb = 0.f, fxp = 0.f;
@@ -2165,14 +2169,15 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
         else
             sm.V[hy][hx] = t;
     }
+    __syncthreads();
+    // ---- y pass in the column-major lanes: P r into the window, rings of P r
+    // and P s; thickness windows into LDS (read after the next barrier)
     sm.HU[tid / 18][tid % 18] = thu0;
     sm.HV[tid / 18][tid % 18] = thv0;
     if (tid + 256 < 324) {
         sm.HU[q1 / 18][q1 % 18] = thu1;
         sm.HV[q1 / 18][q1 % 18] = thv1;
     }
-    __syncthreads();
-    // ---- y pass in the column-major lanes: P r into the window, rings of P r and P s
     {
         const uint8_t m0 = sm.mb[0][cy][cx], m1 = sm.mb[1][cy][cx];
         float y0 = sm.T[0][cy][cx], y1 = sm.T[1][cy][cx];
