@@ -206,6 +206,8 @@ static void burst_collect(wavi_ctx *ctx, int kid, int last_active)
     } while (0)
 
 static inline dim3 flat(int n) { return dim3((unsigned)((n + 255) / 256)); }
+// grid of a u / v pair launch: blockIdx.y 0 for the u arrays, 1 for the v arrays
+static inline dim3 flat_uv(int nu, int nv) { return dim3((unsigned)(((nu > nv ? nu : nv) + 255) / 256), 2); }
 
 static int field_len(const wavi_ctx *ctx, int id)
 {
@@ -853,10 +855,9 @@ static int do_op(wavi_ctx *ctx, const double *xu, const double *xv, double *yu, 
 // get_rhs into ctx->b
 static int do_rhs(wavi_ctx *ctx)
 {
-    hipLaunchKernelGGL((k_select<double, double>), flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_U_U],
-                       ctx->ufix, ctx->fixedv.u, ctx->nu);
-    hipLaunchKernelGGL((k_select<double, double>), flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->f[WAVI_V_V],
-                       ctx->vfix, ctx->fixedv.v, ctx->nv);
+    hipLaunchKernelGGL((k_select_uv<double, double>), flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream,
+                       ctx->f[WAVI_U_U], ctx->ufix, ctx->fixedv.u, ctx->nu, ctx->f[WAVI_V_V], ctx->vfix, ctx->fixedv.v,
+                       ctx->nv);
     CHK(do_op(ctx, ctx->fixedv.u, ctx->fixedv.v, ctx->f3.u, ctx->f3.v));
     RhsArgs a;
     a.hm = ctx->hm;
@@ -1211,8 +1212,8 @@ static int do_vcycle_t(wavi_ctx *ctx, const VSet<T> &w, int *cg_iters)
     const GridDims &g = ctx->g;
     dim3 tg(g.nbx, g.nby);
     CHK(do_diag(ctx, w));
-    hipLaunchKernelGGL(k_fill<T>, flat(ctx->nu), dim3(256), 0, ctx->stream, w.e.u, T(0), ctx->nu);
-    hipLaunchKernelGGL(k_fill<T>, flat(ctx->nv), dim3(256), 0, ctx->stream, w.e.v, T(0), ctx->nv);
+    hipLaunchKernelGGL(k_fill_uv<T>, flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream, w.e.u, ctx->nu, w.e.v,
+                       ctx->nv, T(0));
     CHK(do_smoother(ctx, w, fwd));
     // rho = r - A e
     hipLaunchKernelGGL(k_tile_resid<T>, tg, dim3(256), 0, ctx->stream, g, coef_of(ctx, w), w.e.u, w.e.v, w.r.u,
@@ -1225,18 +1226,32 @@ static int do_vcycle_t(wavi_ctx *ctx, const VSet<T> &w, int *cg_iters)
     // e += P x_c
     hipLaunchKernelGGL((k_tile_haar<T, T>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1, w.xc.u,
                        w.xc.v, ctx->mwu, ctx->mwv, w.t.u, w.t.v, ctx->ui, ctx->vi);
-    hipLaunchKernelGGL(k_add_masked<T>, flat(ctx->nu), dim3(256), 0, ctx->stream, w.e.u, w.t.u, ctx->ui, ctx->nu);
-    hipLaunchKernelGGL(k_add_masked<T>, flat(ctx->nv), dim3(256), 0, ctx->stream, w.e.v, w.t.v, ctx->vi, ctx->nv);
+    hipLaunchKernelGGL(k_add_masked_uv<T>, flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream, w.e.u, w.t.u,
+                       ctx->ui, ctx->nu, w.e.v, w.t.v, ctx->vi, ctx->nv);
     CHK(do_smoother(ctx, w, rev));
     HIPCHK(hipGetLastError());
     return WAVI_OK;
 }
 
+// Conversions of several arrays, one launch.
 template <typename Tin, typename Tout>
-static void convert(wavi_ctx *ctx, const Tin *in, Tout *out, int n)
-{
-    hipLaunchKernelGGL((k_convert<Tin, Tout>), flat(n), dim3(256), 0, ctx->stream, in, out, n);
-}
+struct Conversions {
+    ConvList<Tin, Tout> L{};
+    int count = 0, nmax = 0;
+    void add(const Tin *in, Tout *out, int n)
+    {
+        L.in[count] = in;
+        L.out[count] = out;
+        L.n[count] = n;
+        ++count;
+        nmax = n > nmax ? n : nmax;
+    }
+    void launch(wavi_ctx *ctx) const
+    {
+        hipLaunchKernelGGL((k_convert_list<Tin, Tout>), dim3((unsigned)((nmax + 255) / 256), (unsigned)count),
+                           dim3(256), 0, ctx->stream, L);
+    }
+};
 
 // V-cycle on the FP64 Picard residual ctx->r; correction into ctx->e.  In
 // mixed precision the operator coefficients, the residual and the warm start
@@ -1246,23 +1261,27 @@ static int do_vcycle(wavi_ctx *ctx, int *cg_iters)
 {
     if (ctx->precision == WAVI_PRECISION_FP64) return do_vcycle_t<double>(ctx, vset64(ctx), cg_iters);
     VSet<float> &w = ctx->v32;
-    convert(ctx, ctx->heta_h, w.heta_h, ctx->nh);
-    convert(ctx, ctx->heta_c, w.heta_c, ctx->nc > 0 ? ctx->nc : 1);
-    convert(ctx, ctx->dimp, w.dimp, ctx->nh);
-    convert(ctx, ctx->f[WAVI_U_H], w.h.u, ctx->nu);
-    convert(ctx, ctx->f[WAVI_V_H], w.h.v, ctx->nv);
-    convert(ctx, ctx->f[WAVI_U_BETA_EFF], w.beff.u, ctx->nu);
-    convert(ctx, ctx->f[WAVI_V_BETA_EFF], w.beff.v, ctx->nv);
-    convert(ctx, ctx->r.u, w.r.u, ctx->nu);
-    convert(ctx, ctx->r.v, w.r.v, ctx->nv);
-    convert(ctx, ctx->f[WAVI_WU_CORRECTION_COARSE], w.xc.u, ctx->nu);
-    convert(ctx, ctx->f[WAVI_WV_CORRECTION_COARSE], w.xc.v, ctx->nv);
+    Conversions<double, float> down;
+    down.add(ctx->heta_h, w.heta_h, ctx->nh);
+    down.add(ctx->heta_c, w.heta_c, ctx->nc > 0 ? ctx->nc : 1);
+    down.add(ctx->dimp, w.dimp, ctx->nh);
+    down.add(ctx->f[WAVI_U_H], w.h.u, ctx->nu);
+    down.add(ctx->f[WAVI_V_H], w.h.v, ctx->nv);
+    down.add(ctx->f[WAVI_U_BETA_EFF], w.beff.u, ctx->nu);
+    down.add(ctx->f[WAVI_V_BETA_EFF], w.beff.v, ctx->nv);
+    down.add(ctx->r.u, w.r.u, ctx->nu);
+    down.add(ctx->r.v, w.r.v, ctx->nv);
+    down.add(ctx->f[WAVI_WU_CORRECTION_COARSE], w.xc.u, ctx->nu);
+    down.add(ctx->f[WAVI_WV_CORRECTION_COARSE], w.xc.v, ctx->nv);
+    down.launch(ctx);
     HIPCHK(hipGetLastError());
     CHK(do_vcycle_t<float>(ctx, w, cg_iters));
-    convert(ctx, w.e.u, ctx->e.u, ctx->nu);
-    convert(ctx, w.e.v, ctx->e.v, ctx->nv);
-    convert(ctx, w.xc.u, ctx->f[WAVI_WU_CORRECTION_COARSE], ctx->nu);
-    convert(ctx, w.xc.v, ctx->f[WAVI_WV_CORRECTION_COARSE], ctx->nv);
+    Conversions<float, double> up;
+    up.add(w.e.u, ctx->e.u, ctx->nu);
+    up.add(w.e.v, ctx->e.v, ctx->nv);
+    up.add(w.xc.u, ctx->f[WAVI_WU_CORRECTION_COARSE], ctx->nu);
+    up.add(w.xc.v, ctx->f[WAVI_WV_CORRECTION_COARSE], ctx->nv);
+    up.launch(ctx);
     HIPCHK(hipGetLastError());
     return WAVI_OK;
 }
@@ -1272,19 +1291,16 @@ static int do_precondition(wavi_ctx *ctx, int *converged, double *rel, int *cg_i
 {
     const GridDims &g = ctx->g;
     dim3 tg(g.nbx, g.nby);
-    hipLaunchKernelGGL((k_select<double, double>), flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_U_U],
-                       ctx->ui, ctx->x.u, ctx->nu);
-    hipLaunchKernelGGL((k_select<double, double>), flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->f[WAVI_V_V],
-                       ctx->vi, ctx->x.v, ctx->nv);
+    hipLaunchKernelGGL((k_select_uv<double, double>), flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream,
+                       ctx->f[WAVI_U_U], ctx->ui, ctx->x.u, ctx->nu, ctx->f[WAVI_V_V], ctx->vi, ctx->x.v, ctx->nv);
     CHK(do_rhs(ctx));
     hipLaunchKernelGGL(k_tile_resid<double>, tg, dim3(256), 0, ctx->stream, g, coef64(ctx), ctx->x.u, ctx->x.v,
                        ctx->b.u, ctx->b.v, ctx->r.u, ctx->r.v, ctx->Pr0, ctx->Pb);
     hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, ctx->stream, ctx->Pr0, ctx->Pb, g.npart, ctx->norms);
     // set_residual! (update_velocities.jl:428-433): residual fields at inner points
-    hipLaunchKernelGGL(k_copy_masked<double>, flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_U_RESIDUAL],
-                       ctx->r.u, ctx->ui, ctx->nu);
-    hipLaunchKernelGGL(k_copy_masked<double>, flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->f[WAVI_V_RESIDUAL],
-                       ctx->r.v, ctx->vi, ctx->nv);
+    hipLaunchKernelGGL(k_copy_masked_uv<double>, flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream,
+                       ctx->f[WAVI_U_RESIDUAL], ctx->r.u, ctx->ui, ctx->nu, ctx->f[WAVI_V_RESIDUAL], ctx->r.v, ctx->vi,
+                       ctx->nv);
     HIPCHK(hipMemcpyAsync(ctx->h_norms, ctx->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1294,9 +1310,8 @@ static int do_precondition(wavi_ctx *ctx, int *converged, double *rel, int *cg_i
     *cg_iters = 0;
     if (!*converged) {
         CHK(do_vcycle(ctx, cg_iters));
-        hipLaunchKernelGGL(k_set_velocities, flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_U_U], ctx->x.u,
-                           ctx->e.u, ctx->ui, ctx->nu);
-        hipLaunchKernelGGL(k_set_velocities, flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->f[WAVI_V_V], ctx->x.v,
+        hipLaunchKernelGGL(k_set_velocities_uv, flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream,
+                           ctx->f[WAVI_U_U], ctx->x.u, ctx->e.u, ctx->ui, ctx->nu, ctx->f[WAVI_V_V], ctx->x.v,
                            ctx->e.v, ctx->vi, ctx->nv);
         HIPCHK(hipGetLastError());
     }

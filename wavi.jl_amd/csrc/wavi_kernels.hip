@@ -24,6 +24,86 @@ __global__ void k_fill(T *a, T v, int n)
     if (k < n) a[k] = v;
 }
 
+// u / v pairs of the elementwise kernels above in one launch: blockIdx.y
+// selects the u (0) or v (1) arrays.
+template <typename Tin, typename Tout>
+__global__ void k_select_uv(const Tin *iu, const uint8_t *mu, Tout *ou, int nu, const Tin *iv, const uint8_t *mv,
+                            Tout *ov, int nv)
+{
+    const bool v = blockIdx.y != 0;
+    const Tin *in = v ? iv : iu;
+    const uint8_t *mask = v ? mv : mu;
+    Tout *out = v ? ov : ou;
+    const int n = v ? nv : nu;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = mask[k] ? (Tout)in[k] : Tout(0);
+}
+
+template <typename T>
+__global__ void k_copy_masked_uv(T *du, const T *su, const uint8_t *mu, int nu, T *dv, const T *sv,
+                                 const uint8_t *mv, int nv)
+{
+    const bool v = blockIdx.y != 0;
+    T *dst = v ? dv : du;
+    const T *src = v ? sv : su;
+    const uint8_t *mask = v ? mv : mu;
+    const int n = v ? nv : nu;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && mask[k]) dst[k] = src[k];
+}
+
+__global__ void k_set_velocities_uv(double *U, const double *xu, const double *eu, const uint8_t *iu, int nu,
+                                    double *V, const double *xv, const double *ev, const uint8_t *iv, int nv)
+{
+    const bool v = blockIdx.y != 0;
+    double *W = v ? V : U;
+    const double *x = v ? xv : xu, *e = v ? ev : eu;
+    const uint8_t *inner = v ? iv : iu;
+    const int n = v ? nv : nu;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && inner[k]) W[k] = x[k] + e[k];
+}
+
+template <typename T>
+__global__ void k_add_masked_uv(T *au, const T *bu, const uint8_t *mu, int nu, T *av, const T *bv,
+                                const uint8_t *mv, int nv)
+{
+    const bool v = blockIdx.y != 0;
+    T *a = v ? av : au;
+    const T *b = v ? bv : bu;
+    const uint8_t *mask = v ? mv : mu;
+    const int n = v ? nv : nu;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && mask[k]) a[k] = a[k] + b[k];
+}
+
+template <typename T>
+__global__ void k_fill_uv(T *au, int nu, T *av, int nv, T val)
+{
+    const bool v = blockIdx.y != 0;
+    T *a = v ? av : au;
+    const int n = v ? nv : nu;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = val;
+}
+
+// Up to CONV_MAX conversions in one launch: blockIdx.y selects the array.
+constexpr int CONV_MAX = 12;
+template <typename Tin, typename Tout>
+struct ConvList {
+    const Tin *in[CONV_MAX];
+    Tout *out[CONV_MAX];
+    int n[CONV_MAX];
+};
+
+template <typename Tin, typename Tout>
+__global__ void k_convert_list(ConvList<Tin, Tout> L)
+{
+    const int a = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < L.n[a]) L.out[a][k] = (Tout)L.in[a][k];
+}
+
 // out = mask ? in : 0
 template <typename Tin, typename Tout>
 __global__ void k_select(const Tin *in, const uint8_t *mask, Tout *out, int n)
