@@ -173,7 +173,8 @@ int wavi_update_velocities(wavi_ctx *ctx, wavi_solve_report *report);
  * kernels.  reports and codes (each NULL or n entries) receive each
  * member's report and return code (wavi_last_error of that member's context
  * holds its message).  Returns 0, or the first nonzero member code in member
- * order.  No reference counterpart: the reference solves one model per call. */
+ * order; WAVI_EARG before any member runs when a context is NULL or appears
+ * twice.  No reference counterpart: the reference solves one model per call. */
 int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,
                                  wavi_solve_report *reports, int *codes);
 /* update_state! (update_state.jl:7-21) on each of n contexts, like

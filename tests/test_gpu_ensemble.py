@@ -57,6 +57,15 @@ def test_batch_edge_cases(W):
     W.update_velocities_batch(two, nthreads=1)
     for a, b in zip(alone, two):
         np.testing.assert_array_equal(a.fields.gv.v, b.fields.gv.v)
+    _step("ensemble: a context twice is rejected before any member runs")
+    with pytest.raises(W.ArgumentError):
+        W.update_velocities_batch([two[0], two[0]])
+    import ctypes
+    lib = W._lib.load()
+    c = two[0].parallel_spec.ctx
+    ctxs = (ctypes.c_void_p * 3)(two[1].parallel_spec.ctx, c, c)
+    assert lib.wavi_update_velocities_batch(ctxs, 3, 2, None, None) == W._lib.WAVI_EARG
+    assert b"twice" in lib.wavi_last_error(None)
 
 
 def test_gradient_batch_matches_individual_calls(W):
@@ -85,3 +94,40 @@ def test_gradient_batch_matches_individual_calls(W):
         assert Ja == Jb
         np.testing.assert_array_equal(ga, gb)
         assert ra == rb
+
+
+def test_batch_while_other_contexts_are_created(W):
+    """Contexts created, filled, read and destroyed on the calling thread
+    while a mixed-precision batch (whose coarse CG captures a HIP graph per
+    member) runs on another thread: the library serialises its allocations,
+    frees and synchronous copies with the captures, so the batch completes
+    and equals the members solved alone."""
+    import threading
+    specs = _small_specs(W, 4, seed=11)
+    _step("ensemble under concurrent context churn: individual solves")
+    alone = _members(W, specs, "mixed")
+    for m in alone:
+        W.update_velocities(m)
+    batch = _members(W, specs, "mixed")
+    errors = []
+
+    def run_batch():
+        try:
+            W.update_velocities_batch(batch, nthreads=4)
+        except Exception as ex:  # reported below, on the test's thread
+            errors.append(ex)
+
+    _step("ensemble under concurrent context churn: batch on a thread, contexts created here")
+    t = threading.Thread(target=run_batch)
+    t.start()
+    churn = 0
+    while t.is_alive() or churn < 3:
+        m = _members(W, specs[:1], "fp64")[0]
+        W.update_geometry(m)  # creates the context, sets and reads fields
+        m.parallel_spec.close()
+        churn += 1
+    t.join()
+    assert not errors, errors
+    for a, b in zip(alone, batch):
+        np.testing.assert_array_equal(a.fields.gu.u, b.fields.gu.u)
+        np.testing.assert_array_equal(a.fields.gv.v, b.fields.gv.v)

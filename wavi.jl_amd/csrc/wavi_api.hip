@@ -8,8 +8,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <mutex>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -21,9 +23,10 @@ using namespace wavi;
 
 static thread_local std::string g_create_error;
 
-// Serialises HIP stream captures with the library's device allocations and
-// frees made while other contexts may be solving on other threads (a
-// batched call): an allocation or free can end another thread's capture.
+// Serialises HIP stream captures with the library's device allocations,
+// frees and synchronous copies, which other threads may make while a context
+// captures (a batched call, or a caller creating or filling contexts on
+// another thread): such a call can end another thread's capture.
 static std::mutex g_graph_mu;
 
 struct DevVec {
@@ -337,6 +340,19 @@ static void build_haar_masks(HaarMasks tabs[4])
 // calling thread included), each first making the member's device current.
 // codes (NULL or n entries) receives every member's return code; the
 // result is the first nonzero one in member order.
+// A context may appear only once in a batch: its members run concurrently.
+static bool batch_has_duplicates(wavi_ctx **ctxs, int n)
+{
+    std::vector<wavi_ctx *> v(ctxs, ctxs + n);
+    std::sort(v.begin(), v.end());
+    for (int k = 1; k < n; ++k)
+        if (v[k] == v[k - 1]) {
+            v[k]->err = g_create_error = "the same context appears twice in the batch";
+            return true;
+        }
+    return false;
+}
+
 template <typename F>
 static int run_members(wavi_ctx **ctxs, int n, int nthreads, int *codes, F &&fn)
 {
@@ -350,12 +366,21 @@ static int run_members(wavi_ctx **ctxs, int n, int nthreads, int *codes, F &&fn)
                 rcs[k] = WAVI_EHIP;
                 continue;
             }
-            rcs[k] = fn(ctx, k);
+            try {
+                rcs[k] = fn(ctx, k);
+            } catch (const std::exception &ex) {
+                ctx->err = std::string("host exception: ") + ex.what();
+                rcs[k] = WAVI_ESTATE;
+            }
         }
     };
     const int nt = std::min(nthreads, std::max(n, 1));
     std::vector<std::thread> pool;
-    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+    try {
+        for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+    } catch (const std::system_error &) {
+        // fewer threads: the members left are taken by those running
+    }
     worker();
     for (std::thread &t : pool) t.join();
     if (codes)
@@ -375,7 +400,7 @@ const char *wavi_last_error(const wavi_ctx *ctx)
     return ctx->err.c_str();
 }
 
-int wavi_ctx_destroy(wavi_ctx *ctx);
+static void ctx_free(wavi_ctx *ctx);
 
 int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, const wavi_solver_params *solver,
                     int precision_mode, wavi_ctx **out)
@@ -429,6 +454,7 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
             return WAVI_EDIM;
         }
     }
+    std::lock_guard<std::mutex> lock(g_graph_mu);
     wavi_ctx *ctx = new wavi_ctx();
     ctx->P = *phys;
     ctx->S = *solver;
@@ -459,7 +485,7 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
 
     auto fail = [&](int rc) {
         g_create_error = ctx->err;
-        wavi_ctx_destroy(ctx);
+        ctx_free(ctx);
         return rc;
     };
 #define CREATECHK(expr)                                                           \
@@ -616,6 +642,13 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
 int wavi_ctx_destroy(wavi_ctx *ctx)
 {
     if (!ctx) return WAVI_OK;
+    std::lock_guard<std::mutex> lock(g_graph_mu);
+    ctx_free(ctx);
+    return WAVI_OK;
+}
+
+static void ctx_free(wavi_ctx *ctx)
+{
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (uint8_t *p : {ctx->hm, ctx->hfix, ctx->um, ctx->vm, ctx->cm, ctx->ui, ctx->vi, ctx->ufix, ctx->vfix,
                        ctx->mwu, ctx->mwv, ctx->saved_mwu, ctx->saved_mwv})
@@ -656,7 +689,6 @@ int wavi_ctx_destroy(wavi_ctx *ctx)
     if (ctx->cgm_chunk) (void)hipGraphExecDestroy(ctx->cgm_chunk);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
-    return WAVI_OK;
 }
 
 int wavi_set_dt(wavi_ctx *ctx, double dt)
@@ -691,6 +723,7 @@ int wavi_set_field(wavi_ctx *ctx, int field, const double *host, size_t n)
         ctx->err = "field size mismatch: expected " + std::to_string(len) + ", got " + std::to_string(n);
         return WAVI_EDIM;
     }
+    std::lock_guard<std::mutex> lock(g_graph_mu);
     if (field == WAVI_WU_MASK || field == WAVI_WV_MASK) {
         double *tmp = nullptr;
         HIPCHK(hipMalloc(&tmp, n * sizeof(double)));
@@ -717,6 +750,7 @@ int wavi_get_field(wavi_ctx *ctx, int field, double *host, size_t n)
         ctx->err = "field size mismatch: expected " + std::to_string(len) + ", got " + std::to_string(n);
         return WAVI_EDIM;
     }
+    std::lock_guard<std::mutex> lock(g_graph_mu);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (field == WAVI_WU_MASK || field == WAVI_WV_MASK) {
         double *tmp = nullptr;
@@ -746,6 +780,7 @@ int wavi_state_save(wavi_ctx *ctx)
 {
     if (!ctx) return WAVI_EARG;
     if (!ctx->has_saved) {
+        std::lock_guard<std::mutex> lock(g_graph_mu);
         ctx->saved.assign(WAVI_NFIELDS, nullptr);
         for (int id = 0; id < WAVI_NFIELDS; ++id) {
             if (!ctx->f[id]) continue;
@@ -1749,6 +1784,7 @@ int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,
     if (n < 0 || nthreads < 1 || (n > 0 && !ctxs)) return WAVI_EARG;
     for (int k = 0; k < n; ++k)
         if (!ctxs[k]) return WAVI_EARG;
+    if (batch_has_duplicates(ctxs, n)) return WAVI_EARG;
     return run_members(ctxs, n, nthreads, codes, [&](wavi_ctx *ctx, int k) {
         return wavi_update_velocities(ctx, reports ? &reports[k] : nullptr);
     });
@@ -1759,6 +1795,7 @@ int wavi_update_state_batch(wavi_ctx **ctxs, int n, int nthreads, wavi_solve_rep
     if (n < 0 || nthreads < 1 || (n > 0 && !ctxs)) return WAVI_EARG;
     for (int k = 0; k < n; ++k)
         if (!ctxs[k]) return WAVI_EARG;
+    if (batch_has_duplicates(ctxs, n)) return WAVI_EARG;
     return run_members(ctxs, n, nthreads, codes, [&](wavi_ctx *ctx, int k) {
         return wavi_update_state(ctx, reports ? &reports[k] : nullptr);
     });
@@ -1771,6 +1808,7 @@ int wavi_misfit_gradient_batch(wavi_ctx **ctxs, int n, int nthreads, int n_picar
     if (n < 0 || nthreads < 1 || (n > 0 && (!ctxs || !u_obs || !v_obs || !J))) return WAVI_EARG;
     for (int k = 0; k < n; ++k)
         if (!ctxs[k]) return WAVI_EARG;
+    if (batch_has_duplicates(ctxs, n)) return WAVI_EARG;
     return run_members(ctxs, n, nthreads, codes, [&](wavi_ctx *ctx, int k) {
         return wavi_misfit_gradient(ctx, n_picard, tol_linear, maxit_linear, u_obs[k], v_obs[k], &J[k],
                                     grad_logc ? grad_logc[k] : nullptr, reports ? &reports[k] : nullptr);
@@ -1842,6 +1880,7 @@ int wavi_update_wavelets(wavi_ctx *ctx)
 
 static int upload2(wavi_ctx *ctx, const double *hu, const double *hv, DevVec &d)
 {
+    std::lock_guard<std::mutex> lock(g_graph_mu);
     HIPCHK(hipMemcpy(d.u, hu, ctx->nu * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d.v, hv, ctx->nv * sizeof(double), hipMemcpyHostToDevice));
     return WAVI_OK;
@@ -1850,6 +1889,7 @@ static int upload2(wavi_ctx *ctx, const double *hu, const double *hv, DevVec &d)
 static int download2(wavi_ctx *ctx, const DevVec &d, double *hu, double *hv)
 {
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::lock_guard<std::mutex> lock(g_graph_mu);
     HIPCHK(hipMemcpy(hu, d.u, ctx->nu * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(hv, d.v, ctx->nv * sizeof(double), hipMemcpyDeviceToHost));
     return WAVI_OK;
@@ -1919,13 +1959,17 @@ int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out)
     const int n = comp == 0 ? ctx->nu : ctx->nv;
     double *din = comp == 0 ? ctx->fixedv.u : ctx->fixedv.v;
     double *dout = comp == 0 ? ctx->t.u : ctx->t.v;
-    HIPCHK(hipMemcpy(din, in, n * sizeof(double), hipMemcpyHostToDevice));
+    {
+        std::lock_guard<std::mutex> lock(g_graph_mu);
+        HIPCHK(hipMemcpy(din, in, n * sizeof(double), hipMemcpyHostToDevice));
+    }
     dim3 tg(ctx->g.nbx, ctx->g.nby);
     hipLaunchKernelGGL((k_tile_haar<double, double>), tg, dim3(256), 0, ctx->stream, ctx->g, ctx->tab, ctx->S.levels, dir,
                        ctx->fixedv.u, ctx->fixedv.v, (const uint8_t *)nullptr, (const uint8_t *)nullptr, ctx->t.u,
                        ctx->t.v, (const uint8_t *)nullptr, (const uint8_t *)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::lock_guard<std::mutex> lock(g_graph_mu);
     HIPCHK(hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost));
     return WAVI_OK;
 }
@@ -1963,6 +2007,7 @@ int wavi_math_probe(int fn, const double *x, const double *y, double *out, size_
         return WAVI_EARG;
     }
     if (n == 0) return WAVI_OK;
+    std::lock_guard<std::mutex> lock(g_graph_mu);
     double *d = nullptr;
     hipError_t e = hipMalloc(&d, 3 * n * sizeof(double));
     if (e == hipSuccess) e = hipMemcpy(d, x, n * sizeof(double), hipMemcpyHostToDevice);

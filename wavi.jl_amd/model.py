@@ -388,6 +388,8 @@ def update_velocities_batch(models, nthreads=8, verbose=False):
     small members share the GPU.  Equivalent, member by member, to calling
     update_velocities on each model in turn."""
     models = list(models)
+    if len({id(m) for m in models}) != len(models):
+        raise ArgumentError("update_velocities_batch: a model appears twice in the batch")
     for m in models:
         update_preconditioner(m)
         push_fields(m)
@@ -456,6 +458,8 @@ def misfit_gradient_batch(models, n_picard, observations, tol_linear=1e-13, maxi
     n = len(models)
     if n == 0:
         return []
+    if len({id(m) for m in models}) != n:
+        raise ArgumentError("misfit_gradient_batch: a model appears twice in the batch")
     obs = []
     for m, (uo, vo) in zip(models, observations):
         update_preconditioner(m)
