@@ -7,7 +7,7 @@ do all of the solve's arithmetic.  Julia names ending in "!" drop the "!".
 """
 from .core import (ArgumentError, BoundsError, DimensionMismatch, Grid, InitialConditions, Params,
                    SolverParams, TimesteppingParams, OutputParams, UniformMeltRate, clip, get_c_mask,
-                   get_u_mask, get_v_mask, glen_b, height_above_floatation, mismip_plus_bed,
+                   get_glx, get_u_mask, get_v_mask, glen_b, height_above_floatation, mismip_plus_bed,
                    orientations2bc, volume_above_floatation)
 from .model import (BasicParallelSpec, Fields, HIPSpec, Model, get_op_diag, get_rhs, haar,
                     inner_update, misfit_gradient, misfit_gradient_batch, op_apply, pull_fields,
@@ -27,7 +27,7 @@ __all__ = [
     "misfit_gradient", "misfit_gradient_batch", "timestep", "run_simulation",
     "update_thickness", "op_apply", "get_rhs", "get_op_diag", "vcycle", "haar",
     "push_fields", "pull_fields", "glen_b", "mismip_plus_bed", "orientations2bc",
-    "get_u_mask", "get_v_mask", "get_c_mask", "clip", "height_above_floatation",
+    "get_u_mask", "get_v_mask", "get_c_mask", "get_glx", "clip", "height_above_floatation",
     "volume_above_floatation", "workloads", "BoundsError", "SharedMemorySpec", "schwarz_model",
     "schwarz_partition_of_unity", "schwarz_prolong_velocities", "schwarz_restrict_velocities",
     "OutputParams", "save_checkpoint", "load_checkpoint", "pickup", "write_vel",

@@ -319,6 +319,22 @@ def height_above_floatation(h, b, params):
     return h - (params.density_ocean / params.density_ice) * (params.sea_level_wrt_geoid - b)
 
 
+def get_glx(model):
+    """Grounding-line x positions (src/utilities.jl:417-426).  Wherever the
+    sign of the height above floatation drops from +1 to -1 between the h
+    cells (i, j) and (i+1, j), the zero of the straight line through the two
+    cell centres.  The crossings come in the reference's column-major order:
+    x fastest, then y."""
+    haf = np.asarray(model.fields.gh.haf, dtype=float)
+    xx = np.asarray(model.grid.xxh, dtype=float)
+    s = np.sign(haf)
+    cross = (s[1:, :] - s[:-1, :]) == -2
+    jj, ii = np.nonzero(cross.T)  # row-major over (j, i): i varies fastest
+    x1, x2 = xx[ii, jj], xx[ii + 1, jj]
+    h1, h2 = haf[ii, jj], haf[ii + 1, jj]
+    return x1 + (x2 - x1) * (0.0 - h1) / (h2 - h1)
+
+
 def volume_above_floatation(h, b, params, grid):
     """src/utilities.jl:453."""
     haf = height_above_floatation(h, b, params)
