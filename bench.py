@@ -189,7 +189,9 @@ def timed_solves(W, nx, ny, precision, steps, warmup, picard=None, maxiter_coars
 def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False, n_picard=10):
     """The seeded ensemble (channel solves of mixed sizes, nσ, Glen A and
     basal-drag fields) from the reset state, one batched call per step:
-    forward solves (wavi_update_velocities_batch, 20 Picard iterations), or
+    forward solves (wavi_update_velocities_batch, 20 Picard iterations: the
+    small members in one lockstep group, the large ones one per thread; also
+    timed with every member in one lockstep group), or
     with adjoint=True forward + adjoint gradients (wavi_misfit_gradient_batch,
     n_picard Picard iterations with converged solves; each member's
     observations are its own forward velocities times (1 + 0.01 N(0, 1))).
@@ -232,11 +234,13 @@ def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False
     else:
         reps = (W._lib.SolveReport * n)()
 
-    def run(threads):
+    def run(threads, lockstep=False):
         for m in members:
             assert lib.wavi_state_restore(m.parallel_spec.ctx) == 0
         t0 = time.perf_counter()
-        if adjoint:
+        if lockstep:
+            rc = lib.wavi_update_velocities_lockstep(ctxs, n, reps, codes)
+        elif adjoint:
             rc = lib.wavi_misfit_gradient_batch(ctxs, n, threads, n_picard, 1e-13, 200, up, vp, W._lib.dptr(Js),
                                                 gp, reps, codes)
         else:
@@ -254,9 +258,46 @@ def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False
     for k in range(steps):
         times.append(run(nthreads))
         log("[bench] ensemble step %d: %.3f s (%d threads)" % (k, times[-1], nthreads))
-    serial = run(1)
+    def one_at_a_time():
+        for m in members:
+            assert lib.wavi_state_restore(m.parallel_spec.ctx) == 0
+        t0 = time.perf_counter()
+        for k, m in enumerate(members):
+            ctx = m.parallel_spec.ctx
+            if adjoint:
+                rc = lib.wavi_misfit_gradient(ctx, n_picard, 1e-13, 200, up[k], vp[k], ctypes.byref(ctypes.c_double()),
+                                              gp[k], ctypes.byref(reps[k]))
+            else:
+                rc = lib.wavi_update_velocities(ctx, ctypes.byref(reps[k]))
+            assert rc == 0, lib.wavi_last_error(ctx)
+        return time.perf_counter() - t0
+
+    serial = one_at_a_time()
     log("[bench] ensemble one member at a time: %.3f s" % serial)
     total = float(sum(times))
+
+    def velocities():
+        out = []
+        for sp, m in zip(specs, members):
+            u = np.empty((sp["nx"] + 1) * sp["ny"])
+            assert lib.wavi_get_field(m.parallel_spec.ctx, W._lib.FIELD["U_U"], W._lib.dptr(u), u.size) == 0
+            out.append(u)
+        return out
+
+    lock = None
+    if not adjoint and precision == "mixed":
+        # the same solves with every member in one lockstep group (the
+        # batch itself puts only the small members in one)
+        ref = velocities()
+        run(nthreads, lockstep=True)
+        same = all(np.array_equal(a, b) for a, b in zip(ref, velocities()))
+        lt = []
+        for k in range(steps):
+            lt.append(run(nthreads, lockstep=True))
+            log("[bench] ensemble lockstep step %d: %.3f s" % (k, lt[-1]))
+        lock = {"members_per_s": n * steps / float(sum(lt)), "ms_per_step": 1e3 * float(sum(lt)) / steps,
+                "over_threaded_batch": (total / steps) / (float(sum(lt)) / steps),
+                "bit_identical_to_threaded_batch": bool(same)}
     unknowns = sum(2 * sp["nx"] * sp["ny"] + sp["nx"] + sp["ny"] for sp in specs)
     cfg = {"workload": "ensemble of %d channel %s, grids from %s, nsigma 4 or 8, A log-uniform in [1e-17, 4e-17], "
                        "C = 1e4 exp(GRF std 0.3)"
@@ -272,6 +313,8 @@ def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False
         picard = [reps[k].picard_iters for k in range(n)]
         cfg["picard_iters_min_max"] = [min(picard), max(picard)]
         cfg["all_converged"] = all(reps[k].converged for k in range(n))
+        if lock is not None:
+            cfg["all_members_lockstep"] = lock
     return {
         "metric": "ensemble members per second (%s)" % ("forward + adjoint gradients" if adjoint
                                                         else "forward velocity solves"),

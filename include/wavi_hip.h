@@ -170,13 +170,29 @@ int wavi_update_velocities(wavi_ctx *ctx, wavi_solve_report *report);
 /* update_velocities! for each of n contexts (an ensemble of independent
  * models, of any grid sizes and precisions), run by nthreads host threads;
  * every context has its own stream, so the GPU overlaps the members'
- * kernels.  reports and codes (each NULL or n entries) receive each
+ * kernels.  Small mixed-precision members (at most 256 tiles of 16 x 16,
+ * one device and Haar level count, their coarse-CG arenas within 128 MB)
+ * run as one lockstep group (wavi_update_velocities_lockstep) on one of the
+ * threads, concurrently with the other members on the rest.  reports and codes (each NULL or n entries) receive each
  * member's report and return code (wavi_last_error of that member's context
  * holds its message).  Returns 0, or the first nonzero member code in member
  * order; WAVI_EARG before any member runs when a context is NULL or appears
  * twice.  No reference counterpart: the reference solves one model per call. */
 int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,
                                  wavi_solve_report *reports, int *codes);
+/* update_velocities! for n mixed-precision contexts in lockstep, driven by
+ * the calling thread.  Each Picard iteration runs every member's inner
+ * update and residual on the member's own stream; the members whose residual
+ * asks for a V-cycle then run it up to the coarse right-hand side, and their
+ * coarse CGs run together: one launch per CG iteration over the tiles of all
+ * of them (k_cgm_iter_batch), until every member's CG has stopped; the rest
+ * of each V-cycle follows on the members' streams.  Every member is
+ * bit-identical to its own wavi_update_velocities.  All contexts must be
+ * mixed precision, on one device, with one Haar level count (else
+ * WAVI_EARG, as for a NULL or repeated context).  reports and codes as for
+ * wavi_update_velocities_batch.  No reference counterpart: the reference
+ * solves one model per call. */
+int wavi_update_velocities_lockstep(wavi_ctx **ctxs, int n, wavi_solve_report *reports, int *codes);
 /* update_state! (update_state.jl:7-21) on each of n contexts, like
  * wavi_update_velocities_batch: the subdomain solves of the restricted
  * additive Schwarz preconditioner (src/SharedMemorySpec.jl:96-133 runs

@@ -131,3 +131,36 @@ def test_batch_while_other_contexts_are_created(W):
     for a, b in zip(alone, batch):
         np.testing.assert_array_equal(a.fields.gu.u, b.fields.gu.u)
         np.testing.assert_array_equal(a.fields.gv.v, b.fields.gv.v)
+
+
+def test_lockstep_matches_individual_solves(W):
+    """wavi_update_velocities_lockstep: the members' coarse CGs share one
+    launch per iteration; every member must still be bit-identical to its
+    own solve, also when members stop their Picard loops at different
+    iterations."""
+    specs = _small_specs(W, 6, seed=5)
+    _step("lockstep: %d individual solves" % len(specs))
+
+    def members():
+        ms = _members(W, specs, "mixed")
+        ms[1].solver_params.maxiter_picard = 3  # leaves the lockstep early
+        return ms
+
+    alone = members()
+    for m in alone:
+        W.update_velocities(m)
+    _step("lockstep: one call")
+    lock = members()
+    W.update_velocities_batch(lock, lockstep=True)
+    for a, b in zip(alone, lock):
+        assert a.last_report["picard_iters"] == b.last_report["picard_iters"]
+        assert a.last_report["coarse_cg_iters"] == b.last_report["coarse_cg_iters"]
+        np.testing.assert_array_equal(a.fields.gu.u, b.fields.gu.u)
+        np.testing.assert_array_equal(a.fields.gv.v, b.fields.gv.v)
+    assert lock[1].last_report["picard_iters"] == 3
+    _step("lockstep: one member; FP64 members rejected")
+    one = members()[:1]
+    W.update_velocities_batch(one, lockstep=True)
+    np.testing.assert_array_equal(alone[0].fields.gu.u, one[0].fields.gu.u)
+    with pytest.raises(W.ArgumentError):
+        W.update_velocities_batch(_members(W, specs[:2], "fp64"), lockstep=True)

@@ -1122,12 +1122,31 @@ static void launch_cgm_iter(wavi_ctx *ctx, dim3 tb, const CGMIterArgs &a)
     }
 }
 
-// Mixed-precision coarse CG (Chronopoulos-Gear form, one launch per
-// iteration) on b_c held in v32.cr; solution in the warm start v32.xc.  The
-// CG works in the padded arena: k_cgm_pack fills the coefficients and masks,
-// k_cgm_setup the vectors, k_cgm_unpack returns the solution.
-template <>
-int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
+// The arguments of k_cgm_iter, fixed for the context.
+static CGMIterArgs cgm_iter_args(wavi_ctx *ctx)
+{
+    CGMIterArgs ia;
+    ia.Lo = cgm_layout(ctx);
+    ia.R = ctx->rings;
+    ia.Pgd = ctx->Pgd;
+    ia.st = ctx->stm;
+    ia.hflag = ctx->d_flag;
+    ia.Pb = ctx->Pb;
+    ia.g = ctx->g;
+    ia.npad = ctx->npad;
+    ia.rdx = (float)ctx->rdx;
+    ia.rdy = (float)ctx->rdy;
+    ia.tol_coarse = ctx->S.tol_coarse;
+    ia.reltol = std::sqrt(2.220446049250313e-16);  // IterativeSolvers' default sqrt(eps)
+    ia.maxiter = ctx->S.maxiter_coarse;
+    ia.nblk = ctx->nblk;
+    return ia;
+}
+
+// Set-up of the mixed coarse CG on b_c in w.cr with the warm start w.xc:
+// coefficients and masks into the arena, P x0, r0 = b - A_c x0, the
+// scalars and the host stop report reset.
+static int cgm_prepare(wavi_ctx *ctx, const VSet<float> &w)
 {
     const GridDims &g = ctx->g;
     dim3 tg(g.nbx, g.nby), tb(ctx->nblk);
@@ -1158,21 +1177,22 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     ctx->h_flag->done = 0;
     ctx->h_flag->iters = 0;
     hipLaunchKernelGGL(k_cgm_setup, tb, dim3(256), 0, ctx->stream, sa);
-    CGMIterArgs ia;
-    ia.Lo = Lo;
-    ia.R = ctx->rings;
-    ia.Pgd = ctx->Pgd;
-    ia.st = ctx->stm;
-    ia.hflag = ctx->d_flag;
-    ia.Pb = ctx->Pb;
-    ia.g = g;
-    ia.npad = ctx->npad;
-    ia.rdx = (float)ctx->rdx;
-    ia.rdy = (float)ctx->rdy;
-    ia.tol_coarse = ctx->S.tol_coarse;
-    ia.reltol = std::sqrt(2.220446049250313e-16);  // IterativeSolvers' default sqrt(eps)
-    ia.maxiter = ctx->S.maxiter_coarse;
-    ia.nblk = ctx->nblk;
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+// Mixed-precision coarse CG (Chronopoulos-Gear form, one launch per
+// iteration) on b_c held in v32.cr; solution in the warm start v32.xc.  The
+// CG works in the padded arena: k_cgm_pack fills the coefficients and masks,
+// k_cgm_setup the vectors, k_cgm_unpack returns the solution.
+template <>
+int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
+{
+    const GridDims &g = ctx->g;
+    dim3 tb(ctx->nblk);
+    const CGMLayout Lo = cgm_layout(ctx);
+    CHK(cgm_prepare(ctx, w));
+    const CGMIterArgs ia = cgm_iter_args(ctx);
     launch_cgm_iter<0, true>(ctx, tb, ia);
     HIPCHK(hipGetLastError());
     // Launch q (q >= 1) performs iteration q - 1 unless the CG has stopped;
@@ -1239,11 +1259,13 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
 }
 
 // precondition!(x, p, b) with x = 0, in the arithmetic of T: b in w.r,
-// correction into w.e (update_preconditioners.jl:54-83)
+// correction into w.e (update_preconditioners.jl:54-83).  vcycle_pre runs it
+// up to the coarse right-hand side b_c = R (r - A e) in w.cr: diagonal,
+// pre-smoothing, residual, restriction.
 template <typename T>
-static int do_vcycle_t(wavi_ctx *ctx, const VSet<T> &w, int *cg_iters)
+static int vcycle_pre(wavi_ctx *ctx, const VSet<T> &w)
 {
-    static const int fwd[4] = {1, 3, 2, 4}, rev[4] = {4, 2, 3, 1};
+    static const int fwd[4] = {1, 3, 2, 4};
     const GridDims &g = ctx->g;
     dim3 tg(g.nbx, g.nby);
     CHK(do_diag(ctx, w));
@@ -1257,7 +1279,16 @@ static int do_vcycle_t(wavi_ctx *ctx, const VSet<T> &w, int *cg_iters)
     hipLaunchKernelGGL((k_tile_haar<T, T>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 2, w.rho.u,
                        w.rho.v, ctx->ui, ctx->vi, w.cr.u, w.cr.v, ctx->mwu, ctx->mwv);
     HIPCHK(hipGetLastError());
-    CHK(do_coarse_cg(ctx, w, cg_iters));
+    return WAVI_OK;
+}
+
+// The V-cycle after its coarse solve: e += P x_c, post-smoothing.
+template <typename T>
+static int vcycle_post(wavi_ctx *ctx, const VSet<T> &w)
+{
+    static const int rev[4] = {4, 2, 3, 1};
+    const GridDims &g = ctx->g;
+    dim3 tg(g.nbx, g.nby);
     // e += P x_c
     hipLaunchKernelGGL((k_tile_haar<T, T>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1, w.xc.u,
                        w.xc.v, ctx->mwu, ctx->mwv, w.t.u, w.t.v, ctx->ui, ctx->vi);
@@ -1266,6 +1297,14 @@ static int do_vcycle_t(wavi_ctx *ctx, const VSet<T> &w, int *cg_iters)
     CHK(do_smoother(ctx, w, rev));
     HIPCHK(hipGetLastError());
     return WAVI_OK;
+}
+
+template <typename T>
+static int do_vcycle_t(wavi_ctx *ctx, const VSet<T> &w, int *cg_iters)
+{
+    CHK(vcycle_pre(ctx, w));
+    CHK(do_coarse_cg(ctx, w, cg_iters));
+    return vcycle_post(ctx, w);
 }
 
 // Conversions of several arrays, one launch.
@@ -1292,9 +1331,8 @@ struct Conversions {
 // mixed precision the operator coefficients, the residual and the warm start
 // are rounded to FP32, the V-cycle runs in FP32 (reductions accumulated in
 // FP64) and the correction and warm start are widened back.
-static int do_vcycle(wavi_ctx *ctx, int *cg_iters)
+static int vcycle_narrow(wavi_ctx *ctx)
 {
-    if (ctx->precision == WAVI_PRECISION_FP64) return do_vcycle_t<double>(ctx, vset64(ctx), cg_iters);
     VSet<float> &w = ctx->v32;
     Conversions<double, float> down;
     down.add(ctx->heta_h, w.heta_h, ctx->nh);
@@ -1310,7 +1348,12 @@ static int do_vcycle(wavi_ctx *ctx, int *cg_iters)
     down.add(ctx->f[WAVI_WV_CORRECTION_COARSE], w.xc.v, ctx->nv);
     down.launch(ctx);
     HIPCHK(hipGetLastError());
-    CHK(do_vcycle_t<float>(ctx, w, cg_iters));
+    return WAVI_OK;
+}
+
+static int vcycle_widen(wavi_ctx *ctx)
+{
+    VSet<float> &w = ctx->v32;
     Conversions<float, double> up;
     up.add(w.e.u, ctx->e.u, ctx->nu);
     up.add(w.e.v, ctx->e.v, ctx->nv);
@@ -1321,8 +1364,21 @@ static int do_vcycle(wavi_ctx *ctx, int *cg_iters)
     return WAVI_OK;
 }
 
-// precondition!(model, ::BasicParallelSpec) (update_velocities.jl:50-83)
-static int do_precondition(wavi_ctx *ctx, int *converged, double *rel, int *cg_iters)
+static int do_vcycle(wavi_ctx *ctx, int *cg_iters)
+{
+    if (ctx->precision == WAVI_PRECISION_FP64) return do_vcycle_t<double>(ctx, vset64(ctx), cg_iters);
+    CHK(vcycle_narrow(ctx));
+    VSet<float> &w = ctx->v32;
+    CHK(do_vcycle_t<float>(ctx, w, cg_iters));
+    return vcycle_widen(ctx);
+}
+
+// precondition!(model, ::BasicParallelSpec) (update_velocities.jl:50-83),
+// in three parts: precondition_resid queues the residual r = b - A x, the
+// residual fields and the copy of ||r||, ||b|| to the host;
+// precondition_decide waits for them; precondition_update queues
+// u = x + e after the V-cycle.
+static int precondition_resid(wavi_ctx *ctx)
 {
     const GridDims &g = ctx->g;
     dim3 tg(g.nbx, g.nby);
@@ -1338,17 +1394,35 @@ static int do_precondition(wavi_ctx *ctx, int *converged, double *rel, int *cg_i
                        ctx->nv);
     HIPCHK(hipMemcpyAsync(ctx->h_norms, ctx->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+static int precondition_decide(wavi_ctx *ctx, int *converged, double *rel)
+{
     HIPCHK(hipStreamSynchronize(ctx->stream));
     prof_collect(ctx);
     *rel = ctx->h_norms[0] / ctx->h_norms[1];
     *converged = *rel < ctx->S.tol_picard;
+    return WAVI_OK;
+}
+
+static int precondition_update(wavi_ctx *ctx)
+{
+    hipLaunchKernelGGL(k_set_velocities_uv, flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream,
+                       ctx->f[WAVI_U_U], ctx->x.u, ctx->e.u, ctx->ui, ctx->nu, ctx->f[WAVI_V_V], ctx->x.v,
+                       ctx->e.v, ctx->vi, ctx->nv);
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+static int do_precondition(wavi_ctx *ctx, int *converged, double *rel, int *cg_iters)
+{
+    CHK(precondition_resid(ctx));
+    CHK(precondition_decide(ctx, converged, rel));
     *cg_iters = 0;
     if (!*converged) {
         CHK(do_vcycle(ctx, cg_iters));
-        hipLaunchKernelGGL(k_set_velocities_uv, flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream,
-                           ctx->f[WAVI_U_U], ctx->x.u, ctx->e.u, ctx->ui, ctx->nu, ctx->f[WAVI_V_V], ctx->x.v,
-                           ctx->e.v, ctx->vi, ctx->nv);
-        HIPCHK(hipGetLastError());
+        CHK(precondition_update(ctx));
     }
     return WAVI_OK;
 }
@@ -1749,6 +1823,155 @@ static int do_update_thickness(wavi_ctx *ctx, double dt)
 }
 
 // ---------------------------------------------------------------------------
+// k_cgm_iter_batch at the Haar level of the members.
+template <int PAR, bool INIT>
+static void launch_cgm_batch(hipStream_t s, int levels, int nwg, const CGMBatch &B)
+{
+    switch (levels) {
+    case 0: hipLaunchKernelGGL((k_cgm_iter_batch<PAR, INIT, 0>), dim3(nwg), dim3(256), 0, s, B); break;
+    case 1: hipLaunchKernelGGL((k_cgm_iter_batch<PAR, INIT, 1>), dim3(nwg), dim3(256), 0, s, B); break;
+    case 2: hipLaunchKernelGGL((k_cgm_iter_batch<PAR, INIT, 2>), dim3(nwg), dim3(256), 0, s, B); break;
+    default: hipLaunchKernelGGL((k_cgm_iter_batch<PAR, INIT, 3>), dim3(nwg), dim3(256), 0, s, B); break;
+    }
+}
+
+// Device tables of a lockstep batch and the events that order the members'
+// streams around its shared coarse CG.
+struct LockstepRes {
+    CGMIterArgs *members = nullptr;
+    int *group = nullptr, *first = nullptr;
+    std::vector<hipEvent_t> ev;
+    ~LockstepRes()
+    {
+        {
+            std::lock_guard<std::mutex> lock(g_graph_mu);
+            if (members) (void)hipFree(members);
+            if (group) (void)hipFree(group);
+            if (first) (void)hipFree(first);
+        }
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+struct LockMember {
+    wavi_ctx *ctx;
+    wavi_solve_report rep;
+    int it = 0, total = 0, converged = 0, cg = 0;
+    double rel = INFINITY;
+};
+
+// One Picard iteration of every member that still iterates: each member's
+// inner update and residual on its own stream; then, for the members whose
+// residual asks for a V-cycle, the V-cycle up to the coarse right-hand side
+// on their streams, the coarse CGs of all of them together on the stream of
+// member 0 (one k_cgm_iter_batch launch per iteration over all their tiles,
+// until every member's CG has stopped), and the rest of each V-cycle and the
+// velocity update back on the members' streams.  Returns the number of
+// members that iterated, or a negative error code (*bad = the member).
+static int lockstep_picard(std::vector<LockMember> &M, LockstepRes &R, int it, int *bad)
+{
+    std::vector<int> act, cgm;
+    for (int k = 0; k < (int)M.size(); ++k)
+        if (!M[k].converged && M[k].it < M[k].ctx->S.maxiter_picard) act.push_back(k);
+    for (int k : act) {
+        *bad = k;
+        wavi_ctx *ctx = M[k].ctx;
+        M[k].it = it;
+        M[k].cg = 0;
+        CHK(do_inner_update(ctx));
+        CHK(precondition_resid(ctx));
+    }
+    for (int k : act) {
+        *bad = k;
+        CHK(precondition_decide(M[k].ctx, &M[k].converged, &M[k].rel));
+        if (!M[k].converged) cgm.push_back(k);
+    }
+    if (!cgm.empty()) {
+        wavi_ctx *lead = M[0].ctx;
+        hipStream_t cs = lead->stream;
+        std::vector<CGMIterArgs> hm(cgm.size());
+        std::vector<int> hf(cgm.size()), hg;
+        int nwg = 0, qmax = 0;
+        for (size_t j = 0; j < cgm.size(); ++j) {
+            const int k = cgm[j];
+            *bad = k;
+            wavi_ctx *ctx = M[k].ctx;
+            CHK(vcycle_narrow(ctx));
+            CHK(vcycle_pre(ctx, ctx->v32));
+            CHK(cgm_prepare(ctx, ctx->v32));
+            HIPCHK(hipEventRecord(R.ev[k], ctx->stream));
+            if (ctx != lead) HIPCHK(hipStreamWaitEvent(cs, R.ev[k], 0));
+            hm[j] = cgm_iter_args(ctx);
+            hf[j] = nwg;
+            for (int q = 0; q < ctx->nblk / 8; ++q) hg.push_back((int)j);
+            nwg += ctx->nblk;
+            qmax = std::max(qmax, ctx->S.maxiter_coarse + 1);
+        }
+        *bad = 0;
+        wavi_ctx *ctx = lead;
+        HIPCHK(hipMemcpyAsync(R.members, hm.data(), hm.size() * sizeof(CGMIterArgs), hipMemcpyHostToDevice, cs));
+        HIPCHK(hipMemcpyAsync(R.group, hg.data(), hg.size() * sizeof(int), hipMemcpyHostToDevice, cs));
+        HIPCHK(hipMemcpyAsync(R.first, hf.data(), hf.size() * sizeof(int), hipMemcpyHostToDevice, cs));
+        const CGMBatch B{R.members, R.group, R.first};
+        const int levels = lead->S.levels;
+        launch_cgm_batch<0, true>(cs, levels, nwg, B);
+        HIPCHK(hipGetLastError());
+        // launch q performs iteration q - 1 of every member whose CG has not
+        // stopped; chunks of 32 with at most two in flight, as for one member
+        const int CHUNK = 32;
+        int q = 1, c = 0;
+        while (q <= qmax) {
+            const int n = std::min(CHUNK, qmax - q + 1);
+            for (int s = 0; s < n; ++s, ++q) {
+                if (q & 1)
+                    launch_cgm_batch<1, false>(cs, levels, nwg, B);
+                else
+                    launch_cgm_batch<0, false>(cs, levels, nwg, B);
+            }
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(lead->cg_ev[c % 3], cs));
+            if (c >= 2) {
+                HIPCHK(hipEventSynchronize(lead->cg_ev[(c - 2) % 3]));
+                bool all = true;
+                for (int k : cgm) all = all && ((volatile CGMHostFlag *)M[k].ctx->h_flag)->done;
+                if (all) break;
+            }
+            ++c;
+        }
+        const int ndone = (int)M.size();
+        HIPCHK(hipEventRecord(R.ev[ndone], cs));
+        HIPCHK(hipStreamSynchronize(cs));
+        for (int k : cgm) {
+            *bad = k;
+            wavi_ctx *ctx = M[k].ctx;
+            volatile CGMHostFlag *f = ctx->h_flag;
+            if (!f->done) {
+                ctx->err = "coarse CG did not terminate";
+                return WAVI_ESTATE;
+            }
+            M[k].cg = f->iters;
+            if (ctx != lead) HIPCHK(hipStreamWaitEvent(ctx->stream, R.ev[ndone], 0));
+            const VSet<float> &w = ctx->v32;
+            hipLaunchKernelGGL(k_cgm_unpack, flat(std::max(ctx->nu, ctx->nv)), dim3(256), 0, ctx->stream, ctx->g,
+                               cgm_layout(ctx), w.xc.u, w.xc.v);
+            HIPCHK(hipGetLastError());
+            CHK(vcycle_post(ctx, w));
+            CHK(vcycle_widen(ctx));
+            CHK(precondition_update(ctx));
+        }
+    }
+    for (int k : act) {
+        LockMember &m = M[k];
+        m.total += m.cg;
+        if (it <= WAVI_REPORT_MAX_PICARD) {
+            m.rep.coarse_cg_iters[it - 1] = m.cg;
+            m.rep.rel_resid_hist[it - 1] = m.rel;
+        }
+    }
+    return (int)act.size();
+}
+
 extern "C" {
 #pragma GCC visibility push(default)
 
@@ -1778,6 +2001,82 @@ int wavi_misfit_gradient(wavi_ctx *ctx, int n_picard, double tol_linear, int max
                               report ? report : &local);
 }
 
+int wavi_update_velocities_lockstep(wavi_ctx **ctxs, int n, wavi_solve_report *reports, int *codes)
+{
+    g_create_error.clear();
+    if (n < 0 || (n > 0 && !ctxs)) return WAVI_EARG;
+    for (int k = 0; k < n; ++k)
+        if (!ctxs[k]) return WAVI_EARG;
+    if (batch_has_duplicates(ctxs, n)) return WAVI_EARG;
+    if (n == 0) return WAVI_OK;
+    for (int k = 0; k < n; ++k)
+        if (ctxs[k]->precision != WAVI_PRECISION_MIXED || ctxs[k]->device != ctxs[0]->device ||
+            ctxs[k]->S.levels != ctxs[0]->S.levels) {
+            ctxs[k]->err = g_create_error =
+                "lockstep batch: every context must be mixed precision, on one device, with one Haar level count";
+            return WAVI_EARG;
+        }
+    if (hipSetDevice(ctxs[0]->device) != hipSuccess) {
+        g_create_error = "hipSetDevice failed";
+        return WAVI_EHIP;
+    }
+    std::vector<LockMember> M(n);
+    size_t ngroups = 0;
+    for (int k = 0; k < n; ++k) {
+        M[k].ctx = ctxs[k];
+        memset(&M[k].rep, 0, sizeof(M[k].rep));
+        ngroups += (size_t)(ctxs[k]->nblk / 8);
+    }
+    LockstepRes R;
+    auto setup = [&]() -> int {
+        wavi_ctx *ctx = ctxs[0];
+        {
+            std::lock_guard<std::mutex> lock(g_graph_mu);
+            HIPCHK(hipMalloc(&R.members, n * sizeof(CGMIterArgs)));
+            HIPCHK(hipMalloc(&R.group, ngroups * sizeof(int)));
+            HIPCHK(hipMalloc(&R.first, n * sizeof(int)));
+        }
+        R.ev.assign(n + 1, nullptr);
+        for (hipEvent_t &e : R.ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return WAVI_OK;
+    };
+    int rc = setup(), bad = 0;
+    for (int it = 1; rc == WAVI_OK; ++it) {
+        const int r = lockstep_picard(M, R, it, &bad);
+        if (r < 0) rc = r;
+        if (r <= 0) break;
+    }
+    for (int k = 0; k < n; ++k) {
+        LockMember &m = M[k];
+        const int code = rc == WAVI_OK ? WAVI_OK : (k == bad ? rc : WAVI_ESTATE);
+        if (rc != WAVI_OK && k != bad) m.ctx->err = "lockstep batch stopped by another member's error";
+        if (hipStreamSynchronize(m.ctx->stream) != hipSuccess && rc == WAVI_OK) rc = WAVI_EHIP;
+        m.rep.picard_iters = m.it;
+        m.rep.converged = m.converged;
+        m.rep.rel_resid = m.rel;
+        m.rep.coarse_cg_total = m.total;
+        m.rep.refinements = m.it;
+        if (reports) reports[k] = m.rep;
+        if (codes) codes[k] = code;
+    }
+    if (rc != WAVI_OK && g_create_error.empty()) g_create_error = ctxs[bad]->err;
+    return rc;
+}
+
+// Members that join a batch's lockstep group: mixed precision, at most
+// LOCK_TILES tiles, on the device and Haar level of the group's first
+// member, while the group's arenas stay within LOCK_ARENA_BYTES.  The
+// group's coarse CGs touch all of its arenas in every launch; kept within
+// half the 256 MB Infinity Cache they are served from it, and small members
+// gain most from sharing launches.  Larger members run one per thread.
+constexpr int LOCK_TILES = 256;
+constexpr size_t LOCK_ARENA_BYTES = (size_t)128 << 20;
+
+static size_t arena_bytes(const wavi_ctx *ctx)
+{
+    return (size_t)CGM_NARRAY * ctx->ns * sizeof(float) + 2 * (size_t)ctx->ns;
+}
+
 int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,
                                  wavi_solve_report *reports, int *codes)
 {
@@ -1785,9 +2084,75 @@ int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,
     for (int k = 0; k < n; ++k)
         if (!ctxs[k]) return WAVI_EARG;
     if (batch_has_duplicates(ctxs, n)) return WAVI_EARG;
-    return run_members(ctxs, n, nthreads, codes, [&](wavi_ctx *ctx, int k) {
-        return wavi_update_velocities(ctx, reports ? &reports[k] : nullptr);
-    });
+    std::vector<int> small, large;
+    size_t bytes = 0;
+    for (int k = 0; k < n; ++k) {
+        const wavi_ctx *c = ctxs[k], *ref = small.empty() ? c : ctxs[small[0]];
+        const bool fits = c->precision == WAVI_PRECISION_MIXED && c->g.nbx * c->g.nby <= LOCK_TILES &&
+                          c->device == ref->device && c->S.levels == ref->S.levels &&
+                          bytes + arena_bytes(c) <= LOCK_ARENA_BYTES;
+        if (fits) {
+            small.push_back(k);
+            bytes += arena_bytes(c);
+        } else {
+            large.push_back(k);
+        }
+    }
+    if (small.size() < 2) {
+        return run_members(ctxs, n, nthreads, codes, [&](wavi_ctx *ctx, int k) {
+            return wavi_update_velocities(ctx, reports ? &reports[k] : nullptr);
+        });
+    }
+    // the lockstep group on one thread, the other members on the others
+    std::vector<wavi_ctx *> sc, lc;
+    for (int k : small) sc.push_back(ctxs[k]);
+    for (int k : large) lc.push_back(ctxs[k]);
+    std::vector<wavi_solve_report> srep(sc.size()), lrep(lc.size());
+    std::vector<int> scode(sc.size(), WAVI_OK), lcode(lc.size(), WAVI_OK);
+    int src = WAVI_OK, lrc = WAVI_OK;
+    auto run_small = [&]() {
+        try {
+            src = wavi_update_velocities_lockstep(sc.data(), (int)sc.size(), srep.data(), scode.data());
+        } catch (const std::exception &ex) {
+            sc[0]->err = std::string("host exception: ") + ex.what();
+            src = scode[0] = WAVI_ESTATE;
+        }
+    };
+    auto run_large = [&](int threads) {
+        lrc = run_members(lc.data(), (int)lc.size(), threads, lcode.data(), [&](wavi_ctx *ctx, int k) {
+            return wavi_update_velocities(ctx, &lrep[k]);
+        });
+    };
+    std::thread t;
+    if (nthreads > 1 && !lc.empty()) {
+        try {
+            t = std::thread(run_small);
+        } catch (const std::system_error &) {
+        }
+    }
+    if (t.joinable()) {
+        run_large(nthreads - 1);
+        t.join();
+    } else {
+        run_small();
+        run_large(nthreads);
+    }
+    for (size_t j = 0; j < small.size(); ++j) {
+        if (reports) reports[small[j]] = srep[j];
+        if (codes) codes[small[j]] = scode[j];
+    }
+    for (size_t j = 0; j < large.size(); ++j) {
+        if (reports) reports[large[j]] = lrep[j];
+        if (codes) codes[large[j]] = lcode[j];
+    }
+    if (src == WAVI_OK && lrc == WAVI_OK) return WAVI_OK;
+    for (int k = 0; k < n; ++k) {
+        const bool is_small = std::find(small.begin(), small.end(), k) != small.end();
+        const int c = is_small ? scode[std::find(small.begin(), small.end(), k) - small.begin()]
+                               : lcode[std::find(large.begin(), large.end(), k) - large.begin()];
+        if (c != WAVI_OK) return c;
+    }
+    return src != WAVI_OK ? src : lrc;
 }
 
 int wavi_update_state_batch(wavi_ctx **ctxs, int n, int nthreads, wavi_solve_report *reports, int *codes)

@@ -2042,7 +2042,7 @@ struct CGMIterArgs {
 // y pass straight into the window); intermediates; operator and R x pass;
 // R y pass and P y pass; P x pass and the new partials.
 template <int PAR, bool INIT, int LEV>
-__global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
+__device__ __forceinline__ void cgm_iter_tile(const CGMIterArgs &a, const int L)
 {
     static_assert(!INIT || PAR == 0, "the initial launch reads set 0");
     struct Smem {
@@ -2057,7 +2057,6 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     };
     __shared__ Smem sm;
     const GridDims &g = a.g;
-    const int L = xcd_tile(a.nblk);
     const int ntile = g.nbx * g.nby;
     if (L >= ntile) return;
     const int tid = threadIdx.x;
@@ -2354,6 +2353,35 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
     if (tid == 0)
         a.Pgd[NP * a.npad + L] = make_float2((sm.red1[0] + sm.red1[1]) + (sm.red1[2] + sm.red1[3]),
                                              (sm.red1[4] + sm.red1[5]) + (sm.red1[6] + sm.red1[7]));
+}
+
+// One coarse-CG iteration of one CG problem (the tile of workgroup
+// blockIdx.x, XCD-aware).
+template <int PAR, bool INIT, int LEV>
+__global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
+{
+    cgm_iter_tile<PAR, INIT, LEV>(a, xcd_tile(a.nblk));
+}
+
+// The same iteration of several independent CG problems (an ensemble's
+// members, each with its own arena, rings, partials and scalars) in one
+// launch.  Member m owns the workgroups first[m] .. first[m] + nblk_m - 1;
+// every first[m] and nblk_m is a multiple of 8, so group_member, indexed by
+// blockIdx.x / 8, names the member, and a member's workgroups map to its
+// tiles exactly as k_cgm_iter's do (blockIdx.x mod 8 is preserved).
+struct CGMBatch {
+    const CGMIterArgs *members;
+    const int *group_member;
+    const int *first;
+};
+
+template <int PAR, bool INIT, int LEV>
+__global__ void __launch_bounds__(256) k_cgm_iter_batch(CGMBatch B)
+{
+    const int m = B.group_member[blockIdx.x >> 3];
+    const CGMIterArgs a = B.members[m];
+    const int w = (int)blockIdx.x - B.first[m];
+    cgm_iter_tile<PAR, INIT, LEV>(a, (w & 7) * (a.nblk >> 3) + (w >> 3));
 }
 
 // out = mask_out ? H(mask_in ? in : 0) : 0 on the u grid (comp 0) and v grid

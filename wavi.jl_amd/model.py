@@ -382,11 +382,15 @@ def update_velocities(model, verbose=False):
     return model
 
 
-def update_velocities_batch(models, nthreads=8, verbose=False):
+def update_velocities_batch(models, nthreads=8, verbose=False, lockstep=False):
     """update_velocities! on every model of an ensemble in one native call:
     host threads drive the members concurrently, each on its own stream, so
-    small members share the GPU.  Equivalent, member by member, to calling
-    update_velocities on each model in turn."""
+    small members share the GPU.  With lockstep=True (mixed-precision models
+    with one Haar level count) the calling thread drives all members through
+    each Picard iteration together instead, and their coarse CGs share one
+    launch per CG iteration (wavi_update_velocities_lockstep).  Either way
+    equivalent, member by member, to calling update_velocities on each model
+    in turn."""
     models = list(models)
     if len({id(m) for m in models}) != len(models):
         raise ArgumentError("update_velocities_batch: a model appears twice in the batch")
@@ -399,7 +403,10 @@ def update_velocities_batch(models, nthreads=8, verbose=False):
     ctxs = (ctypes.c_void_p * len(models))(*[m.parallel_spec.ctx for m in models])
     reps = (_lib.SolveReport * len(models))()
     codes = (ctypes.c_int * len(models))()
-    rc = lib.wavi_update_velocities_batch(ctxs, len(models), int(nthreads), reps, codes)
+    if lockstep:
+        rc = lib.wavi_update_velocities_lockstep(ctxs, len(models), reps, codes)
+    else:
+        rc = lib.wavi_update_velocities_batch(ctxs, len(models), int(nthreads), reps, codes)
     if rc != _lib.WAVI_OK:
         for m, code in zip(models, codes):
             _check(lib, m.parallel_spec.ctx, code)
