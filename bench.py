@@ -186,7 +186,7 @@ def timed_solves(W, nx, ny, precision, steps, warmup, picard=None, maxiter_coars
     return times, rep, velocities(W, m, spec), roof
 
 
-def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False, n_picard=10):
+def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False, n_picard=10, max_nx=None):
     """The seeded ensemble (channel solves of mixed sizes, nσ, Glen A and
     basal-drag fields) from the reset state, one batched call per step:
     forward solves (wavi_update_velocities_batch, 20 Picard iterations: the
@@ -197,7 +197,11 @@ def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False
     observations are its own forward velocities times (1 + 0.01 N(0, 1))).
     Also timed member after member with one thread, for the concurrency
     gain.  Returns the JSON line."""
-    specs = W.workloads.ensemble_specs(n)
+    if max_nx:
+        # the same seeded draw, keeping the first n members at most max_nx wide
+        specs = [sp for sp in W.workloads.ensemble_specs(16 * n) if sp["nx"] <= max_nx][:n]
+    else:
+        specs = W.workloads.ensemble_specs(n)
     log("[bench] ensemble: %d members, setup" % n)
     members = []
     for sp in specs:
@@ -303,7 +307,7 @@ def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False
                        "C = 1e4 exp(GRF std 0.3)"
                        % (n, "forward + adjoint gradients (%d Picard iterations, solves to 1e-13)" % n_picard
                           if adjoint else "solves (20 Picard iterations)",
-                          ["%dx%d" % s for s in W.workloads.ENSEMBLE_SIZES]),
+                          ["%dx%d" % s for s in W.workloads.ENSEMBLE_SIZES if not max_nx or s[0] <= max_nx]),
            "precision": precision, "host_threads": nthreads, "unknowns_total": unknowns,
            "serial_members_per_s": n / serial, "batched_over_serial": serial / (total / steps)}
     if adjoint:
@@ -472,6 +476,8 @@ def main():
     ap.add_argument("--ensemble", type=int, default=0,
                     help="time the seeded ensemble of this many members instead of the single solve")
     ap.add_argument("--threads", type=int, default=8, help="host threads of the ensemble batch")
+    ap.add_argument("--ensemble-max-nx", type=int, default=None,
+                    help="ensemble of members at most this many cells wide (same seeded draw)")
     ap.add_argument("--with-adjoint", action="store_true",
                     help="with --ensemble: forward + adjoint gradient per member")
     ap.add_argument("--adjoint", action="store_true",
@@ -491,7 +497,7 @@ def main():
     if args.ensemble:
         prec = "fp64" if args.precision == "fp64" else "mixed"
         print(json.dumps(ensemble_run(W, args.ensemble, args.threads, args.steps, args.warmup, prec,
-                                      args.with_adjoint)), flush=True)
+                                      args.with_adjoint, max_nx=args.ensemble_max_nx)), flush=True)
         return
     nx, ny = args.nx, args.ny
     if args.calibration_copy_mib:
