@@ -1037,6 +1037,16 @@ static bool use_cg_graph()
     return on;
 }
 
+// Coarse-CG chunk k holds min(32, 2^(k+1)) iterations.  A CG that stops after
+// a few iterations (the usual case from the second step of a transient run
+// on) then leaves few launches queued past its stop, while a long CG runs in
+// chunks of 32 after its first 30 iterations.  The host reads the stop
+// report of chunk k - 1 once chunk k is queued, so one chunk keeps the GPU
+// busy while the host waits.  Chunk lengths are even, so every chunk of 32
+// starts at the same launch parity.
+constexpr int CG_CHUNK = 32;
+static int cg_chunk_len(int k) { return k >= 4 ? CG_CHUNK : 2 << k; }
+
 // coarse CG on b_c held in w.cr; solution in the warm start w.xc
 template <typename T>
 static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
@@ -1052,19 +1062,18 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
     hipLaunchKernelGGL(k_cg_setup<T>, tb, dim3(256), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
     // Triplet q performs iteration q unless the CG has stopped; triplet
-    // maxiter always stops, so no more are needed.  Chunks of 32 triplets are
-    // queued while at most two chunks are in flight; the host reads the stop
-    // report between chunks and queues no more after a stop.
+    // maxiter always stops, so no more are needed.  Triplets are queued in
+    // chunks (cg_chunk_len); the host reads the stop report between chunks
+    // and queues no more after a stop.
     volatile CGMHostFlag *hf = ctx->h_flag;
-    const int CHUNK = 32;
     const int nq = ctx->S.maxiter_coarse + 1;
     int q = 0, k = 0;
     while (q < nq) {
-        const int n = std::min(CHUNK, nq - q);
-        // even chunks: one burst over the chunk; odd chunks: sampled launches
-        // when profiling
+        const int n = std::min(cg_chunk_len(k), nq - q);
+        // when profiling, odd chunks time sampled launches and even chunks
+        // of full length time the whole chunk
         const bool sampled = ctx->prof && (k & 1) != 0;
-        const int bev = sampled ? -1 : burst_begin(ctx);
+        const int bev = sampled || n != CG_CHUNK ? -1 : burst_begin(ctx);
         for (int s = 0; s < n; ++s, ++q) {
             const int par = q & 1;
             {
@@ -1083,8 +1092,8 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
         burst_end(ctx, bev, q - 1, n);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ctx->cg_ev[k % 3], ctx->stream));
-        if (k >= 2) {
-            HIPCHK(hipEventSynchronize(ctx->cg_ev[(k - 2) % 3]));
+        if (k >= 1) {
+            HIPCHK(hipEventSynchronize(ctx->cg_ev[(k - 1) % 3]));
             if (hf->done) break;
         }
         ++k;
@@ -1196,24 +1205,24 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     launch_cgm_iter<0, true>(ctx, tb, ia);
     HIPCHK(hipGetLastError());
     // Launch q (q >= 1) performs iteration q - 1 unless the CG has stopped;
-    // launch maxiter + 1 always stops, so no more are needed.  Chunks of 32
-    // launches are queued while at most two chunks are in flight; the host
-    // reads the stop report between chunks and queues no more after a stop.
+    // launch maxiter + 1 always stops, so no more are needed.  Launches are
+    // queued in chunks (cg_chunk_len); the host reads the stop report
+    // between chunks and queues no more after a stop.  Full chunks are
+    // timed (bursts) and replay one captured graph.
     volatile CGMHostFlag *hf = ctx->h_flag;
-    const int CHUNK = 32;
     const int qmax = ctx->S.maxiter_coarse + 1;
     const bool use_graph = use_cg_graph();
     int q = 1, k = 0;
     while (q <= qmax) {
-        const int n = std::min(CHUNK, qmax - q + 1);
-        const int bev = burst_begin(ctx);
-        if (n == CHUNK && use_graph) {
-            // a full chunk starts at an odd q (CHUNK is even): the graph's parities
+        const int n = std::min(cg_chunk_len(k), qmax - q + 1);
+        const int bev = n == CG_CHUNK ? burst_begin(ctx) : -1;
+        if (n == CG_CHUNK && use_graph) {
+            // a full chunk starts at an odd q (chunk lengths are even): the graph's parities
             if (!ctx->cgm_chunk) {
                 std::lock_guard<std::mutex> lock(g_graph_mu);
                 hipGraph_t gr = nullptr;
                 HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
-                for (int s = 0; s < CHUNK; ++s) {
+                for (int s = 0; s < CG_CHUNK; ++s) {
                     if (s & 1)
                         launch_cgm_iter<0, false>(ctx, tb, ia);
                     else
@@ -1225,7 +1234,7 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
                 HIPCHK(e);
             }
             HIPCHK(hipGraphLaunch(ctx->cgm_chunk, ctx->stream));
-            q += CHUNK;
+            q += CG_CHUNK;
         } else {
             for (int s = 0; s < n; ++s, ++q) {
                 if (q & 1)
@@ -1237,8 +1246,8 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
         burst_end(ctx, bev, q - 1, n);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ctx->cg_ev[k % 3], ctx->stream));
-        if (k >= 2) {
-            HIPCHK(hipEventSynchronize(ctx->cg_ev[(k - 2) % 3]));
+        if (k >= 1) {
+            HIPCHK(hipEventSynchronize(ctx->cg_ev[(k - 1) % 3]));
             if (hf->done) break;
         }
         ++k;
@@ -1918,11 +1927,10 @@ static int lockstep_picard(std::vector<LockMember> &M, LockstepRes &R, int it, i
         launch_cgm_batch<0, true>(cs, levels, nwg, B);
         HIPCHK(hipGetLastError());
         // launch q performs iteration q - 1 of every member whose CG has not
-        // stopped; chunks of 32 with at most two in flight, as for one member
-        const int CHUNK = 32;
+        // stopped; chunks as for one member (cg_chunk_len)
         int q = 1, c = 0;
         while (q <= qmax) {
-            const int n = std::min(CHUNK, qmax - q + 1);
+            const int n = std::min(cg_chunk_len(c), qmax - q + 1);
             for (int s = 0; s < n; ++s, ++q) {
                 if (q & 1)
                     launch_cgm_batch<1, false>(cs, levels, nwg, B);
@@ -1931,8 +1939,8 @@ static int lockstep_picard(std::vector<LockMember> &M, LockstepRes &R, int it, i
             }
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(lead->cg_ev[c % 3], cs));
-            if (c >= 2) {
-                HIPCHK(hipEventSynchronize(lead->cg_ev[(c - 2) % 3]));
+            if (c >= 1) {
+                HIPCHK(hipEventSynchronize(lead->cg_ev[(c - 1) % 3]));
                 bool all = true;
                 for (int k : cgm) all = all && ((volatile CGMHostFlag *)M[k].ctx->h_flag)->done;
                 if (all) break;
