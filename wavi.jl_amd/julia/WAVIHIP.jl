@@ -231,12 +231,17 @@ function run_steps!(simulation, nsteps::Integer)
 end
 
 """
-    update_velocities_batch!(models; nthreads = 8)
+    update_velocities_batch!(models; nthreads = 8, lockstep = false)
 
 `update_velocities!` on every model of an ensemble in one native call (host
-threads, one stream per model).  No counterpart in WAVI.jl.
+threads, one stream per model; the small mixed-precision models as one
+lockstep group).  `lockstep = true` puts every model (all mixed precision)
+in one lockstep group: their coarse CGs share one launch per CG iteration.
+No counterpart in WAVI.jl.
 """
-function update_velocities_batch!(models::AbstractVector; nthreads::Integer = 8)
+function update_velocities_batch!(models::AbstractVector; nthreads::Integer = 8, lockstep::Bool = false)
+    length(unique(objectid.(models))) == length(models) ||
+        throw(ArgumentError("update_velocities_batch!: a model appears twice in the batch"))
     for m in models
         WAVI.update_preconditioner!(m)
         push_fields!(m)
@@ -244,11 +249,22 @@ function update_velocities_batch!(models::AbstractVector; nthreads::Integer = 8)
     ctxs = Ptr{Cvoid}[m.parallel_spec.ctx for m in models]
     reps = Vector{SolveReport}(undef, length(models))
     codes = zeros(Cint, length(models))
-    ccall((:wavi_update_velocities_batch, LIB), Cint,
-          (Ptr{Ptr{Cvoid}}, Cint, Cint, Ptr{SolveReport}, Ptr{Cint}),
-          ctxs, length(models), nthreads, reps, codes)
-    for (m, c) in zip(models, codes)
-        check(m.parallel_spec.ctx, c)
+    rc = if lockstep
+        ccall((:wavi_update_velocities_lockstep, LIB), Cint,
+              (Ptr{Ptr{Cvoid}}, Cint, Ptr{SolveReport}, Ptr{Cint}),
+              ctxs, length(models), reps, codes)
+    else
+        ccall((:wavi_update_velocities_batch, LIB), Cint,
+              (Ptr{Ptr{Cvoid}}, Cint, Cint, Ptr{SolveReport}, Ptr{Cint}),
+              ctxs, length(models), nthreads, reps, codes)
+    end
+    if rc != 0
+        for (m, c) in zip(models, codes)
+            check(m.parallel_spec.ctx, c)
+        end
+        check(C_NULL, rc)
+    end
+    for m in models
         pull_fields!(m)
     end
     return models
