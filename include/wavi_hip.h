@@ -237,7 +237,10 @@ int wavi_update_wavelets(wavi_ctx *ctx);
  * systems A_k lambda = ubar solved like the forward ones: A_k is symmetric).
  * u_obs: (nx+1)*ny, v_obs: nx*(ny+1).  The context is left in the state of
  * the forward's last Picard iteration.  Fixed velocities (u_isfixed,
- * v_isfixed) are rejected with WAVI_EARG. */
+ * v_isfixed) are rejected with WAVI_EARG.  The first call allocates a device
+ * workspace of (n_picard + 1) (N_u + N_v + nsigma N_h + 2 N_h) doubles plus a
+ * few vectors, kept with the context (grown when a later call needs more)
+ * until wavi_ctx_destroy. */
 typedef struct wavi_adjoint_report {
     int32_t n_picard;
     int32_t fwd_vcycles;   /* V-cycles of all forward linear solves */

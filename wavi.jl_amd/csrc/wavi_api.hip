@@ -61,6 +61,8 @@ struct wavi_ctx {
     uint8_t *ui = nullptr, *vi = nullptr, *ufix = nullptr, *vfix = nullptr, *mwu = nullptr, *mwv = nullptr;
     bool has_fixed = false;     // any u_isfixed / v_isfixed point
     std::vector<uint8_t> h_ui, h_vi;  // host copies of the inner masks
+    double *adj_base = nullptr;       // adjoint workspace, kept between gradient calls
+    size_t adj_cap = 0;               // its length in doubles
     // fields (double, device); mask fields use mwu/mwv instead
     double *f[WAVI_NFIELDS] = {};
     // sigma-level constants
@@ -687,6 +689,7 @@ static void ctx_free(wavi_ctx *ctx)
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->burst_ev) (void)hipEventDestroy(e);
     if (ctx->cgm_chunk) (void)hipGraphExecDestroy(ctx->cgm_chunk);
+    if (ctx->adj_base) (void)hipFree(ctx->adj_base);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1579,20 +1582,13 @@ static int do_linear_solve(wavi_ctx *ctx, const DevVec &b, const DevVec &x, cons
 
 // Device storage of the adjoint: checkpoints of (u, v, eta, beta, f2) for
 // npic + 1 states, the state adjoints and the per-cell work arrays.
+// Views into the context's adjoint workspace (ctx->adj_base).
 struct AdjStore {
-    double *base = nullptr;
     std::vector<double *> u, v, eta, beta, f2;
     double *ubar = nullptr, *vbar = nullptr, *lu = nullptr, *lv = nullptr;
     double *etab = nullptr, *betab = nullptr, *f2b = nullptr, *wcb = nullptr, *grad = nullptr;
     double *cub = nullptr, *cvb = nullptr, *uxb = nullptr, *vyb = nullptr, *shb = nullptr;
     LinScratch ls;
-    ~AdjStore()
-    {
-        if (base) {
-            std::lock_guard<std::mutex> lock(g_graph_mu);
-            (void)hipFree(base);
-        }
-    }
 };
 
 static int adj_alloc(wavi_ctx *ctx, int npic, AdjStore &A)
@@ -1601,11 +1597,18 @@ static int adj_alloc(wavi_ctx *ctx, int npic, AdjStore &A)
     const size_t per = nu + nv + ns3 + 2 * nh;
     const size_t npart = ctx->g.npart;
     const size_t total = per * (npic + 1) + 4 * (nu + nv) + ns3 + 4 * nh + 5 * nh + 2 * npart;
-    {
+    // The workspace is allocated on the first gradient call and kept with the
+    // context (grown when a call needs more): a free, which can wait for the
+    // whole device, would otherwise stall the other members of a batch.
+    if (ctx->adj_cap < total) {
         std::lock_guard<std::mutex> lock(g_graph_mu);
-        HIPCHK(hipMalloc(&A.base, total * sizeof(double)));
+        if (ctx->adj_base) HIPCHK(hipFree(ctx->adj_base));
+        ctx->adj_base = nullptr;
+        ctx->adj_cap = 0;
+        HIPCHK(hipMalloc(&ctx->adj_base, total * sizeof(double)));
+        ctx->adj_cap = total;
     }
-    double *q = A.base;
+    double *q = ctx->adj_base;
     auto take = [&](size_t n) {
         double *r = q;
         q += n;
