@@ -258,7 +258,12 @@ int wavi_misfit_gradient(wavi_ctx *ctx, int n_picard, double tol_linear, int max
  * observations; J (n entries) receives the misfits; grad_logc (NULL, or n
  * pointers of which any may be NULL) the gradients; reports and codes (each
  * NULL or n entries) the members' reports and return codes.  Returns 0 or
- * the first nonzero member code.  No reference counterpart. */
+ * the first nonzero member code.  The small mixed-precision members (as in
+ * wavi_update_velocities_batch) run on up to 32 threads of their own and
+ * meet at every coarse CG: when all of them that are running wait there,
+ * their CGs run as one batch (one launch per CG iteration over all their
+ * tiles); every member stays bit-identical to its own call.  No reference
+ * counterpart. */
 int wavi_misfit_gradient_batch(wavi_ctx **ctxs, int n, int nthreads, int n_picard, double tol_linear,
                                int maxit_linear, const double *const *u_obs, const double *const *v_obs,
                                double *J, double *const *grad_logc, wavi_adjoint_report *reports, int *codes);

@@ -4,11 +4,13 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <system_error>
@@ -63,6 +65,10 @@ struct wavi_ctx {
     std::vector<uint8_t> h_ui, h_vi;  // host copies of the inner masks
     double *adj_base = nullptr;       // adjoint workspace, kept between gradient calls
     size_t adj_cap = 0;               // its length in doubles
+    struct CgRendezvous *rdv = nullptr;  // set while a batch runs this context's coarse CGs with others'
+    int rdv_index = -1;               // the context's index in that rendezvous
+    bool rdv_done = false;            // its current coarse CG has been run
+    int rdv_rc = 0, rdv_iters = 0;    // that CG's return code and iteration count
     // fields (double, device); mask fields use mwu/mwv instead
     double *f[WAVI_NFIELDS] = {};
     // sigma-level constants
@@ -1197,9 +1203,12 @@ static int cgm_prepare(wavi_ctx *ctx, const VSet<float> &w)
 // iteration) on b_c held in v32.cr; solution in the warm start v32.xc.  The
 // CG works in the padded arena: k_cgm_pack fills the coefficients and masks,
 // k_cgm_setup the vectors, k_cgm_unpack returns the solution.
+static int rdv_coarse_cg(wavi_ctx *ctx, const VSet<float> &w, int *iters);
+
 template <>
 int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
 {
+    if (ctx->rdv) return rdv_coarse_cg(ctx, w, iters);
     const GridDims &g = ctx->g;
     dim3 tb(ctx->nblk);
     const CGMLayout Lo = cgm_layout(ctx);
@@ -1866,12 +1875,181 @@ struct LockstepRes {
     }
 };
 
+// Tables and events of a batch of the n members ctxs (tables sized for all
+// of them at once; n + 1 events).
+static int lockstep_res_alloc(LockstepRes &R, wavi_ctx *const *ctxs, int n)
+{
+    size_t ngroups = 0;
+    for (int k = 0; k < n; ++k) ngroups += (size_t)(ctxs[k]->nblk / 8);
+    wavi_ctx *ctx = ctxs[0];
+    {
+        std::lock_guard<std::mutex> lock(g_graph_mu);
+        HIPCHK(hipMalloc(&R.members, n * sizeof(CGMIterArgs)));
+        HIPCHK(hipMalloc(&R.group, ngroups * sizeof(int)));
+        HIPCHK(hipMalloc(&R.first, n * sizeof(int)));
+    }
+    R.ev.assign(n + 1, nullptr);
+    for (hipEvent_t &e : R.ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return WAVI_OK;
+}
+
 struct LockMember {
     wavi_ctx *ctx;
     wavi_solve_report rep;
     int it = 0, total = 0, converged = 0, cg = 0;
     double rel = INFINITY;
 };
+
+// Coarse CGs of the members mem (each set up by cgm_prepare on its own
+// stream and marked there by the event pre[j]) run together on the stream of
+// mem[0]: one k_cgm_iter_batch launch per CG iteration over the tiles of all
+// of them, in chunks as for one member (cg_chunk_len), until every member's
+// CG has stopped.  On return the CGs are complete, iters[j] holds member j's
+// iteration count, and every member's stream waits for the event done
+// before its next work.  On an error *bad names the member concerned.
+static int run_cg_batch(const std::vector<wavi_ctx *> &mem, const std::vector<hipEvent_t> &pre, hipEvent_t done,
+                        LockstepRes &R, std::vector<int> &iters, int *bad)
+{
+    wavi_ctx *lead = mem[0];
+    hipStream_t cs = lead->stream;
+    std::vector<CGMIterArgs> hm(mem.size());
+    std::vector<int> hf(mem.size()), hg;
+    int nwg = 0, qmax = 0;
+    for (size_t j = 0; j < mem.size(); ++j) {
+        *bad = (int)j;
+        wavi_ctx *ctx = mem[j];
+        if (j > 0) HIPCHK(hipStreamWaitEvent(cs, pre[j], 0));
+        hm[j] = cgm_iter_args(ctx);
+        hf[j] = nwg;
+        for (int q = 0; q < ctx->nblk / 8; ++q) hg.push_back((int)j);
+        nwg += ctx->nblk;
+        qmax = std::max(qmax, ctx->S.maxiter_coarse + 1);
+    }
+    *bad = 0;
+    wavi_ctx *ctx = lead;
+    HIPCHK(hipMemcpyAsync(R.members, hm.data(), hm.size() * sizeof(CGMIterArgs), hipMemcpyHostToDevice, cs));
+    HIPCHK(hipMemcpyAsync(R.group, hg.data(), hg.size() * sizeof(int), hipMemcpyHostToDevice, cs));
+    HIPCHK(hipMemcpyAsync(R.first, hf.data(), hf.size() * sizeof(int), hipMemcpyHostToDevice, cs));
+    const CGMBatch B{R.members, R.group, R.first};
+    const int levels = lead->S.levels;
+    launch_cgm_batch<0, true>(cs, levels, nwg, B);
+    HIPCHK(hipGetLastError());
+    // launch q performs iteration q - 1 of every member whose CG has not stopped
+    int q = 1, c = 0;
+    while (q <= qmax) {
+        const int n = std::min(cg_chunk_len(c), qmax - q + 1);
+        for (int s = 0; s < n; ++s, ++q) {
+            if (q & 1)
+                launch_cgm_batch<1, false>(cs, levels, nwg, B);
+            else
+                launch_cgm_batch<0, false>(cs, levels, nwg, B);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(lead->cg_ev[c % 3], cs));
+        if (c >= 1) {
+            HIPCHK(hipEventSynchronize(lead->cg_ev[(c - 1) % 3]));
+            bool all = true;
+            for (wavi_ctx *m : mem) all = all && ((volatile CGMHostFlag *)m->h_flag)->done;
+            if (all) break;
+        }
+        ++c;
+    }
+    HIPCHK(hipEventRecord(done, cs));
+    HIPCHK(hipStreamSynchronize(cs));
+    iters.assign(mem.size(), 0);
+    for (size_t j = 0; j < mem.size(); ++j) {
+        *bad = (int)j;
+        wavi_ctx *ctx = mem[j];
+        volatile CGMHostFlag *f = ctx->h_flag;
+        if (!f->done) {
+            ctx->err = "coarse CG did not terminate";
+            return WAVI_ESTATE;
+        }
+        iters[j] = f->iters;
+        if (j > 0) HIPCHK(hipStreamWaitEvent(ctx->stream, done, 0));
+    }
+    return WAVI_OK;
+}
+
+// A rendezvous of the coarse CGs of ensemble members that run concurrently,
+// each driven by its own thread.  A member whose V-cycle reaches its coarse
+// CG sets it up on its own stream and waits.  When every registered member
+// waits, the thread that completed the set runs all of their CGs as one
+// batch (run_cg_batch) and releases them.  Members register when a thread
+// starts them and leave when it is done with them, so the set always
+// completes: the last member to arrive, or to leave, runs the batch.
+struct CgRendezvous {
+    std::mutex mu;
+    std::condition_variable cv;
+    int active = 0;
+    std::vector<wavi_ctx *> pending;
+    LockstepRes res;  // tables for all members; ev[j]: member j's set-up, ev.back(): a batch's end
+};
+
+// Runs the waiting members' CGs; called with lk held, returns with it held.
+static void rdv_run(CgRendezvous &R, std::unique_lock<std::mutex> &lk)
+{
+    std::vector<wavi_ctx *> batch;
+    batch.swap(R.pending);
+    lk.unlock();
+    std::vector<hipEvent_t> pre;
+    for (wavi_ctx *c : batch) pre.push_back(R.res.ev[c->rdv_index]);
+    std::vector<int> iters;
+    int bad = 0, rc;
+    try {
+        rc = run_cg_batch(batch, pre, R.res.ev.back(), R.res, iters, &bad);
+    } catch (const std::exception &ex) {
+        batch[0]->err = std::string("host exception: ") + ex.what();
+        rc = WAVI_ESTATE;
+        bad = 0;
+    }
+    lk.lock();
+    for (size_t j = 0; j < batch.size(); ++j) {
+        wavi_ctx *c = batch[j];
+        c->rdv_iters = rc == WAVI_OK ? iters[j] : 0;
+        c->rdv_rc = rc == WAVI_OK ? WAVI_OK : ((int)j == bad ? rc : WAVI_ESTATE);
+        if (rc != WAVI_OK && (int)j != bad) c->err = "a batched coarse CG failed for another member";
+        c->rdv_done = true;
+    }
+    R.cv.notify_all();
+}
+
+static void rdv_join(wavi_ctx *ctx, CgRendezvous *R, int index)
+{
+    std::lock_guard<std::mutex> lk(R->mu);
+    ctx->rdv = R;
+    ctx->rdv_index = index;
+    ++R->active;
+}
+
+static void rdv_leave(wavi_ctx *ctx)
+{
+    CgRendezvous &R = *ctx->rdv;
+    std::unique_lock<std::mutex> lk(R.mu);
+    ctx->rdv = nullptr;
+    --R.active;
+    if (R.active > 0 && (int)R.pending.size() == R.active) rdv_run(R, lk);
+}
+
+static int rdv_coarse_cg(wavi_ctx *ctx, const VSet<float> &w, int *iters)
+{
+    CHK(cgm_prepare(ctx, w));
+    CgRendezvous &R = *ctx->rdv;
+    HIPCHK(hipEventRecord(R.res.ev[ctx->rdv_index], ctx->stream));
+    {
+        std::unique_lock<std::mutex> lk(R.mu);
+        ctx->rdv_done = false;
+        R.pending.push_back(ctx);
+        if ((int)R.pending.size() == R.active) rdv_run(R, lk);
+        R.cv.wait(lk, [&] { return ctx->rdv_done; });
+    }
+    CHK(ctx->rdv_rc);
+    hipLaunchKernelGGL(k_cgm_unpack, flat(std::max(ctx->nu, ctx->nv)), dim3(256), 0, ctx->stream, ctx->g,
+                       cgm_layout(ctx), w.xc.u, w.xc.v);
+    HIPCHK(hipGetLastError());
+    *iters = ctx->rdv_iters;
+    return WAVI_OK;
+}
 
 // One Picard iteration of every member that still iterates: each member's
 // inner update and residual on its own stream; then, for the members whose
@@ -1900,69 +2078,30 @@ static int lockstep_picard(std::vector<LockMember> &M, LockstepRes &R, int it, i
         if (!M[k].converged) cgm.push_back(k);
     }
     if (!cgm.empty()) {
-        wavi_ctx *lead = M[0].ctx;
-        hipStream_t cs = lead->stream;
-        std::vector<CGMIterArgs> hm(cgm.size());
-        std::vector<int> hf(cgm.size()), hg;
-        int nwg = 0, qmax = 0;
-        for (size_t j = 0; j < cgm.size(); ++j) {
-            const int k = cgm[j];
+        std::vector<wavi_ctx *> mem;
+        std::vector<hipEvent_t> pre;
+        for (int k : cgm) {
             *bad = k;
             wavi_ctx *ctx = M[k].ctx;
             CHK(vcycle_narrow(ctx));
             CHK(vcycle_pre(ctx, ctx->v32));
             CHK(cgm_prepare(ctx, ctx->v32));
             HIPCHK(hipEventRecord(R.ev[k], ctx->stream));
-            if (ctx != lead) HIPCHK(hipStreamWaitEvent(cs, R.ev[k], 0));
-            hm[j] = cgm_iter_args(ctx);
-            hf[j] = nwg;
-            for (int q = 0; q < ctx->nblk / 8; ++q) hg.push_back((int)j);
-            nwg += ctx->nblk;
-            qmax = std::max(qmax, ctx->S.maxiter_coarse + 1);
+            mem.push_back(ctx);
+            pre.push_back(R.ev[k]);
         }
-        *bad = 0;
-        wavi_ctx *ctx = lead;
-        HIPCHK(hipMemcpyAsync(R.members, hm.data(), hm.size() * sizeof(CGMIterArgs), hipMemcpyHostToDevice, cs));
-        HIPCHK(hipMemcpyAsync(R.group, hg.data(), hg.size() * sizeof(int), hipMemcpyHostToDevice, cs));
-        HIPCHK(hipMemcpyAsync(R.first, hf.data(), hf.size() * sizeof(int), hipMemcpyHostToDevice, cs));
-        const CGMBatch B{R.members, R.group, R.first};
-        const int levels = lead->S.levels;
-        launch_cgm_batch<0, true>(cs, levels, nwg, B);
-        HIPCHK(hipGetLastError());
-        // launch q performs iteration q - 1 of every member whose CG has not
-        // stopped; chunks as for one member (cg_chunk_len)
-        int q = 1, c = 0;
-        while (q <= qmax) {
-            const int n = std::min(cg_chunk_len(c), qmax - q + 1);
-            for (int s = 0; s < n; ++s, ++q) {
-                if (q & 1)
-                    launch_cgm_batch<1, false>(cs, levels, nwg, B);
-                else
-                    launch_cgm_batch<0, false>(cs, levels, nwg, B);
-            }
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipEventRecord(lead->cg_ev[c % 3], cs));
-            if (c >= 1) {
-                HIPCHK(hipEventSynchronize(lead->cg_ev[(c - 1) % 3]));
-                bool all = true;
-                for (int k : cgm) all = all && ((volatile CGMHostFlag *)M[k].ctx->h_flag)->done;
-                if (all) break;
-            }
-            ++c;
+        std::vector<int> iters;
+        int j = 0;
+        const int rc = run_cg_batch(mem, pre, R.ev[M.size()], R, iters, &j);
+        if (rc != WAVI_OK) {
+            *bad = cgm[j];
+            return rc;
         }
-        const int ndone = (int)M.size();
-        HIPCHK(hipEventRecord(R.ev[ndone], cs));
-        HIPCHK(hipStreamSynchronize(cs));
-        for (int k : cgm) {
+        for (size_t jj = 0; jj < cgm.size(); ++jj) {
+            const int k = cgm[jj];
             *bad = k;
             wavi_ctx *ctx = M[k].ctx;
-            volatile CGMHostFlag *f = ctx->h_flag;
-            if (!f->done) {
-                ctx->err = "coarse CG did not terminate";
-                return WAVI_ESTATE;
-            }
-            M[k].cg = f->iters;
-            if (ctx != lead) HIPCHK(hipStreamWaitEvent(ctx->stream, R.ev[ndone], 0));
+            M[k].cg = iters[jj];
             const VSet<float> &w = ctx->v32;
             hipLaunchKernelGGL(k_cgm_unpack, flat(std::max(ctx->nu, ctx->nv)), dim3(256), 0, ctx->stream, ctx->g,
                                cgm_layout(ctx), w.xc.u, w.xc.v);
@@ -2032,26 +2171,12 @@ int wavi_update_velocities_lockstep(wavi_ctx **ctxs, int n, wavi_solve_report *r
         return WAVI_EHIP;
     }
     std::vector<LockMember> M(n);
-    size_t ngroups = 0;
     for (int k = 0; k < n; ++k) {
         M[k].ctx = ctxs[k];
         memset(&M[k].rep, 0, sizeof(M[k].rep));
-        ngroups += (size_t)(ctxs[k]->nblk / 8);
     }
     LockstepRes R;
-    auto setup = [&]() -> int {
-        wavi_ctx *ctx = ctxs[0];
-        {
-            std::lock_guard<std::mutex> lock(g_graph_mu);
-            HIPCHK(hipMalloc(&R.members, n * sizeof(CGMIterArgs)));
-            HIPCHK(hipMalloc(&R.group, ngroups * sizeof(int)));
-            HIPCHK(hipMalloc(&R.first, n * sizeof(int)));
-        }
-        R.ev.assign(n + 1, nullptr);
-        for (hipEvent_t &e : R.ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        return WAVI_OK;
-    };
-    int rc = setup(), bad = 0;
+    int rc = lockstep_res_alloc(R, ctxs, n), bad = 0;
     for (int it = 1; rc == WAVI_OK; ++it) {
         const int r = lockstep_picard(M, R, it, &bad);
         if (r < 0) rc = r;
@@ -2088,6 +2213,22 @@ static size_t arena_bytes(const wavi_ctx *ctx)
     return (size_t)CGM_NARRAY * ctx->ns * sizeof(float) + 2 * (size_t)ctx->ns;
 }
 
+// The members of a batch that share coarse-CG launches, in member order.
+static std::vector<int> lock_group(wavi_ctx *const *ctxs, int n)
+{
+    std::vector<int> small;
+    size_t bytes = 0;
+    for (int k = 0; k < n; ++k) {
+        const wavi_ctx *c = ctxs[k], *ref = small.empty() ? c : ctxs[small[0]];
+        if (c->precision == WAVI_PRECISION_MIXED && c->g.nbx * c->g.nby <= LOCK_TILES && c->device == ref->device &&
+            c->S.levels == ref->S.levels && bytes + arena_bytes(c) <= LOCK_ARENA_BYTES) {
+            small.push_back(k);
+            bytes += arena_bytes(c);
+        }
+    }
+    return small;
+}
+
 int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,
                                  wavi_solve_report *reports, int *codes)
 {
@@ -2095,20 +2236,9 @@ int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,
     for (int k = 0; k < n; ++k)
         if (!ctxs[k]) return WAVI_EARG;
     if (batch_has_duplicates(ctxs, n)) return WAVI_EARG;
-    std::vector<int> small, large;
-    size_t bytes = 0;
-    for (int k = 0; k < n; ++k) {
-        const wavi_ctx *c = ctxs[k], *ref = small.empty() ? c : ctxs[small[0]];
-        const bool fits = c->precision == WAVI_PRECISION_MIXED && c->g.nbx * c->g.nby <= LOCK_TILES &&
-                          c->device == ref->device && c->S.levels == ref->S.levels &&
-                          bytes + arena_bytes(c) <= LOCK_ARENA_BYTES;
-        if (fits) {
-            small.push_back(k);
-            bytes += arena_bytes(c);
-        } else {
-            large.push_back(k);
-        }
-    }
+    std::vector<int> small = lock_group(ctxs, n), large;
+    for (int k = 0; k < n; ++k)
+        if (std::find(small.begin(), small.end(), k) == small.end()) large.push_back(k);
     if (small.size() < 2) {
         return run_members(ctxs, n, nthreads, codes, [&](wavi_ctx *ctx, int k) {
             return wavi_update_velocities(ctx, reports ? &reports[k] : nullptr);
@@ -2185,10 +2315,76 @@ int wavi_misfit_gradient_batch(wavi_ctx **ctxs, int n, int nthreads, int n_picar
     for (int k = 0; k < n; ++k)
         if (!ctxs[k]) return WAVI_EARG;
     if (batch_has_duplicates(ctxs, n)) return WAVI_EARG;
-    return run_members(ctxs, n, nthreads, codes, [&](wavi_ctx *ctx, int k) {
-        return wavi_misfit_gradient(ctx, n_picard, tol_linear, maxit_linear, u_obs[k], v_obs[k], &J[k],
-                                    grad_logc ? grad_logc[k] : nullptr, reports ? &reports[k] : nullptr);
-    });
+    // the small mixed-precision members meet at every coarse CG, so the
+    // members that run at the same time share the CG launches
+    const std::vector<int> small = lock_group(ctxs, n);
+    std::unique_ptr<CgRendezvous> R;
+    std::vector<int> slot(n, -1);
+    if (small.size() >= 2) {
+        R.reset(new CgRendezvous);
+        std::vector<wavi_ctx *> sc;
+        for (size_t j = 0; j < small.size(); ++j) {
+            slot[small[j]] = (int)j;
+            sc.push_back(ctxs[small[j]]);
+        }
+        if (lockstep_res_alloc(R->res, sc.data(), (int)sc.size()) != WAVI_OK) R.reset();  // then alone
+    }
+    auto gradient = [&](wavi_ctx *ctx, int k) {
+        const bool meet = R && slot[k] >= 0;
+        if (meet) rdv_join(ctx, R.get(), slot[k]);
+        int rc;
+        try {
+            rc = wavi_misfit_gradient(ctx, n_picard, tol_linear, maxit_linear, u_obs[k], v_obs[k], &J[k],
+                                      grad_logc ? grad_logc[k] : nullptr, reports ? &reports[k] : nullptr);
+        } catch (const std::exception &ex) {
+            ctx->err = std::string("host exception: ") + ex.what();
+            rc = WAVI_ESTATE;
+        }
+        if (meet) rdv_leave(ctx);
+        return rc;
+    };
+    if (!R) return run_members(ctxs, n, nthreads, codes, gradient);
+    // The small members get threads of their own, up to RDV_THREADS, beside
+    // the nthreads that take the others: more of them then meet at each
+    // coarse CG (a member waiting there holds its thread but no GPU time).
+    constexpr int RDV_THREADS = 32;
+    std::vector<int> large;
+    for (int k = 0; k < n; ++k)
+        if (slot[k] < 0) large.push_back(k);
+    std::vector<wavi_ctx *> sc, lc;
+    for (int k : small) sc.push_back(ctxs[k]);
+    for (int k : large) lc.push_back(ctxs[k]);
+    std::vector<int> scode(sc.size(), WAVI_OK), lcode(lc.size(), WAVI_OK);
+    int src = WAVI_OK, lrc = WAVI_OK;
+    auto run_small = [&]() {
+        src = run_members(sc.data(), (int)sc.size(), std::min(RDV_THREADS, (int)sc.size()), scode.data(),
+                          [&](wavi_ctx *ctx, int j) { return gradient(ctx, small[j]); });
+    };
+    auto run_large = [&]() {
+        lrc = run_members(lc.data(), (int)lc.size(), nthreads, lcode.data(),
+                          [&](wavi_ctx *ctx, int j) { return gradient(ctx, large[j]); });
+    };
+    std::thread t;
+    if (!lc.empty()) {
+        try {
+            t = std::thread(run_large);
+        } catch (const std::system_error &) {
+        }
+    }
+    run_small();
+    if (t.joinable())
+        t.join();
+    else
+        run_large();
+    for (size_t j = 0; j < small.size(); ++j)
+        if (codes) codes[small[j]] = scode[j];
+    for (size_t j = 0; j < large.size(); ++j)
+        if (codes) codes[large[j]] = lcode[j];
+    for (int k = 0; k < n; ++k) {
+        const int c = slot[k] >= 0 ? scode[slot[k]] : lcode[std::find(large.begin(), large.end(), k) - large.begin()];
+        if (c != WAVI_OK) return c;
+    }
+    return src != WAVI_OK ? src : lrc;
 }
 
 int wavi_inner_update(wavi_ctx *ctx)
