@@ -164,3 +164,33 @@ def test_lockstep_matches_individual_solves(W):
     np.testing.assert_array_equal(alone[0].fields.gu.u, one[0].fields.gu.u)
     with pytest.raises(W.ArgumentError):
         W.update_velocities_batch(_members(W, specs[:2], "fp64"), lockstep=True)
+
+
+def test_gradient_batch_two_pools(W):
+    """Small mixed members meet at their coarse CGs on threads of their own
+    while an FP64 member (not eligible to meet) runs on the other pool: all
+    bit-identical to their own calls."""
+    specs = _small_specs(W, 4, seed=13)
+    precs = ["mixed", "fp64", "mixed", "mixed"]
+    _step("ensemble gradients, two pools: individual calls")
+
+    def members():
+        out = []
+        for sp, p in zip(specs, precs):
+            m = W.workloads.ensemble_member(sp, p, maxiter_picard=2)
+            W.update_geometry(m)
+            out.append(m)
+        return out
+
+    obs = []
+    for m, sp in zip(members(), specs):
+        nx, ny = sp["nx"], sp["ny"]
+        W.misfit_gradient(m, 2, np.zeros((nx + 1, ny)), np.zeros((nx, ny + 1)), gradient=False)
+        obs.append(W.workloads.inversion_observations(m.fields.gu.u, m.fields.gv.v, seed=sp["seed"]))
+    alone = [W.misfit_gradient(m, 2, uo, vo) for m, (uo, vo) in zip(members(), obs)]
+    _step("ensemble gradients, two pools: one batched call")
+    batch = W.misfit_gradient_batch(members(), 2, obs, nthreads=2)
+    for (Ja, ga, ra), (Jb, gb, rb) in zip(alone, batch):
+        assert Ja == Jb
+        np.testing.assert_array_equal(ga, gb)
+        assert ra == rb
