@@ -194,3 +194,32 @@ def test_gradient_batch_two_pools(W):
         assert Ja == Jb
         np.testing.assert_array_equal(ga, gb)
         assert ra == rb
+
+
+def test_gradient_batch_more_members_than_rendezvous_threads(W):
+    """36 small members: more than the 32 threads the meeting members get, so
+    threads leave the rendezvous with one member and join with the next."""
+    specs = [sp for sp in W.workloads.ensemble_specs(512, 17) if sp["nx"] == 160][:36]
+    assert len(specs) == 36
+    _step("ensemble gradients: 36 small members, individual calls")
+
+    def members():
+        out = []
+        for sp in specs:
+            m = W.workloads.ensemble_member(sp, "mixed", maxiter_picard=2)
+            W.update_geometry(m)
+            out.append(m)
+        return out
+
+    obs = []
+    for m, sp in zip(members(), specs):
+        nx, ny = sp["nx"], sp["ny"]
+        W.misfit_gradient(m, 2, np.zeros((nx + 1, ny)), np.zeros((nx, ny + 1)), gradient=False)
+        obs.append(W.workloads.inversion_observations(m.fields.gu.u, m.fields.gv.v, seed=sp["seed"]))
+    alone = [W.misfit_gradient(m, 2, uo, vo) for m, (uo, vo) in zip(members(), obs)]
+    _step("ensemble gradients: 36 small members, one batched call")
+    batch = W.misfit_gradient_batch(members(), 2, obs, nthreads=4)
+    for (Ja, ga, ra), (Jb, gb, rb) in zip(alone, batch):
+        assert Ja == Jb
+        np.testing.assert_array_equal(ga, gb)
+        assert ra == rb
