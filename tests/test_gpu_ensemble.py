@@ -223,3 +223,28 @@ def test_gradient_batch_more_members_than_rendezvous_threads(W):
         assert Ja == Jb
         np.testing.assert_array_equal(ga, gb)
         assert ra == rb
+
+
+@pytest.mark.parametrize("levels", [0, 1, 2])
+def test_lockstep_other_haar_levels(W, levels):
+    """k_cgm_iter_batch at Haar levels 0..2 (blocks of 2, 4, 8): lockstep
+    members bit-identical to their own solves."""
+    specs = _small_specs(W, 3, seed=21)
+    _step("lockstep levels %d" % levels)
+
+    def members():
+        ms = _members(W, specs, "mixed")
+        for m in ms:
+            m.solver_params.levels = levels
+            m.solver_params.maxiter_picard = 3
+        return ms
+
+    alone = members()
+    for m in alone:
+        W.update_velocities(m)
+    lock = members()
+    W.update_velocities_batch(lock, lockstep=True)
+    for a, b in zip(alone, lock):
+        assert a.last_report["coarse_cg_iters"] == b.last_report["coarse_cg_iters"]
+        np.testing.assert_array_equal(a.fields.gu.u, b.fields.gu.u)
+        np.testing.assert_array_equal(a.fields.gv.v, b.fields.gv.v)
