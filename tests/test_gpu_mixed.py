@@ -68,7 +68,7 @@ def test_mixed_vcycle_close_to_fp64(W, shape):
         assert rel_l2(e32, e64) <= 1e-2, (k, rel_l2(e32, e64))
 
 
-@pytest.mark.parametrize("shape", [(96, 24), (320, 40)])
+@pytest.mark.parametrize("shape", [(96, 24), (320, 40), (63, 31)])
 def test_mixed_update_velocities_within_1e6(W, shape):
     nx, ny = shape
     _step("mixed update_velocities %s" % (shape,))

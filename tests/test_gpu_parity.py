@@ -152,7 +152,7 @@ def test_vcycle(W, oracle):
     _assert_fields_equal(W, m, o, names={"WU_CORRECTION_COARSE", "WV_CORRECTION_COARSE"})
 
 
-@pytest.mark.parametrize("shape,picard", [((48, 12), 4), ((40, 24), 3)])
+@pytest.mark.parametrize("shape,picard", [((48, 12), 4), ((40, 24), 3), ((47, 15), 3)])
 def test_update_velocities(W, oracle, shape, picard):
     nx, ny = shape
     _step("update_velocities %s" % (shape,))
