@@ -213,6 +213,25 @@ def test_channel_transient_200_steps(W, oracle):
     _assert_fields_equal(W, m, o)
 
 
+def test_transient_with_fixed_thickness(W, oracle):
+    """update_thickness! with h_isfixed (run_simulation.jl:24-42): 30 steps
+    of the channel with the upstream column and an interior patch fixed:
+    bit-equal to the oracle, and the fixed cells keep their initial 100 m."""
+    _step("channel transient with fixed thickness")
+    m = W.workloads.channel_transient()
+    m.grid.h_isfixed[0, :] = True
+    m.grid.h_isfixed[30:34, 4:6] = True
+    sim = W.Simulation(model=m, timestepping_params=W.TimesteppingParams(dt=0.5, n_iter_total=30))
+    o = oracle.OracleModel(m)
+    W.run_simulation(sim)
+    o.run_steps(30)
+    _assert_fields_equal(W, m, o)
+    fixed = m.grid.h_isfixed & m.fields.gh.mask
+    assert fixed.sum() == 18
+    np.testing.assert_array_equal(m.fields.gh.h[fixed], 100.0)
+    assert np.any(m.fields.gh.h[~fixed & m.fields.gh.mask] != 100.0)
+
+
 @pytest.mark.parametrize("levels", [0, 1, 2])
 def test_update_velocities_other_haar_levels(W, oracle, levels):
     """Haar levels 0..2 (blocks of 2, 4, 8): bit-equal to the oracle."""

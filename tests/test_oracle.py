@@ -302,3 +302,20 @@ def test_butterfly_haar_matches_wavelet_matrix(oracle, levels):
         x = rng.integers(-8, 9, n).astype(float)  # small integers: sums are exact in any order
         assert np.array_equal(_butterfly(x, levels, False), Wi @ x), n
         assert np.array_equal(_butterfly(x, levels, True), Wi.T @ x), n
+
+
+def test_fixed_thickness_cells_keep_their_thickness(W, oracle):
+    # update_thickness! (src/Simulations/run_simulation.jl:24-42) zeroes the
+    # update on h_isfixed cells: over 10 steps of the channel they keep their
+    # initial 100 m while the free cells evolve
+    m = W.workloads.channel_transient()
+    m.grid.h_isfixed[0, :] = True
+    m.grid.h_isfixed[30:34, 4:6] = True
+    m.params.dt = 0.5
+    o = oracle.OracleModel(m)
+    o.set_dt(0.5)
+    o.run_steps(10)
+    h = o.field("H_H").reshape((80, 10), order="F")
+    fixed = m.grid.h_isfixed
+    np.testing.assert_array_equal(h[fixed], 100.0)
+    assert np.all(h[~fixed] != 100.0)
