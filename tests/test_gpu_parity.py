@@ -232,6 +232,20 @@ def test_transient_with_fixed_thickness(W, oracle):
     assert np.any(m.fields.gh.h[~fixed & m.fields.gh.mask] != 100.0)
 
 
+def test_transient_other_implicitness(W, oracle):
+    """super_implicitness 0.6 (the theta of the implicit thickness term in
+    the operator and the right-hand side): 20 channel steps bit-equal to the
+    oracle."""
+    _step("channel transient, super_implicitness 0.6")
+    m = W.workloads.channel_transient()
+    m.solver_params.super_implicitness = 0.6
+    sim = W.Simulation(model=m, timestepping_params=W.TimesteppingParams(dt=0.5, n_iter_total=20))
+    o = oracle.OracleModel(m)
+    W.run_simulation(sim)
+    o.run_steps(20)
+    _assert_fields_equal(W, m, o)
+
+
 @pytest.mark.parametrize("levels", [0, 1, 2])
 def test_update_velocities_other_haar_levels(W, oracle, levels):
     """Haar levels 0..2 (blocks of 2, 4, 8): bit-equal to the oracle."""

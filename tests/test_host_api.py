@@ -172,3 +172,17 @@ def test_batch_and_gradient_argument_checks_without_gpu(W):
     J = ctypes.c_double()
     assert lib.wavi_misfit_gradient(None, 3, 1e-13, 60, None, None, ctypes.byref(J), None, None) == EARG
     assert lib.wavi_after_solve(None) == EARG
+
+
+def test_params_reject_unknown_fields(W):
+    """As Julia structs do: a misspelt or misplaced field is an error, not a
+    silently ignored new attribute (super_implicitness is a solver
+    parameter, src/SolverParams.jl:2-14)."""
+    p, sp = W.Params(), W.SolverParams()
+    with pytest.raises(AttributeError):
+        p.super_implicitness = 0.6
+    with pytest.raises(AttributeError):
+        sp.tol_coarses = 1e-6
+    sp.super_implicitness = 0.6
+    p.dt = 0.5
+    assert sp.super_implicitness == 0.6 and p.dt == 0.5

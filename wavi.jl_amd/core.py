@@ -112,8 +112,19 @@ class Grid:
         self.quadrature_weights = quadrature_weights
 
 
+class _Sealed:
+    """Fields fixed at construction, as for a Julia struct: assigning a name
+    that is not a field raises AttributeError instead of adding it."""
+    _sealed = False
+
+    def __setattr__(self, name, value):
+        if self._sealed and not hasattr(self, name):
+            raise AttributeError("%s has no field %s" % (type(self).__name__, name))
+        object.__setattr__(self, name, value)
+
+
 @dataclass
-class SolverParams:
+class SolverParams(_Sealed):
     """SolverParams (src/SolverParams.jl:2-14)."""
     n_iter_viscosity: int = 2
     tol_picard: float = 1e-5
@@ -130,9 +141,10 @@ class SolverParams:
     def __post_init__(self):
         if self.n_iter_viscosity != 2:
             raise AssertionError("n_iter_viscosity == 2")
+        self._sealed = True
 
 
-class Params:
+class Params(_Sealed):
     """Params(; ...) (src/Params.jl:280-331). dt defaults to 1.0 and is set
     by Simulation."""
 
@@ -167,6 +179,7 @@ class Params:
         self.minimum_thickness = minimum_thickness
         self.evolveShelves = evolveShelves
         self.smallHAF = smallHAF
+        self._sealed = True
 
 
 class InitialConditions:
