@@ -319,3 +319,29 @@ def test_fixed_thickness_cells_keep_their_thickness(W, oracle):
     fixed = m.grid.h_isfixed
     np.testing.assert_array_equal(h[fixed], 100.0)
     assert np.all(h[~fixed] != 100.0)
+
+
+def test_implicitness_enters_affinely(W, oracle):
+    # theta (super_implicitness) multiplies dt dsdh in the implicit thickness
+    # term of the operator (update_velocities.jl:381) and of the right-hand
+    # side (:110), and nothing else depends on it: both are affine in theta,
+    # so at theta = 0.6 they equal 0.4 x (theta = 0) + 0.6 x (theta = 1)
+    nx, ny = 12, 6
+    rng = np.random.default_rng(5)
+    xu, xv = rng.standard_normal((nx + 1, ny)), rng.standard_normal((nx, ny + 1))
+    out = {}
+    for th in (0.0, 0.6, 1.0):
+        m = channel(W, nx=nx, ny=ny)
+        m.solver_params.super_implicitness = th
+        m.params.dt = 2.0
+        m.params.accumulation_rate = 0.3 * np.ones((nx, ny))
+        o = oracle.OracleModel(m)
+        o.set_dt(2.0)
+        o.update_geometry()
+        o.inner_update()
+        out[th] = o.op_apply(xu, xv) + o.rhs()
+    for k in range(4):  # operator u, v, then right-hand side u, v
+        a0, a6, a1 = (out[t][k] for t in (0.0, 0.6, 1.0))
+        scale = max(np.abs(a0).max(), np.abs(a1).max())
+        assert np.abs(a1 - a0).max() > 1e-6 * scale  # theta does enter
+        assert np.abs(a6 - (0.4 * a0 + 0.6 * a1)).max() <= 1e-10 * scale
