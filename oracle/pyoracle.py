@@ -43,6 +43,8 @@ def load():
         "om_rhs": (None, [vp, _dp, _dp]),
         "om_op_diag": (None, [vp, _dp, _dp]),
         "om_vcycle": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp]),
+        "om_coarse_op": (None, [vp, _dp, _dp, _dp, _dp]),
+        "om_coarse_cg": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int]),
         "om_precondition": (ctypes.c_int, [vp, _dp, ctypes.POINTER(ctypes.c_int)]),
         "om_update_velocities": (ctypes.c_int, [vp, vp]),
         "om_update_state": (None, [vp, vp]),
@@ -175,6 +177,20 @@ class OracleModel:
         bv = np.empty(self.nx * (self.ny + 1))
         self._lib.om_rhs(self.h, _d(bu), _d(bv))
         return self._shape(bu, "u"), self._shape(bv, "v")
+
+    def coarse_op(self, yu, yv):
+        au, av = self._u(yu), self._u(yv)
+        ou, ov = np.empty_like(au), np.empty_like(av)
+        self._lib.om_coarse_op(self.h, _d(au), _d(av), _d(ou), _d(ov))
+        return self._shape(ou, "u"), self._shape(ov, "v")
+
+    def coarse_cg(self, bu, bv, xu, xv, tol_coarse, maxiter):
+        """The coarse CG alone from the warm start (xu, xv): returns
+        (iterations, xu, xv)."""
+        b1, b2 = self._u(bu), self._u(bv)
+        x1, x2 = self._u(xu).copy(), self._u(xv).copy()
+        it = self._lib.om_coarse_cg(self.h, _d(b1), _d(b2), _d(x1), _d(x2), float(tol_coarse), int(maxiter))
+        return it, self._shape(x1, "u"), self._shape(x2, "v")
 
     def op_diag(self):
         du = np.empty((self.nx + 1) * self.ny)

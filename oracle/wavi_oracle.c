@@ -978,6 +978,27 @@ static int coarse_cg(om_model *m, double *xu, double *xv, const double *bu, cons
     return it;
 }
 
+/* Test hooks: the coarse operator A_c, and the coarse CG alone with a given
+ * tolerance and iteration cap (x: the warm start on entry, the solution on
+ * return; returns the iterations). */
+EXPORT void om_coarse_op(om_model *m, const double *yu, const double *yv, double *ou, double *ov)
+{
+    coarse_op(m, yu, yv, ou, ov);
+}
+
+EXPORT int om_coarse_cg(om_model *m, const double *bu, const double *bv, double *xu, double *xv, double tol_coarse,
+                        int maxiter)
+{
+    const double t = m->S.tol_coarse;
+    const int mi = m->S.maxiter_coarse;
+    m->S.tol_coarse = tol_coarse;
+    m->S.maxiter_coarse = maxiter;
+    const int it = coarse_cg(m, xu, xv, bu, bv);
+    m->S.tol_coarse = t;
+    m->S.maxiter_coarse = mi;
+    return it;
+}
+
 /* precondition!(x, p, b) (:54-83) with x = 0 on entry. */
 EXPORT int om_vcycle(om_model *m, const double *bu, const double *bv, double *xu, double *xv)
 {

@@ -345,3 +345,67 @@ def test_implicitness_enters_affinely(W, oracle):
         scale = max(np.abs(a0).max(), np.abs(a1).max())
         assert np.abs(a1 - a0).max() > 1e-6 * scale  # theta does enter
         assert np.abs(a6 - (0.4 * a0 + 0.6 * a1)).max() <= 1e-10 * scale
+
+
+def _textbook_cg(A, b, x0, tol_coarse, maxiter):
+    # Hestenes-Stiefel CG with the stop rule IterativeSolvers documents for
+    # cg!: iterate while ||r|| > max(reltol ||r0||, abstol) and fewer than
+    # maxiter iterations, reltol = sqrt(eps), abstol = tol_coarse ||b||
+    x = x0.copy()
+    r = b - A @ x
+    tol = max(math.sqrt(np.finfo(float).eps) * np.linalg.norm(r), tol_coarse * np.linalg.norm(b))
+    p = np.zeros_like(b)
+    rr_old, it = 1.0, 0
+    while np.linalg.norm(r) > tol and it < maxiter:
+        rr = r @ r
+        p = r + (rr / rr_old) * p
+        q = A @ p
+        a = rr / (p @ q)
+        x += a * p
+        r -= a * q
+        rr_old = rr
+        it += 1
+    return it, x
+
+
+def test_coarse_cg_follows_the_documented_cg(W, oracle):
+    # The coarse solve (update_preconditioners.jl:73 calls IterativeSolvers'
+    # cg!, which is not vendored) against a textbook CG on the assembled
+    # coarse operator A_c = R A P: the absolute-tolerance stop, the
+    # relative-tolerance stop, the iteration cap and the warm start
+    nx, ny = 12, 6
+    m = channel(W, nx=nx, ny=ny)
+    o = oracle.OracleModel(m)
+    o.update_geometry()
+    o.inner_update()
+    nu, nv = (nx + 1) * ny, nx * (ny + 1)
+    n = nu + nv
+    split = lambda z: (z[:nu].reshape((nx + 1, ny), order="F"), z[nu:].reshape((nx, ny + 1), order="F"))  # noqa: E731
+    join = lambda a, b: np.r_[a.ravel(order="F"), b.ravel(order="F")]  # noqa: E731
+    A = np.zeros((n, n))
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = 1.0
+        A[:, k] = join(*o.coarse_op(*split(e)))
+    assert np.abs(A - A.T).max() <= 1e-10 * np.abs(A).max()
+    rng = np.random.default_rng(8)
+    b = A @ rng.standard_normal(n)  # in the range of A_c
+    x0 = 0.1 * rng.standard_normal(n)
+    for tol_coarse, maxiter in ((1e-3, 1000), (0.0, 1000), (0.0, 5)):
+        it, xu, xv = o.coarse_cg(*split(b), *split(x0), tol_coarse, maxiter)
+        it_ref, x_ref = _textbook_cg(A, b, x0, tol_coarse, maxiter)
+        x = join(xu, xv)
+        if maxiter == 5:
+            assert it == it_ref == 5
+            np.testing.assert_allclose(x, x_ref, rtol=0, atol=1e-10 * np.abs(x_ref).max())
+        else:
+            # rounding makes the two drift apart after many iterations on
+            # this small, ill-conditioned coarse system: within 3%
+            assert abs(it - it_ref) <= max(2, 0.03 * it_ref) and it < maxiter, (tol_coarse, it, it_ref)
+            assert np.linalg.norm(b - A @ x) <= 1.01 * max(1.5e-8 * np.linalg.norm(b - A @ x0),
+                                                           tol_coarse * np.linalg.norm(b))
+    # a warm start that already meets the tolerance: no iteration, x unchanged
+    _, x_ref = _textbook_cg(A, b, x0, 0.0, 1000)
+    it, xu, xv = o.coarse_cg(*split(b), *split(x_ref), 1e-6, 1000)
+    assert it == 0
+    np.testing.assert_array_equal(join(xu, xv), x_ref)
