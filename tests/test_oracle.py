@@ -409,3 +409,51 @@ def test_coarse_cg_follows_the_documented_cg(W, oracle):
     it, xu, xv = o.coarse_cg(*split(b), *split(x_ref), 1e-6, 1000)
     assert it == 0
     np.testing.assert_array_equal(join(xu, xv), x_ref)
+
+
+def test_viscosity_closed_form_on_a_floating_slab(W, oracle):
+    # inner_update_viscosity! (update_velocities.jl:264-283) on a floating
+    # slab (grounded fraction 0, so no basal drag and tau = 0) with u = a x
+    # and v = 0: the strain rate is a in every cell, so on every level the
+    # viscosity is the fixed point eta = 1/2 B (a^2 + eps_reg^2)^((1/n-1)/2)
+    nx, ny, ns = 10, 6, 4
+    a = 2.0e-3
+    grid = W.Grid(nx=nx, ny=ny, nσ=ns, dx=1000.0, dy=1000.0)
+    params = W.Params(default_thickness=100.0, default_temperature=265.700709)
+    m = W.Model(grid=grid, bed_elevation=-2000.0 * np.ones((nx, ny)), params=params)
+    o = oracle.OracleModel(m)
+    o._view(W._lib.FIELD["U_U"])[:] = (a * grid.xxu).ravel(order="F")
+    o._view(W._lib.FIELD["V_V"])[:] = 0.0
+    o.update_geometry()
+    assert np.all(o.field("H_GROUNDED_FRACTION") == 0.0)
+    o.inner_update()
+    eta = o.field("S_ETA").reshape((nx, ny, ns), order="F")
+    B = o.field("S_GLEN_B").reshape((nx, ny, ns), order="F")
+    p = m.params
+    expect = 0.5 * B * (a * a + p.glen_reg_strain_rate ** 2) ** ((1.0 / p.glen_n - 1.0) / 2.0)
+    assert np.abs(eta / expect - 1.0).max() <= 1e-13
+    # and n enters: at n = 3 the viscosity differs from 1/2 B
+    assert np.abs(eta / (0.5 * B) - 1.0).min() > 0.5
+
+
+def test_basal_drag_closed_form_on_a_sliding_slab(W, oracle):
+    # update_bed_speed! and update_beta! (update_velocities.jl:227-255) from
+    # the cold start (beta = 0): a grounded slab sliding uniformly, u = U,
+    # v = 0, has bed speed U in every cell, so
+    # beta = C (U^2 + reg^2)^((1/m - 1)/2) with C = weertman_c (grounded fraction 1)
+    nx, ny = 10, 6
+    U = 250.0
+    grid = W.Grid(nx=nx, ny=ny, nσ=4, dx=1000.0, dy=1000.0)
+    params = W.Params(default_thickness=100.0, default_temperature=265.700709, weertman_c=2.0e4)
+    m = W.Model(grid=grid, bed_elevation=np.zeros((nx, ny)), params=params)
+    o = oracle.OracleModel(m)
+    o._view(W._lib.FIELD["U_U"])[:] = U
+    o._view(W._lib.FIELD["V_V"])[:] = 0.0
+    o.update_geometry()
+    assert np.all(o.field("H_GROUNDED_FRACTION") == 1.0)
+    assert np.all(o.field("H_BETA") == 0.0)
+    o.inner_update()
+    p = m.params
+    expect = 2.0e4 * (U * U + p.weertman_reg_speed ** 2) ** ((1.0 / p.weertman_m - 1.0) / 2.0)
+    beta = o.field("H_BETA")
+    assert np.abs(beta / expect - 1.0).max() <= 1e-14
