@@ -49,3 +49,38 @@ def iceberg(W, precision="fp64", nx=20, ny=20):
     model = W.Model(grid=grid, bed_elevation=-500.0 * np.ones((nx, ny)), params=params,
                     solver_params=sp, initial_conditions=ic, parallel_spec=W.HIPSpec(precision))
     return model, solid_u, solid_v
+
+
+def ragged(W, nx=40, ny=24, precision="fp64", seed=7, maxiter_picard=3, nσ=4):
+    """An irregular domain: a calving front whose position varies from row to
+    row, interior holes (ice-free cells inside the ice, as around nunataks),
+    a retrograde bed that grounds the upstream half and floats the rest, and
+    interior Dirichlet points on u with nonzero values (the inflow of a
+    tributary).  The walls follow the channel (u zero on the north edge, v zero
+    on the west and east edges)."""
+    rng = np.random.default_rng(seed)
+    dx = dy = 4000.0
+    h_mask = np.ones((nx, ny), bool)
+    front = nx - 2 - rng.integers(0, max(nx // 5, 2), ny)
+    for j in range(ny):
+        h_mask[front[j]:, j] = False
+    for _ in range(3):
+        i = int(rng.integers(nx // 5, nx // 2))
+        j = int(rng.integers(2, ny - 4))
+        h_mask[i:i + 2, j:j + 2] = False
+    h_mask[nx // 3, ny // 2] = False
+    u_isfixed = np.zeros((nx + 1, ny), bool)
+    u_isfixed[nx // 4, 1:4] = True
+    grid = W.Grid(nx=nx, ny=ny, nσ=nσ, x0=0.0, y0=-ny * dy / 2, dx=dx, dy=dy, h_mask=h_mask,
+                  u_iszero=["north"], v_iszero=["west", "east"], u_isfixed=u_isfixed)
+    x = grid.xxh
+    bed = 150.0 - 900.0 * x / (nx * dx) + rng.normal(0.0, 10.0, x.shape)
+    thick = np.where(h_mask, np.maximum(1200.0 * (1.0 - 0.8 * x / (nx * dx)), 50.0), 0.0)
+    thick = np.asfortranarray(thick + np.where(h_mask, rng.normal(0.0, 5.0, x.shape), 0.0))
+    u0 = np.zeros((nx + 1, ny))
+    u0[u_isfixed] = 80.0
+    ic = W.InitialConditions(initial_thickness=thick, initial_u_veloc=u0)
+    params = W.Params(default_temperature=265.700709, accumulation_rate=0.3, weertman_c=1e4)
+    sp = W.SolverParams(maxiter_picard=maxiter_picard, tol_coarse=1e-8)
+    return W.Model(grid=grid, bed_elevation=np.asfortranarray(bed), params=params, solver_params=sp,
+                   initial_conditions=ic, parallel_spec=W.HIPSpec(precision))

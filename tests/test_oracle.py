@@ -209,6 +209,31 @@ def test_operator_symmetric_negative_definite_local(W, oracle):
     assert np.array_equal(d, np.diag(A))
 
 
+def test_operator_invariants_on_a_ragged_domain(W, oracle):
+    """The operator on an irregular domain (a calving front that moves from
+    row to row, interior holes, interior Dirichlet points, grounded and
+    floating parts) keeps the invariants of the rectangular case: symmetric,
+    negative definite, probed diagonal equal to diag(A)."""
+    from helpers import ragged
+    m = ragged(W, nx=20, ny=12)
+    o = oracle.OracleModel(m)
+    o.update_geometry()
+    gf = o.field("H_GROUNDED_FRACTION").reshape((20, 12), order="F")[m.fields.gh.mask]
+    assert 0.0 < gf.mean() < 1.0
+    rng = np.random.default_rng(4)
+    o._view(W._lib.FIELD["U_U"])[:] = 100 + 50 * rng.standard_normal(21 * 12)
+    o._view(W._lib.FIELD["V_V"])[:] = 10 * rng.standard_normal(20 * 13)
+    o.inner_update()
+    masks = o.masks()
+    assert masks["u_inner"].sum() < m.fields.gu.mask.sum()  # Dirichlet points are not unknowns
+    A, idx = _assemble(o, masks, 20, 12)
+    assert np.abs(A - A.T).max() <= 1e-12 * np.abs(A).max()
+    assert np.linalg.eigvalsh(0.5 * (A + A.T)).max() < 0
+    du, dv = o.op_diag()
+    d = np.r_[du.ravel(order="F")[masks["u_inner"].ravel(order="F")], dv.ravel(order="F")[masks["v_inner"].ravel(order="F")]]
+    assert np.array_equal(d, np.diag(A))
+
+
 def test_linear_rheology_solve_matches_direct_solve(W, oracle):
     # glen_n = weertman_m = 1 make viscosity and drag constant, so Picard
     # iterations of one V-cycle each converge to the solution of one linear
