@@ -1056,6 +1056,9 @@ static bool use_cg_graph()
 constexpr int CG_CHUNK = 32;
 static int cg_chunk_len(int k) { return k >= 4 ? CG_CHUNK : 2 << k; }
 
+// Relative stop of the mixed-precision coarse CG: 2^-19 = 16 FP32 ulps.
+constexpr double CGM_FP32_RELTOL = 0x1p-19;
+
 // coarse CG on b_c held in w.cr; solution in the warm start w.xc
 template <typename T>
 static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
@@ -1155,7 +1158,12 @@ static CGMIterArgs cgm_iter_args(wavi_ctx *ctx)
     ia.rdx = (float)ctx->rdx;
     ia.rdy = (float)ctx->rdy;
     ia.tol_coarse = ctx->S.tol_coarse;
-    ia.reltol = std::sqrt(2.220446049250313e-16);  // IterativeSolvers' default sqrt(eps)
+    // IterativeSolvers' default reltol sqrt(eps), raised to FP32's attainable
+    // accuracy: the FP32 recurrences track the true residual down to about
+    // 16 ulps of ||r0||, and iterating past that point lets the recursive
+    // residual drift from the true one and the iterate grow (the outer
+    // Picard loop then stalls near 1e-8 instead of converging).
+    ia.reltol = std::max(std::sqrt(2.220446049250313e-16), CGM_FP32_RELTOL);
     ia.maxiter = ctx->S.maxiter_coarse;
     ia.nblk = ctx->nblk;
     return ia;
