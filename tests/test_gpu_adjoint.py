@@ -69,6 +69,40 @@ def test_gradient_matches_finite_differences(W, precision):
         assert abs(fd - gd) <= 1e-5 * abs(gd)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gradient_on_a_ragged_domain(W, oracle, precision):
+    """An irregular domain (a calving front that moves from row to row,
+    interior holes, grounded and floating parts): the gradient against the
+    oracle's reverse sweep."""
+    from helpers import ragged
+    nx, ny = 40, 24
+    _step("adjoint on a ragged domain %s %dx%d" % (precision, nx, ny))
+
+    def make(logc, prec):
+        return ragged(W, nx=nx, ny=ny, precision=prec, maxiter_picard=N_PICARD, dirichlet=False,
+                      weertman_c=np.asfortranarray(np.exp(logc)))
+
+    l0 = np.log(1e4) * np.ones((nx, ny))
+    lt = l0 + 0.3 * np.random.default_rng(8).standard_normal((nx, ny))
+    ot = oracle.OracleModel(make(lt, "fp64"))
+    ot.update_geometry()
+    ot.misfit_gradient(N_PICARD, TOL_LIN, 60, np.zeros((nx + 1) * ny), np.zeros(nx * (ny + 1)), gradient=False)
+    uo, vo = W.workloads.inversion_observations(ot.field("U_U").reshape((nx + 1, ny), order="F"),
+                                                ot.field("V_V").reshape((nx, ny + 1), order="F"), seed=8)
+    o = oracle.OracleModel(make(l0, "fp64"))
+    o.update_geometry()
+    Jo, go, repo = o.misfit_gradient(N_PICARD, TOL_LIN, 60, uo, vo)
+    assert repo["unconverged"] == 0
+    m = make(l0, precision)
+    W.update_geometry(m)
+    J, g, rep = W.misfit_gradient(m, N_PICARD, uo, vo, TOL_LIN)
+    err = np.linalg.norm(g - go) / np.linalg.norm(go)
+    print("  J rel %.2e, gradient rel %.2e" % (abs(J - Jo) / abs(Jo), err), flush=True)
+    assert rep["unconverged"] == 0
+    assert abs(J - Jo) <= 1e-10 * abs(Jo)
+    assert err <= (1e-8 if precision == "fp64" else 1e-6)
+
+
 def test_fixed_velocities_rejected(W):
     _step("adjoint rejects fixed velocities")
     nx, ny = 32, 8
