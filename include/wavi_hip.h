@@ -45,7 +45,10 @@ extern "C" {
 #define WAVI_PRECISION_MIXED 1  /* FP32 V-cycle (smoother, Haar transfers and a one-launch-per-
                                   * iteration coarse CG with FP64 cross-workgroup sums) inside
                                   * FP64 residual refinement: every Picard iteration forms the
-                                  * FP64 residual b - A(u)u and applies the FP32 correction */
+                                  * FP64 residual b - A(u)u and applies the FP32 correction.
+                                  * The FP32 coarse CG's relative stop is 2^-19 ||r0|| (FP32's
+                                  * attainable accuracy) in place of sqrt(eps), so its
+                                  * coarse_cg_iters can be lower than FP64's */
 
 /* Grid: the reference's Grid(;...) keyword arguments (src/Grid.jl:66-82).
  * Masks are 0/1 bytes.  u_iszero/v_iszero remove points from the u/v masks
