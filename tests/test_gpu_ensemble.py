@@ -225,6 +225,33 @@ def test_gradient_batch_more_members_than_rendezvous_threads(W):
         assert ra == rb
 
 
+def test_lockstep_with_irregular_members(W):
+    """Lockstep members on irregular domains (calving fronts that move from
+    row to row, interior holes, interior Dirichlet points) beside channel
+    members: each bit-identical to its own solve."""
+    from helpers import ragged
+    specs = _small_specs(W, 2, seed=9)
+    _step("lockstep with irregular members")
+
+    def members():
+        ms = _members(W, specs, "mixed")
+        ms += [ragged(W, nx=40, ny=24, precision="mixed", seed=s, maxiter_picard=6) for s in (7, 11)]
+        ms.append(ragged(W, nx=24, ny=12, precision="mixed", seed=3, maxiter_picard=4, dirichlet=False))
+        for m in ms:
+            W.update_geometry(m)
+        return ms
+
+    alone = members()
+    for m in alone:
+        W.update_velocities(m)
+    lock = members()
+    W.update_velocities_batch(lock, lockstep=True)
+    for a, b in zip(alone, lock):
+        assert a.last_report["coarse_cg_iters"] == b.last_report["coarse_cg_iters"]
+        np.testing.assert_array_equal(a.fields.gu.u, b.fields.gu.u)
+        np.testing.assert_array_equal(a.fields.gv.v, b.fields.gv.v)
+
+
 @pytest.mark.parametrize("levels", [0, 1, 2])
 def test_lockstep_other_haar_levels(W, levels):
     """k_cgm_iter_batch at Haar levels 0..2 (blocks of 2, 4, 8): lockstep
