@@ -1,0 +1,172 @@
+"""Parity at the benchmark shapes: the channel single solve at 1280 x 160 x 8
+and 2560 x 320 x 8 (the bench's solve and its double-resolution version) and
+the full 64-member seeded ensemble.
+
+At these sizes the FP64 coarse CG folds 1,690 (1280 x 160) and 6,580
+(2560 x 320) per-tile partials per reduction, so every lane of the
+256-lane fold sums several partials in order, the tile grid covers the
+whole chip, and nσ is 8.  The oracle runs a full 20-Picard solve of these
+grids in about 20 minutes, so the FP64 comparison caps the coarse CG (40 and
+20 iterations) and takes two Picard iterations: the second starts from the
+first's velocities, viscosity, drag and coarse warm start.  The bar is bit
+equality with the oracle, with equal reports.
+
+The mixed-precision solve is compared after the full 20 Picard iterations
+with the FP64 GPU solve of the same state (the path shown bit-equal to the
+oracle above): relative L2 and max |du| / max |u| at most 1e-6.  Reference
+path: update_velocities! (src/Models/update_velocities.jl:11-83) with the
+V-cycle of src/Wavelets/update_preconditioners.jl:54-83."""
+import copy
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _step(name):
+    print("[gpu] %s" % name, flush=True)
+    sys.stdout.flush()
+
+
+def _fields_equal(W, model, o):
+    bad = []
+    for cont, attr, fid in W.model.FIELD_MAP:
+        a = np.asarray(model._array(cont, attr), dtype=float)
+        b = o.field(fid).reshape(a.shape, order="F")
+        if not np.array_equal(a, b, equal_nan=True):
+            bad.append("%s max|diff| %.3e rel %.3e" % (fid, np.nanmax(np.abs(a - b)), rel_l2(a, b)))
+    return bad
+
+
+def _with_precision(W, base, precision):
+    m = copy.deepcopy(base)
+    m.parallel_spec = W.HIPSpec(precision)
+    return m
+
+
+def _vel(m):
+    return np.concatenate([m.fields.gu.u.ravel(order="F"), m.fields.gv.v.ravel(order="F")])
+
+
+def _errors(got, ref):
+    return rel_l2(got, ref), float(np.abs(got - ref).max() / np.abs(ref).max())
+
+
+@pytest.mark.parametrize("nx,ny,cap", [(1280, 160, 40), (2560, 320, 20)])
+def test_fp64_solve_bit_equal_to_oracle_at_bench_shape(W, oracle, nx, ny, cap):
+    _step("fp64 %dx%dx8, 2 Picard, coarse CG capped at %d: GPU vs oracle" % (nx, ny, cap))
+    m = W.workloads.channel_solve(nx=nx, ny=ny, maxiter_picard=2)
+    assert m.grid.nσ == 8 and m.solver_params.levels == 3
+    m.solver_params.maxiter_coarse = cap
+    o = oracle.OracleModel(m)
+    W.update_geometry(m)
+    W.update_velocities(m)
+    t = time.time()
+    o.update_geometry()
+    orep = o.update_velocities()
+    _step("oracle %.1f s; report %s" % (time.time() - t, orep["coarse_cg_iters"][:2]))
+    assert orep["picard_iters"] == 2 and orep["coarse_cg_iters"][:2] == [cap, cap]
+    assert m.last_report == orep
+    bad = _fields_equal(W, m, o)
+    assert not bad, bad
+    m.parallel_spec.close()
+
+
+@pytest.mark.parametrize("nx,ny", [(1280, 160), (2560, 320)])
+def test_mixed_full_solve_within_1e6_at_bench_shape(W, nx, ny):
+    base = W.workloads.channel_solve(nx=nx, ny=ny)
+    vel, reps = {}, {}
+    for prec in ("fp64", "mixed"):
+        _step("%s %dx%dx8, 20 Picard" % (prec, nx, ny))
+        m = _with_precision(W, base, prec)
+        W.update_geometry(m)
+        t = time.time()
+        W.update_velocities(m)
+        reps[prec] = m.last_report
+        vel[prec] = _vel(m)
+        m.parallel_spec.close()
+        _step("%s: %.2f s (with host copies), picard %d, coarse CG %d, rel resid %.3e" % (
+            prec, time.time() - t, reps[prec]["picard_iters"], reps[prec]["coarse_cg_total"],
+            reps[prec]["rel_resid"]))
+    assert np.all(np.isfinite(vel["mixed"]))
+    l2, mx = _errors(vel["mixed"], vel["fp64"])
+    _step("mixed vs FP64 at %dx%dx8: rel L2 %.3e, max|du|/max|u| %.3e, refinements %d" % (
+        nx, ny, l2, mx, reps["mixed"]["refinements"]))
+    assert reps["mixed"]["picard_iters"] == reps["fp64"]["picard_iters"] == 20
+    assert reps["mixed"]["refinements"] == 20
+    assert l2 <= 1e-6 and mx <= 1e-6
+
+
+def _ensemble_base(W):
+    specs = W.workloads.ensemble_specs(64)
+    sizes = {(sp["nx"], sp["ny"]) for sp in specs}
+    assert sizes == set(W.workloads.ENSEMBLE_SIZES) and {sp["nσ"] for sp in specs} == {4, 8}
+    return specs, [W.workloads.ensemble_member(sp, "mixed") for sp in specs]
+
+
+def _solved(W, m):
+    W.update_geometry(m)
+    return m
+
+
+@pytest.mark.timeout(900)
+def test_full_ensemble_batch_matches_lone_solves(W):
+    """All 64 members of the seeded ensemble (160 x 20 up to 1280 x 160,
+    nσ 4 and 8), mixed precision, in one batched call: every member
+    bit-identical to its own wavi_update_velocities; and every member's
+    velocities within 1e-6 (relative L2 and max) of its FP64 solve."""
+    _step("ensemble 64: building members")
+    specs, base = _ensemble_base(W)
+    _step("ensemble 64: lone mixed and FP64 solves")
+    lone, fp64 = [], []
+    for b in base:
+        for prec, out in (("mixed", lone), ("fp64", fp64)):
+            m = _solved(W, _with_precision(W, b, prec))
+            W.update_velocities(m)
+            out.append((_vel(m), m.last_report))
+            m.parallel_spec.close()
+    _step("ensemble 64: one batched call, 8 threads")
+    batch = [_solved(W, _with_precision(W, b, "mixed")) for b in base]
+    t = time.time()
+    W.update_velocities_batch(batch, nthreads=8)
+    _step("ensemble 64: batch %.2f s (with host copies)" % (time.time() - t))
+    worst = (0.0, 0.0, None)
+    for k, (sp, m) in enumerate(zip(specs, batch)):
+        got = _vel(m)
+        assert np.array_equal(got, lone[k][0]), (k, sp)
+        assert m.last_report == lone[k][1], (k, sp)
+        l2, mx = _errors(got, fp64[k][0])
+        if mx > worst[1]:
+            worst = (l2, mx, (sp["nx"], sp["ny"], sp["nσ"]))
+        assert l2 <= 1e-6 and mx <= 1e-6, (k, sp, l2, mx)
+        m.parallel_spec.close()
+    _step("ensemble 64: worst member vs FP64: rel L2 %.3e, max %.3e at %s" % worst)
+
+
+def test_smallest_ensemble_members_fp64_batch_bit_equal_to_oracle(W, oracle):
+    """The 160 x 20 members of the ensemble in FP64, batched (3 Picard
+    iterations, coarse CG capped at 200): each bit-equal to the oracle."""
+    specs = [sp for sp in W.workloads.ensemble_specs(64) if sp["nx"] == 160]
+    assert len(specs) >= 8
+    _step("ensemble fp64: %d members of 160x20 vs oracle" % len(specs))
+    ms, os_ = [], []
+    for sp in specs:
+        m = W.workloads.ensemble_member(sp, "fp64", maxiter_picard=3)
+        m.solver_params.maxiter_coarse = 200
+        o = oracle.OracleModel(m)
+        W.update_geometry(m)
+        o.update_geometry()
+        ms.append(m)
+        os_.append(o)
+    W.update_velocities_batch(ms, nthreads=4)
+    for sp, m, o in zip(specs, ms, os_):
+        orep = o.update_velocities()
+        assert m.last_report == orep, sp
+        bad = _fields_equal(W, m, o)
+        assert not bad, (sp, bad)
+        m.parallel_spec.close()
