@@ -5,8 +5,12 @@ MI355X velocity solve for WAVI.jl models.  `HIPSpec` is a new
 `AbstractParallelSpec` (WAVI.jl `src/WAVI.jl:17`); models built with
 `parallel_spec = HIPSpec()` dispatch `update_velocities!` and
 `update_state!` to `libwavi_hip.so` through `ccall`.  The fields stay in the
-model's `Fields` struct and travel to the device before a call and back after
-it; `run_steps!` keeps them resident over many time steps.
+model's `Fields` struct.  `update_velocities!` moves only the fields the
+Picard loop reads to the device and only those it writes back;
+`run_simulation!` keeps the whole state resident on the device between the
+steps at which the reference writes something (checkpoints, outputs, the
+velocity dump), where it brings the fields back and writes the reference's
+own JLD2 checkpoints with `@save`.
 
 This file is written against the C ABI in `include/wavi_hip.h`.  It has not
 been run: the build image has no Julia.
@@ -15,6 +19,9 @@ module WAVIHIP
 
 using WAVI
 using ChainRulesCore
+using JLD2
+using SparseArrays
+using LinearAlgebra
 import WAVI: update_velocities!, update_state!, AbstractParallelSpec, AbstractModel
 
 const LIB = get(ENV, "WAVI_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libwavi_hip.so"))
@@ -164,33 +171,79 @@ function WAVI.update_preconditioner!(model::AbstractModel{T,N,M,<:HIPSpec}) wher
     return model
 end
 
-function push_fields!(model)
-    ctx = model.parallel_spec.ctx
-    for (id, a) in enumerate(fieldlist(model))
-        check(ctx, ccall((:wavi_set_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t),
-                         ctx, id - 1, a, length(a)))
-    end
+# Field ids (0-based, enum wavi_field) that the Picard loop of
+# wavi_update_velocities reads (inner update: velocities, thickness, C * gf,
+# grounded fractions, dsdh, glen_b, viscosity, drag, f2; right-hand side:
+# surface, accumulation, melt, thickness and surface on u and v) and writes
+# (the fields inner_update! sets, velocities, residuals).  Everything else
+# (bed, haf, dh/dt, velocities on h, parameters, wavelet coefficients) is
+# neither read nor written by the solve.
+const SOLVE_IN = (1, 2, 4, 5, 7, 8, 18, 19, 24, 27, 28, 29, 31, 33, 34, 35, 37, 39, 40)
+const SOLVE_OUT = (9, 10, 17, 19, 20, 21, 22, 23, 24, 30, 31, 32, 36, 37, 38, 39)
+const ALL_FIELDS = 0:40
+
+put_field(ctx, id, a) = check(ctx, ccall((:wavi_set_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t),
+                                        ctx, id, a, length(a)))
+take_field!(ctx, id, a) = check(ctx, ccall((:wavi_get_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t),
+                                         ctx, id, a, length(a)))
+
+# The coarse warm start: compacted on the wavelet mask on the host (the
+# reference's correction_coarse[], update_preconditioners.jl:195-216), full
+# grid with zeros off the mask on the device.
+function push_wavelets!(ctx, model; coefficients::Bool = false)
     wu, wv = model.fields.wu, model.fields.wv
-    for (id, a) in ((41, wu.wavelets), (42, Float64.(wu.mask)), (44, wv.wavelets), (45, Float64.(wv.mask)))
-        check(ctx, ccall((:wavi_set_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t), ctx, id, a, length(a)))
+    for (w, idw, idm, idc) in ((wu, 41, 42, 43), (wv, 44, 45, 46))
+        coefficients && put_field(ctx, idw, w.wavelets)
+        put_field(ctx, idm, Float64.(w.mask))
+        full = zeros(Float64, size(w.mask))
+        c = w.correction_coarse[]
+        length(c) == count(w.mask) && (full[w.mask] .= c)
+        put_field(ctx, idc, full)
     end
+end
+
+# The wavelet masks, warm start and (coefficients = true) coefficients from
+# the device, with the sampling operators rebuilt from the mask as
+# update_wavelets! builds them (src/Wavelets/update_wavelets.jl:16-29).
+function pull_wavelets!(ctx, model; coefficients::Bool = false)
+    wu, wv = model.fields.wu, model.fields.wv
+    for (w, idw, idm, idc) in ((wu, 41, 42, 43), (wv, 44, 45, 46))
+        coefficients && take_field!(ctx, idw, w.wavelets)
+        m = similar(w.wavelets)
+        take_field!(ctx, idm, m)
+        mask = m .> 0.5
+        if mask != w.mask
+            w.mask .= mask
+            n = count(mask)
+            w.n[] = n
+            N = length(mask)
+            w.crop[] = Diagonal(float(mask[:]))
+            w.samp[] = sparse(1:n, (1:N)[mask[:]], ones(n), n, N)
+            w.spread[] = sparse(w.samp[]')
+        end
+        full = similar(w.wavelets)
+        take_field!(ctx, idc, full)
+        w.correction_coarse[] = full[w.mask]
+    end
+end
+
+function push_fields!(model, ids = ALL_FIELDS)
+    ctx = model.parallel_spec.ctx
+    fl = fieldlist(model)
+    for id in ids
+        put_field(ctx, id, fl[id + 1])
+    end
+    push_wavelets!(ctx, model; coefficients = ids === ALL_FIELDS)
     return model
 end
 
-function pull_fields!(model)
+function pull_fields!(model, ids = ALL_FIELDS)
     ctx = model.parallel_spec.ctx
-    for (id, a) in enumerate(fieldlist(model))
-        check(ctx, ccall((:wavi_get_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t),
-                         ctx, id - 1, a, length(a)))
+    fl = fieldlist(model)
+    for id in ids
+        take_field!(ctx, id, fl[id + 1])
     end
-    wu, wv = model.fields.wu, model.fields.wv
-    for (id, w, mask) in ((41, wu, 42), (44, wv, 45))
-        check(ctx, ccall((:wavi_get_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t), ctx, id, w.wavelets, length(w.wavelets)))
-        m = similar(w.wavelets)
-        check(ctx, ccall((:wavi_get_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t), ctx, mask, m, length(m)))
-        w.mask .= m .> 0.5
-        w.n[] = count(w.mask)
-    end
+    pull_wavelets!(ctx, model; coefficients = ids === ALL_FIELDS)
     return model
 end
 
@@ -201,11 +254,11 @@ end
 """
 function WAVI.update_velocities!(model::AbstractModel{T,N,M,<:HIPSpec}) where {T,N,M}
     WAVI.update_preconditioner!(model)
-    push_fields!(model)
+    push_fields!(model, SOLVE_IN)
     rep = Ref{SolveReport}()
     check(model.parallel_spec.ctx, ccall((:wavi_update_velocities, LIB), Cint, (Ptr{Cvoid}, Ref{SolveReport}),
                                          model.parallel_spec.ctx, rep))
-    pull_fields!(model)
+    pull_fields!(model, SOLVE_OUT)
     println("Solved momentum equation on the GPU with residual ", round(rep[].rel_resid, sigdigits=3),
             " at iteration ", rep[].picard_iters)
     return model
@@ -215,18 +268,74 @@ end
     run_steps!(simulation, nsteps)
 
 `nsteps` of `timestep!` (`src/Simulations/run_simulation.jl:6-17`) with the
-state resident on the device.
+state resident on the device: every field goes to the device once and comes
+back once.  With `step_thickness = false` the steps are `update_state!` only.
 """
 function run_steps!(simulation, nsteps::Integer)
     model = simulation.model
+    nsteps > 0 || return simulation
     WAVI.update_preconditioner!(model)
     push_fields!(model)
     rep = Ref{SolveReport}()
-    check(model.parallel_spec.ctx, ccall((:wavi_run_steps, LIB), Cint, (Ptr{Cvoid}, Cint, Ref{SolveReport}),
-                                         model.parallel_spec.ctx, nsteps, rep))
+    ctx = model.parallel_spec.ctx
+    if simulation.timestepping_params.step_thickness
+        check(ctx, ccall((:wavi_run_steps, LIB), Cint, (Ptr{Cvoid}, Cint, Ref{SolveReport}), ctx, nsteps, rep))
+    else
+        for _ in 1:nsteps
+            check(ctx, ccall((:wavi_update_state, LIB), Cint, (Ptr{Cvoid}, Ref{SolveReport}), ctx, rep))
+        end
+    end
     pull_fields!(model)
     simulation.clock.n_iter += nsteps
     simulation.clock.time += nsteps * simulation.timestepping_params.dt
+    return simulation
+end
+
+"""
+    run_simulation!(simulation) for a model with a `HIPSpec`
+
+`run_simulation!` (`src/Simulations/run_simulation.jl:61-100`) with the time
+loop on the device: the steps run by `run_steps!` in stretches that end at
+the steps where the reference writes something (temporary and permanent
+checkpoints, outputs, the velocity dump at the last step).  There the fields
+come back to the host and the reference's own writers run: `@save` of the
+whole simulation into `ChkptA/B.jld2` and `PChkpt_<10 digits>.jld2` under
+`chkpt_path` (so the reference's `pickup!`, `Simulation.jl:68-86`, reads
+them), `write_output` and `write_vel`; then `zip_output`.
+"""
+function WAVI.run_simulation!(simulation::WAVI.Simulation{<:AbstractModel{T,N,M,<:HIPSpec}}) where {T,N,M}
+    tp = simulation.timestepping_params
+    op = simulation.output_params
+    events = (tp.n_iter_chkpt, tp.n_iter_pchkpt, op.n_iter_out)
+    chkpt_tag = "A"
+    i = simulation.clock.n_iter
+    while i < tp.n_iter_total
+        nxt = tp.n_iter_total
+        for e in events
+            isfinite(e) && (nxt = min(nxt, (i ÷ Int(e) + 1) * Int(e)))
+        end
+        run_steps!(simulation, nxt - i)
+        i = nxt
+        if isfinite(tp.n_iter_chkpt) && mod(i, tp.n_iter_chkpt) == 0
+            fname = string(tp.chkpt_path, "Chkpt", chkpt_tag, ".jld2")
+            @save fname simulation
+            chkpt_tag = (chkpt_tag == "A") ? "B" : "A"
+            println("making temporary checkpoint at timestep number $(simulation.clock.n_iter)")
+        end
+        if isfinite(tp.n_iter_pchkpt) && mod(i, tp.n_iter_pchkpt) == 0
+            n_iter_string = lpad(simulation.clock.n_iter, 10, "0")
+            fname = string(tp.chkpt_path, "PChkpt_", n_iter_string, ".jld2")
+            @save fname simulation
+            println("making permanent checkpoint at timestep number $(simulation.clock.n_iter)")
+        end
+        if isfinite(op.n_iter_out) && mod(i, op.n_iter_out) == 0
+            WAVI.write_output(simulation)
+        end
+        if i == tp.n_iter_total && op.dump_vel
+            WAVI.write_vel(simulation)
+        end
+    end
+    WAVI.zip_output(simulation)
     return simulation
 end
 
@@ -346,6 +455,7 @@ function ChainRulesCore.rrule(::typeof(misfit), problem::MisfitProblem, logC::Ab
     return J, misfit_pullback
 end
 
-export HIPSpec, run_steps!, update_velocities_batch!, update_geometry!, misfit_gradient, MisfitProblem, misfit
+export HIPSpec, run_steps!, update_velocities_batch!, update_geometry!, misfit_gradient, MisfitProblem, misfit,
+       push_fields!, pull_fields!
 
 end # module
