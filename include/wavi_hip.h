@@ -196,6 +196,22 @@ int wavi_update_velocities_batch(wavi_ctx **ctxs, int n, int nthreads,
  * wavi_update_velocities_batch.  No reference counterpart: the reference
  * solves one model per call. */
 int wavi_update_velocities_lockstep(wavi_ctx **ctxs, int n, wavi_solve_report *reports, int *codes);
+/* wavi_update_velocities_lockstep without any device allocation: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, checks and results as
+ * wavi_update_velocities_lockstep, bit for bit.  The device tables of the
+ * shared coarse-CG launches (per member about 100 bytes plus one int per
+ * 8 tiles) live in a static buffer of WAVI_NOALLOC_SCRATCH_BYTES on each
+ * GPU, held until the process ends and set up by the first wavi_ctx_create
+ * on that GPU; the events are the contexts' own.  At most
+ * WAVI_NOALLOC_MAX_MEMBERS members, and the tables must fit the buffer:
+ * WAVI_EARG otherwise, before any work.  Every call on one GPU uses that
+ * GPU's one buffer, which keeps the last call's tables: calls that overlap
+ * on one GPU race, so a caller serialises them. */
+#define WAVI_NOALLOC_MAX_MEMBERS 64
+#define WAVI_NOALLOC_SCRATCH_BYTES (256 * 1024)
+int wavi_update_velocities_lockstep_noalloc(wavi_ctx **ctxs, int n, wavi_solve_report *reports, int *codes);
 /* update_state! (update_state.jl:7-21) on each of n contexts, like
  * wavi_update_velocities_batch: the subdomain solves of the restricted
  * additive Schwarz preconditioner (src/SharedMemorySpec.jl:96-133 runs

@@ -31,6 +31,17 @@ static thread_local std::string g_create_error;
 // another thread): such a call can end another thread's capture.
 static std::mutex g_graph_mu;
 
+// The scratch of wavi_update_velocities_lockstep_noalloc: the device tables
+// of its shared coarse-CG launches, in a fixed buffer that every GPU the
+// library runs on holds until the process ends (each GPU has its own copy at
+// its own address).  In host code `scratch` is only a stand-in for the
+// symbol: its device address on each GPU is looked up once, at the first
+// context creation on that GPU, and kept in g_scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[WAVI_NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[MAX_DEVICES] = {};
+
 struct DevVec {
     double *u = nullptr, *v = nullptr;
 };
@@ -94,6 +105,7 @@ struct wavi_ctx {
     CGMHostFlag *h_flag = nullptr;  // pinned, host-mapped stop report of the mixed coarse CG
     CGMHostFlag *d_flag = nullptr;  // its device address
     hipEvent_t cg_ev[3] = {};       // chunk completion events of the mixed coarse CG
+    hipEvent_t ev_ls = nullptr, ev_ls_done = nullptr;  // set-up and batch end in a lockstep batch without allocation
     hipGraphExec_t cgm_chunk = nullptr;  // 32 k_cgm_iter launches (parities 1, 0, ...) as one graph
     double *Pr0 = nullptr, *Pr1 = nullptr, *Pd = nullptr, *Pb = nullptr, *Pn = nullptr;
     double *quad = nullptr;  // 4 * nh pos_fraction quadrants
@@ -505,6 +517,15 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
         }                                                                         \
     } while (0)
     CREATECHK(hipGetDevice(&ctx->device));
+    if (ctx->device >= 0 && ctx->device < MAX_DEVICES) {
+        // the scratch address of the entry point without allocation, once per device
+        std::lock_guard<std::mutex> slock(g_scratch_mu);
+        if (!g_scratch_ptr[ctx->device]) {
+            void *p = nullptr;
+            CREATECHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            g_scratch_ptr[ctx->device] = (unsigned char *)p;
+        }
+    }
     CREATECHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
 
     // host masks (setup_fields, src/Fields/Fields.jl:28-35)
@@ -607,6 +628,8 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
     CREATECHK(hipHostMalloc((void **)&ctx->h_flag, sizeof(CGMHostFlag), hipHostMallocMapped | hipHostMallocCoherent));
     CREATECHK(hipHostGetDevicePointer((void **)&ctx->d_flag, ctx->h_flag, 0));
     for (hipEvent_t &e : ctx->cg_ev) CREATECHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    CREATECHK(hipEventCreateWithFlags(&ctx->ev_ls, hipEventDisableTiming));
+    CREATECHK(hipEventCreateWithFlags(&ctx->ev_ls_done, hipEventDisableTiming));
     if (precision_mode == WAVI_PRECISION_MIXED) {
         VSet<float> &w = ctx->v32;
         auto falloc = [&](float **p, size_t n) -> hipError_t {
@@ -687,6 +710,8 @@ static void ctx_free(wavi_ctx *ctx)
     if (ctx->stm) (void)hipFree(ctx->stm);
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     for (hipEvent_t e : ctx->cg_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {ctx->ev_ls, ctx->ev_ls_done})
         if (e) (void)hipEventDestroy(e);
     if (ctx->st) (void)hipFree(ctx->st);
     if (ctx->tab) (void)hipFree(ctx->tab);
@@ -1865,11 +1890,15 @@ static void launch_cgm_batch(hipStream_t s, int levels, int nwg, const CGMBatch 
 }
 
 // Device tables of a lockstep batch and the events that order the members'
-// streams around its shared coarse CG.
-struct LockstepRes {
+// streams around its shared coarse CG (ev[k]: member k's set-up, ev[n]: a
+// batch's end).  LockTables only refers to them; LockstepRes owns them.
+struct LockTables {
     CGMIterArgs *members = nullptr;
     int *group = nullptr, *first = nullptr;
     std::vector<hipEvent_t> ev;
+};
+
+struct LockstepRes : LockTables {
     ~LockstepRes()
     {
         {
@@ -1916,7 +1945,7 @@ struct LockMember {
 // iteration count, and every member's stream waits for the event done
 // before its next work.  On an error *bad names the member concerned.
 static int run_cg_batch(const std::vector<wavi_ctx *> &mem, const std::vector<hipEvent_t> &pre, hipEvent_t done,
-                        LockstepRes &R, std::vector<int> &iters, int *bad)
+                        LockTables &R, std::vector<int> &iters, int *bad)
 {
     wavi_ctx *lead = mem[0];
     hipStream_t cs = lead->stream;
@@ -2067,7 +2096,7 @@ static int rdv_coarse_cg(wavi_ctx *ctx, const VSet<float> &w, int *iters)
 // until every member's CG has stopped), and the rest of each V-cycle and the
 // velocity update back on the members' streams.  Returns the number of
 // members that iterated, or a negative error code (*bad = the member).
-static int lockstep_picard(std::vector<LockMember> &M, LockstepRes &R, int it, int *bad)
+static int lockstep_picard(std::vector<LockMember> &M, LockTables &R, int it, int *bad)
 {
     std::vector<int> act, cgm;
     for (int k = 0; k < (int)M.size(); ++k)
@@ -2159,14 +2188,14 @@ int wavi_misfit_gradient(wavi_ctx *ctx, int n_picard, double tol_linear, int max
                               report ? report : &local);
 }
 
-int wavi_update_velocities_lockstep(wavi_ctx **ctxs, int n, wavi_solve_report *reports, int *codes)
+// Argument checks shared by the lockstep entry points (0 or WAVI_EARG, the
+// message in g_create_error and, where a context is at fault, its error).
+static int lockstep_check(wavi_ctx **ctxs, int n)
 {
-    g_create_error.clear();
     if (n < 0 || (n > 0 && !ctxs)) return WAVI_EARG;
     for (int k = 0; k < n; ++k)
         if (!ctxs[k]) return WAVI_EARG;
     if (batch_has_duplicates(ctxs, n)) return WAVI_EARG;
-    if (n == 0) return WAVI_OK;
     for (int k = 0; k < n; ++k)
         if (ctxs[k]->precision != WAVI_PRECISION_MIXED || ctxs[k]->device != ctxs[0]->device ||
             ctxs[k]->S.levels != ctxs[0]->S.levels) {
@@ -2174,19 +2203,21 @@ int wavi_update_velocities_lockstep(wavi_ctx **ctxs, int n, wavi_solve_report *r
                 "lockstep batch: every context must be mixed precision, on one device, with one Haar level count";
             return WAVI_EARG;
         }
-    if (hipSetDevice(ctxs[0]->device) != hipSuccess) {
-        g_create_error = "hipSetDevice failed";
-        return WAVI_EHIP;
-    }
+    return WAVI_OK;
+}
+
+// The Picard iterations of a lockstep batch of n >= 1 checked members with
+// the tables and events T; fills reports and codes.
+static int lockstep_run(wavi_ctx **ctxs, int n, LockTables &T, int rc, wavi_solve_report *reports, int *codes)
+{
     std::vector<LockMember> M(n);
     for (int k = 0; k < n; ++k) {
         M[k].ctx = ctxs[k];
         memset(&M[k].rep, 0, sizeof(M[k].rep));
     }
-    LockstepRes R;
-    int rc = lockstep_res_alloc(R, ctxs, n), bad = 0;
+    int bad = 0;
     for (int it = 1; rc == WAVI_OK; ++it) {
-        const int r = lockstep_picard(M, R, it, &bad);
+        const int r = lockstep_picard(M, T, it, &bad);
         if (r < 0) rc = r;
         if (r <= 0) break;
     }
@@ -2205,6 +2236,65 @@ int wavi_update_velocities_lockstep(wavi_ctx **ctxs, int n, wavi_solve_report *r
     }
     if (rc != WAVI_OK && g_create_error.empty()) g_create_error = ctxs[bad]->err;
     return rc;
+}
+
+int wavi_update_velocities_lockstep(wavi_ctx **ctxs, int n, wavi_solve_report *reports, int *codes)
+{
+    g_create_error.clear();
+    CHK(lockstep_check(ctxs, n));
+    if (n == 0) return WAVI_OK;
+    if (hipSetDevice(ctxs[0]->device) != hipSuccess) {
+        g_create_error = "hipSetDevice failed";
+        return WAVI_EHIP;
+    }
+    LockstepRes R;
+    const int rc = lockstep_res_alloc(R, ctxs, n);
+    return lockstep_run(ctxs, n, R, rc, reports, codes);
+}
+
+int wavi_update_velocities_lockstep_noalloc(wavi_ctx **ctxs, int n, wavi_solve_report *reports, int *codes)
+{
+    g_create_error.clear();
+    if (n > WAVI_NOALLOC_MAX_MEMBERS) {
+        g_create_error = "wavi_update_velocities_lockstep_noalloc: at most " +
+                         std::to_string(WAVI_NOALLOC_MAX_MEMBERS) + " members (got " + std::to_string(n) + ")";
+        return WAVI_EARG;
+    }
+    CHK(lockstep_check(ctxs, n));
+    if (n == 0) return WAVI_OK;
+    const int dev = ctxs[0]->device;
+    // the tables in the static scratch: member arguments, workgroup group ->
+    // member, first workgroup of each member, each section 256-byte aligned
+    size_t ngroups = 0;
+    for (int k = 0; k < n; ++k) ngroups += (size_t)(ctxs[k]->nblk / 8);
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t off_group = up((size_t)n * sizeof(CGMIterArgs));
+    const size_t off_first = up(off_group + ngroups * sizeof(int));
+    const size_t need = off_first + (size_t)n * sizeof(int);
+    if (need > WAVI_NOALLOC_SCRATCH_BYTES) {
+        g_create_error = "wavi_update_velocities_lockstep_noalloc: the batch's coarse-CG tables need " +
+                         std::to_string(need) + " bytes, more than the " +
+                         std::to_string((size_t)WAVI_NOALLOC_SCRATCH_BYTES) + "-byte static scratch";
+        return WAVI_EARG;
+    }
+    unsigned char *base = dev >= 0 && dev < MAX_DEVICES ? g_scratch_ptr[dev] : nullptr;
+    if (!base) {
+        g_create_error = "wavi_update_velocities_lockstep_noalloc: no scratch address for this device "
+                         "(the contexts' creation did not set it up)";
+        return WAVI_ESTATE;
+    }
+    if (hipSetDevice(dev) != hipSuccess) {
+        g_create_error = "hipSetDevice failed";
+        return WAVI_EHIP;
+    }
+    LockTables T;
+    T.members = (CGMIterArgs *)base;
+    T.group = (int *)(base + off_group);
+    T.first = (int *)(base + off_first);
+    T.ev.resize(n + 1);
+    for (int k = 0; k < n; ++k) T.ev[k] = ctxs[k]->ev_ls;
+    T.ev[n] = ctxs[0]->ev_ls_done;
+    return lockstep_run(ctxs, n, T, WAVI_OK, reports, codes);
 }
 
 // Members that join a batch's lockstep group: mixed precision, at most
