@@ -312,7 +312,8 @@ int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out);
  * includes the cost of the two event records): WAVI_KT_CG_APPLY (FP64
  * coarse CG: A then R on one tile, dot partials), WAVI_KT_CG_DIR (direction
  * update and prolongation), WAVI_KT_CG_UPDATE (axpys and norm partials),
- * WAVI_KT_COLOUR (one smoother colour sweep).  Bursts (events around each
+ * WAVI_KT_COLOUR (one smoother iteration: its four colour sweeps in one
+ * launch).  Bursts (events around each
  * chunk of 32 back-to-back iterations; only chunks in which every launch
  * still iterates are counted, and the count is of launches):
  * WAVI_KT_CG_ITER_MIXED (mixed-precision coarse CG, one whole iteration per

@@ -915,8 +915,8 @@ static int do_inner_update(wavi_ctx *ctx)
 
 static int do_op(wavi_ctx *ctx, const double *xu, const double *xv, double *yu, double *yv)
 {
-    hipLaunchKernelGGL(k_op<double>, flat(ctx->nu + ctx->nv), dim3(256), 0, ctx->stream, ctx->g, coef64(ctx), xu, xv,
-                       yu, yv);
+    hipLaunchKernelGGL(k_op<double>, dim3(ctx->g.nbx, ctx->g.nby), dim3(256), 0, ctx->stream, ctx->g, coef64(ctx),
+                       xu, xv, yu, yv);
     HIPCHK(hipGetLastError());
     return WAVI_OK;
 }
@@ -1011,20 +1011,29 @@ static int do_diag(wavi_ctx *ctx, const VSet<T> &w)
     return WAVI_OK;
 }
 
+// gauss_seidel_smoother! (update_preconditioners.jl:133-150) on w.e with
+// right-hand side w.r and diagonal w.d: nsmooth iterations of the four
+// colours in `order`, one k_smooth4 launch per iteration.  A launch reads
+// one buffer and writes the other, so the iterations alternate between w.e
+// and w.t (free during smoothing); after an odd count the result is copied
+// back into w.e.
 template <typename T>
 static int do_smoother(wavi_ctx *ctx, const VSet<T> &w, const int order[4])
 {
     const GridDims &g = ctx->g;
-    int nthreads_u = ((g.nxu + 1) / 2) * g.nyu, nthreads_v = ((g.nxv + 1) / 2) * g.nyv;
     OpCoef<T> c = coef_of(ctx, w);
-    for (int it = 0; it < ctx->S.nsmooth; ++it)
-        for (int q = 0; q < 4; ++q) {
-            int col = order[q];
-            int n = col <= 2 ? nthreads_u : nthreads_v;
-            ProfScope ps(ctx, WAVI_KT_COLOUR);
-            hipLaunchKernelGGL(k_colour<T>, flat(n), dim3(256), 0, ctx->stream, g, c, w.e.u, w.e.v, w.r.u, w.r.v,
-                               w.d.u, w.d.v, (T)ctx->S.smoother_omega, col);
-        }
+    const int packed = order[0] | order[1] << 4 | order[2] << 8 | order[3] << 12;
+    T *xu = w.e.u, *xv = w.e.v, *yu = w.t.u, *yv = w.t.v;
+    for (int it = 0; it < ctx->S.nsmooth; ++it) {
+        ProfScope ps(ctx, WAVI_KT_COLOUR);
+        hipLaunchKernelGGL(k_smooth4<T>, dim3(g.nbx, g.nby), dim3(256), 0, ctx->stream, g, c, xu, xv, yu, yv, w.r.u,
+                           w.r.v, w.d.u, w.d.v, (T)ctx->S.smoother_omega, packed);
+        std::swap(xu, yu);
+        std::swap(xv, yv);
+    }
+    if (ctx->S.nsmooth & 1)
+        hipLaunchKernelGGL(k_copy_masked_uv<T>, flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream, w.e.u, w.t.u,
+                           ctx->ui, ctx->nu, w.e.v, w.t.v, ctx->vi, ctx->nv);
     HIPCHK(hipGetLastError());
     return WAVI_OK;
 }
@@ -1266,7 +1275,7 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
             if (!ctx->cgm_chunk) {
                 std::lock_guard<std::mutex> lock(g_graph_mu);
                 hipGraph_t gr = nullptr;
-                HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+                HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
                 for (int s = 0; s < CG_CHUNK; ++s) {
                     if (s & 1)
                         launch_cgm_iter<0, false>(ctx, tb, ia);
@@ -1584,8 +1593,8 @@ static int do_linear_solve(wavi_ctx *ctx, const DevVec &b, const DevVec &x, cons
             CHK(dots(ctx, s, 1, au, av, bu, bv, &rz));
         }
         for (;;) {
-            hipLaunchKernelGGL(k_op<double>, flat(nu + nv), dim3(256), 0, ctx->stream, g, coef64(ctx), s.p.u, s.p.v,
-                               s.q.u, s.q.v);
+            hipLaunchKernelGGL(k_op<double>, dim3(g.nbx, g.nby), dim3(256), 0, ctx->stream, g, coef64(ctx), s.p.u,
+                               s.p.v, s.q.u, s.q.v);
             double pq;
             {
                 const double *au[1] = {s.p.u}, *av[1] = {s.p.v}, *bu[1] = {s.q.u}, *bv[1] = {s.q.v};

@@ -893,69 +893,6 @@ __global__ void k_rhs(GridDims g, RhsArgs a)
 // ===========================================================================
 // Operator application, residual, diagonal, colour sweeps.
 
-// y = A x at inner points, 0 elsewhere (flat over u then v points)
-template <typename T>
-__global__ void k_op(GridDims g, OpCoef<T> c, const T *xu, const T *xv, T *yu, T *yv)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nu = g.nxu * g.nyu, nv = g.nxv * g.nyv;
-    GlobalIn<T> in{xu, xv, g.nxu, g.nx};
-    if (t < nu) {
-        int i = t % g.nxu, j = t / g.nxu;
-        yu[t] = c.ui[t] ? op_u(c, in, i, j) : T(0);
-    } else if (t < nu + nv) {
-        int tv = t - nu;
-        int i = tv % g.nxv, j = tv / g.nxv;
-        yv[tv] = c.vi[tv] ? op_v(c, in, i, j) : T(0);
-    }
-}
-
-// r = b - A x at inner points (get_resid!, utilities.jl:549-552); partials of
-// sum r^2 into Pr and, if Pb != nullptr, of sum b^2 into Pb.
-template <typename T>
-__global__ void k_tile_resid(GridDims g, OpCoef<T> c, const T *xu, const T *xv, const T *bu, const T *bv,
-                             T *ru, T *rv, double *Pr, double *Pb)
-{
-    __shared__ double lds[256];
-    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int bx = blockIdx.x, by = blockIdx.y;
-    const int i = bx * TILE + lx, j = by * TILE + ly;
-    GlobalIn<T> in{xu, xv, g.nxu, g.nx};
-    if (bx < g.nbxu && by < g.nbyu) {
-        double r2 = 0.0, b2 = 0.0;
-        if (i < g.nxu && j < g.nyu) {
-            int k = i + g.nxu * j;
-            T r = c.ui[k] ? bu[k] - op_u(c, in, i, j) : T(0);
-            ru[k] = r;
-            r2 = (double)r * (double)r;
-            b2 = (double)bu[k] * (double)bu[k];
-        }
-        double s = tree256(r2, lds);
-        if (threadIdx.x == 0) Pr[bx + g.nbxu * by] = s;
-        if (Pb) {
-            s = tree256(b2, lds);
-            if (threadIdx.x == 0) Pb[bx + g.nbxu * by] = s;
-        }
-    }
-    if (bx < g.nbxv && by < g.nbyv) {
-        double r2 = 0.0, b2 = 0.0;
-        if (i < g.nxv && j < g.nyv) {
-            int k = i + g.nxv * j;
-            T r = c.vi[k] ? bv[k] - op_v(c, in, i, j) : T(0);
-            rv[k] = r;
-            r2 = (double)r * (double)r;
-            b2 = (double)bv[k] * (double)bv[k];
-        }
-        const int o = g.nbxu * g.nbyu + bx + g.nbxv * by;
-        double s = tree256(r2, lds);
-        if (threadIdx.x == 0) Pr[o] = s;
-        if (Pb) {
-            s = tree256(b2, lds);
-            if (threadIdx.x == 0) Pb[o] = s;
-        }
-    }
-}
-
 // diag(A) at inner points: the stencil evaluated on a unit vector, which is
 // what get_op_diag's colour probes (update_preconditioners.jl:92-109) see.
 template <typename T>
@@ -972,43 +909,6 @@ __global__ void k_diag(GridDims g, OpCoef<T> c, T *du, T *dv)
         int i = tv % g.nxv, j = tv / g.nxv;
         DeltaIn<T> in{i, j, 1};
         dv[tv] = c.vi[tv] ? op_v(c, in, i, j) : T(0);
-    }
-}
-
-// One colour of gauss_seidel_smoother! (update_preconditioners.jl:133-150):
-// x[c] += omega * (b - A x)[c] / diag[c].  Colours 1,2 are u points with
-// 1 + mod(i-j, 2) == colour, 3,4 are v points with 3 + mod(i-j, 2).  No point
-// of a colour is in the stencil of another point of the same colour, so the
-// residual evaluated here equals the reference's full recompute.
-template <typename T>
-__global__ void k_colour(GridDims g, OpCoef<T> c, T *xu, T *xv, const T *bu, const T *bv,
-                         const T *du, const T *dv, T omega, int colour)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    GlobalIn<T> in{xu, xv, g.nxu, g.nx};
-    if (colour <= 2) {
-        // half of the u points: row j, every other i
-        const int half = (g.nxu + 1) / 2;
-        int j = t / half, ii = t % half;
-        if (j >= g.nyu) return;
-        int par = (colour - 1);  // mod(i - j, 2) == par
-        int i = 2 * ii + ((j + par) & 1);
-        if (i >= g.nxu) return;
-        int k = i + g.nxu * j;
-        if (!c.ui[k]) return;
-        T r = bu[k] - op_u(c, in, i, j);
-        xu[k] = xu[k] + (omega * r) / du[k];
-    } else {
-        const int half = (g.nxv + 1) / 2;
-        int j = t / half, ii = t % half;
-        if (j >= g.nyv) return;
-        int par = (colour - 3);
-        int i = 2 * ii + ((j + par) & 1);
-        if (i >= g.nxv) return;
-        int k = i + g.nxv * j;
-        if (!c.vi[k]) return;
-        T r = bv[k] - op_v(c, in, i, j);
-        xv[k] = xv[k] + (omega * r) / dv[k];
     }
 }
 
@@ -1214,15 +1114,15 @@ struct CellCoef {
     bool cell, corner;
 };
 
+// Coefficients of h cell (a, b) and its c corner; in_window = false leaves
+// both absent.
 template <typename T>
-__device__ __forceinline__ void cell_prefetch(const GridDims &g, const OpCoef<T> &c, int i0, int j0, int q,
-                                              CellCoef<T> &cc)
+__device__ __forceinline__ void cell_prefetch_at(const GridDims &g, const OpCoef<T> &c, int a, int b,
+                                                 bool in_window, CellCoef<T> &cc)
 {
     const int nx = g.nx, ny = g.ny, nxu = g.nxu;
-    const int x = q % 17, y = q / 17;
-    const int a = i0 - 1 + x, b = j0 - 1 + y;
-    cc.cell = q < 289 && a >= 0 && a < nx && b >= 0 && b < ny;
-    cc.corner = q < 289 && a >= 0 && a < nx - 1 && b >= 0 && b < ny - 1;
+    cc.cell = in_window && a >= 0 && a < nx && b >= 0 && b < ny;
+    cc.corner = in_window && a >= 0 && a < nx - 1 && b >= 0 && b < ny - 1;
     if (cc.cell) {
         const int k = a + nx * b, ku0 = a + nxu * b, kv0 = a + nx * b, kv1 = kv0 + nx;
         cc.hm = c.hm[k];
@@ -1244,13 +1144,19 @@ __device__ __forceinline__ void cell_prefetch(const GridDims &g, const OpCoef<T>
     }
 }
 
-// Intermediates of one cell / corner from the staged inputs (same
-// expressions as op_rxx, op_ryy, op_extra, op_rxy).
 template <typename T>
-__device__ __forceinline__ void cell_compute(const OpCoef<T> &c, const CellCoef<T> &cc, int q, OpTile<T> &s)
+__device__ __forceinline__ void cell_prefetch(const GridDims &g, const OpCoef<T> &c, int i0, int j0, int q,
+                                              CellCoef<T> &cc)
 {
-    if (q >= 289) return;
-    const int x = q % 17, y = q / 17;
+    cell_prefetch_at(g, c, i0 - 1 + q % 17, j0 - 1 + q / 17, q < 289, cc);
+}
+
+// Intermediates of the cell / corner at (x, y) of a staged window whose
+// points U[y][x] and cells share one origin (same expressions as op_rxx,
+// op_ryy, op_extra, op_rxy).
+template <typename T, class S>
+__device__ __forceinline__ void cell_eval(const OpCoef<T> &c, const CellCoef<T> &cc, int x, int y, S &s)
+{
     T rxx = T(0), ryy = T(0), ext = T(0), rxy = T(0);
     if (cc.cell) {
         const T u0 = s.U[y][x], u1 = s.U[y][x + 1], v0 = s.V[y][x], v1 = s.V[y + 1][x];
@@ -1276,6 +1182,13 @@ __device__ __forceinline__ void cell_compute(const OpCoef<T> &c, const CellCoef<
     s.ryy[y][x] = ryy;
     s.ext[y][x] = ext;
     s.rxy[y][x] = rxy;
+}
+
+template <typename T>
+__device__ __forceinline__ void cell_compute(const OpCoef<T> &c, const CellCoef<T> &cc, int q, OpTile<T> &s)
+{
+    if (q >= 289) return;
+    cell_eval(c, cc, q % 17, q / 17, s);
 }
 
 // Coefficients of the thread's own u and v output points.
@@ -1313,9 +1226,10 @@ __device__ __forceinline__ void point_prefetch(const GridDims &g, const OpCoef<T
     }
 }
 
-// fx at the u point (i0 + x, j0 + y)
-template <typename T>
-__device__ __forceinline__ T optile_u(const OpCoef<T> &c, const OpTile<T> &s, const PointCoef<T> &pc, int x, int y)
+// fx at the u point (i0 + x, j0 + y), whose value is s.U[y + 1][x + 1] and
+// whose cell (i, j) is s.rxx[y + 1][x + 1]
+template <typename T, class S>
+__device__ __forceinline__ T optile_u(const OpCoef<T> &c, const S &s, const PointCoef<T> &pc, int x, int y)
 {
     const T d_rxx_dx = c.rdx * s.rxx[y + 1][x + 1] - c.rdx * s.rxx[y + 1][x];
     const T d_rxy_dy = c.rdy * s.rxy[y + 1][x] - c.rdy * s.rxy[y][x];
@@ -1324,9 +1238,9 @@ __device__ __forceinline__ T optile_u(const OpCoef<T> &c, const OpTile<T> &s, co
     return ((d_rxx_dx + d_rxy_dy) - taubx) - pc.hu * d_extra_dx;
 }
 
-// fy at the v point (i0 + x, j0 + y)
-template <typename T>
-__device__ __forceinline__ T optile_v(const OpCoef<T> &c, const OpTile<T> &s, const PointCoef<T> &pc, int x, int y)
+// fy at the v point (i0 + x, j0 + y), with the same window convention
+template <typename T, class S>
+__device__ __forceinline__ T optile_v(const OpCoef<T> &c, const S &s, const PointCoef<T> &pc, int x, int y)
 {
     const T d_ryy_dy = c.rdy * s.ryy[y + 1][x + 1] - c.rdy * s.ryy[y][x + 1];
     const T d_rxy_dx = c.rdx * s.rxy[y][x + 1] - c.rdx * s.rxy[y][x];
@@ -1393,6 +1307,252 @@ __device__ __forceinline__ void optile_apply(const GridDims &g, const OpCoef<T> 
     tu[ly][lx] = (pc.oku && pc.ui) ? optile_u(c, s, pc, lx, ly) : T(0);
     tv[ly][lx] = (pc.okv && pc.vi) ? optile_v(c, s, pc, lx, ly) : T(0);
     __syncthreads();
+}
+
+// ===========================================================================
+// Operator application and residual on 16 x 16 tiles of u and v points: the
+// tile stages its inputs with a one-point halo and its coefficients once
+// (OpTile), computes every intermediate once, and each thread evaluates its
+// u and its v point.  The arithmetic is that of op_u / op_v, so the results
+// equal the per-point forms bit for bit.  Launched on the (nbx, nby) tile
+// grid.
+
+// The coefficients a tile's own u and v points read (no wavelet masks).
+template <typename T>
+__device__ __forceinline__ void point_prefetch_op(const GridDims &g, const OpCoef<T> &c, int i, int j, bool has_u,
+                                                  bool has_v, PointCoef<T> &pc)
+{
+    pc.oku = has_u && i < g.nxu && j < g.nyu;
+    pc.okv = has_v && i < g.nxv && j < g.nyv;
+    pc.ku = i + g.nxu * j;
+    pc.kv = i + g.nxv * j;
+    pc.ui = pc.um = pc.vi = pc.vm = pc.mwu = pc.mwv = false;
+    pc.bu = pc.hu = pc.bv = pc.hv = T(0);
+    if (pc.oku) {
+        pc.ui = c.ui[pc.ku];
+        pc.um = c.um[pc.ku];
+        pc.bu = c.bu[pc.ku];
+        pc.hu = c.hu[pc.ku];
+    }
+    if (pc.okv) {
+        pc.vi = c.vi[pc.kv];
+        pc.vm = c.vm[pc.kv];
+        pc.bv = c.bv[pc.kv];
+        pc.hv = c.hv[pc.kv];
+    }
+}
+
+// Stage the tile's window of x and its intermediates (ends with a barrier).
+template <typename T>
+__device__ __forceinline__ void optile_prepare(const GridDims &g, const OpCoef<T> &c, const T *xu, const T *xv,
+                                               int i0, int j0, OpTile<T> &s)
+{
+    StageRegs<T> sr;
+    CellCoef<T> c0, c1;
+    optile_fetch(g, xu, xv, i0, j0, sr);
+    cell_prefetch(g, c, i0, j0, threadIdx.x, c0);
+    cell_prefetch(g, c, i0, j0, threadIdx.x + 256, c1);
+    optile_store(sr, s);
+    __syncthreads();
+    cell_compute(c, c0, threadIdx.x, s);
+    cell_compute(c, c1, threadIdx.x + 256, s);
+    __syncthreads();
+}
+
+// y = A x at inner points, 0 elsewhere
+template <typename T>
+__global__ void __launch_bounds__(256) k_op(GridDims g, OpCoef<T> c, const T *xu, const T *xv, T *yu, T *yv)
+{
+    __shared__ OpTile<T> s;
+    const int bx = blockIdx.x, by = blockIdx.y;
+    const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
+    const int i0 = bx * TILE, j0 = by * TILE, lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    PointCoef<T> pc;
+    point_prefetch_op(g, c, i0 + lx, j0 + ly, has_u, has_v, pc);
+    optile_prepare(g, c, xu, xv, i0, j0, s);
+    if (pc.oku) yu[pc.ku] = pc.ui ? optile_u(c, s, pc, lx, ly) : T(0);
+    if (pc.okv) yv[pc.kv] = pc.vi ? optile_v(c, s, pc, lx, ly) : T(0);
+}
+
+// r = b - A x at inner points (get_resid!, utilities.jl:549-552); per-tile
+// partials of sum r^2 into Pr and, if Pb != nullptr, of sum b^2 into Pb (u
+// tiles first, then v tiles).
+template <typename T>
+__global__ void __launch_bounds__(256) k_tile_resid(GridDims g, OpCoef<T> c, const T *xu, const T *xv, const T *bu,
+                                                    const T *bv, T *ru, T *rv, double *Pr, double *Pb)
+{
+    __shared__ OpTile<T> s;
+    __shared__ double lds[8];
+    const int bx = blockIdx.x, by = blockIdx.y;
+    const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
+    const int i0 = bx * TILE, j0 = by * TILE, lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    PointCoef<T> pc;
+    point_prefetch_op(g, c, i0 + lx, j0 + ly, has_u, has_v, pc);
+    const T b_u = pc.oku ? bu[pc.ku] : T(0), b_v = pc.okv ? bv[pc.kv] : T(0);
+    optile_prepare(g, c, xu, xv, i0, j0, s);
+    if (has_u) {
+        double r2 = 0.0, b2 = 0.0;
+        if (pc.oku) {
+            const T r = pc.ui ? b_u - optile_u(c, s, pc, lx, ly) : T(0);
+            ru[pc.ku] = r;
+            r2 = (double)r * (double)r;
+            b2 = (double)b_u * (double)b_u;
+        }
+        tree256x2(r2, b2, lds);
+        if (threadIdx.x == 0) {
+            Pr[bx + g.nbxu * by] = r2;
+            if (Pb) Pb[bx + g.nbxu * by] = b2;
+        }
+    }
+    if (has_v) {
+        double r2 = 0.0, b2 = 0.0;
+        if (pc.okv) {
+            const T r = pc.vi ? b_v - optile_v(c, s, pc, lx, ly) : T(0);
+            rv[pc.kv] = r;
+            r2 = (double)r * (double)r;
+            b2 = (double)b_v * (double)b_v;
+        }
+        tree256x2(r2, b2, lds);
+        if (threadIdx.x == 0) {
+            const int o = g.nbxu * g.nbyu + bx + g.nbxv * by;
+            Pr[o] = r2;
+            if (Pb) Pb[o] = b2;
+        }
+    }
+}
+
+// ===========================================================================
+// One iteration of gauss_seidel_smoother! (update_preconditioners.jl:133-150)
+// in one launch: the four colour sweeps x[c] += omega (b - A x)[c] / diag[c]
+// in the order given, on the u and v points of one 16 x 16 tile.  Colours 1
+// and 2 are the u points with 1 + mod(i - j, 2) == colour, 3 and 4 the v
+// points with 3 + mod(i - j, 2) (get_preconditioner,
+// update_preconditioners.jl:7-43).  A point's update reads its neighbours
+// within one point, so the tile stages a window with a halo of SM_H = 4
+// points; sweep s updates its colour inside the window shrunk by s points,
+// whose neighbours hold the values of sweep s - 1, and the fourth sweep
+// covers the tile.  Halo points are updated by the neighbouring tiles with
+// the same inputs and arithmetic, and only the tile's own points are
+// written, into a second buffer (xu_out, xv_out): the other tiles read the
+// inputs of the iteration from the first.  Every update has the arithmetic of
+// op_u / op_v and of the one-colour sweep, so the result is bit-identical to
+// four sweeps of one colour each.
+constexpr int SM_H = 4;                  // halo points
+constexpr int SM_P = TILE + 2 * SM_H;    // window points per direction (24)
+constexpr int SM_C = SM_P - 1;           // cells per direction computed each sweep (23)
+
+// Points at [py][px] = (i0 - SM_H - 1 + px, j0 - SM_H - 1 + py), px, py in
+// 1..SM_P (index 0 unused); cells and corners with the same origin, the
+// SM_C x SM_C of them at indices 1..SM_C computed.  Window position (x, y)
+// of a point is px = x + 1, py = y + 1, as optile_u / optile_v expect.
+template <typename T>
+struct SmoothWin {
+    T U[SM_P + 1][SM_P + 2];
+    T V[SM_P + 1][SM_P + 2];
+    T rxx[SM_C + 1][SM_C + 2];
+    T ryy[SM_C + 1][SM_C + 2];
+    T ext[SM_C + 1][SM_C + 2];
+    T rxy[SM_C + 1][SM_C + 2];
+};
+
+// The coefficients one window point's update reads.
+template <typename T>
+struct SmoothPoint {
+    PointCoef<T> pc;
+    T bu, du, bv, dv;  // right-hand side and diagonal at the u and the v point
+};
+
+constexpr int SM_SLOTS = (SM_P * SM_P + 255) / 256;  // window points per thread (3)
+constexpr int SM_CSLOTS = (SM_C * SM_C + 255) / 256; // cells per thread (3)
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_smooth4(GridDims g, OpCoef<T> c, const T *xu, const T *xv, T *xu_out,
+                                                 T *xv_out, const T *bu, const T *bv, const T *du, const T *dv,
+                                                 T omega, int order)
+{
+    __shared__ SmoothWin<T> s;
+    const int i0 = blockIdx.x * TILE, j0 = blockIdx.y * TILE;
+    const int ox = i0 - SM_H, oy = j0 - SM_H;  // window position 0
+    // stage the window and load the coefficients the four sweeps read
+    SmoothPoint<T> sp[SM_SLOTS];
+#pragma unroll
+    for (int k = 0; k < SM_SLOTS; ++k) {
+        const int q = threadIdx.x + 256 * k;
+        const int x = q % SM_P, y = q / SM_P, i = ox + x, j = oy + y;
+        PointCoef<T> &pc = sp[k].pc;
+        pc.oku = q < SM_P * SM_P && i >= 0 && j >= 0 && i < g.nxu && j < g.nyu;
+        pc.okv = q < SM_P * SM_P && i >= 0 && j >= 0 && i < g.nxv && j < g.nyv;
+        pc.ku = i + g.nxu * j;
+        pc.kv = i + g.nxv * j;
+        pc.ui = pc.um = pc.vi = pc.vm = pc.mwu = pc.mwv = false;
+        pc.bu = pc.hu = pc.bv = pc.hv = T(0);
+        sp[k].bu = sp[k].du = sp[k].bv = sp[k].dv = T(0);
+        T u = T(0), v = T(0);
+        if (pc.oku) {
+            u = xu[pc.ku];
+            pc.ui = c.ui[pc.ku];
+            pc.um = c.um[pc.ku];
+            pc.bu = c.bu[pc.ku];
+            pc.hu = c.hu[pc.ku];
+            sp[k].bu = bu[pc.ku];
+            sp[k].du = du[pc.ku];
+        }
+        if (pc.okv) {
+            v = xv[pc.kv];
+            pc.vi = c.vi[pc.kv];
+            pc.vm = c.vm[pc.kv];
+            pc.bv = c.bv[pc.kv];
+            pc.hv = c.hv[pc.kv];
+            sp[k].bv = bv[pc.kv];
+            sp[k].dv = dv[pc.kv];
+        }
+        if (q < SM_P * SM_P) {
+            s.U[y + 1][x + 1] = u;
+            s.V[y + 1][x + 1] = v;
+        }
+    }
+    CellCoef<T> cc[SM_CSLOTS];
+#pragma unroll
+    for (int k = 0; k < SM_CSLOTS; ++k) {
+        const int q = threadIdx.x + 256 * k;
+        cell_prefetch_at(g, c, ox + q % SM_C, oy + q / SM_C, q < SM_C * SM_C, cc[k]);
+    }
+    for (int sw = 1; sw <= 4; ++sw) {
+        const int colour = (order >> (4 * (sw - 1))) & 15;
+        __syncthreads();  // the window holds the values after sweep sw - 1
+#pragma unroll
+        for (int k = 0; k < SM_CSLOTS; ++k) {
+            const int q = threadIdx.x + 256 * k;
+            if (q < SM_C * SM_C) cell_eval(c, cc[k], 1 + q % SM_C, 1 + q / SM_C, s);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SM_SLOTS; ++k) {
+            const int q = threadIdx.x + 256 * k;
+            const int x = q % SM_P, y = q / SM_P;
+            if (q >= SM_P * SM_P || x < sw || y < sw || x >= SM_P - sw || y >= SM_P - sw) continue;
+            const PointCoef<T> &pc = sp[k].pc;
+            const int i = ox + x, j = oy + y;
+            if (colour <= 2) {
+                if (pc.oku && pc.ui && ((i - j) & 1) == colour - 1) {
+                    const T r = sp[k].bu - optile_u(c, s, pc, x, y);
+                    s.U[y + 1][x + 1] = s.U[y + 1][x + 1] + (omega * r) / sp[k].du;
+                }
+            } else if (pc.okv && pc.vi && ((i - j) & 1) == colour - 3) {
+                const T r = sp[k].bv - optile_v(c, s, pc, x, y);
+                s.V[y + 1][x + 1] = s.V[y + 1][x + 1] + (omega * r) / sp[k].dv;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SM_SLOTS; ++k) {
+        const int q = threadIdx.x + 256 * k;
+        const int x = q % SM_P, y = q / SM_P;
+        if (q >= SM_P * SM_P || x < SM_H || y < SM_H || x >= SM_H + TILE || y >= SM_H + TILE) continue;
+        if (sp[k].pc.oku) xu_out[sp[k].pc.ku] = s.U[y + 1][x + 1];
+        if (sp[k].pc.okv) xv_out[sp[k].pc.kv] = s.V[y + 1][x + 1];
+    }
 }
 
 // ===========================================================================
