@@ -590,6 +590,8 @@ def main():
                     help="with --ensemble: forward + adjoint gradient per member")
     ap.add_argument("--adjoint", action="store_true",
                     help="time the basal-drag gradient (FP64 and mixed) and check it by finite differences")
+    ap.add_argument("--transient", action="store_true",
+                    help="time only the 80x10 channel transient (200 steps, GPU FP64 and mixed, CPU oracle)")
     ap.add_argument("--fd-directions", type=int, default=16)
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="profiling runs only: issue one device-to-device copy of this size first")
@@ -601,6 +603,9 @@ def main():
     W = _package()
     if args.adjoint:
         print(json.dumps(adjoint_run(W, args.nx, args.ny, args.steps, args.warmup, args.fd_directions)), flush=True)
+        return
+    if args.transient:
+        print(json.dumps(transient_run(W, runs=max(args.steps, 1))), flush=True)
         return
     if args.ensemble:
         prec = "fp64" if args.precision == "fp64" else "mixed"

@@ -307,6 +307,20 @@ int wavi_vcycle(wavi_ctx *ctx, const double *ru, const double *rv,
  * 2: transposed inverse idwt^T (UWavelets.jl:49). */
 int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out);
 
+/* Tile shape of the smoother kernel (one launch per smoothing iteration
+ * runs the four colour sweeps of gauss_seidel_smoother!,
+ * src/Wavelets/update_preconditioners.jl:133-150, on tiles of tx x ty
+ * points).  The shapes are 16 x 16, 32 x 16 and 16 x 32; every shape gives
+ * the same bits.  By default the library's autotuner picks one on the
+ * context's first V-cycle: it times each shape on the context's own grid
+ * and keeps the fastest, and remembers the choice for later contexts of the
+ * same grid size and precision in the process.  wavi_set_smoother_tile
+ * fixes a shape (0, 0: tune again); wavi_smoother_tile reports the shape in
+ * use for the context's V-cycle precision (0, 0 before the first V-cycle).
+ * No reference counterpart. */
+int wavi_set_smoother_tile(wavi_ctx *ctx, int tx, int ty);
+int wavi_smoother_tile(const wavi_ctx *ctx, int *tx, int *ty);
+
 /* Kernel timing with HIP events on the context's stream, for measurement.
  * Sampled single launches (events around every 16th launch, so each time
  * includes the cost of the two event records): WAVI_KT_CG_APPLY (FP64

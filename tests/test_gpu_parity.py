@@ -295,3 +295,53 @@ def test_update_velocities_other_haar_levels(W, oracle, levels):
     orep = o.update_velocities()
     assert rep.as_dict()["coarse_cg_iters"] == orep["coarse_cg_iters"]
     _assert_fields_equal(W, m, o)
+
+
+@pytest.mark.parametrize("tile", [(16, 16), (32, 16), (16, 32)])
+def test_smoother_tile_shapes_bit_equal(W, oracle, tile):
+    """Each tile shape of the smoother kernel (the autotuner's candidates)
+    gives the oracle's bits: a 3-Picard solve on 70 x 40, whose tiles are
+    partial in both directions for every shape."""
+    _step("smoother tile %dx%d" % tile)
+    m = channel(W, nx=70, ny=40, maxiter_picard=3)
+    o = oracle.OracleModel(m)
+    spec = _gpu_model(W, m)
+    lib = spec._lib
+    assert lib.wavi_set_smoother_tile(spec.ctx, *tile) == 0
+    assert lib.wavi_update_geometry(spec.ctx) == 0
+    o.update_geometry()
+    rep = W._lib.SolveReport()
+    assert lib.wavi_update_velocities(spec.ctx, ctypes.byref(rep)) == 0
+    orep = o.update_velocities()
+    assert rep.as_dict() == orep
+    _assert_fields_equal(W, m, o)
+    tx, ty = ctypes.c_int(), ctypes.c_int()
+    assert lib.wavi_smoother_tile(spec.ctx, ctypes.byref(tx), ctypes.byref(ty)) == 0
+    assert (tx.value, ty.value) == tile
+
+
+def test_smoother_autotuner_and_shapes_in_mixed_precision(W):
+    """The mixed solve gives the same bits whatever the smoother tile shape;
+    the autotuner picks one of the shapes; an unknown shape is rejected."""
+    _step("smoother autotuner, mixed")
+    runs = []
+    for tile in ((0, 0), (16, 16), (32, 16), (16, 32)):
+        m = channel(W, nx=96, ny=40, precision="mixed", maxiter_picard=3)
+        spec = _gpu_model(W, m)
+        lib = spec._lib
+        assert lib.wavi_set_smoother_tile(spec.ctx, *tile) == 0
+        assert lib.wavi_update_geometry(spec.ctx) == 0
+        rep = W._lib.SolveReport()
+        assert lib.wavi_update_velocities(spec.ctx, ctypes.byref(rep)) == 0
+        W.pull_fields(m)
+        tx, ty = ctypes.c_int(), ctypes.c_int()
+        assert lib.wavi_smoother_tile(spec.ctx, ctypes.byref(tx), ctypes.byref(ty)) == 0
+        if tile == (0, 0):
+            assert (tx.value, ty.value) in ((16, 16), (32, 16), (16, 32))
+            _step("autotuned smoother tile %dx%d" % (tx.value, ty.value))
+        else:
+            assert (tx.value, ty.value) == tile
+        runs.append((m.fields.gu.u.copy(), m.fields.gv.v.copy(), rep.as_dict()))
+    for u, v, r in runs[1:]:
+        assert np.array_equal(u, runs[0][0]) and np.array_equal(v, runs[0][1]) and r == runs[0][2]
+    assert lib.wavi_set_smoother_tile(spec.ctx, 8, 8) == W._lib.WAVI_EARG

@@ -151,6 +151,8 @@ SIGNATURES = {
     "wavi_op_diag": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "wavi_vcycle": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int32)]),
     "wavi_haar": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "wavi_set_smoother_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "wavi_smoother_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "wavi_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "wavi_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "wavi_math_probe": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]),

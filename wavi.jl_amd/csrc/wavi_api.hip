@@ -10,11 +10,13 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
 #include <system_error>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/wavi_hip.h"
@@ -106,6 +108,8 @@ struct wavi_ctx {
     CGMHostFlag *d_flag = nullptr;  // its device address
     hipEvent_t cg_ev[3] = {};       // chunk completion events of the mixed coarse CG
     hipEvent_t ev_ls = nullptr, ev_ls_done = nullptr;  // set-up and batch end in a lockstep batch without allocation
+    hipEvent_t ev_tune[2] = {};     // timing events of the smoother's tile-shape tuning
+    int smooth_shape[2] = {-1, -1}; // smoother tile shape by V-cycle precision (0 FP64, 1 FP32); -1: tune
     hipGraphExec_t cgm_chunk = nullptr;  // 32 k_cgm_iter launches (parities 1, 0, ...) as one graph
     double *Pr0 = nullptr, *Pr1 = nullptr, *Pd = nullptr, *Pb = nullptr, *Pn = nullptr;
     double *quad = nullptr;  // 4 * nh pos_fraction quadrants
@@ -629,6 +633,7 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
     CREATECHK(hipHostGetDevicePointer((void **)&ctx->d_flag, ctx->h_flag, 0));
     for (hipEvent_t &e : ctx->cg_ev) CREATECHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     CREATECHK(hipEventCreateWithFlags(&ctx->ev_ls, hipEventDisableTiming));
+    for (hipEvent_t &e : ctx->ev_tune) CREATECHK(hipEventCreate(&e));
     CREATECHK(hipEventCreateWithFlags(&ctx->ev_ls_done, hipEventDisableTiming));
     if (precision_mode == WAVI_PRECISION_MIXED) {
         VSet<float> &w = ctx->v32;
@@ -711,7 +716,7 @@ static void ctx_free(wavi_ctx *ctx)
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     for (hipEvent_t e : ctx->cg_ev)
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {ctx->ev_ls, ctx->ev_ls_done})
+    for (hipEvent_t e : {ctx->ev_ls, ctx->ev_ls_done, ctx->ev_tune[0], ctx->ev_tune[1]})
         if (e) (void)hipEventDestroy(e);
     if (ctx->st) (void)hipFree(ctx->st);
     if (ctx->tab) (void)hipFree(ctx->tab);
@@ -1011,6 +1016,80 @@ static int do_diag(wavi_ctx *ctx, const VSet<T> &w)
     return WAVI_OK;
 }
 
+// Tile shapes of the smoother kernel k_smooth4<T, TX, TY>.
+constexpr int SMOOTH_SHAPES = 3;
+static const int SMOOTH_TX[SMOOTH_SHAPES] = {16, 32, 16};
+static const int SMOOTH_TY[SMOOTH_SHAPES] = {16, 16, 32};
+
+template <typename T>
+static void launch_smooth(wavi_ctx *ctx, int shape, const OpCoef<T> &c, const VSet<T> &w, const T *xu, const T *xv,
+                          T *yu, T *yv, int packed)
+{
+    const GridDims &g = ctx->g;
+    const int tx = SMOOTH_TX[shape], ty = SMOOTH_TY[shape];
+    const dim3 grid((unsigned)((std::max(g.nxu, g.nxv) + tx - 1) / tx), (unsigned)((std::max(g.nyu, g.nyv) + ty - 1) / ty));
+    const T om = (T)ctx->S.smoother_omega;
+    switch (shape) {
+    case 1:
+        hipLaunchKernelGGL((k_smooth4<T, 32, 16>), grid, dim3(256), 0, ctx->stream, g, c, xu, xv, yu, yv, w.r.u,
+                           w.r.v, w.d.u, w.d.v, om, packed);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_smooth4<T, 16, 32>), grid, dim3(256), 0, ctx->stream, g, c, xu, xv, yu, yv, w.r.u,
+                           w.r.v, w.d.u, w.d.v, om, packed);
+        break;
+    default:
+        hipLaunchKernelGGL((k_smooth4<T, 16, 16>), grid, dim3(256), 0, ctx->stream, g, c, xu, xv, yu, yv, w.r.u,
+                           w.r.v, w.d.u, w.d.v, om, packed);
+        break;
+    }
+}
+
+// The autotuner's results, shared by the contexts of one process: the
+// fastest smoother tile shape per (nx, ny, V-cycle precision).
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int>, int> g_smooth_tuned;
+
+// The smoother tile shape of the context, chosen on first use: the shape
+// set by wavi_set_smoother_tile, else one tuned before for a grid of this
+// size and precision, else measured now: after a warm-up launch of each,
+// three launches of every candidate shape from the current iterate w.e into
+// w.t (free while smoothing; the results are not used) between two events,
+// and the fastest shape is kept.  Every shape gives the same bits, so the
+// choice never changes a result.
+template <typename T>
+static int smooth_shape(wavi_ctx *ctx, const VSet<T> &w, const OpCoef<T> &c, int packed)
+{
+    const int pi = sizeof(T) == sizeof(double) ? 0 : 1;
+    if (ctx->smooth_shape[pi] >= 0) return ctx->smooth_shape[pi];
+    const auto key = std::make_tuple(ctx->g.nx, ctx->g.ny, pi);
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_smooth_tuned.find(key);
+        if (it != g_smooth_tuned.end()) return ctx->smooth_shape[pi] = it->second;
+    }
+    int best = 0;
+    float best_ms = 0.f;
+    for (int sh = 0; sh < SMOOTH_SHAPES; ++sh) {
+        launch_smooth(ctx, sh, c, w, w.e.u, w.e.v, w.t.u, w.t.v, packed);
+        if (hipEventRecord(ctx->ev_tune[0], ctx->stream) != hipSuccess) return 0;
+        for (int r = 0; r < 3; ++r) launch_smooth(ctx, sh, c, w, w.e.u, w.e.v, w.t.u, w.t.v, packed);
+        if (hipEventRecord(ctx->ev_tune[1], ctx->stream) != hipSuccess) return 0;
+        if (hipEventSynchronize(ctx->ev_tune[1]) != hipSuccess) return 0;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev_tune[0], ctx->ev_tune[1]) != hipSuccess) return 0;
+        if (sh == 0 || ms < best_ms) {
+            best = sh;
+            best_ms = ms;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_smooth_tuned.emplace(key, best);
+    }
+    return ctx->smooth_shape[pi] = best;
+}
+
 // gauss_seidel_smoother! (update_preconditioners.jl:133-150) on w.e with
 // right-hand side w.r and diagonal w.d: nsmooth iterations of the four
 // colours in `order`, one k_smooth4 launch per iteration.  A launch reads
@@ -1020,14 +1099,13 @@ static int do_diag(wavi_ctx *ctx, const VSet<T> &w)
 template <typename T>
 static int do_smoother(wavi_ctx *ctx, const VSet<T> &w, const int order[4])
 {
-    const GridDims &g = ctx->g;
     OpCoef<T> c = coef_of(ctx, w);
     const int packed = order[0] | order[1] << 4 | order[2] << 8 | order[3] << 12;
+    const int shape = smooth_shape(ctx, w, c, packed);
     T *xu = w.e.u, *xv = w.e.v, *yu = w.t.u, *yv = w.t.v;
     for (int it = 0; it < ctx->S.nsmooth; ++it) {
         ProfScope ps(ctx, WAVI_KT_COLOUR);
-        hipLaunchKernelGGL(k_smooth4<T>, dim3(g.nbx, g.nby), dim3(256), 0, ctx->stream, g, c, xu, xv, yu, yv, w.r.u,
-                           w.r.v, w.d.u, w.d.v, (T)ctx->S.smoother_omega, packed);
+        launch_smooth(ctx, shape, c, w, xu, xv, yu, yv, packed);
         std::swap(xu, yu);
         std::swap(xv, yv);
     }
@@ -2650,6 +2728,30 @@ int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out)
     HIPCHK(hipStreamSynchronize(ctx->stream));
     std::lock_guard<std::mutex> lock(g_graph_mu);
     HIPCHK(hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost));
+    return WAVI_OK;
+}
+
+int wavi_set_smoother_tile(wavi_ctx *ctx, int tx, int ty)
+{
+    if (!ctx) return WAVI_EARG;
+    int shape = -1;
+    for (int k = 0; k < SMOOTH_SHAPES; ++k)
+        if (SMOOTH_TX[k] == tx && SMOOTH_TY[k] == ty) shape = k;
+    if (shape < 0 && !(tx == 0 && ty == 0)) {
+        ctx->err = "wavi_set_smoother_tile: the tile shapes are 16 x 16, 32 x 16 and 16 x 32 (0 x 0: tune)";
+        return WAVI_EARG;
+    }
+    ctx->smooth_shape[0] = ctx->smooth_shape[1] = shape;
+    return WAVI_OK;
+}
+
+int wavi_smoother_tile(const wavi_ctx *ctx, int *tx, int *ty)
+{
+    if (!ctx || !tx || !ty) return WAVI_EARG;
+    const int pi = ctx->precision == WAVI_PRECISION_MIXED ? 1 : 0;
+    const int sh = ctx->smooth_shape[pi];
+    *tx = sh >= 0 ? SMOOTH_TX[sh] : 0;
+    *ty = sh >= 0 ? SMOOTH_TY[sh] : 0;
     return WAVI_OK;
 }
 

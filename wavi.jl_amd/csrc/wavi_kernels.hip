@@ -1424,7 +1424,7 @@ __global__ void __launch_bounds__(256) k_tile_resid(GridDims g, OpCoef<T> c, con
 // ===========================================================================
 // One iteration of gauss_seidel_smoother! (update_preconditioners.jl:133-150)
 // in one launch: the four colour sweeps x[c] += omega (b - A x)[c] / diag[c]
-// in the order given, on the u and v points of one 16 x 16 tile.  Colours 1
+// in the order given, on the u and v points of one tile.  Colours 1
 // and 2 are the u points with 1 + mod(i - j, 2) == colour, 3 and 4 the v
 // points with 3 + mod(i - j, 2) (get_preconditioner,
 // update_preconditioners.jl:7-43).  A point's update reads its neighbours
@@ -1437,22 +1437,30 @@ __global__ void __launch_bounds__(256) k_tile_resid(GridDims g, OpCoef<T> c, con
 // inputs of the iteration from the first.  Every update has the arithmetic of
 // op_u / op_v and of the one-colour sweep, so the result is bit-identical to
 // four sweeps of one colour each.
-constexpr int SM_H = 4;                  // halo points
-constexpr int SM_P = TILE + 2 * SM_H;    // window points per direction (24)
-constexpr int SM_C = SM_P - 1;           // cells per direction computed each sweep (23)
+constexpr int SM_H = 4;  // halo points
 
-// Points at [py][px] = (i0 - SM_H - 1 + px, j0 - SM_H - 1 + py), px, py in
-// 1..SM_P (index 0 unused); cells and corners with the same origin, the
-// SM_C x SM_C of them at indices 1..SM_C computed.  Window position (x, y)
+// A TX x TY tile: window of PX x PY points (the tile and its halo), of which
+// the CX x CY cells starting one point in are computed.  Points at
+// [py][px] = (i0 - SM_H - 1 + px, j0 - SM_H - 1 + py), px in 1..PX (index 0
+// unused); cells and corners with the same origin.  Window position (x, y)
 // of a point is px = x + 1, py = y + 1, as optile_u / optile_v expect.
-template <typename T>
+template <int TX, int TY>
+struct SmoothShape {
+    static constexpr int PX = TX + 2 * SM_H, PY = TY + 2 * SM_H;
+    static constexpr int CX = PX - 1, CY = PY - 1;
+    static constexpr int SLOTS = (PX * PY + 255) / 256;   // window points per thread
+    static constexpr int CSLOTS = (CX * CY + 255) / 256;  // cells per thread
+};
+
+template <typename T, int TX, int TY>
 struct SmoothWin {
-    T U[SM_P + 1][SM_P + 2];
-    T V[SM_P + 1][SM_P + 2];
-    T rxx[SM_C + 1][SM_C + 2];
-    T ryy[SM_C + 1][SM_C + 2];
-    T ext[SM_C + 1][SM_C + 2];
-    T rxy[SM_C + 1][SM_C + 2];
+    using S = SmoothShape<TX, TY>;
+    T U[S::PY + 1][S::PX + 2];
+    T V[S::PY + 1][S::PX + 2];
+    T rxx[S::CY + 1][S::CX + 2];
+    T ryy[S::CY + 1][S::CX + 2];
+    T ext[S::CY + 1][S::CX + 2];
+    T rxy[S::CY + 1][S::CX + 2];
 };
 
 // The coefficients one window point's update reads.
@@ -1462,26 +1470,27 @@ struct SmoothPoint {
     T bu, du, bv, dv;  // right-hand side and diagonal at the u and the v point
 };
 
-constexpr int SM_SLOTS = (SM_P * SM_P + 255) / 256;  // window points per thread (3)
-constexpr int SM_CSLOTS = (SM_C * SM_C + 255) / 256; // cells per thread (3)
-
-template <typename T>
+// Tile shapes of the smoother (the autotuner's candidates): 16 x 16, 32 x 16
+// and 16 x 32 points.  Any shape gives the same bits: a point's update does
+// not depend on the tile it is computed in.
+template <typename T, int TX, int TY>
 __global__ void __launch_bounds__(256) k_smooth4(GridDims g, OpCoef<T> c, const T *xu, const T *xv, T *xu_out,
                                                  T *xv_out, const T *bu, const T *bv, const T *du, const T *dv,
                                                  T omega, int order)
 {
-    __shared__ SmoothWin<T> s;
-    const int i0 = blockIdx.x * TILE, j0 = blockIdx.y * TILE;
+    using S = SmoothShape<TX, TY>;
+    __shared__ SmoothWin<T, TX, TY> s;
+    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
     const int ox = i0 - SM_H, oy = j0 - SM_H;  // window position 0
     // stage the window and load the coefficients the four sweeps read
-    SmoothPoint<T> sp[SM_SLOTS];
+    SmoothPoint<T> sp[S::SLOTS];
 #pragma unroll
-    for (int k = 0; k < SM_SLOTS; ++k) {
+    for (int k = 0; k < S::SLOTS; ++k) {
         const int q = threadIdx.x + 256 * k;
-        const int x = q % SM_P, y = q / SM_P, i = ox + x, j = oy + y;
+        const int x = q % S::PX, y = q / S::PX, i = ox + x, j = oy + y;
         PointCoef<T> &pc = sp[k].pc;
-        pc.oku = q < SM_P * SM_P && i >= 0 && j >= 0 && i < g.nxu && j < g.nyu;
-        pc.okv = q < SM_P * SM_P && i >= 0 && j >= 0 && i < g.nxv && j < g.nyv;
+        pc.oku = q < S::PX * S::PY && i >= 0 && j >= 0 && i < g.nxu && j < g.nyu;
+        pc.okv = q < S::PX * S::PY && i >= 0 && j >= 0 && i < g.nxv && j < g.nyv;
         pc.ku = i + g.nxu * j;
         pc.kv = i + g.nxv * j;
         pc.ui = pc.um = pc.vi = pc.vm = pc.mwu = pc.mwv = false;
@@ -1506,31 +1515,36 @@ __global__ void __launch_bounds__(256) k_smooth4(GridDims g, OpCoef<T> c, const 
             sp[k].bv = bv[pc.kv];
             sp[k].dv = dv[pc.kv];
         }
-        if (q < SM_P * SM_P) {
+        if (q < S::PX * S::PY) {
             s.U[y + 1][x + 1] = u;
             s.V[y + 1][x + 1] = v;
         }
     }
-    CellCoef<T> cc[SM_CSLOTS];
+    // cell (a, b) = (ox + q % CX, oy + q / CX) at [1 + q / CX][1 + q % CX]
+    CellCoef<T> cc[S::CSLOTS];
 #pragma unroll
-    for (int k = 0; k < SM_CSLOTS; ++k) {
+    for (int k = 0; k < S::CSLOTS; ++k) {
         const int q = threadIdx.x + 256 * k;
-        cell_prefetch_at(g, c, ox + q % SM_C, oy + q / SM_C, q < SM_C * SM_C, cc[k]);
+        cell_prefetch_at(g, c, ox + q % S::CX, oy + q / S::CX, q < S::CX * S::CY, cc[k]);
     }
     for (int sw = 1; sw <= 4; ++sw) {
         const int colour = (order >> (4 * (sw - 1))) & 15;
         __syncthreads();  // the window holds the values after sweep sw - 1
+        // sweep sw updates window positions [sw, P - sw), which read the
+        // cells at indices [sw, P - sw]
 #pragma unroll
-        for (int k = 0; k < SM_CSLOTS; ++k) {
+        for (int k = 0; k < S::CSLOTS; ++k) {
             const int q = threadIdx.x + 256 * k;
-            if (q < SM_C * SM_C) cell_eval(c, cc[k], 1 + q % SM_C, 1 + q / SM_C, s);
+            const int cx = 1 + q % S::CX, cy = 1 + q / S::CX;
+            if (q < S::CX * S::CY && cx >= sw && cy >= sw && cx <= S::PX - sw && cy <= S::PY - sw)
+                cell_eval(c, cc[k], cx, cy, s);
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < SM_SLOTS; ++k) {
+        for (int k = 0; k < S::SLOTS; ++k) {
             const int q = threadIdx.x + 256 * k;
-            const int x = q % SM_P, y = q / SM_P;
-            if (q >= SM_P * SM_P || x < sw || y < sw || x >= SM_P - sw || y >= SM_P - sw) continue;
+            const int x = q % S::PX, y = q / S::PX;
+            if (q >= S::PX * S::PY || x < sw || y < sw || x >= S::PX - sw || y >= S::PY - sw) continue;
             const PointCoef<T> &pc = sp[k].pc;
             const int i = ox + x, j = oy + y;
             if (colour <= 2) {
@@ -1546,13 +1560,84 @@ __global__ void __launch_bounds__(256) k_smooth4(GridDims g, OpCoef<T> c, const 
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < SM_SLOTS; ++k) {
+    for (int k = 0; k < S::SLOTS; ++k) {
         const int q = threadIdx.x + 256 * k;
-        const int x = q % SM_P, y = q / SM_P;
-        if (q >= SM_P * SM_P || x < SM_H || y < SM_H || x >= SM_H + TILE || y >= SM_H + TILE) continue;
+        const int x = q % S::PX, y = q / S::PX;
+        if (q >= S::PX * S::PY || x < SM_H || y < SM_H || x >= SM_H + TX || y >= SM_H + TY) continue;
         if (sp[k].pc.oku) xu_out[sp[k].pc.ku] = s.U[y + 1][x + 1];
         if (sp[k].pc.okv) xv_out[sp[k].pc.kv] = s.V[y + 1][x + 1];
     }
+}
+
+// ===========================================================================
+// Restricted additive Schwarz (SharedMemorySpec, src/SharedMemorySpec.jl):
+// moving velocities between the full grid and a subdomain context on the
+// same device.  A subdomain covers the cells [i0, i0 + nxg) x [j0, j0 + nyg)
+// of the full grid (its u points i0 .. i0 + nxg, its v points j0 .. j0 + nyg).
+struct RasPart {
+    int i0, j0, nxg, nyg;
+    int leave_i1, leave_im, leave_j1, leave_jn;  // edges where the partition of unity stays 1
+    int overlap;
+};
+
+// schwarzRestrictVelocities! (SharedMemorySpec.jl:306-331): the subdomain's
+// u and v are the full model's on its points (blockIdx.y 0: u, 1: v).
+__global__ void k_ras_restrict(RasPart p, int nxuf, int nxvf, const double *Uf, const double *Vf, double *Ug,
+                               double *Vg)
+{
+    const bool v = blockIdx.y != 0;
+    const int mx = v ? p.nxg : p.nxg + 1, my = v ? p.nyg + 1 : p.nyg;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= mx * my) return;
+    const int i = k % mx, j = k / mx;
+    if (v)
+        Vg[k] = Vf[(p.i0 + i) + nxvf * (p.j0 + j)];
+    else
+        Ug[k] = Uf[(p.i0 + i) + nxuf * (p.j0 + j)];
+}
+
+// One factor of schwarzPartitionOfUnity (SharedMemorySpec.jl:395-407) at the
+// 1-based index i of m: min(1, (i - 1) / ov, (m - i) / ov), an edge left out
+// where it is kept at 1.
+__device__ __forceinline__ double ras_pou(int i, int m, bool leave_lo, bool leave_hi, int ov)
+{
+    const double lo = leave_lo ? INFINITY : (double)(i - 1) / (double)ov;
+    const double hi = leave_hi ? INFINITY : (double)(m - i) / (double)ov;
+    return fmin(1.0, fmin(lo, hi));
+}
+
+// schwarzProlongVelocities! (SharedMemorySpec.jl:351-382) for one
+// subdomain: u += ((1 - damping) pou) u_g on its points, with the partition
+// of unity ramping over 2 overlap points across u's x edges and 2 overlap - 1
+// across its y edges (the other way round for v).
+__global__ void k_ras_prolong(RasPart p, int nxuf, int nxvf, double *Uf, double *Vf, const double *Ug,
+                              const double *Vg, double one_minus_damping)
+{
+    const bool v = blockIdx.y != 0;
+    const int mx = v ? p.nxg : p.nxg + 1, my = v ? p.nyg + 1 : p.nyg;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= mx * my) return;
+    const int i = k % mx, j = k / mx;
+    const int ovx = v ? 2 * p.overlap - 1 : 2 * p.overlap, ovy = v ? 2 * p.overlap : 2 * p.overlap - 1;
+    const double pou = ras_pou(i + 1, mx, p.leave_i1, p.leave_im, ovx) * ras_pou(j + 1, my, p.leave_j1, p.leave_jn, ovy);
+    const double t = one_minus_damping * pou;
+    if (v) {
+        double *a = &Vf[(p.i0 + i) + nxvf * (p.j0 + j)];
+        *a = *a + t * Vg[k];
+    } else {
+        double *a = &Uf[(p.i0 + i) + nxuf * (p.j0 + j)];
+        *a = *a + t * Ug[k];
+    }
+}
+
+// u = damping u, v = damping v on the full grid (SharedMemorySpec.jl:134-135).
+__global__ void k_scale_uv(double *u, int nu, double *v, int nv, double a)
+{
+    const bool isv = blockIdx.y != 0;
+    double *x = isv ? v : u;
+    const int n = isv ? nv : nu;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] = a * x[k];
 }
 
 // ===========================================================================
