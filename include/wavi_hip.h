@@ -218,6 +218,22 @@ int wavi_update_velocities_lockstep_noalloc(wavi_ctx **ctxs, int n, wavi_solve_r
  * update_state! on every subdomain model, one task each). */
 int wavi_update_state_batch(wavi_ctx **ctxs, int n, int nthreads, wavi_solve_report *reports, int *codes);
 
+/* precondition!(model, ::SharedMemorySpec) (src/SharedMemorySpec.jl:76-153)
+ * on the device: the residual of the full model's start guess (residual
+ * fields set, relative residual into *rel_resid, *converged = rel_resid <
+ * tol_picard), and unless converged, niterations rounds of restricted
+ * additive Schwarz: every subdomain's u and v from the full model's
+ * (schwarzRestrictVelocities!, :306-331), update_state! of every
+ * subdomain (nthreads host threads, one stream each), the full model's
+ * velocities times damping, and each subdomain's velocities weighted by the
+ * partition of unity and (1 - damping) added in subdomain order
+ * (schwarzProlongVelocities!, :351-382).  subs holds the ngridsx * ngridsy
+ * subdomain contexts, subdomain (igrid, jgrid) at (igrid - 1) * ngridsy +
+ * jgrid - 1, each built on the grid and fields schwarzModel gives it
+ * (:181-287) and on the full model's device.  No field crosses to the host. */
+int wavi_schwarz_precondition(wavi_ctx *full, wavi_ctx **subs, int ngridsx, int ngridsy, int overlap,
+                              int niterations, double damping, int nthreads, int *converged, double *rel_resid);
+
 /* inner_update!(model) (update_velocities.jl:32-45). */
 int wavi_inner_update(wavi_ctx *ctx);
 

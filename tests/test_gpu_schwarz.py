@@ -56,3 +56,65 @@ def test_shared_memory_spec_few_steps_close_to_plain_solver(W):
     err = np.linalg.norm(h1 - h0) / np.linalg.norm(h0)
     print("[gpu] 10 steps: thickness rel %.3e" % err, flush=True)
     assert err <= 0.01
+
+
+def test_device_schwarz_round_equals_host_restatement(W):
+    """One precondition!(model, ::SharedMemorySpec) on the device
+    (wavi_schwarz_precondition: two Schwarz rounds over 4 x 2 subdomains,
+    overlap 2, damping 0.3) against the same rounds restated on the host
+    with the reference's steps (schwarz_restrict_velocities, update_state! of
+    every subdomain through wavi_update_state_batch,
+    schwarz_prolong_velocities in subdomain order): the velocities and the
+    residual fields bit for bit, the relative residual to rounding."""
+    import copy
+    import ctypes
+    _step("schwarz: device round vs host restatement")
+
+    def make():
+        grid = W.Grid(nx=80, ny=10, nσ=4, x0=0.0, y0=-40000.0, dx=8000.0, dy=8000.0,
+                      u_iszero=["north"], v_iszero=["west", "east"])
+        spec = W.SharedMemorySpec(ngridsx=4, ngridsy=2, overlap=2, niterations=2, damping=0.3)
+        params = W.Params(default_thickness=100.0, accumulation_rate=0.3, default_temperature=265.700709)
+        m = W.Model(grid=grid, bed_elevation=W.mismip_plus_bed, params=params,
+                    solver_params=W.SolverParams(maxiter_picard=1), parallel_spec=spec)
+        m.params.dt = 0.1
+        W.update_geometry(m)
+        W.inner_update(m)
+        return m
+
+    S = W.schwarz
+    dev, host = make(), make()
+    for m in (dev, host):
+        S.update_preconditioner_schwarz(m)
+        W.push_fields(m)
+    conv, rel = S.precondition_schwarz(dev)
+    W.pull_fields(dev)
+    # the host restatement
+    spec = host.parallel_spec
+    lib = spec._lib
+    bu, bv = W.get_rhs(host)
+    f = host.fields
+    xu = np.where(f.gu.mask_inner, f.gu.u, 0.0)
+    xv = np.where(f.gv.mask_inner, f.gv.v, 0.0)
+    au, av = W.op_apply(host, xu, xv)
+    ru, rv = bu - au, bv - av
+    rel_h = np.linalg.norm(np.concatenate([ru[f.gu.mask_inner], rv[f.gv.mask_inner]])) / \
+        np.linalg.norm(np.concatenate([bu[f.gu.mask_inner], bv[f.gv.mask_inner]]))
+    assert not conv and abs(rel - rel_h) <= 1e-13 * rel_h
+    np.testing.assert_array_equal(dev.fields.gu.residual[f.gu.mask_inner], ru[f.gu.mask_inner])
+    np.testing.assert_array_equal(dev.fields.gv.residual[f.gv.mask_inner], rv[f.gv.mask_inner])
+    members = [(ig, jg, spec.schwarz_model_array[ig][jg]) for ig in range(4) for jg in range(2)]
+    for _ in range(spec.niterations):
+        for ig, jg, mg in members:
+            S.schwarz_restrict_velocities(mg, host, ig + 1, jg + 1, 4, 2, 2)
+            W.push_fields(mg)
+        ctxs = (ctypes.c_void_p * 8)(*[mg.parallel_spec.ctx for _, _, mg in members])
+        assert lib.wavi_update_state_batch(ctxs, 8, 4, None, None) == 0
+        for _, _, mg in members:
+            W.pull_fields(mg)
+        f.gu.u[:, :] = spec.damping * f.gu.u
+        f.gv.v[:, :] = spec.damping * f.gv.v
+        for ig, jg, mg in members:
+            S.schwarz_prolong_velocities(host, mg, ig + 1, jg + 1, 4, 2, 2, spec.damping)
+    np.testing.assert_array_equal(dev.fields.gu.u, host.fields.gu.u)
+    np.testing.assert_array_equal(dev.fields.gv.v, host.fields.gv.v)

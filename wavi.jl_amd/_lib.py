@@ -138,6 +138,10 @@ SIGNATURES = {
     "wavi_update_velocities_lockstep_noalloc": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                                                ctypes.POINTER(SolveReport),
                                                                ctypes.POINTER(ctypes.c_int)]),
+    "wavi_schwarz_precondition": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                 ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(ctypes.c_double)]),
     "wavi_inner_update": (ctypes.c_int, [ctypes.c_void_p]),
     "wavi_update_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveReport)]),
     "wavi_update_geometry": (ctypes.c_int, [ctypes.c_void_p]),

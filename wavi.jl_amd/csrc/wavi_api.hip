@@ -233,6 +233,8 @@ static void burst_collect(wavi_ctx *ctx, int kid, int last_active)
     } while (0)
 
 static inline dim3 flat(int n) { return dim3((unsigned)((n + 255) / 256)); }
+// workgroups of a launch over an nbx x nby tile grid with xcd_band_tile: a multiple of 8
+static inline int band_blocks(int nbx, int nby) { return (nbx * nby + 7) / 8 * 8; }
 // grid of a u / v pair launch: blockIdx.y 0 for the u arrays, 1 for the v arrays
 static inline dim3 flat_uv(int nu, int nv) { return dim3((unsigned)(((nu > nv ? nu : nv) + 255) / 256), 2); }
 
@@ -920,8 +922,8 @@ static int do_inner_update(wavi_ctx *ctx)
 
 static int do_op(wavi_ctx *ctx, const double *xu, const double *xv, double *yu, double *yv)
 {
-    hipLaunchKernelGGL(k_op<double>, dim3(ctx->g.nbx, ctx->g.nby), dim3(256), 0, ctx->stream, ctx->g, coef64(ctx),
-                       xu, xv, yu, yv);
+    const int nb = band_blocks(ctx->g.nbx, ctx->g.nby);
+    hipLaunchKernelGGL(k_op<double>, dim3(nb), dim3(256), 0, ctx->stream, ctx->g, coef64(ctx), xu, xv, yu, yv, nb);
     HIPCHK(hipGetLastError());
     return WAVI_OK;
 }
@@ -1027,20 +1029,22 @@ static void launch_smooth(wavi_ctx *ctx, int shape, const OpCoef<T> &c, const VS
 {
     const GridDims &g = ctx->g;
     const int tx = SMOOTH_TX[shape], ty = SMOOTH_TY[shape];
-    const dim3 grid((unsigned)((std::max(g.nxu, g.nxv) + tx - 1) / tx), (unsigned)((std::max(g.nyu, g.nyv) + ty - 1) / ty));
+    const int ntx = (std::max(g.nxu, g.nxv) + tx - 1) / tx, nty = (std::max(g.nyu, g.nyv) + ty - 1) / ty;
+    const int nb = band_blocks(ntx, nty);
+    const dim3 grid((unsigned)nb);
     const T om = (T)ctx->S.smoother_omega;
     switch (shape) {
     case 1:
         hipLaunchKernelGGL((k_smooth4<T, 32, 16>), grid, dim3(256), 0, ctx->stream, g, c, xu, xv, yu, yv, w.r.u,
-                           w.r.v, w.d.u, w.d.v, om, packed);
+                           w.r.v, w.d.u, w.d.v, om, packed, ntx, nty, nb);
         break;
     case 2:
         hipLaunchKernelGGL((k_smooth4<T, 16, 32>), grid, dim3(256), 0, ctx->stream, g, c, xu, xv, yu, yv, w.r.u,
-                           w.r.v, w.d.u, w.d.v, om, packed);
+                           w.r.v, w.d.u, w.d.v, om, packed, ntx, nty, nb);
         break;
     default:
         hipLaunchKernelGGL((k_smooth4<T, 16, 16>), grid, dim3(256), 0, ctx->stream, g, c, xu, xv, yu, yv, w.r.u,
-                           w.r.v, w.d.u, w.d.v, om, packed);
+                           w.r.v, w.d.u, w.d.v, om, packed, ntx, nty, nb);
         break;
     }
 }
@@ -1414,8 +1418,9 @@ static int vcycle_pre(wavi_ctx *ctx, const VSet<T> &w)
                        ctx->nv, T(0));
     CHK(do_smoother(ctx, w, fwd));
     // rho = r - A e
-    hipLaunchKernelGGL(k_tile_resid<T>, tg, dim3(256), 0, ctx->stream, g, coef_of(ctx, w), w.e.u, w.e.v, w.r.u,
-                       w.r.v, w.rho.u, w.rho.v, ctx->Pn, (double *)nullptr);
+    hipLaunchKernelGGL(k_tile_resid<T>, dim3(band_blocks(g.nbx, g.nby)), dim3(256), 0, ctx->stream, g, coef_of(ctx, w),
+                       w.e.u, w.e.v, w.r.u, w.r.v, w.rho.u, w.rho.v, ctx->Pn, (double *)nullptr,
+                       band_blocks(g.nbx, g.nby));
     // b_c = R rho, into the CG residual buffer
     hipLaunchKernelGGL((k_tile_haar<T, T>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 2, w.rho.u,
                        w.rho.v, ctx->ui, ctx->vi, w.cr.u, w.cr.v, ctx->mwu, ctx->mwv);
@@ -1526,8 +1531,9 @@ static int precondition_resid(wavi_ctx *ctx)
     hipLaunchKernelGGL((k_select_uv<double, double>), flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream,
                        ctx->f[WAVI_U_U], ctx->ui, ctx->x.u, ctx->nu, ctx->f[WAVI_V_V], ctx->vi, ctx->x.v, ctx->nv);
     CHK(do_rhs(ctx));
-    hipLaunchKernelGGL(k_tile_resid<double>, tg, dim3(256), 0, ctx->stream, g, coef64(ctx), ctx->x.u, ctx->x.v,
-                       ctx->b.u, ctx->b.v, ctx->r.u, ctx->r.v, ctx->Pr0, ctx->Pb);
+    hipLaunchKernelGGL(k_tile_resid<double>, dim3(band_blocks(g.nbx, g.nby)), dim3(256), 0, ctx->stream, g,
+                       coef64(ctx), ctx->x.u, ctx->x.v, ctx->b.u, ctx->b.v, ctx->r.u, ctx->r.v, ctx->Pr0, ctx->Pb,
+                       band_blocks(g.nbx, g.nby));
     hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, ctx->stream, ctx->Pr0, ctx->Pb, g.npart, ctx->norms);
     // set_residual! (update_velocities.jl:428-433): residual fields at inner points
     hipLaunchKernelGGL(k_copy_masked_uv<double>, flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream,
@@ -1643,8 +1649,9 @@ static int do_linear_solve(wavi_ctx *ctx, const DevVec &b, const DevVec &x, cons
     hipLaunchKernelGGL(k_fill<double>, flat(nu), dim3(256), 0, ctx->stream, x.u, 0.0, nu);
     hipLaunchKernelGGL(k_fill<double>, flat(nv), dim3(256), 0, ctx->stream, x.v, 0.0, nv);
     auto true_rel = [&](double *rel) -> int {
-        hipLaunchKernelGGL(k_tile_resid<double>, tg, dim3(256), 0, ctx->stream, g, coef64(ctx), x.u, x.v, b.u, b.v,
-                           ctx->r.u, ctx->r.v, ctx->Pr0, ctx->Pb);
+        hipLaunchKernelGGL(k_tile_resid<double>, dim3(band_blocks(g.nbx, g.nby)), dim3(256), 0, ctx->stream, g,
+                           coef64(ctx), x.u, x.v, b.u, b.v, ctx->r.u, ctx->r.v, ctx->Pr0, ctx->Pb,
+                           band_blocks(g.nbx, g.nby));
         hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, ctx->stream, ctx->Pr0, ctx->Pb, g.npart,
                            ctx->norms);
         HIPCHK(hipMemcpyAsync(ctx->h_norms, ctx->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1671,8 +1678,8 @@ static int do_linear_solve(wavi_ctx *ctx, const DevVec &b, const DevVec &x, cons
             CHK(dots(ctx, s, 1, au, av, bu, bv, &rz));
         }
         for (;;) {
-            hipLaunchKernelGGL(k_op<double>, dim3(g.nbx, g.nby), dim3(256), 0, ctx->stream, g, coef64(ctx), s.p.u,
-                               s.p.v, s.q.u, s.q.v);
+            hipLaunchKernelGGL(k_op<double>, dim3(band_blocks(g.nbx, g.nby)), dim3(256), 0, ctx->stream, g,
+                               coef64(ctx), s.p.u, s.p.v, s.q.u, s.q.v, band_blocks(g.nbx, g.nby));
             double pq;
             {
                 const double *au[1] = {s.p.u}, *av[1] = {s.p.v}, *bu[1] = {s.q.u}, *bv[1] = {s.q.v};
@@ -2593,6 +2600,111 @@ int wavi_update_state(wavi_ctx *ctx, wavi_solve_report *report)
     if (!ctx) return WAVI_EARG;
     CHK(do_update_state(ctx, report));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAVI_OK;
+}
+
+// Subdomain (ig, jg) (1-based) of an ngx x ngy restricted additive Schwarz
+// partition with `overlap` cells: the bounds of schwarzModel
+// (SharedMemorySpec.jl:183-199).  False when the grid does not split into
+// whole subdomains.
+static bool ras_part(const GridDims &g, int ig, int jg, int ngx, int ngy, int overlap, RasPart &p)
+{
+    if (g.nx % ngx != 0 || g.ny % ngy != 0) return false;
+    const int nxd = g.nx / ngx, nyd = g.ny / ngy;
+    const int i0 = std::max((ig - 1) * nxd + 1 - overlap, 1), i1 = std::min(ig * nxd + overlap, g.nx);
+    const int j0 = std::max((jg - 1) * nyd + 1 - overlap, 1), j1 = std::min(jg * nyd + overlap, g.ny);
+    p.i0 = i0 - 1;
+    p.j0 = j0 - 1;
+    p.nxg = i1 - i0 + 1;
+    p.nyg = j1 - j0 + 1;
+    p.leave_i1 = ig == 1;
+    p.leave_im = ig == ngx;
+    p.leave_j1 = jg == 1;
+    p.leave_jn = jg == ngy;
+    p.overlap = overlap;
+    return true;
+}
+
+int wavi_schwarz_precondition(wavi_ctx *full, wavi_ctx **subs, int ngridsx, int ngridsy, int overlap,
+                              int niterations, double damping, int nthreads, int *converged, double *rel_resid)
+{
+    g_create_error.clear();
+    if (!full) return WAVI_EARG;
+    wavi_ctx *ctx = full;
+    if (!subs || !converged || !rel_resid || ngridsx < 1 || ngridsy < 1 || overlap < 1 || niterations < 0 ||
+        nthreads < 1) {
+        ctx->err = "wavi_schwarz_precondition: subs, converged and rel_resid are required; ngridsx, ngridsy, "
+                   "overlap and nthreads must be at least 1 and niterations at least 0";
+        return WAVI_EARG;
+    }
+    const int n = ngridsx * ngridsy;
+    std::vector<RasPart> parts(n);
+    for (int ig = 1; ig <= ngridsx; ++ig)
+        for (int jg = 1; jg <= ngridsy; ++jg) {
+            const int k = (ig - 1) * ngridsy + (jg - 1);
+            if (!subs[k] || subs[k] == full || subs[k]->device != full->device) {
+                ctx->err = "wavi_schwarz_precondition: every subdomain needs a context of its own on the full "
+                           "model's device";
+                return WAVI_EARG;
+            }
+            if (!ras_part(full->g, ig, jg, ngridsx, ngridsy, overlap, parts[k])) {
+                ctx->err = "Model domain is not an integer number of subdomains";
+                return WAVI_EARG;
+            }
+            if (subs[k]->g.nx != parts[k].nxg || subs[k]->g.ny != parts[k].nyg) {
+                ctx->err = "wavi_schwarz_precondition: subdomain " + std::to_string(k) + " has the wrong grid size";
+                return WAVI_EDIM;
+            }
+        }
+    if (batch_has_duplicates(subs, n)) {
+        ctx->err = g_create_error;
+        return WAVI_EARG;
+    }
+    HIPCHK(hipSetDevice(full->device));
+    int conv = 0;
+    double rel = 0.0;
+    CHK(precondition_resid(full));
+    CHK(precondition_decide(full, &conv, &rel));
+    const int nxuf = full->g.nxu, nxvf = full->g.nxv;
+    for (int it = 0; !conv && it < niterations; ++it) {
+        // schwarzRestrictVelocities! into every subdomain
+        for (int k = 0; k < n; ++k) {
+            const RasPart &p = parts[k];
+            const int m = std::max((p.nxg + 1) * p.nyg, p.nxg * (p.nyg + 1));
+            hipLaunchKernelGGL(k_ras_restrict, dim3((unsigned)((m + 255) / 256), 2), dim3(256), 0, full->stream, p,
+                               nxuf, nxvf, full->f[WAVI_U_U], full->f[WAVI_V_V], subs[k]->f[WAVI_U_U],
+                               subs[k]->f[WAVI_V_V]);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(full->stream));
+        // update_state! of every subdomain, one stream and host thread each
+        std::vector<int> codes(n, WAVI_OK);
+        const int rc = run_members(subs, n, nthreads, codes.data(),
+                                   [](wavi_ctx *c, int) { return wavi_update_state(c, nullptr); });
+        if (rc != WAVI_OK) {
+            for (int k = 0; k < n; ++k)
+                if (codes[k] != WAVI_OK) {
+                    ctx->err = "subdomain " + std::to_string(k) + ": " + subs[k]->err;
+                    break;
+                }
+            return rc;
+        }
+        HIPCHK(hipSetDevice(full->device));
+        // damping, then schwarzProlongVelocities! of the subdomains in order
+        hipLaunchKernelGGL(k_scale_uv, flat_uv(full->nu, full->nv), dim3(256), 0, full->stream, full->f[WAVI_U_U],
+                           full->nu, full->f[WAVI_V_V], full->nv, damping);
+        for (int k = 0; k < n; ++k) {
+            const RasPart &p = parts[k];
+            const int m = std::max((p.nxg + 1) * p.nyg, p.nxg * (p.nyg + 1));
+            hipLaunchKernelGGL(k_ras_prolong, dim3((unsigned)((m + 255) / 256), 2), dim3(256), 0, full->stream, p,
+                               nxuf, nxvf, full->f[WAVI_U_U], full->f[WAVI_V_V], subs[k]->f[WAVI_U_U],
+                               subs[k]->f[WAVI_V_V], 1.0 - damping);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(full->stream));
+    *converged = conv;
+    *rel_resid = rel;
     return WAVI_OK;
 }
 

@@ -1314,8 +1314,23 @@ __device__ __forceinline__ void optile_apply(const GridDims &g, const OpCoef<T> 
 // tile stages its inputs with a one-point halo and its coefficients once
 // (OpTile), computes every intermediate once, and each thread evaluates its
 // u and its v point.  The arithmetic is that of op_u / op_v, so the results
-// equal the per-point forms bit for bit.  Launched on the (nbx, nby) tile
-// grid.
+// equal the per-point forms bit for bit.  Launched as nblk workgroups
+// (xcd_band_tile).
+
+// Tile (bx, by) of this workgroup in a one-dimensional launch of nblk (a
+// multiple of 8) workgroups over an nbx x nby tile grid.  Workgroups are
+// dispatched round-robin over the 8 XCDs; workgroup w takes tile
+// L = (w % 8) (nblk / 8) + w / 8 in column-major order, so each XCD walks one
+// band of whole tile columns and the halos that neighbouring tiles share stay
+// in that XCD's L2.  Returns false for the padding workgroups.
+__device__ __forceinline__ bool xcd_band_tile(int nbx, int nby, int nblk, int &bx, int &by)
+{
+    const int w = blockIdx.x;
+    const int L = (w & 7) * (nblk >> 3) + (w >> 3);
+    bx = L / nby;
+    by = L % nby;
+    return L < nbx * nby;
+}
 
 // The coefficients a tile's own u and v points read (no wavelet masks).
 template <typename T>
@@ -1361,10 +1376,12 @@ __device__ __forceinline__ void optile_prepare(const GridDims &g, const OpCoef<T
 
 // y = A x at inner points, 0 elsewhere
 template <typename T>
-__global__ void __launch_bounds__(256) k_op(GridDims g, OpCoef<T> c, const T *xu, const T *xv, T *yu, T *yv)
+__global__ void __launch_bounds__(256) k_op(GridDims g, OpCoef<T> c, const T *xu, const T *xv, T *yu, T *yv,
+                                            int nblk)
 {
     __shared__ OpTile<T> s;
-    const int bx = blockIdx.x, by = blockIdx.y;
+    int bx, by;
+    if (!xcd_band_tile(g.nbx, g.nby, nblk, bx, by)) return;
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
     const int i0 = bx * TILE, j0 = by * TILE, lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     PointCoef<T> pc;
@@ -1379,11 +1396,12 @@ __global__ void __launch_bounds__(256) k_op(GridDims g, OpCoef<T> c, const T *xu
 // tiles first, then v tiles).
 template <typename T>
 __global__ void __launch_bounds__(256) k_tile_resid(GridDims g, OpCoef<T> c, const T *xu, const T *xv, const T *bu,
-                                                    const T *bv, T *ru, T *rv, double *Pr, double *Pb)
+                                                    const T *bv, T *ru, T *rv, double *Pr, double *Pb, int nblk)
 {
     __shared__ OpTile<T> s;
     __shared__ double lds[8];
-    const int bx = blockIdx.x, by = blockIdx.y;
+    int bx, by;
+    if (!xcd_band_tile(g.nbx, g.nby, nblk, bx, by)) return;
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
     const int i0 = bx * TILE, j0 = by * TILE, lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     PointCoef<T> pc;
@@ -1476,11 +1494,13 @@ struct SmoothPoint {
 template <typename T, int TX, int TY>
 __global__ void __launch_bounds__(256) k_smooth4(GridDims g, OpCoef<T> c, const T *xu, const T *xv, T *xu_out,
                                                  T *xv_out, const T *bu, const T *bv, const T *du, const T *dv,
-                                                 T omega, int order)
+                                                 T omega, int order, int ntx, int nty, int nblk)
 {
     using S = SmoothShape<TX, TY>;
     __shared__ SmoothWin<T, TX, TY> s;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    int bx, by;
+    if (!xcd_band_tile(ntx, nty, nblk, bx, by)) return;
+    const int i0 = bx * TX, j0 = by * TY;
     const int ox = i0 - SM_H, oy = j0 - SM_H;  // window position 0
     // stage the window and load the coefficients the four sweeps read
     SmoothPoint<T> sp[S::SLOTS];

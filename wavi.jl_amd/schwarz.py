@@ -4,12 +4,16 @@
 The full-domain model keeps one device context (SharedMemorySpec is a
 HIPSpec) for its inner update, right-hand side, residual, geometry and the
 steps after the solve.  Every subdomain is a Model of its own with its own
-device context; update_state! of all subdomains runs in one batched call
-(wavi_update_state_batch: host threads, one stream per subdomain), as the
-reference spawns one task per subdomain.  Restriction, damping and the
-partition-of-unity prolongation are host array operations, as in the
-reference.  Subdomain contexts persist across solves while the subdomain
-grid is unchanged.
+device context.  update_preconditioner! builds the subdomain models on the
+host from the full model's fields, once per solve, as the reference does.
+Each Picard iteration then runs on the device (wavi_schwarz_precondition):
+restriction, update_state! of every subdomain (host threads, one stream
+per subdomain, as the reference spawns one task per subdomain), damping and
+the partition-of-unity prolongation, with no field crossing to the host.
+The host functions below (schwarz_restrict_velocities,
+schwarz_prolong_velocities, schwarz_partition_of_unity) restate the
+reference's for host Models.  Subdomain contexts persist across solves while
+the subdomain grid is unchanged.
 """
 import copy
 import ctypes
@@ -19,8 +23,7 @@ import numpy as np
 
 from . import _lib
 from .core import ArgumentError, BoundsError, Grid, InitialConditions
-from .model import HIPSpec, Model, _check, _report_solve, op_apply, pull_fields, push_fields, update_preconditioner
-from .model import get_rhs as _get_rhs
+from .model import HIPSpec, Model, _check, _report_solve, pull_fields, push_fields, update_preconditioner
 
 
 class SharedMemorySpec(HIPSpec):
@@ -177,73 +180,42 @@ def _same_grid(a, b):
                           "quadrature_weights", "σ")))
 
 
-def _inner_vectors(model, au, av):
-    f = model.fields
-    return np.concatenate([au[f.gu.mask_inner], av[f.gv.mask_inner]])
-
-
 def precondition_schwarz(model):
-    """precondition!(model, ::SharedMemorySpec) (SharedMemorySpec.jl:76-153):
-    the residual of the start guess, then niterations of restrict, subdomain
-    update_state!, damping and prolongation.  Returns (converged, rel_resid)."""
+    """precondition!(model, ::SharedMemorySpec) (SharedMemorySpec.jl:76-153)
+    on the device (wavi_schwarz_precondition): the residual of the start
+    guess, then niterations of restriction, subdomain update_state!, damping
+    and prolongation, with every field on the device.  The model's device
+    context must hold its current fields.  Returns (converged, rel_resid)."""
     spec = model.parallel_spec
-    f = model.fields
-    xu = np.where(f.gu.mask_inner, f.gu.u, 0.0)
-    xv = np.where(f.gv.mask_inner, f.gv.v, 0.0)
-    bu, bv = _get_rhs(model)
-    au, av = op_apply(model, xu, xv, sampled=True)
-    ru, rv = bu - au, bv - av
-    # set_residial! (update_velocities.jl:428-433)
-    f.gu.residual[f.gu.mask_inner] = ru[f.gu.mask_inner]
-    f.gv.residual[f.gv.mask_inner] = rv[f.gv.mask_inner]
-    rel = float(np.linalg.norm(_inner_vectors(model, ru, rv)) / np.linalg.norm(_inner_vectors(model, bu, bv)))
-    converged = rel < model.solver_params.tol_picard
-    if not converged:
-        members = [(ig, jg, spec.schwarz_model_array[ig][jg])
-                   for ig in range(spec.ngridsx) for jg in range(spec.ngridsy)]
-        lib = spec._lib
-        for _ in range(spec.niterations):
-            for ig, jg, mg in members:
-                schwarz_restrict_velocities(mg, model, ig + 1, jg + 1, spec.ngridsx, spec.ngridsy, spec.overlap)
-                for fid, arr in (("U_U", mg.fields.gu.u), ("V_V", mg.fields.gv.v)):
-                    buf = np.ascontiguousarray(arr.ravel(order="F"))
-                    _check(lib, mg.parallel_spec.ctx,
-                           lib.wavi_set_field(mg.parallel_spec.ctx, _lib.FIELD[fid], _lib.dptr(buf), buf.size))
-            n = len(members)
-            ctxs = (ctypes.c_void_p * n)(*[mg.parallel_spec.ctx for _, _, mg in members])
-            codes = (ctypes.c_int * n)()
-            rc = lib.wavi_update_state_batch(ctxs, n, spec.nthreads, None, codes)
-            if rc != 0:
-                for (_, _, mg), code in zip(members, codes):
-                    _check(lib, mg.parallel_spec.ctx, code)
-                _check(lib, None, rc)
-            for _, _, mg in members:
-                for fid, arr in (("U_U", mg.fields.gu.u), ("V_V", mg.fields.gv.v)):
-                    buf = np.empty(arr.size)
-                    _check(lib, mg.parallel_spec.ctx,
-                           lib.wavi_get_field(mg.parallel_spec.ctx, _lib.FIELD[fid], _lib.dptr(buf), buf.size))
-                    arr[:, :] = buf.reshape(arr.shape, order="F")
-            f.gu.u[:, :] = spec.damping * f.gu.u
-            f.gv.v[:, :] = spec.damping * f.gv.v
-            for ig, jg, mg in members:
-                schwarz_prolong_velocities(model, mg, ig + 1, jg + 1, spec.ngridsx, spec.ngridsy, spec.overlap,
-                                           spec.damping)
-    return converged, rel
+    lib = spec._lib
+    members = [spec.schwarz_model_array[ig][jg] for ig in range(spec.ngridsx) for jg in range(spec.ngridsy)]
+    n = len(members)
+    subs = (ctypes.c_void_p * n)(*[mg.parallel_spec.ctx for mg in members])
+    conv = ctypes.c_int(0)
+    rel = ctypes.c_double(0.0)
+    _check(lib, spec.ctx, lib.wavi_schwarz_precondition(spec.ctx, subs, spec.ngridsx, spec.ngridsy, spec.overlap,
+                                                        spec.niterations, spec.damping, spec.nthreads,
+                                                        ctypes.byref(conv), ctypes.byref(rel)))
+    return bool(conv.value), rel.value
 
 
 def update_velocities_schwarz(model, verbose=False):
     """update_velocities! (update_velocities.jl:11-29) with the RAS
-    preconditioner: the Picard loop of inner_update! (on the full domain's
-    device context) and precondition!(model, ::SharedMemorySpec)."""
+    preconditioner: update_preconditioner! builds the subdomain models from
+    the model's fields, then the Picard loop of inner_update! and
+    precondition!(model, ::SharedMemorySpec) runs on the device, the fields
+    going to the device once and coming back once.  The subdomain Models'
+    host fields are those update_preconditioner! gave them; their device
+    contexts hold the subdomain solves."""
     update_preconditioner_schwarz(model)
     spec = model.parallel_spec
+    push_fields(model)
     converged, it, rel = False, 0, math.inf
     while not converged and it < model.solver_params.maxiter_picard:
         it += 1
-        push_fields(model)
         _check(spec._lib, spec.ctx, spec._lib.wavi_inner_update(spec.ctx))
-        pull_fields(model)
         converged, rel = precondition_schwarz(model)
+    pull_fields(model)
     rep = _lib.SolveReport()
     rep.picard_iters, rep.converged, rep.rel_resid = it, int(converged), rel
     _report_solve(model, rep, verbose)
