@@ -234,6 +234,17 @@ int wavi_update_state_batch(wavi_ctx **ctxs, int n, int nthreads, wavi_solve_rep
 int wavi_schwarz_precondition(wavi_ctx *full, wavi_ctx **subs, int ngridsx, int ngridsy, int overlap,
                               int niterations, double damping, int nthreads, int *converged, double *rel_resid);
 
+/* update_preconditioner!(model, ::SharedMemorySpec) (SharedMemorySpec.jl:51-68)
+ * on existing subdomain contexts: every subdomain's fields as schwarzModel
+ * (:181-287) and setup_fields (src/Fields/Fields.jl:26-115) build them from
+ * the full model's current fields (bed, thickness, grounded fraction, u, v,
+ * viscosity, glen_b, Weertman coefficient and accumulation sliced; eta_av
+ * the bottom viscosity level, quad_f2 = h / (3 eta_av) on the mask, dsdh and
+ * the u/v grounded fractions 1, the wavelet masks set, every other field 0),
+ * on the device.  The subdomain contexts must have been created on
+ * schwarzModel's grids; subs as for wavi_schwarz_precondition. */
+int wavi_schwarz_refresh(wavi_ctx *full, wavi_ctx **subs, int ngridsx, int ngridsy, int overlap);
+
 /* inner_update!(model) (update_velocities.jl:32-45). */
 int wavi_inner_update(wavi_ctx *ctx);
 
@@ -327,12 +338,14 @@ int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out);
  * runs the four colour sweeps of gauss_seidel_smoother!,
  * src/Wavelets/update_preconditioners.jl:133-150, on tiles of tx x ty
  * points).  The shapes are 16 x 16, 32 x 16 and 16 x 32; every shape gives
- * the same bits.  By default the library's autotuner picks one on the
- * context's first V-cycle: it times each shape on the context's own grid
- * and keeps the fastest, and remembers the choice for later contexts of the
- * same grid size and precision in the process.  wavi_set_smoother_tile
- * fixes a shape (0, 0: tune again); wavi_smoother_tile reports the shape in
- * use for the context's V-cycle precision (0, 0 before the first V-cycle).
+ * the same bits.  By default the library's autotuner picks the shape, and
+ * whether the tiles are walked in bands of tile columns or of tile rows (one
+ * band per XCD), on the context's first V-cycle: it times each of the six
+ * variants on the context's own grid and keeps the fastest, and remembers
+ * the choice for later contexts of the same grid size and precision in the
+ * process.  wavi_set_smoother_tile fixes a shape, walked in column bands
+ * (0, 0: tune again); wavi_smoother_tile reports the shape in use for the
+ * context's V-cycle precision (0, 0 before the first V-cycle).
  * No reference counterpart. */
 int wavi_set_smoother_tile(wavi_ctx *ctx, int tx, int ty);
 int wavi_smoother_tile(const wavi_ctx *ctx, int *tx, int *ty);

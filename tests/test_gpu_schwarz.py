@@ -118,3 +118,54 @@ def test_device_schwarz_round_equals_host_restatement(W):
             S.schwarz_prolong_velocities(host, mg, ig + 1, jg + 1, 4, 2, 2, spec.damping)
     np.testing.assert_array_equal(dev.fields.gu.u, host.fields.gu.u)
     np.testing.assert_array_equal(dev.fields.gv.v, host.fields.gv.v)
+
+
+def test_device_subdomain_refresh_equals_host_build(W):
+    """update_preconditioner!(model, ::SharedMemorySpec) on the device
+    (wavi_schwarz_refresh) gives every subdomain context the fields that
+    schwarzModel and setup_fields build on the host, bit for bit: the host
+    build is pushed, the subdomains are then stepped (update_state!) so that
+    every field changes, and the device refresh from the same full state
+    brings each field back."""
+    import ctypes
+    _step("schwarz: device refresh vs host build")
+    S = W.schwarz
+    grid = W.Grid(nx=80, ny=10, nσ=4, x0=0.0, y0=-40000.0, dx=8000.0, dy=8000.0,
+                  u_iszero=["north"], v_iszero=["west", "east"])
+    spec = W.SharedMemorySpec(ngridsx=4, ngridsy=2, overlap=2, niterations=1)
+    params = W.Params(default_thickness=100.0, accumulation_rate=0.3, default_temperature=265.700709)
+    m = W.Model(grid=grid, bed_elevation=W.mismip_plus_bed, params=params,
+                solver_params=W.SolverParams(maxiter_picard=2), parallel_spec=spec)
+    m.params.dt = 0.1
+    W.update_geometry(m)
+    W.update_velocities(m)  # builds the subdomains on the host and solves once
+    W.update_geometry(m)
+    W.inner_update(m)
+    spec.__dict__.pop("_subdomains_built")
+    S.update_preconditioner_schwarz(m)  # host build from the current state
+    members = [spec.schwarz_model_array[ig][jg] for ig in range(4) for jg in range(2)]
+    names = [fid for _, _, fid in W.model.FIELD_MAP]
+    lib = spec._lib
+
+    def device_fields(mg):
+        out = {}
+        for fid in W._lib.FIELD_NAMES:
+            n = ctypes.c_size_t(0)
+            assert lib.wavi_field_size(mg.parallel_spec.ctx, W._lib.FIELD[fid], ctypes.byref(n)) == 0
+            a = np.empty(n.value)
+            assert lib.wavi_get_field(mg.parallel_spec.ctx, W._lib.FIELD[fid], W._lib.dptr(a), a.size) == 0
+            out[fid] = a
+        return out
+
+    built = [device_fields(mg) for mg in members]
+    ctxs = (ctypes.c_void_p * 8)(*[mg.parallel_spec.ctx for mg in members])
+    assert lib.wavi_update_state_batch(ctxs, 8, 4, None, None) == 0
+    stepped = [device_fields(mg) for mg in members]
+    assert any(not np.array_equal(stepped[0][f], built[0][f]) for f in names)
+    W.push_fields(m)
+    assert lib.wavi_schwarz_refresh(spec.ctx, ctxs, 4, 2, 2) == 0
+    for k, mg in enumerate(members):
+        got = device_fields(mg)
+        for fid in W._lib.FIELD_NAMES:
+            assert np.array_equal(got[fid], built[k][fid]), (k, fid)
+    assert lib.wavi_schwarz_refresh(spec.ctx, ctxs, 3, 2, 2) == W._lib.WAVI_EARG

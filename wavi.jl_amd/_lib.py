@@ -142,6 +142,8 @@ SIGNATURES = {
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                                  ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                                  ctypes.POINTER(ctypes.c_double)]),
+    "wavi_schwarz_refresh": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int]),
     "wavi_inner_update": (ctypes.c_int, [ctypes.c_void_p]),
     "wavi_update_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveReport)]),
     "wavi_update_geometry": (ctypes.c_int, [ctypes.c_void_p]),

@@ -1018,16 +1018,19 @@ static int do_diag(wavi_ctx *ctx, const VSet<T> &w)
     return WAVI_OK;
 }
 
-// Tile shapes of the smoother kernel k_smooth4<T, TX, TY>.
-constexpr int SMOOTH_SHAPES = 3;
+// Tile shapes of the smoother kernel k_smooth4<T, TX, TY>, each walked in
+// XCD bands of tile columns or of tile rows: variant v is shape v % 3,
+// column bands when v < 3.
+constexpr int SMOOTH_SHAPES = 3, SMOOTH_VARIANTS = 6;
 static const int SMOOTH_TX[SMOOTH_SHAPES] = {16, 32, 16};
 static const int SMOOTH_TY[SMOOTH_SHAPES] = {16, 16, 32};
 
 template <typename T>
-static void launch_smooth(wavi_ctx *ctx, int shape, const OpCoef<T> &c, const VSet<T> &w, const T *xu, const T *xv,
+static void launch_smooth(wavi_ctx *ctx, int variant, const OpCoef<T> &c, const VSet<T> &w, const T *xu, const T *xv,
                           T *yu, T *yv, int packed)
 {
     const GridDims &g = ctx->g;
+    const int shape = variant % SMOOTH_SHAPES, cols = variant < SMOOTH_SHAPES ? 1 : 0;
     const int tx = SMOOTH_TX[shape], ty = SMOOTH_TY[shape];
     const int ntx = (std::max(g.nxu, g.nxv) + tx - 1) / tx, nty = (std::max(g.nyu, g.nyv) + ty - 1) / ty;
     const int nb = band_blocks(ntx, nty);
@@ -1036,31 +1039,35 @@ static void launch_smooth(wavi_ctx *ctx, int shape, const OpCoef<T> &c, const VS
     switch (shape) {
     case 1:
         hipLaunchKernelGGL((k_smooth4<T, 32, 16>), grid, dim3(256), 0, ctx->stream, g, c, xu, xv, yu, yv, w.r.u,
-                           w.r.v, w.d.u, w.d.v, om, packed, ntx, nty, nb);
+                           w.r.v, w.d.u, w.d.v, om, packed, ntx, nty, nb, cols);
         break;
     case 2:
         hipLaunchKernelGGL((k_smooth4<T, 16, 32>), grid, dim3(256), 0, ctx->stream, g, c, xu, xv, yu, yv, w.r.u,
-                           w.r.v, w.d.u, w.d.v, om, packed, ntx, nty, nb);
+                           w.r.v, w.d.u, w.d.v, om, packed, ntx, nty, nb, cols);
         break;
     default:
         hipLaunchKernelGGL((k_smooth4<T, 16, 16>), grid, dim3(256), 0, ctx->stream, g, c, xu, xv, yu, yv, w.r.u,
-                           w.r.v, w.d.u, w.d.v, om, packed, ntx, nty, nb);
+                           w.r.v, w.d.u, w.d.v, om, packed, ntx, nty, nb, cols);
         break;
     }
 }
 
 // The autotuner's results, shared by the contexts of one process: the
-// fastest smoother tile shape per (nx, ny, V-cycle precision).
+// fastest smoother variant (tile shape and band order) per (nx, ny, V-cycle
+// precision).
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, int>, int> g_smooth_tuned;
 
 // The smoother tile shape of the context, chosen on first use: the shape
 // set by wavi_set_smoother_tile, else one tuned before for a grid of this
 // size and precision, else measured now: after a warm-up launch of each,
-// three launches of every candidate shape from the current iterate w.e into
-// w.t (free while smoothing; the results are not used) between two events,
-// and the fastest shape is kept.  Every shape gives the same bits, so the
-// choice never changes a result.
+// three timed runs of four launches of every candidate shape from the
+// current iterate w.e into w.t (free while smoothing; the results are not
+// used), each run between two events; a shape's time is its fastest run
+// (launches of small grids take a few microseconds, and the fastest of
+// several runs filters out the jitter of the others), and the fastest shape
+// is kept.  Every shape gives the same bits, so the choice never changes a
+// result.
 template <typename T>
 static int smooth_shape(wavi_ctx *ctx, const VSet<T> &w, const OpCoef<T> &c, int packed)
 {
@@ -1074,14 +1081,18 @@ static int smooth_shape(wavi_ctx *ctx, const VSet<T> &w, const OpCoef<T> &c, int
     }
     int best = 0;
     float best_ms = 0.f;
-    for (int sh = 0; sh < SMOOTH_SHAPES; ++sh) {
+    for (int sh = 0; sh < SMOOTH_VARIANTS; ++sh) {
         launch_smooth(ctx, sh, c, w, w.e.u, w.e.v, w.t.u, w.t.v, packed);
-        if (hipEventRecord(ctx->ev_tune[0], ctx->stream) != hipSuccess) return 0;
-        for (int r = 0; r < 3; ++r) launch_smooth(ctx, sh, c, w, w.e.u, w.e.v, w.t.u, w.t.v, packed);
-        if (hipEventRecord(ctx->ev_tune[1], ctx->stream) != hipSuccess) return 0;
-        if (hipEventSynchronize(ctx->ev_tune[1]) != hipSuccess) return 0;
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, ctx->ev_tune[0], ctx->ev_tune[1]) != hipSuccess) return 0;
+        for (int run = 0; run < 3; ++run) {
+            if (hipEventRecord(ctx->ev_tune[0], ctx->stream) != hipSuccess) return 0;
+            for (int r = 0; r < 4; ++r) launch_smooth(ctx, sh, c, w, w.e.u, w.e.v, w.t.u, w.t.v, packed);
+            if (hipEventRecord(ctx->ev_tune[1], ctx->stream) != hipSuccess) return 0;
+            if (hipEventSynchronize(ctx->ev_tune[1]) != hipSuccess) return 0;
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ctx->ev_tune[0], ctx->ev_tune[1]) != hipSuccess) return 0;
+            ms = run == 0 ? t : std::min(ms, t);
+        }
         if (sh == 0 || ms < best_ms) {
             best = sh;
             best_ms = ms;
@@ -2625,6 +2636,81 @@ static bool ras_part(const GridDims &g, int ig, int jg, int ngx, int ngy, int ov
     return true;
 }
 
+// Checks of a restricted additive Schwarz call: the subdomain contexts
+// (subdomain (igrid, jgrid) at (igrid - 1) ngridsy + jgrid - 1) exist, are
+// distinct, on the full model's device and on schwarzModel's grids; their
+// parts into `parts`.
+static int ras_check(wavi_ctx *full, wavi_ctx **subs, int ngridsx, int ngridsy, int overlap,
+                     std::vector<RasPart> &parts)
+{
+    wavi_ctx *ctx = full;
+    const int n = ngridsx * ngridsy;
+    parts.assign(n, RasPart{});
+    for (int ig = 1; ig <= ngridsx; ++ig)
+        for (int jg = 1; jg <= ngridsy; ++jg) {
+            const int k = (ig - 1) * ngridsy + (jg - 1);
+            if (!subs[k] || subs[k] == full || subs[k]->device != full->device) {
+                ctx->err = "restricted additive Schwarz: every subdomain needs a context of its own on the full "
+                           "model's device";
+                return WAVI_EARG;
+            }
+            if (!ras_part(full->g, ig, jg, ngridsx, ngridsy, overlap, parts[k])) {
+                ctx->err = "Model domain is not an integer number of subdomains";
+                return WAVI_EARG;
+            }
+            if (subs[k]->g.nx != parts[k].nxg || subs[k]->g.ny != parts[k].nyg || subs[k]->ns3 / subs[k]->nh != full->ns3 / full->nh) {
+                ctx->err = "restricted additive Schwarz: subdomain " + std::to_string(k) + " has the wrong grid size";
+                return WAVI_EDIM;
+            }
+        }
+    if (batch_has_duplicates(subs, n)) {
+        ctx->err = g_create_error;
+        return WAVI_EARG;
+    }
+    return WAVI_OK;
+}
+
+int wavi_schwarz_refresh(wavi_ctx *full, wavi_ctx **subs, int ngridsx, int ngridsy, int overlap)
+{
+    g_create_error.clear();
+    if (!full) return WAVI_EARG;
+    wavi_ctx *ctx = full;
+    if (!subs || ngridsx < 1 || ngridsy < 1 || overlap < 1) {
+        ctx->err = "wavi_schwarz_refresh: subs is required; ngridsx, ngridsy and overlap must be at least 1";
+        return WAVI_EARG;
+    }
+    std::vector<RasPart> parts;
+    CHK(ras_check(full, subs, ngridsx, ngridsy, overlap, parts));
+    HIPCHK(hipSetDevice(full->device));
+    const int n = ngridsx * ngridsy;
+    for (int k = 0; k < n; ++k) {
+        wavi_ctx *sc = subs[k];
+        RasRefresh r{};
+        r.p = parts[k];
+        r.nxf = full->g.nx;
+        r.nyf = full->g.ny;
+        r.ns = full->ns3 / full->nh;
+        r.b = full->f[WAVI_H_B];
+        r.h = full->f[WAVI_H_H];
+        r.gf = full->f[WAVI_H_GROUNDED_FRACTION];
+        r.wc = full->f[WAVI_P_WEERTMAN_C];
+        r.acc = full->f[WAVI_P_ACCUMULATION_RATE];
+        r.u = full->f[WAVI_U_U];
+        r.v = full->f[WAVI_V_V];
+        r.eta = full->f[WAVI_S_ETA];
+        r.glenb = full->f[WAVI_S_GLEN_B];
+        for (int id = 0; id < WAVI_NFIELDS; ++id) r.f[id] = sc->f[id];
+        r.mwu = sc->mwu;
+        r.mwv = sc->mwv;
+        r.hm = sc->hm;
+        const int m = std::max(std::max(sc->nu, sc->nv), sc->ns3);
+        hipLaunchKernelGGL(k_ras_refresh, flat(m), dim3(256), 0, full->stream, r);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(full->stream));
+    return WAVI_OK;
+}
+
 int wavi_schwarz_precondition(wavi_ctx *full, wavi_ctx **subs, int ngridsx, int ngridsy, int overlap,
                               int niterations, double damping, int nthreads, int *converged, double *rel_resid)
 {
@@ -2638,28 +2724,8 @@ int wavi_schwarz_precondition(wavi_ctx *full, wavi_ctx **subs, int ngridsx, int 
         return WAVI_EARG;
     }
     const int n = ngridsx * ngridsy;
-    std::vector<RasPart> parts(n);
-    for (int ig = 1; ig <= ngridsx; ++ig)
-        for (int jg = 1; jg <= ngridsy; ++jg) {
-            const int k = (ig - 1) * ngridsy + (jg - 1);
-            if (!subs[k] || subs[k] == full || subs[k]->device != full->device) {
-                ctx->err = "wavi_schwarz_precondition: every subdomain needs a context of its own on the full "
-                           "model's device";
-                return WAVI_EARG;
-            }
-            if (!ras_part(full->g, ig, jg, ngridsx, ngridsy, overlap, parts[k])) {
-                ctx->err = "Model domain is not an integer number of subdomains";
-                return WAVI_EARG;
-            }
-            if (subs[k]->g.nx != parts[k].nxg || subs[k]->g.ny != parts[k].nyg) {
-                ctx->err = "wavi_schwarz_precondition: subdomain " + std::to_string(k) + " has the wrong grid size";
-                return WAVI_EDIM;
-            }
-        }
-    if (batch_has_duplicates(subs, n)) {
-        ctx->err = g_create_error;
-        return WAVI_EARG;
-    }
+    std::vector<RasPart> parts;
+    CHK(ras_check(full, subs, ngridsx, ngridsy, overlap, parts));
     HIPCHK(hipSetDevice(full->device));
     int conv = 0;
     double rel = 0.0;
@@ -2853,6 +2919,7 @@ int wavi_set_smoother_tile(wavi_ctx *ctx, int tx, int ty)
         ctx->err = "wavi_set_smoother_tile: the tile shapes are 16 x 16, 32 x 16 and 16 x 32 (0 x 0: tune)";
         return WAVI_EARG;
     }
+    // a fixed shape is walked in column bands
     ctx->smooth_shape[0] = ctx->smooth_shape[1] = shape;
     return WAVI_OK;
 }
@@ -2862,8 +2929,8 @@ int wavi_smoother_tile(const wavi_ctx *ctx, int *tx, int *ty)
     if (!ctx || !tx || !ty) return WAVI_EARG;
     const int pi = ctx->precision == WAVI_PRECISION_MIXED ? 1 : 0;
     const int sh = ctx->smooth_shape[pi];
-    *tx = sh >= 0 ? SMOOTH_TX[sh] : 0;
-    *ty = sh >= 0 ? SMOOTH_TY[sh] : 0;
+    *tx = sh >= 0 ? SMOOTH_TX[sh % SMOOTH_SHAPES] : 0;
+    *ty = sh >= 0 ? SMOOTH_TY[sh % SMOOTH_SHAPES] : 0;
     return WAVI_OK;
 }
 

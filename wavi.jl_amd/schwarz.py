@@ -4,8 +4,10 @@
 The full-domain model keeps one device context (SharedMemorySpec is a
 HIPSpec) for its inner update, right-hand side, residual, geometry and the
 steps after the solve.  Every subdomain is a Model of its own with its own
-device context.  update_preconditioner! builds the subdomain models on the
-host from the full model's fields, once per solve, as the reference does.
+device context.  update_preconditioner! builds the subdomain models from
+the full model's fields once per solve, as the reference does: on the host
+the first time, then on the device into the persistent subdomain contexts
+(wavi_schwarz_refresh).
 Each Picard iteration then runs on the device (wavi_schwarz_precondition):
 restriction, update_state! of every subdomain (host threads, one stream
 per subdomain, as the reference spawns one task per subdomain), damping and
@@ -152,10 +154,24 @@ def schwarz_prolong_velocities(model, model_g, igrid=1, jgrid=1, ngridsx=1, ngri
 
 def update_preconditioner_schwarz(model):
     """update_preconditioner!(model, ::SharedMemorySpec) (SharedMemorySpec.jl:51-68):
-    fresh subdomain models from the full model's fields, on persistent device
-    contexts."""
+    fresh subdomain models from the full model's fields.  The first time,
+    each subdomain's Model and device context are built on the host
+    (schwarz_model); after that the contexts persist (the subdomain grids
+    follow the full model's grid, which does not change) and their fields are
+    rebuilt on the device from the full model's (wavi_schwarz_refresh): the
+    full model's context must hold its current fields."""
     spec = model.parallel_spec
     update_preconditioner(model)  # the full domain's context
+    lib = spec._lib
+    ready = getattr(spec, "_subdomains_built", False)
+    if ready:
+        members = [spec.schwarz_model_array[ig][jg] for ig in range(spec.ngridsx) for jg in range(spec.ngridsy)]
+        for mg in members:
+            mg.params.dt = model.params.dt
+            update_preconditioner(mg)  # keeps the subdomain's dt in step
+        subs = (ctypes.c_void_p * len(members))(*[mg.parallel_spec.ctx for mg in members])
+        _check(lib, spec.ctx, lib.wavi_schwarz_refresh(spec.ctx, subs, spec.ngridsx, spec.ngridsy, spec.overlap))
+        return model
     for ig in range(spec.ngridsx):
         for jg in range(spec.ngridsy):
             old = spec.schwarz_model_array[ig][jg]
@@ -170,6 +186,7 @@ def update_preconditioner_schwarz(model):
             spec.schwarz_model_array[ig][jg] = mg
             update_preconditioner(mg)
             push_fields(mg)
+    spec._subdomains_built = True
     return model
 
 
@@ -199,40 +216,66 @@ def precondition_schwarz(model):
     return bool(conv.value), rel.value
 
 
-def update_velocities_schwarz(model, verbose=False):
-    """update_velocities! (update_velocities.jl:11-29) with the RAS
-    preconditioner: update_preconditioner! builds the subdomain models from
-    the model's fields, then the Picard loop of inner_update! and
-    precondition!(model, ::SharedMemorySpec) runs on the device, the fields
-    going to the device once and coming back once.  The subdomain Models'
-    host fields are those update_preconditioner! gave them; their device
-    contexts hold the subdomain solves."""
-    update_preconditioner_schwarz(model)
+def _velocities_on_device(model, verbose=False):
+    """update_velocities! with the RAS preconditioner on fields that are
+    already on the full model's device context (and stay there): the
+    subdomain models from those fields (the first time from the host fields,
+    which must then be current), then the Picard loop of inner_update! and
+    precondition!(model, ::SharedMemorySpec)."""
     spec = model.parallel_spec
-    push_fields(model)
+    if not getattr(spec, "_subdomains_built", False):
+        pull_fields(model)
+    update_preconditioner_schwarz(model)
     converged, it, rel = False, 0, math.inf
     while not converged and it < model.solver_params.maxiter_picard:
         it += 1
         _check(spec._lib, spec.ctx, spec._lib.wavi_inner_update(spec.ctx))
         converged, rel = precondition_schwarz(model)
-    pull_fields(model)
     rep = _lib.SolveReport()
     rep.picard_iters, rep.converged, rep.rel_resid = it, int(converged), rel
     _report_solve(model, rep, verbose)
     return model
 
 
+def update_velocities_schwarz(model, verbose=False):
+    """update_velocities! (update_velocities.jl:11-29) with the RAS
+    preconditioner: update_preconditioner! builds the subdomain models from
+    the model's fields, then the Picard loop of inner_update! and
+    precondition!(model, ::SharedMemorySpec) runs on the device, the fields
+    going to the device once and coming back once.  The subdomain Models'
+    host fields are those update_preconditioner! gave them the first time;
+    their device contexts hold the subdomain solves."""
+    update_preconditioner(model)
+    push_fields(model)
+    _velocities_on_device(model, verbose)
+    pull_fields(model)
+    return model
+
+
+def state_steps_on_device(model, nsteps, dt, step_thickness=True, verbose=False):
+    """nsteps of timestep! (run_simulation.jl:6-17) of a model whose
+    parallel_spec is a SharedMemorySpec, on fields already on its device
+    context: update_state! (geometry, the RAS velocity solve, the steps after
+    the solve) and, with step_thickness, update_thickness!, each step on the
+    device."""
+    spec = model.parallel_spec
+    lib = spec._lib
+    for _ in range(nsteps):
+        _check(lib, spec.ctx, lib.wavi_update_geometry(spec.ctx))
+        _velocities_on_device(model, verbose)
+        _check(lib, spec.ctx, lib.wavi_after_solve(spec.ctx))
+        if step_thickness:
+            _check(lib, spec.ctx, lib.wavi_update_thickness(spec.ctx, dt))
+    return model
+
+
 def update_state_schwarz(model, verbose=False):
     """update_state! (update_state.jl:7-21) of a model whose parallel_spec is
     a SharedMemorySpec: geometry and the steps after the solve on the full
-    domain's context, the velocity solve by RAS."""
+    domain's context, the velocity solve by RAS, with the fields on the
+    device throughout."""
     update_preconditioner(model)
-    spec = model.parallel_spec
     push_fields(model)
-    _check(spec._lib, spec.ctx, spec._lib.wavi_update_geometry(spec.ctx))
-    pull_fields(model)
-    update_velocities_schwarz(model, verbose)
-    push_fields(model)
-    _check(spec._lib, spec.ctx, spec._lib.wavi_after_solve(spec.ctx))
+    state_steps_on_device(model, 1, model.params.dt, step_thickness=False, verbose=verbose)
     pull_fields(model)
     return model

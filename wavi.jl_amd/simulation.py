@@ -104,11 +104,15 @@ def update_clock(simulation):
 def _run_steps(simulation, nsteps, verbose=False):
     model = simulation.model
     if _schwarz(model):
-        # restricted additive Schwarz: the steps are driven from the host
+        # restricted additive Schwarz: each step's pieces are launched from the
+        # host, on fields that stay on the device for the whole stretch
+        from .schwarz import state_steps_on_device
+        update_preconditioner(model)
+        push_fields(model)
+        state_steps_on_device(model, nsteps, simulation.timestepping_params.dt,
+                              simulation.timestepping_params.step_thickness, verbose)
+        pull_fields(model)
         for _ in range(nsteps):
-            update_state(model, verbose=verbose)
-            if simulation.timestepping_params.step_thickness:
-                update_thickness(simulation)
             update_clock(simulation)
         return simulation
     update_preconditioner(model)
