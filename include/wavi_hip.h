@@ -350,6 +350,22 @@ int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out);
 int wavi_set_smoother_tile(wavi_ctx *ctx, int tx, int ty);
 int wavi_smoother_tile(const wavi_ctx *ctx, int *tx, int *ty);
 
+/* How the coarse CG of the V-cycle runs (update_preconditioners.jl:73).
+ * WAVI_CG_LAUNCHES: one launch per CG iteration over all tiles (three in
+ * FP64, one in mixed precision), queued in chunks.  WAVI_CG_WORKGROUP: the
+ * whole CG in one launch of one workgroup, which runs each iteration on the
+ * tiles in turn, with no launch and no host round trip between iterations.
+ * Both give the same bits.  WAVI_CG_AUTO (the default) is the library's
+ * choice, at present WAVI_CG_LAUNCHES for every grid: measured on MI355X,
+ * the workgroup is 3.4 to 3.6 times slower per iteration even on a grid of
+ * six 16 x 16 tiles.  wavi_coarse_cg reports the mode in effect for the
+ * context.  No reference counterpart. */
+#define WAVI_CG_AUTO 0
+#define WAVI_CG_LAUNCHES 1
+#define WAVI_CG_WORKGROUP 2
+int wavi_set_coarse_cg(wavi_ctx *ctx, int mode);
+int wavi_coarse_cg(const wavi_ctx *ctx, int *mode);
+
 /* Kernel timing with HIP events on the context's stream, for measurement.
  * Sampled single launches (events around every 16th launch, so each time
  * includes the cost of the two event records): WAVI_KT_CG_APPLY (FP64

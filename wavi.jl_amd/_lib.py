@@ -17,6 +17,11 @@ WAVI_EDIM = -2
 WAVI_ESTATE = -3
 WAVI_EHIP = 1
 
+# coarse-CG modes (wavi_set_coarse_cg)
+WAVI_CG_AUTO = 0
+WAVI_CG_LAUNCHES = 1
+WAVI_CG_WORKGROUP = 2
+
 PRECISION = {"fp64": 0, "mixed": 1}
 
 REPORT_MAX_PICARD = 64
@@ -159,6 +164,8 @@ SIGNATURES = {
     "wavi_haar": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "wavi_set_smoother_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "wavi_smoother_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "wavi_set_coarse_cg": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "wavi_coarse_cg": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "wavi_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "wavi_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "wavi_math_probe": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]),

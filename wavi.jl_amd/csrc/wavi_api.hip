@@ -110,6 +110,7 @@ struct wavi_ctx {
     hipEvent_t ev_ls = nullptr, ev_ls_done = nullptr;  // set-up and batch end in a lockstep batch without allocation
     hipEvent_t ev_tune[2] = {};     // timing events of the smoother's tile-shape tuning
     int smooth_shape[2] = {-1, -1}; // smoother tile shape by V-cycle precision (0 FP64, 1 FP32); -1: tune
+    int cg_mode = WAVI_CG_AUTO;      // coarse CG: one launch per iteration or one workgroup (wavi_set_coarse_cg)
     hipGraphExec_t cgm_chunk = nullptr;  // 32 k_cgm_iter launches (parities 1, 0, ...) as one graph
     double *Pr0 = nullptr, *Pr1 = nullptr, *Pd = nullptr, *Pb = nullptr, *Pn = nullptr;
     double *quad = nullptr;  // 4 * nh pos_fraction quadrants
@@ -1186,6 +1187,17 @@ static int cg_chunk_len(int k) { return k >= 4 ? CG_CHUNK : 2 << k; }
 // Relative stop of the mixed-precision coarse CG: 2^-19 = 16 FP32 ulps.
 constexpr double CGM_FP32_RELTOL = 0x1p-19;
 
+// The coarse CG runs in one workgroup (k_cg_solve1, k_cgm_solve1) only when
+// wavi_set_coarse_cg asks for it: on MI355X one workgroup spends about 3 us
+// per tile and iteration (a tile's chain of loads, barriers and stores),
+// so even a 6-tile grid (80 x 10) takes 3.4 (mixed) to 3.6 (FP64) times as
+// long per CG iteration as the launches over all tiles, whose tiles run
+// side by side.
+static bool cg_one_workgroup(const wavi_ctx *ctx)
+{
+    return ctx->cg_mode == WAVI_CG_WORKGROUP;
+}
+
 // coarse CG on b_c held in w.cr; solution in the warm start w.xc
 template <typename T>
 static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
@@ -1200,11 +1212,22 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
     ctx->h_flag->iters = 0;
     hipLaunchKernelGGL(k_cg_setup<T>, tb, dim3(256), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
+    volatile CGMHostFlag *hf = ctx->h_flag;
+    if (cg_one_workgroup(ctx)) {
+        hipLaunchKernelGGL(k_cg_solve1<T>, dim3(1), dim3(256), 0, ctx->stream, a);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (!hf->done) {
+            ctx->err = "coarse CG did not terminate";
+            return WAVI_ESTATE;
+        }
+        *iters = hf->iters;
+        return WAVI_OK;
+    }
     // Triplet q performs iteration q unless the CG has stopped; triplet
     // maxiter always stops, so no more are needed.  Triplets are queued in
     // chunks (cg_chunk_len); the host reads the stop report between chunks
     // and queues no more after a stop.
-    volatile CGMHostFlag *hf = ctx->h_flag;
     const int nq = ctx->S.maxiter_coarse + 1;
     int q = 0, k = 0;
     while (q < nq) {
@@ -1349,6 +1372,26 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     const CGMLayout Lo = cgm_layout(ctx);
     CHK(cgm_prepare(ctx, w));
     const CGMIterArgs ia = cgm_iter_args(ctx);
+    volatile CGMHostFlag *hf = ctx->h_flag;
+    if (cg_one_workgroup(ctx)) {
+        switch (ctx->S.levels) {
+        case 0: hipLaunchKernelGGL(k_cgm_solve1<0>, dim3(1), dim3(256), 0, ctx->stream, ia); break;
+        case 1: hipLaunchKernelGGL(k_cgm_solve1<1>, dim3(1), dim3(256), 0, ctx->stream, ia); break;
+        case 2: hipLaunchKernelGGL(k_cgm_solve1<2>, dim3(1), dim3(256), 0, ctx->stream, ia); break;
+        default: hipLaunchKernelGGL(k_cgm_solve1<3>, dim3(1), dim3(256), 0, ctx->stream, ia); break;
+        }
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_cgm_unpack, flat(std::max(ctx->nu, ctx->nv)), dim3(256), 0, ctx->stream, g, Lo, w.xc.u,
+                           w.xc.v);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (!hf->done) {
+            ctx->err = "coarse CG did not terminate";
+            return WAVI_ESTATE;
+        }
+        *iters = hf->iters;
+        return WAVI_OK;
+    }
     launch_cgm_iter<0, true>(ctx, tb, ia);
     HIPCHK(hipGetLastError());
     // Launch q (q >= 1) performs iteration q - 1 unless the CG has stopped;
@@ -1356,7 +1399,6 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     // queued in chunks (cg_chunk_len); the host reads the stop report
     // between chunks and queues no more after a stop.  Full chunks are
     // timed (bursts) and replay one captured graph.
-    volatile CGMHostFlag *hf = ctx->h_flag;
     const int qmax = ctx->S.maxiter_coarse + 1;
     const bool use_graph = use_cg_graph();
     int q = 1, k = 0;
@@ -2931,6 +2973,24 @@ int wavi_smoother_tile(const wavi_ctx *ctx, int *tx, int *ty)
     const int sh = ctx->smooth_shape[pi];
     *tx = sh >= 0 ? SMOOTH_TX[sh % SMOOTH_SHAPES] : 0;
     *ty = sh >= 0 ? SMOOTH_TY[sh % SMOOTH_SHAPES] : 0;
+    return WAVI_OK;
+}
+
+int wavi_set_coarse_cg(wavi_ctx *ctx, int mode)
+{
+    if (!ctx) return WAVI_EARG;
+    if (mode != WAVI_CG_AUTO && mode != WAVI_CG_LAUNCHES && mode != WAVI_CG_WORKGROUP) {
+        ctx->err = "wavi_set_coarse_cg: the modes are WAVI_CG_AUTO, WAVI_CG_LAUNCHES and WAVI_CG_WORKGROUP";
+        return WAVI_EARG;
+    }
+    ctx->cg_mode = mode;
+    return WAVI_OK;
+}
+
+int wavi_coarse_cg(const wavi_ctx *ctx, int *mode)
+{
+    if (!ctx || !mode) return WAVI_EARG;
+    *mode = cg_one_workgroup(ctx) ? WAVI_CG_WORKGROUP : WAVI_CG_LAUNCHES;
     return WAVI_OK;
 }
 

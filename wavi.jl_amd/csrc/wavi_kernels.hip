@@ -1861,13 +1861,12 @@ __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
 // recomputed from the partials each time, so it needs no flag: after a stop
 // the partials are frozen and it[1 - par] repeats the count.
 template <typename T>
-__global__ void __launch_bounds__(256) k_cg_dir(CGArgs<T> a, int par)
+__device__ __forceinline__ bool cg_dir_tile(const CGArgs<T> &a, int par, const int L)
 {
     __shared__ T tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
     __shared__ double lds[8];
     const GridDims &g = a.g;
-    const int L = xcd_tile(a.nblk);
-    if (L >= g.nbx * g.nby) return;
+    if (L >= g.nbx * g.nby) return false;
     int bx, by;
     const int tile_id = band_tile(g, L, bx, by);
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
@@ -1906,7 +1905,7 @@ __global__ void __launch_bounds__(256) k_cg_dir(CGArgs<T> a, int par)
             }
         }
     }
-    if (done) return;
+    if (done) return true;
     const double prev = it == 0 ? 1.0 : prev_res;
     const T beta = (T)((res * res) / (prev * prev));
     T unu = T(0), unv = T(0);
@@ -1924,17 +1923,17 @@ __global__ void __launch_bounds__(256) k_cg_dir(CGArgs<T> a, int par)
     haar_pm_tile2<T>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
     if (oku) a.tu[ku] = iu ? tu[ly][lx] : T(0);
     if (okv) a.tv[kv] = iv ? tv[ly][lx] : T(0);
+    return false;
 }
 
 // part 2: c = R A t and partials of dot(u, c)
 template <typename T>
-__global__ void __launch_bounds__(256) k_cg_apply(CGArgs<T> a)
+__device__ __forceinline__ void cg_apply_tile(const CGArgs<T> &a, const int L)
 {
     __shared__ OpTile<T> s;
     __shared__ T tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
     __shared__ double lds[8];
     const GridDims &g = a.g;
-    const int L = xcd_tile(a.nblk);
     if (L >= g.nbx * g.nby) return;
     if (a.st->done) return;  // written only by earlier kernels
     int bx, by;
@@ -1967,11 +1966,10 @@ __global__ void __launch_bounds__(256) k_cg_apply(CGArgs<T> a)
 // part 3: alpha = res^2 / dot(u, c), x += alpha u, r -= alpha c, partials of
 // ||r||^2 and the iteration count
 template <typename T>
-__global__ void __launch_bounds__(256) k_cg_update(CGArgs<T> a, int par)
+__device__ __forceinline__ void cg_update_tile(const CGArgs<T> &a, int par, const int L)
 {
     __shared__ double lds[8];
     const GridDims &g = a.g;
-    const int L = xcd_tile(a.nblk);
     if (L >= g.nbx * g.nby) return;
     if (a.st->done) return;  // written only by earlier kernels
     int bx, by;
@@ -2005,6 +2003,55 @@ __global__ void __launch_bounds__(256) k_cg_update(CGArgs<T> a, int par)
     if (threadIdx.x == 0) {
         if (has_u) a.Pr[bx + g.nbxu * by] = r2u;
         if (has_v) a.Pr[g.nbxu * g.nbyu + bx + g.nbxv * by] = r2v;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_cg_dir(CGArgs<T> a, int par)
+{
+    (void)cg_dir_tile<T>(a, par, xcd_tile(a.nblk));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_cg_apply(CGArgs<T> a)
+{
+    cg_apply_tile<T>(a, xcd_tile(a.nblk));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_cg_update(CGArgs<T> a, int par)
+{
+    cg_update_tile<T>(a, par, xcd_tile(a.nblk));
+}
+
+// The whole coarse CG of a small grid in one workgroup (after k_cg_setup):
+// the workgroup runs each part of an iteration on every tile in turn, so
+// the barrier between two tiles stands in for the kernel boundary.  Tiles
+// read only what earlier parts wrote and write only their own points,
+// partials and parity slots, so the results are those of the three
+// launches per iteration, bit for bit.  Ends at the stop, which every
+// thread of every tile decides alike from the same partials and scalars,
+// after maxiter + 1 iterations at most.
+template <typename T>
+__global__ void __launch_bounds__(256) k_cg_solve1(CGArgs<T> a)
+{
+    const int ntile = a.g.nbx * a.g.nby;
+    for (int q = 0; q <= a.maxiter; ++q) {
+        const int par = q & 1;
+        bool stop = false;
+        for (int L = 0; L < ntile; ++L) {
+            stop = cg_dir_tile<T>(a, par, L);
+            __syncthreads();
+        }
+        if (stop) break;
+        for (int L = 0; L < ntile; ++L) {
+            cg_apply_tile<T>(a, L);
+            __syncthreads();
+        }
+        for (int L = 0; L < ntile; ++L) {
+            cg_update_tile<T>(a, par, L);
+            __syncthreads();
+        }
     }
 }
 
@@ -2387,7 +2434,7 @@ struct CGMIterArgs {
 // y pass straight into the window); intermediates; operator and R x pass;
 // R y pass and P y pass; P x pass and the new partials.
 template <int PAR, bool INIT, int LEV>
-__device__ __forceinline__ void cgm_iter_tile(const CGMIterArgs &a, const int L)
+__device__ __forceinline__ bool cgm_iter_tile(const CGMIterArgs &a, const int L)
 {
     static_assert(!INIT || PAR == 0, "the initial launch reads set 0");
     struct Smem {
@@ -2403,7 +2450,7 @@ __device__ __forceinline__ void cgm_iter_tile(const CGMIterArgs &a, const int L)
     __shared__ Smem sm;
     const GridDims &g = a.g;
     const int ntile = g.nbx * g.nby;
-    if (L >= ntile) return;
+    if (L >= ntile) return false;
     const int tid = threadIdx.x;
     int bx, by;
     const int tile_id = band_tile(g, L, bx, by);
@@ -2509,7 +2556,7 @@ __device__ __forceinline__ void cgm_iter_tile(const CGMIterArgs &a, const int L)
         wg_sum2(gamma, delta, sm.red0);
         if (S.done) {
             if (L == 0 && tid == 0) a.st->s[NP] = S;
-            return;
+            return true;
         }
         // stop when ||r|| <= tol, tested as ||r||^2 <= tol^2
         double tol2 = S.tol2;
@@ -2535,7 +2582,7 @@ __device__ __forceinline__ void cgm_iter_tile(const CGMIterArgs &a, const int L)
                 el(arr(CGM_X), ob) = x_u0 + fp * p_u0;
                 el(arr(CGM_X + 1), ob) = x_v0 + fp * p_v0;
             }
-            return;
+            return true;
         }
         const double beta = S.it == 0 ? 0.0 : gamma * S.igamma;
         const double alpha = gamma / (S.it == 0 ? delta : delta - beta * gamma * S.ialpha);
@@ -2699,6 +2746,7 @@ __device__ __forceinline__ void cgm_iter_tile(const CGMIterArgs &a, const int L)
     if (tid == 0)
         a.Pgd[NP * a.npad + tile_id] = make_float2((sm.red1[0] + sm.red1[1]) + (sm.red1[2] + sm.red1[3]),
                                              (sm.red1[4] + sm.red1[5]) + (sm.red1[6] + sm.red1[7]));
+    return false;
 }
 
 // One coarse-CG iteration of one CG problem (the tile of workgroup
@@ -2706,7 +2754,32 @@ __device__ __forceinline__ void cgm_iter_tile(const CGMIterArgs &a, const int L)
 template <int PAR, bool INIT, int LEV>
 __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
 {
-    cgm_iter_tile<PAR, INIT, LEV>(a, xcd_tile(a.nblk));
+    (void)cgm_iter_tile<PAR, INIT, LEV>(a, xcd_tile(a.nblk));
+}
+
+// The whole mixed coarse CG of a small grid in one workgroup (after
+// k_cgm_setup): the initial launch's work and then each iteration on every
+// tile in turn, the barrier between two tiles standing in for the kernel
+// boundary.  A tile reads the other tiles' rings and partials of the
+// previous iteration only, so the results are those of one launch per
+// iteration, bit for bit.  Every thread decides the stop alike (same
+// partials, same scalars); at most maxiter + 1 iterations.
+template <int LEV>
+__global__ void __launch_bounds__(256) k_cgm_solve1(CGMIterArgs a)
+{
+    const int ntile = a.g.nbx * a.g.nby;
+    for (int L = 0; L < ntile; ++L) {
+        (void)cgm_iter_tile<0, true, LEV>(a, L);
+        __syncthreads();
+    }
+    for (int q = 1; q <= a.maxiter + 1; ++q) {
+        bool stop = false;
+        for (int L = 0; L < ntile; ++L) {
+            stop = (q & 1) ? cgm_iter_tile<1, false, LEV>(a, L) : cgm_iter_tile<0, false, LEV>(a, L);
+            __syncthreads();
+        }
+        if (stop) break;
+    }
 }
 
 // The same iteration of several independent CG problems (an ensemble's
@@ -2727,7 +2800,7 @@ __global__ void __launch_bounds__(256) k_cgm_iter_batch(CGMBatch B)
     const int m = B.group_member[blockIdx.x >> 3];
     const CGMIterArgs a = B.members[m];
     const int w = (int)blockIdx.x - B.first[m];
-    cgm_iter_tile<PAR, INIT, LEV>(a, (w & 7) * (a.nblk >> 3) + (w >> 3));
+    (void)cgm_iter_tile<PAR, INIT, LEV>(a, (w & 7) * (a.nblk >> 3) + (w >> 3));
 }
 
 // out = mask_out ? H(mask_in ? in : 0) : 0 on the u grid (comp 0) and v grid
