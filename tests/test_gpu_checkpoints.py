@@ -16,18 +16,19 @@ def _step(name):
     sys.stdout.flush()
 
 
-def _sim(W, path, niter0=0, n=20, dump=False):
-    m = W.workloads.mismip_plus_fixture()
+def _sim(W, path, niter0=0, n=20, dump=False, precision="fp64"):
+    m = W.workloads.mismip_plus_fixture(precision)
     tp = W.TimesteppingParams(niter0=niter0, dt=0.1, n_iter_total=n, chkpt_freq=0.5, pchkpt_freq=1.0,
                               chkpt_path=str(path))
     op = W.OutputParams(prefix=os.path.join(str(path), "run"), dump_vel=dump)
     return W.Simulation(model=m, timestepping_params=tp, output_params=op)
 
 
-def test_pickup_continues_bit_equal(W, tmp_path, monkeypatch):
-    _step("checkpoint / pickup")
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pickup_continues_bit_equal(W, tmp_path, monkeypatch, precision):
+    _step("checkpoint / pickup, %s" % precision)
     monkeypatch.chdir(tmp_path)
-    full = _sim(W, tmp_path, dump=True)
+    full = _sim(W, tmp_path, dump=True, precision=precision)
     W.run_simulation(full)
     names = sorted(os.listdir(tmp_path))
     assert "PChkpt_0000000010.npz" in names and "PChkpt_0000000020.npz" in names
@@ -37,7 +38,7 @@ def test_pickup_continues_bit_equal(W, tmp_path, monkeypatch):
     v = np.fromfile(os.path.join(str(tmp_path), "run_V.bin"), dtype=">f8")
     np.testing.assert_array_equal(u, full.model.fields.gu.u[:-1, :].ravel(order="F"))
     np.testing.assert_array_equal(v, full.model.fields.gv.v[:, :-1].ravel(order="F"))
-    resumed = _sim(W, tmp_path, niter0=10)
+    resumed = _sim(W, tmp_path, niter0=10, precision=precision)
     assert resumed.clock.n_iter == 10
     W.run_simulation(resumed)
     assert resumed.clock.n_iter == 20

@@ -65,3 +65,23 @@ def test_timestepping_checkpoint_steps_and_output_params(W, tmp_path):
         assert W.TimesteppingParams(dt=1.0, n_iter_total=2, chkpt_path=str(tmp_path / "missing")).chkpt_path == "./"
     with pytest.raises(W.ArgumentError):
         W.OutputParams(outputs={"h": None})
+
+
+def test_pickup_restores_fields_clock_and_saved_dt(W, tmp_path, monkeypatch):
+    """pickup! from PChkpt_<niter0>: the fields and the clock come back, and
+    the model's dt is the saved one (the reference replaces the model with
+    the saved model, Simulation.jl:77-79), while the new TimesteppingParams
+    keeps its own dt for the thickness steps."""
+    monkeypatch.chdir(tmp_path)
+    m = W.workloads.mismip_plus_fixture()
+    sim = W.Simulation(model=m, timestepping_params=W.TimesteppingParams(dt=0.1, n_iter_total=20))
+    m.fields.gh.h[:] = m.fields.gh.h + 1.25
+    m.fields.gu.u[:] = 3.0
+    sim.clock = W.Clock(10, 1.0)
+    W.save_checkpoint(sim, "PChkpt_%010d.npz" % 10)
+    m2 = W.workloads.mismip_plus_fixture()
+    sim2 = W.Simulation(model=m2, timestepping_params=W.TimesteppingParams(niter0=10, dt=0.2, n_iter_total=20))
+    assert (sim2.clock.n_iter, sim2.clock.time) == (10, 1.0)
+    assert m2.params.dt == 0.1 and sim2.timestepping_params.dt == 0.2
+    np.testing.assert_array_equal(m2.fields.gh.h, m.fields.gh.h)
+    np.testing.assert_array_equal(m2.fields.gu.u, m.fields.gu.u)

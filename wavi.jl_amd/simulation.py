@@ -59,14 +59,20 @@ def save_checkpoint(simulation, fname):
 
 
 def load_checkpoint(simulation, fname):
-    """Restore the fields and the clock written by save_checkpoint into the
-    simulation's model (built with the same grid and parameters)."""
+    """Restore the fields, the model's dt and the clock written by
+    save_checkpoint into the simulation's model (built with the same grid and
+    parameters).  pickup! replaces the model with the saved one
+    (Simulation.jl:77-79), so the model's dt, which the velocity solve's
+    implicit terms use, is the saved dt even where the new
+    TimesteppingParams has another; the thickness steps use the new
+    TimesteppingParams.dt, as in the reference (run_simulation.jl:28)."""
     from .model import FIELD_MAP
     m = simulation.model
     with np.load(fname, allow_pickle=False) as d:
         for c, a, fid in FIELD_MAP:
             m._set_array(c, a, d[fid])
         n_iter, time = d["clock"]
+        m.params.dt = float(d["dt"][0])
     simulation.clock = Clock(int(n_iter), float(time))
     return simulation
 
