@@ -111,6 +111,7 @@ struct wavi_ctx {
     hipEvent_t ev_tune[2] = {};     // timing events of the smoother's tile-shape tuning
     int smooth_shape[2] = {-1, -1}; // smoother tile shape by V-cycle precision (0 FP64, 1 FP32); -1: tune
     int cg_mode = WAVI_CG_AUTO;      // coarse CG: one launch per iteration or one workgroup (wavi_set_coarse_cg)
+    int cg_prev = 0;                 // iterations of the context's previous coarse CG (0: none yet)
     hipGraphExec_t cgm_chunk = nullptr;  // 32 k_cgm_iter launches (parities 1, 0, ...) as one graph
     double *Pr0 = nullptr, *Pr1 = nullptr, *Pd = nullptr, *Pb = nullptr, *Pn = nullptr;
     double *quad = nullptr;  // 4 * nh pos_fraction quadrants
@@ -1174,15 +1175,23 @@ static bool use_cg_graph()
     return on;
 }
 
-// Coarse-CG chunk k holds min(32, 2^(k+1)) iterations.  A CG that stops after
-// a few iterations (the usual case from the second step of a transient run
-// on) then leaves few launches queued past its stop, while a long CG runs in
-// chunks of 32 after its first 30 iterations.  The host reads the stop
-// report of chunk k - 1 once chunk k is queued, so one chunk keeps the GPU
-// busy while the host waits.  Chunk lengths are even, so every chunk of 32
-// starts at the same launch parity.
+// Iterations (launches, or triplets of launches in FP64) of coarse-CG chunk
+// k.  The host reads the stop report of chunk k - 1 once chunk k is queued,
+// so one chunk keeps the GPU busy while the host waits, and every launch
+// queued past the stop runs as a no-op (the kernels read the stop from the
+// partials).  Without a prediction (prev = 0) chunk k holds min(32,
+// 2^(k+1)), so a CG that stops after a few iterations leaves few launches
+// past its stop while a long CG runs in chunks of 32 after its first 30.
+// With prev = the iterations of the context's previous coarse CG (a
+// transient's CGs change length slowly from step to step), chunk 0 holds
+// prev + 1, at most 32 (the launches up to and including the one that
+// stops, if this CG is as long), and chunk k >= 1 holds min(32, 2^k).
 constexpr int CG_CHUNK = 32;
-static int cg_chunk_len(int k) { return k >= 4 ? CG_CHUNK : 2 << k; }
+static int cg_chunk_len(int k, int prev)
+{
+    if (prev > 0) return k == 0 ? std::min(prev + 1, CG_CHUNK) : (k >= 5 ? CG_CHUNK : 1 << k);
+    return k >= 4 ? CG_CHUNK : 2 << k;
+}
 
 // Relative stop of the mixed-precision coarse CG: 2^-19 = 16 FP32 ulps.
 constexpr double CGM_FP32_RELTOL = 0x1p-19;
@@ -1231,7 +1240,7 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
     const int nq = ctx->S.maxiter_coarse + 1;
     int q = 0, k = 0;
     while (q < nq) {
-        const int n = std::min(cg_chunk_len(k), nq - q);
+        const int n = std::min(cg_chunk_len(k, ctx->cg_prev), nq - q);
         // when profiling, odd chunks time sampled launches and even chunks
         // of full length time the whole chunk
         const bool sampled = ctx->prof && (k & 1) != 0;
@@ -1260,8 +1269,12 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
         }
         ++k;
     }
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    prof_collect(ctx);
+    // A stop seen between chunks needs no wait for the launches queued past
+    // it (they change nothing); the timing events do.
+    if (!hf->done || ctx->prof) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        prof_collect(ctx);
+    }
     // triplet q iterates when q < iters
     burst_collect(ctx, WAVI_KT_CG_ITER_FP64, hf->done ? hf->iters - 1 : -1);
     if (!hf->done) {
@@ -1269,6 +1282,7 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
         return WAVI_ESTATE;
     }
     *iters = hf->iters;
+    ctx->cg_prev = hf->iters;
     return WAVI_OK;
 }
 
@@ -1349,6 +1363,7 @@ static int cgm_prepare(wavi_ctx *ctx, const VSet<float> &w)
     sa.R = ctx->rings;
     sa.Pb = ctx->Pb;
     sa.st = ctx->stm;
+    sa.hflag = ctx->d_flag;
     sa.nblk = ctx->nblk;
     ctx->h_flag->done = 0;
     ctx->h_flag->iters = 0;
@@ -1403,10 +1418,10 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     const bool use_graph = use_cg_graph();
     int q = 1, k = 0;
     while (q <= qmax) {
-        const int n = std::min(cg_chunk_len(k), qmax - q + 1);
+        const int n = std::min(cg_chunk_len(k, ctx->cg_prev), qmax - q + 1);
         const int bev = n == CG_CHUNK ? burst_begin(ctx) : -1;
-        if (n == CG_CHUNK && use_graph) {
-            // a full chunk starts at an odd q (chunk lengths are even): the graph's parities
+        if (n == CG_CHUNK && use_graph && (q & 1)) {
+            // the graph's launches have parities 1, 0, 1, ...: it replays a chunk that starts at an odd q
             if (!ctx->cgm_chunk) {
                 std::lock_guard<std::mutex> lock(g_graph_mu);
                 hipGraph_t gr = nullptr;
@@ -1444,8 +1459,12 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     hipLaunchKernelGGL(k_cgm_unpack, flat(std::max(ctx->nu, ctx->nv)), dim3(256), 0, ctx->stream, g, Lo, w.xc.u,
                        w.xc.v);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    prof_collect(ctx);
+    // A stop seen between chunks needs no wait for the launches queued past
+    // it (they change nothing); the timing events do.
+    if (!hf->done || ctx->prof) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        prof_collect(ctx);
+    }
     // launch q iterates when q <= iters
     burst_collect(ctx, WAVI_KT_CG_ITER_MIXED, hf->done ? hf->iters : -1);
     if (!hf->done) {
@@ -1453,6 +1472,7 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
         return WAVI_ESTATE;
     }
     *iters = hf->iters;
+    ctx->cg_prev = hf->iters;
     return WAVI_OK;
 }
 
@@ -2121,7 +2141,7 @@ static int run_cg_batch(const std::vector<wavi_ctx *> &mem, const std::vector<hi
     // launch q performs iteration q - 1 of every member whose CG has not stopped
     int q = 1, c = 0;
     while (q <= qmax) {
-        const int n = std::min(cg_chunk_len(c), qmax - q + 1);
+        const int n = std::min(cg_chunk_len(c, 0), qmax - q + 1);
         for (int s = 0; s < n; ++s, ++q) {
             if (q & 1)
                 launch_cgm_batch<1, false>(cs, levels, nwg, B);

@@ -1816,6 +1816,11 @@ __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
         a.st->iters = 0;
         a.st->it[0] = 0;
         a.st->it[1] = 0;
+        // also reset on the stream: launches queued past the previous
+        // CG's stop, which rewrite the report, may still have been running
+        // when the host reset it
+        a.hflag->done = 0;
+        a.hflag->iters = 0;
     }
     int bx, by;
     const int tile_id = band_tile(g, L, bx, by);
@@ -2206,6 +2211,7 @@ struct CGMArgs {
     float *R;                // ring images
     double *Pb;              // partials of ||b||^2 per u / v block
     CGMScalars *st;
+    CGMHostFlag *hflag;      // host-mapped stop report, reset here
     int nblk;
 };
 
@@ -2231,6 +2237,8 @@ __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
         }
         a.st->done = 0;
         a.st->iters = 0;
+        a.hflag->done = 0;
+        a.hflag->iters = 0;
     }
     int bx, by;
     const int tile_id = band_tile(g, L, bx, by);
