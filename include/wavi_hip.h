@@ -90,7 +90,7 @@ typedef struct wavi_solver_params {
     double tol_picard;
     int32_t maxiter_picard;
     double tol_coarse;
-    int32_t maxiter_coarse;
+    int32_t maxiter_coarse;       /* 0..2^30 */
     int32_t levels;               /* Haar levels: blocks of 2^(levels+1), 0..3 */
     double wavelet_threshold;
     int32_t nsmooth;
@@ -359,7 +359,9 @@ int wavi_smoother_tile(const wavi_ctx *ctx, int *tx, int *ty);
  * choice, at present WAVI_CG_LAUNCHES for every grid: measured on MI355X,
  * the workgroup is 3.4 to 3.6 times slower per iteration even on a grid of
  * six 16 x 16 tiles.  wavi_coarse_cg reports the mode in effect for the
- * context.  No reference counterpart. */
+ * context.  Lockstep batches (wavi_update_velocities_lockstep*) and the
+ * small members of wavi_misfit_gradient_batch share one launch per CG
+ * iteration whatever the mode.  No reference counterpart. */
 #define WAVI_CG_AUTO 0
 #define WAVI_CG_LAUNCHES 1
 #define WAVI_CG_WORKGROUP 2

@@ -135,6 +135,11 @@ def test_create_errors_without_gpu(W):
     assert lib.wavi_ctx_create(ctypes.byref(desc), ctypes.byref(phys), ctypes.byref(solver), 0,
                                ctypes.byref(out)) == W._lib.WAVI_EARG
     solver.levels = 3
+    solver.maxiter_coarse = (1 << 30) + 1
+    assert lib.wavi_ctx_create(ctypes.byref(desc), ctypes.byref(phys), ctypes.byref(solver), 0,
+                               ctypes.byref(out)) == W._lib.WAVI_EARG
+    assert b"maxiter_coarse" in lib.wavi_last_error(None)
+    solver.maxiter_coarse = 1000
     assert lib.wavi_ctx_create(ctypes.byref(desc), ctypes.byref(phys), ctypes.byref(solver), 7,
                                ctypes.byref(out)) == W._lib.WAVI_EARG
     assert lib.wavi_ctx_create(None, ctypes.byref(phys), ctypes.byref(solver), 0, ctypes.byref(out)) == W._lib.WAVI_EARG

@@ -463,6 +463,10 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
         g_create_error = "iteration counts must be non-negative";
         return WAVI_EARG;
     }
+    if (solver->maxiter_coarse > (1 << 30)) {
+        g_create_error = "maxiter_coarse must be at most 2^30";
+        return WAVI_EARG;
+    }
     if (precision_mode != WAVI_PRECISION_FP64 && precision_mode != WAVI_PRECISION_MIXED) {
         g_create_error = "unknown precision mode";
         return WAVI_EARG;
