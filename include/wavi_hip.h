@@ -313,6 +313,13 @@ int wavi_misfit_gradient(wavi_ctx *ctx, int n_picard, double tol_linear, int max
 int wavi_misfit_gradient_batch(wavi_ctx **ctxs, int n, int nthreads, int n_picard, double tol_linear,
                                int maxit_linear, const double *const *u_obs, const double *const *v_obs,
                                double *J, double *const *grad_logc, wavi_adjoint_report *reports, int *codes);
+/* The linear solves of wavi_misfit_gradient (flexible CG, whose
+ * preconditioner may be any fixed or varying SPD map) run the V-cycle with
+ * its coarse CG preconditioned by the diagonal of the coarse operator
+ * (Jacobi), computed for each linear system; on = 0 restores the
+ * update_velocities! V-cycle there.  The gradient is the same to the solve
+ * tolerance either way.  Default on.  No reference counterpart. */
+int wavi_set_linear_coarse_jacobi(wavi_ctx *ctx, int on);
 
 /* ---- test hooks: single pieces of the solve on host arrays ---- */
 /* op_fun! (src/utilities.jl:61-145) on full-grid inputs xu ((nx+1)*ny) and
@@ -329,6 +336,11 @@ int wavi_op_diag(wavi_ctx *ctx, double *du, double *dv);
  * x starting at zero; uses and updates the coarse warm start. */
 int wavi_vcycle(wavi_ctx *ctx, const double *ru, const double *rv,
                 double *eu, double *ev, int32_t *coarse_iters);
+/* Diagonal of the coarse-CG operator A_c = R A P (with the masks the CG
+ * applies; update_preconditioners.jl:73) for the current coefficients
+ * (call wavi_inner_update first), in the full-grid layout, zero off the
+ * wavelet masks. */
+int wavi_coarse_diag(wavi_ctx *ctx, double *du, double *dv);
 /* 2-D Haar transform of one component (comp 0 = u grid, 1 = v grid).
  * dir 0: forward dwt (UGrid.jl:89); 1: inverse idwt (UWavelets.jl:48);
  * 2: transposed inverse idwt^T (UWavelets.jl:49). */

@@ -103,6 +103,63 @@ def test_gradient_on_a_ragged_domain(W, oracle, precision):
     assert err <= (1e-8 if precision == "fp64" else 1e-6)
 
 
+@pytest.mark.parametrize("levels", [1, 3])
+def test_coarse_diagonal_matches_the_oracle_operator(W, oracle, levels):
+    """wavi_coarse_diag, the Jacobi scaling of the coarse CG in the
+    adjoint's linear solves, against the diagonal of the oracle's coarse
+    operator R A P (update_preconditioners.jl:73) read off column by column:
+    a ragged domain of 37 x 22 (partial tiles and Haar blocks, holes, a
+    moving front), Haar levels 1 and 3."""
+    from helpers import ragged
+    nx, ny = 37, 22
+    _step("coarse diagonal vs oracle, levels %d" % levels)
+    m = ragged(W, nx=nx, ny=ny, maxiter_picard=1, dirichlet=False)
+    m.solver_params.levels = levels
+    W.update_geometry(m)
+    spec = m.parallel_spec
+    lib = spec._lib
+    o = oracle.OracleModel(m)
+    assert lib.wavi_inner_update(spec.ctx) == 0
+    o.inner_update()
+    nu, nv = (nx + 1) * ny, nx * (ny + 1)
+    du, dv = np.zeros(nu), np.zeros(nv)
+    assert lib.wavi_coarse_diag(spec.ctx, W._lib.dptr(du), W._lib.dptr(dv)) == 0
+    ref = np.zeros(nu + nv)
+    for k in range(nu + nv):
+        e = np.zeros(nu + nv)
+        e[k] = 1.0
+        au, av = o.coarse_op(e[:nu].reshape((nx + 1, ny), order="F"), e[nu:].reshape((nx, ny + 1), order="F"))
+        ref[k] = au.ravel(order="F")[k] if k < nu else av.ravel(order="F")[k - nu]
+    got = np.concatenate([du, dv])
+    scale = np.abs(ref).max()
+    assert scale > 0 and np.count_nonzero(ref) > (nu + nv) // 2
+    assert np.array_equal(got == 0.0, ref == 0.0)
+    err = np.abs(got - ref)
+    assert np.all(err <= 1e-10 * np.abs(ref) + 1e-13 * scale), err.max() / scale
+
+
+def test_jacobi_coarse_cg_gives_the_same_gradient_with_fewer_coarse_iterations(W):
+    """The adjoint's linear solves with the Jacobi-scaled coarse CG (the
+    default) and with the V-cycle of update_velocities!: the same J and
+    gradient to the solve tolerance, in fewer V-cycles."""
+    nx, ny = 96, 24
+    _step("Jacobi coarse CG in the adjoint solves %dx%d" % (nx, ny))
+    l0, uo, vo = _setup(W, nx, ny)
+    res = {}
+    for on in (0, 1):
+        m = _model(W, l0, "fp64")
+        assert m.parallel_spec._lib.wavi_set_linear_coarse_jacobi(m.parallel_spec.ctx, on) == 0
+        res[on] = W.misfit_gradient(m, N_PICARD, uo, vo, TOL_LIN)
+    (J0, g0, r0), (J1, g1, r1) = res[0], res[1]
+    v0, v1 = r0["fwd_vcycles"] + r0["adj_vcycles"], r1["fwd_vcycles"] + r1["adj_vcycles"]
+    err = np.linalg.norm(g1 - g0) / np.linalg.norm(g0)
+    print("  V-cycles %d plain, %d Jacobi; J rel %.2e, gradient rel %.2e" % (v0, v1, abs(J1 - J0) / abs(J0), err))
+    assert r0["unconverged"] == 0 and r1["unconverged"] == 0
+    assert abs(J1 - J0) <= 1e-11 * abs(J0)
+    assert err <= 1e-9
+    assert v1 < v0
+
+
 def test_fixed_velocities_rejected(W):
     _step("adjoint rejects fixed velocities")
     nx, ny = 32, 8

@@ -165,6 +165,8 @@ SIGNATURES = {
     "wavi_set_smoother_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "wavi_smoother_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "wavi_set_coarse_cg": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "wavi_set_linear_coarse_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "wavi_coarse_diag": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "wavi_coarse_cg": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "wavi_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "wavi_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int64)]),

@@ -114,6 +114,10 @@ struct wavi_ctx {
     int cg_prev = 0;                 // iterations of the context's previous coarse CG (0: none yet)
     hipGraphExec_t cgm_chunk = nullptr;  // 32 k_cgm_iter launches (parities 1, 0, ...) as one graph
     double *Pr0 = nullptr, *Pr1 = nullptr, *Pd = nullptr, *Pb = nullptr, *Pn = nullptr;
+    double *Pz = nullptr;           // partials of (r, D^-1 r) of the Jacobi-preconditioned coarse CG
+    DevVec cdinv;                   // 1 / |diag(A_c)| for the Jacobi-preconditioned coarse CG
+    bool cg_pre = false;            // the V-cycles of do_linear_solve run the Jacobi-preconditioned coarse CG
+    bool lin_jacobi = true;         // ... when set (wavi_set_linear_coarse_jacobi)
     double *quad = nullptr;  // 4 * nh pos_fraction quadrants
     double *norms = nullptr; // device: 2 doubles
     double *h_norms = nullptr; // pinned host: 2 doubles
@@ -630,6 +634,9 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
     CREATECHK(zalloc(&ctx->Pd, g.npart));
     CREATECHK(zalloc(&ctx->Pb, g.npart));
     CREATECHK(zalloc(&ctx->Pn, g.npart));
+    CREATECHK(zalloc(&ctx->Pz, g.npart));
+    CREATECHK(zalloc(&ctx->cdinv.u, ctx->nu));
+    CREATECHK(zalloc(&ctx->cdinv.v, ctx->nv));
     CREATECHK(zalloc(&ctx->quad, 4 * (size_t)ctx->nh));
     CREATECHK(zalloc(&ctx->norms, 2));
     CREATECHK(hipHostMalloc((void **)&ctx->h_norms, 2 * sizeof(double), hipHostMallocDefault));
@@ -702,7 +709,7 @@ static void ctx_free(wavi_ctx *ctx)
     for (double *p : ctx->saved)
         if (p) (void)hipFree(p);
     for (double *p : {ctx->zeta, ctx->qw, ctx->heta_h, ctx->heta_c, ctx->dimp, ctx->Pr0, ctx->Pr1, ctx->Pd, ctx->Pb,
-                      ctx->Pn, ctx->quad, ctx->norms})
+                      ctx->Pn, ctx->Pz, ctx->cdinv.u, ctx->cdinv.v, ctx->quad, ctx->norms})
         if (p) (void)hipFree(p);
     for (DevVec *dv : {&ctx->x, &ctx->b, &ctx->r, &ctx->e, &ctx->d, &ctx->rho, &ctx->t, &ctx->f3, &ctx->fixedv,
                        &ctx->cr, &ctx->cp, &ctx->cq, &ctx->ct}) {
@@ -1161,6 +1168,9 @@ static CGArgs<T> cg_args(wavi_ctx *ctx, const VSet<T> &w)
     a.Pr = ctx->Pr0;
     a.Pd = ctx->Pd;
     a.Pb = ctx->Pb;
+    a.Pz = ctx->Pz;
+    a.dinvu = ctx->cdinv.u;
+    a.dinvv = ctx->cdinv.v;
     a.st = ctx->st;
     a.hflag = ctx->d_flag;
     a.tol_coarse = ctx->S.tol_coarse;
@@ -1223,11 +1233,18 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
                        ctx->mwu, ctx->mwv, a.tu, a.tv, ctx->ui, ctx->vi);
     ctx->h_flag->done = 0;
     ctx->h_flag->iters = 0;
-    hipLaunchKernelGGL(k_cg_setup<T>, tb, dim3(256), 0, ctx->stream, a);
+    const bool pre = ctx->cg_pre;
+    if (pre)
+        hipLaunchKernelGGL((k_cg_setup<T, true>), tb, dim3(256), 0, ctx->stream, a);
+    else
+        hipLaunchKernelGGL((k_cg_setup<T, false>), tb, dim3(256), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
     volatile CGMHostFlag *hf = ctx->h_flag;
     if (cg_one_workgroup(ctx)) {
-        hipLaunchKernelGGL(k_cg_solve1<T>, dim3(1), dim3(256), 0, ctx->stream, a);
+        if (pre)
+            hipLaunchKernelGGL((k_cg_solve1<T, true>), dim3(1), dim3(256), 0, ctx->stream, a);
+        else
+            hipLaunchKernelGGL((k_cg_solve1<T, false>), dim3(1), dim3(256), 0, ctx->stream, a);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->stream));
         if (!hf->done) {
@@ -1253,7 +1270,10 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
             const int par = q & 1;
             {
                 ProfScope ps(ctx, WAVI_KT_CG_DIR, sampled);
-                hipLaunchKernelGGL(k_cg_dir<T>, tb, dim3(256), 0, ctx->stream, a, par);
+                if (pre)
+                    hipLaunchKernelGGL((k_cg_dir<T, true>), tb, dim3(256), 0, ctx->stream, a, par);
+                else
+                    hipLaunchKernelGGL((k_cg_dir<T, false>), tb, dim3(256), 0, ctx->stream, a, par);
             }
             {
                 ProfScope ps(ctx, WAVI_KT_CG_APPLY, sampled);
@@ -1261,7 +1281,10 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
             }
             {
                 ProfScope ps(ctx, WAVI_KT_CG_UPDATE, sampled);
-                hipLaunchKernelGGL(k_cg_update<T>, tb, dim3(256), 0, ctx->stream, a, par);
+                if (pre)
+                    hipLaunchKernelGGL((k_cg_update<T, true>), tb, dim3(256), 0, ctx->stream, a, par);
+                else
+                    hipLaunchKernelGGL((k_cg_update<T, false>), tb, dim3(256), 0, ctx->stream, a, par);
             }
         }
         burst_end(ctx, bev, q - 1, n);
@@ -1710,12 +1733,38 @@ static int dots(wavi_ctx *ctx, const LinScratch &s, int count, const double *con
 // zero: the warm start of the Picard loop (the previous correction) would
 // be orders of magnitude larger than the shrinking residual's correction,
 // and cancelling it costs the digits the solve is after.
+// 1 / |diag(A_c)| of the current coefficients into ctx->cdinv (and the
+// diagonal into diag when given).
+static int do_coarse_diag(wavi_ctx *ctx, const DevVec *diag)
+{
+    const GridDims &g = ctx->g;
+    hipLaunchKernelGGL(k_coarse_diag, dim3(g.nbx * g.nby, 2 * TILE), dim3(256), 0, ctx->stream, g, coef64(ctx),
+                       ctx->mwu, ctx->mwv, ctx->S.levels, ctx->cdinv.u, ctx->cdinv.v, diag ? diag->u : nullptr,
+                       diag ? diag->v : nullptr);
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
+// The V-cycles of one linear solve run the Jacobi-preconditioned coarse CG.
+struct CgPreScope {
+    wavi_ctx *ctx;
+    CgPreScope(wavi_ctx *c, bool on) : ctx(c) { ctx->cg_pre = on; }
+    ~CgPreScope() { ctx->cg_pre = false; }
+};
+
 static int do_linear_solve(wavi_ctx *ctx, const DevVec &b, const DevVec &x, const LinScratch &s, double tol,
                            int maxit, int *vcycles, double *rel_out, bool *converged)
 {
     const GridDims &g = ctx->g;
     const int nu = ctx->nu, nv = ctx->nv;
     dim3 tg(g.nbx, g.nby);
+    // The solve is the flexible CG's, so its preconditioner is free: the
+    // V-cycle's coarse CG is scaled by the coarse operator's diagonal, which
+    // the coarse problem (Haar levels of very different scale) needs far
+    // more than the reference's plain CG can give it.
+    const bool jacobi = ctx->lin_jacobi && ctx->precision == WAVI_PRECISION_FP64;
+    if (jacobi) CHK(do_coarse_diag(ctx, nullptr));
+    CgPreScope pre(ctx, jacobi);
     auto vcycle = [&](int *cg) -> int {
         hipLaunchKernelGGL(k_fill<double>, flat(nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_WU_CORRECTION_COARSE],
                            0.0, nu);
@@ -2934,6 +2983,24 @@ int wavi_op_diag(wavi_ctx *ctx, double *du, double *dv)
     }
     CHK(do_diag(ctx, vset64(ctx)));
     return download2(ctx, ctx->d, du, dv);
+}
+
+int wavi_coarse_diag(wavi_ctx *ctx, double *du, double *dv)
+{
+    if (!ctx) return WAVI_EARG;
+    if (!du || !dv) {
+        ctx->err = "null pointer";
+        return WAVI_EARG;
+    }
+    CHK(do_coarse_diag(ctx, &ctx->t));
+    return download2(ctx, ctx->t, du, dv);
+}
+
+int wavi_set_linear_coarse_jacobi(wavi_ctx *ctx, int on)
+{
+    if (!ctx) return WAVI_EARG;
+    ctx->lin_jacobi = on != 0;
+    return WAVI_OK;
 }
 
 int wavi_vcycle(wavi_ctx *ctx, const double *ru, const double *rv, double *eu, double *ev, int32_t *coarse_iters)

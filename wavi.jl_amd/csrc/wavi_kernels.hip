@@ -1750,6 +1750,7 @@ struct CGScalars {
     int it[2];        // iterations done, by parity
     int done;         // sticky stop flag, written only by k_cg_dir
     int iters;        // iterations at termination
+    double rz[2];     // Jacobi-preconditioned CG: (r, D^-1 r) by parity (written by k_cg_dir)
 };
 
 template <typename T>
@@ -1764,6 +1765,8 @@ struct CGArgs {
     T *qu, *qv;       // c = A_c u
     T *tu, *tv;       // P u on the fine grid
     double *Pr, *Pd, *Pb;
+    double *Pz;                       // Jacobi: partials of (r, D^-1 r)
+    const double *dinvu, *dinvv;      // Jacobi: 1 / |diag(A_c)|, 0 off the wavelet masks
     CGScalars *st;
     CGMHostFlag *hflag;   // host-mapped stop report
     double tol_coarse, reltol;
@@ -1802,7 +1805,7 @@ __device__ __forceinline__ void tile_apply_restrict(const CGArgs<T> &a, int i0, 
 
 // setup (t = P x0 computed beforehand): r = b - R A t (b arrives in r),
 // u = 0, partials of ||r0||^2 and ||b||^2; resets the scalars.
-template <typename T>
+template <typename T, bool PRE>
 __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
 {
     __shared__ OpTile<T> s;
@@ -1831,13 +1834,14 @@ __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
     point_prefetch(g, a.c, a.mwu, a.mwv, i0 + lx, j0 + ly, has_u, has_v, pc);
     const T bu = pc.oku ? a.ru[pc.ku] : T(0), bv = pc.okv ? a.rv[pc.kv] : T(0);
     tile_apply_restrict<T>(a, i0, j0, has_u, has_v, pc, s, tu, mu, tv, mv);
-    double r2u = 0.0, b2u = 0.0, r2v = 0.0, b2v = 0.0;
+    double r2u = 0.0, b2u = 0.0, r2v = 0.0, b2v = 0.0, z2u = 0.0, z2v = 0.0;
     if (pc.oku) {
         const T r = bu - (pc.mwu ? tu[ly][lx] : T(0));
         a.ru[pc.ku] = r;
         a.pu[pc.ku] = T(0);
         r2u = (double)r * (double)r;
         b2u = (double)bu * (double)bu;
+        if constexpr (PRE) z2u = (double)r * (a.dinvu[pc.ku] * (double)r);
     }
     if (pc.okv) {
         const T r = bv - (pc.mwv ? tv[ly][lx] : T(0));
@@ -1845,18 +1849,22 @@ __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
         a.pv[pc.kv] = T(0);
         r2v = (double)r * (double)r;
         b2v = (double)bv * (double)bv;
+        if constexpr (PRE) z2v = (double)r * (a.dinvv[pc.kv] * (double)r);
     }
     tree256x2(r2u, r2v, lds);
     tree256x2(b2u, b2v, lds);
+    if constexpr (PRE) tree256x2(z2u, z2v, lds);
     if (threadIdx.x == 0) {
         if (has_u) {
             a.Pr[bx + g.nbxu * by] = r2u;
             a.Pb[bx + g.nbxu * by] = b2u;
+            if constexpr (PRE) a.Pz[bx + g.nbxu * by] = z2u;
         }
         if (has_v) {
             const int o = g.nbxu * g.nbyu + bx + g.nbxv * by;
             a.Pr[o] = r2v;
             a.Pb[o] = b2v;
+            if constexpr (PRE) a.Pz[o] = z2v;
         }
     }
 }
@@ -1865,7 +1873,7 @@ __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
 // u = r + beta u with beta = res^2 / prev^2, t = P u.  The stop decision is
 // recomputed from the partials each time, so it needs no flag: after a stop
 // the partials are frozen and it[1 - par] repeats the count.
-template <typename T>
+template <typename T, bool PRE>
 __device__ __forceinline__ bool cg_dir_tile(const CGArgs<T> &a, int par, const int L)
 {
     __shared__ T tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
@@ -1887,7 +1895,20 @@ __device__ __forceinline__ bool cg_dir_tile(const CGArgs<T> &a, int par, const i
     const T rv = okv ? a.rv[kv] : T(0), pv = okv ? a.pv[kv] : T(0);
     const bool mwu = oku && a.mwu[ku], mwv = okv && a.mwv[kv];
     const bool iu = oku && a.ui[ku], iv = okv && a.vi[kv];
-    const double res = sqrt(lanes_sum(a.Pr, g.npart, lds));
+    double dzu = 0.0, dzv = 0.0, rz = 0.0, prev_rz = 0.0;
+    if constexpr (PRE) {
+        dzu = oku ? a.dinvu[ku] : 0.0;
+        dzv = okv ? a.dinvv[kv] : 0.0;
+        prev_rz = a.st->rz[1 - par];
+    }
+    double res;
+    if constexpr (PRE) {
+        double r2;
+        lanes_sum2(a.Pr, a.Pz, g.npart, lds, r2, rz);
+        res = sqrt(r2);
+    } else {
+        res = sqrt(lanes_sum(a.Pr, g.npart, lds));
+    }
     double tol;
     if (it == 0) {
         const double abstol = a.tol_coarse * sqrt(lanes_sum(a.Pb, g.npart, lds));
@@ -1900,6 +1921,7 @@ __device__ __forceinline__ bool cg_dir_tile(const CGArgs<T> &a, int par, const i
     if (threadIdx.x == 0) {
         if (it == 0) a.st->tol = tol;
         a.st->res[par] = res;
+        if constexpr (PRE) a.st->rz[par] = rz;
         if (done) {
             a.st->done = 1;
             a.st->iters = it;
@@ -1911,15 +1933,26 @@ __device__ __forceinline__ bool cg_dir_tile(const CGArgs<T> &a, int par, const i
         }
     }
     if (done) return true;
-    const double prev = it == 0 ? 1.0 : prev_res;
-    const T beta = (T)((res * res) / (prev * prev));
+    T beta;
+    if constexpr (PRE) {
+        beta = it == 0 ? T(0) : (T)(rz / prev_rz);
+    } else {
+        const double prev = it == 0 ? 1.0 : prev_res;
+        beta = (T)((res * res) / (prev * prev));
+    }
     T unu = T(0), unv = T(0);
     if (oku) {
-        unu = ru + beta * pu;
+        if constexpr (PRE)
+            unu = (T)(dzu * (double)ru) + beta * pu;
+        else
+            unu = ru + beta * pu;
         a.pu[ku] = unu;
     }
     if (okv) {
-        unv = rv + beta * pv;
+        if constexpr (PRE)
+            unv = (T)(dzv * (double)rv) + beta * pv;
+        else
+            unv = rv + beta * pv;
         a.pv[kv] = unv;
     }
     tu[ly][lx] = mwu ? unu : T(0);
@@ -1970,7 +2003,7 @@ __device__ __forceinline__ void cg_apply_tile(const CGArgs<T> &a, const int L)
 
 // part 3: alpha = res^2 / dot(u, c), x += alpha u, r -= alpha c, partials of
 // ||r||^2 and the iteration count
-template <typename T>
+template <typename T, bool PRE>
 __device__ __forceinline__ void cg_update_tile(const CGArgs<T> &a, int par, const int L)
 {
     __shared__ double lds[8];
@@ -1988,33 +2021,50 @@ __device__ __forceinline__ void cg_update_tile(const CGArgs<T> &a, int par, cons
     const double res = a.st->res[par];
     const T xu = oku ? a.xu[ku] : T(0), pu = oku ? a.pu[ku] : T(0), ru = oku ? a.ru[ku] : T(0), qu = oku ? a.qu[ku] : T(0);
     const T xv = okv ? a.xv[kv] : T(0), pv = okv ? a.pv[kv] : T(0), rv = okv ? a.rv[kv] : T(0), qv = okv ? a.qv[kv] : T(0);
+    double dzu = 0.0, dzv = 0.0, rzc = 0.0;
+    if constexpr (PRE) {
+        dzu = oku ? a.dinvu[ku] : 0.0;
+        dzv = okv ? a.dinvv[kv] : 0.0;
+        rzc = a.st->rz[par];
+    }
     const double dot = lanes_sum(a.Pd, g.npart, lds);
-    const T alpha = (T)((res * res) / dot);
+    T alpha;
+    if constexpr (PRE)
+        alpha = (T)(rzc / dot);
+    else
+        alpha = (T)((res * res) / dot);
     if (threadIdx.x == 0) a.st->it[1 - par] = it + 1;
-    double r2u = 0.0, r2v = 0.0;
+    double r2u = 0.0, r2v = 0.0, z2u = 0.0, z2v = 0.0;
     if (oku) {
         a.xu[ku] = xu + alpha * pu;
         const T r = ru - alpha * qu;
         a.ru[ku] = r;
         r2u = (double)r * (double)r;
+        if constexpr (PRE) z2u = (double)r * (dzu * (double)r);
     }
     if (okv) {
         a.xv[kv] = xv + alpha * pv;
         const T r = rv - alpha * qv;
         a.rv[kv] = r;
         r2v = (double)r * (double)r;
+        if constexpr (PRE) z2v = (double)r * (dzv * (double)r);
     }
     tree256x2(r2u, r2v, lds);
+    if constexpr (PRE) tree256x2(z2u, z2v, lds);
     if (threadIdx.x == 0) {
         if (has_u) a.Pr[bx + g.nbxu * by] = r2u;
         if (has_v) a.Pr[g.nbxu * g.nbyu + bx + g.nbxv * by] = r2v;
+        if constexpr (PRE) {
+            if (has_u) a.Pz[bx + g.nbxu * by] = z2u;
+            if (has_v) a.Pz[g.nbxu * g.nbyu + bx + g.nbxv * by] = z2v;
+        }
     }
 }
 
-template <typename T>
+template <typename T, bool PRE>
 __global__ void __launch_bounds__(256) k_cg_dir(CGArgs<T> a, int par)
 {
-    (void)cg_dir_tile<T>(a, par, xcd_tile(a.nblk));
+    (void)cg_dir_tile<T, PRE>(a, par, xcd_tile(a.nblk));
 }
 
 template <typename T>
@@ -2023,10 +2073,85 @@ __global__ void __launch_bounds__(256) k_cg_apply(CGArgs<T> a)
     cg_apply_tile<T>(a, xcd_tile(a.nblk));
 }
 
-template <typename T>
+template <typename T, bool PRE>
 __global__ void __launch_bounds__(256) k_cg_update(CGArgs<T> a, int par)
 {
-    cg_update_tile<T>(a, par, xcd_tile(a.nblk));
+    cg_update_tile<T, PRE>(a, par, xcd_tile(a.nblk));
+}
+
+// Diagonal of the coarse-CG operator A_c = M_w R M_i A M_i P M_w
+// (update_preconditioners.jl:73, with the masks the CG applies) at the
+// points k of the wavelet masks: e_k' A_c e_k = phi' M_i A M_i phi with
+// phi = P e_k, the Haar basis function of k, which is separable
+// (Py (x) Px, from the inverse-transform rows of c_haar) and lives in k's
+// Haar block, inside k's tile.  Workgroup (tile, component, coarse row ky)
+// stages phi for each coarse column kx with a zero halo, evaluates the
+// operator at the tile's points and sums phi (A phi) (FP64, canonical
+// tree).  Writes 1 / |diagonal| (0 where the diagonal is 0 or off the
+// masks) and, if diag is given, the diagonal.  Grid: (tiles, 32).
+__global__ void __launch_bounds__(256) k_coarse_diag(GridDims g, OpCoef<double> c, const uint8_t *mwu,
+                                                     const uint8_t *mwv, int levels, double *dinvu, double *dinvv,
+                                                     double *diagu, double *diagv)
+{
+    __shared__ OpTile<double> s;
+    __shared__ double lds[8];
+    const int L = blockIdx.x;
+    const int comp = blockIdx.y & 1, ky = blockIdx.y >> 1;
+    const int bx = L % g.nbx, by = L / g.nbx;
+    const bool has = comp == 0 ? (bx < g.nbxu && by < g.nbyu) : (bx < g.nbxv && by < g.nbyv);
+    const int nxw = comp == 0 ? g.nxu : g.nxv, nyw = comp == 0 ? g.nyu : g.nyv;
+    const int i0 = bx * TILE, j0 = by * TILE;
+    if (!has || j0 + ky >= nyw) return;  // the whole workgroup
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int B = 2 << levels;
+    PointCoef<double> pc;
+    point_prefetch_op(g, c, i0 + lx, j0 + ly, comp == 0, comp == 1, pc);
+    CellCoef<double> c0, c1;
+    cell_prefetch(g, c, i0, j0, threadIdx.x, c0);
+    cell_prefetch(g, c, i0, j0, threadIdx.x + 256, c1);
+    const bool live = comp == 0 ? (pc.oku && pc.ui) : (pc.okv && pc.vi);
+    // y factor of phi at this thread's row
+    const int jb = ly & ~(B - 1), kjb = ky & ~(B - 1);
+    double fy = 0.0;
+    if (jb == kjb && j0 + ly < nyw) {
+        const int my = min(nyw - (j0 + jb), B), col = ky - kjb;
+        const uint32_t nz = c_haar[levels].nz[0][my][ly - jb], ng = c_haar[levels].neg[0][my][ly - jb];
+        if ((nz >> col) & 1u) fy = ((ng >> col) & 1u) ? -1.0 : 1.0;
+    }
+    const int ib = lx & ~(B - 1);
+    const int mx = i0 + lx < nxw ? min(nxw - (i0 + ib), B) : 0;
+    const uint32_t nzx = mx > 0 ? c_haar[levels].nz[0][mx][lx - ib] : 0u;
+    const uint32_t ngx = mx > 0 ? c_haar[levels].neg[0][mx][lx - ib] : 0u;
+    // zero windows (the halo and the other component stay zero)
+    for (int q = threadIdx.x; q < 18 * 19; q += 256) {
+        (&s.U[0][0])[q] = 0.0;
+        (&s.V[0][0])[q] = 0.0;
+    }
+    const uint8_t *mw = comp == 0 ? mwu : mwv;
+    for (int kx = 0; kx < TILE && i0 + kx < nxw; ++kx) {
+        const int kib = kx & ~(B - 1), col = kx - kib;
+        double fx = 0.0;
+        if (ib == kib && ((nzx >> col) & 1u)) fx = ((ngx >> col) & 1u) ? -1.0 : 1.0;
+        const double phi = live ? fx * fy : 0.0;
+        __syncthreads();  // previous intermediates read
+        if (comp == 0)
+            s.U[ly + 1][lx + 1] = phi;
+        else
+            s.V[ly + 1][lx + 1] = phi;
+        __syncthreads();
+        cell_compute(c, c0, threadIdx.x, s);
+        cell_compute(c, c1, threadIdx.x + 256, s);
+        __syncthreads();
+        double v = 0.0;
+        if (phi != 0.0) v = phi * (comp == 0 ? optile_u(c, s, pc, lx, ly) : optile_v(c, s, pc, lx, ly));
+        v = tree256(v, lds);
+        if (threadIdx.x == 0) {
+            const int k = (i0 + kx) + nxw * (j0 + ky);
+            const double d = mw[k] ? v : 0.0;
+            (comp == 0 ? dinvu : dinvv)[k] = d != 0.0 ? 1.0 / fabs(d) : 0.0;
+            if (diagu) (comp == 0 ? diagu : diagv)[k] = d;
+        }
+    }
 }
 
 // The whole coarse CG of a small grid in one workgroup (after k_cg_setup):
@@ -2037,7 +2162,7 @@ __global__ void __launch_bounds__(256) k_cg_update(CGArgs<T> a, int par)
 // launches per iteration, bit for bit.  Ends at the stop, which every
 // thread of every tile decides alike from the same partials and scalars,
 // after maxiter + 1 iterations at most.
-template <typename T>
+template <typename T, bool PRE>
 __global__ void __launch_bounds__(256) k_cg_solve1(CGArgs<T> a)
 {
     const int ntile = a.g.nbx * a.g.nby;
@@ -2045,7 +2170,7 @@ __global__ void __launch_bounds__(256) k_cg_solve1(CGArgs<T> a)
         const int par = q & 1;
         bool stop = false;
         for (int L = 0; L < ntile; ++L) {
-            stop = cg_dir_tile<T>(a, par, L);
+            stop = cg_dir_tile<T, PRE>(a, par, L);
             __syncthreads();
         }
         if (stop) break;
@@ -2054,7 +2179,7 @@ __global__ void __launch_bounds__(256) k_cg_solve1(CGArgs<T> a)
             __syncthreads();
         }
         for (int L = 0; L < ntile; ++L) {
-            cg_update_tile<T>(a, par, L);
+            cg_update_tile<T, PRE>(a, par, L);
             __syncthreads();
         }
     }
