@@ -294,6 +294,7 @@ typedef struct wavi_adjoint_report {
     int32_t unconverged;   /* linear solves stopped by maxit_linear */
     double max_fwd_rel;    /* largest final ||b - A u|| / ||b|| of a forward solve */
     double max_adj_rel;    /* the same for the adjoint solves */
+    int64_t coarse_cg_iters; /* coarse-CG iterations of all those V-cycles */
 } wavi_adjoint_report;
 int wavi_misfit_gradient(wavi_ctx *ctx, int n_picard, double tol_linear, int maxit_linear,
                          const double *u_obs, const double *v_obs, double *J, double *grad_logc,

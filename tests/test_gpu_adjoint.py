@@ -138,26 +138,31 @@ def test_coarse_diagonal_matches_the_oracle_operator(W, oracle, levels):
     assert np.all(err <= 1e-10 * np.abs(ref) + 1e-13 * scale), err.max() / scale
 
 
-def test_jacobi_coarse_cg_gives_the_same_gradient_with_fewer_coarse_iterations(W):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_jacobi_coarse_cg_gives_the_same_gradient_with_fewer_coarse_iterations(W, precision):
     """The adjoint's linear solves with the Jacobi-scaled coarse CG (the
     default) and with the V-cycle of update_velocities!: the same J and
     gradient to the solve tolerance, in fewer V-cycles."""
     nx, ny = 96, 24
-    _step("Jacobi coarse CG in the adjoint solves %dx%d" % (nx, ny))
+    _step("Jacobi coarse CG in the adjoint solves %s %dx%d" % (precision, nx, ny))
     l0, uo, vo = _setup(W, nx, ny)
     res = {}
     for on in (0, 1):
-        m = _model(W, l0, "fp64")
+        m = _model(W, l0, precision)
         assert m.parallel_spec._lib.wavi_set_linear_coarse_jacobi(m.parallel_spec.ctx, on) == 0
         res[on] = W.misfit_gradient(m, N_PICARD, uo, vo, TOL_LIN)
     (J0, g0, r0), (J1, g1, r1) = res[0], res[1]
     v0, v1 = r0["fwd_vcycles"] + r0["adj_vcycles"], r1["fwd_vcycles"] + r1["adj_vcycles"]
+    c0, c1 = r0["coarse_cg_iters"], r1["coarse_cg_iters"]
     err = np.linalg.norm(g1 - g0) / np.linalg.norm(g0)
-    print("  V-cycles %d plain, %d Jacobi; J rel %.2e, gradient rel %.2e" % (v0, v1, abs(J1 - J0) / abs(J0), err))
+    print("  V-cycles %d plain, %d Jacobi; coarse-CG iterations %d plain, %d Jacobi; J rel %.2e, gradient rel %.2e"
+          % (v0, v1, c0, c1, abs(J1 - J0) / abs(J0), err))
     assert r0["unconverged"] == 0 and r1["unconverged"] == 0
     assert abs(J1 - J0) <= 1e-11 * abs(J0)
     assert err <= 1e-9
-    assert v1 < v0
+    assert v1 <= v0 and c1 < c0
+    if precision == "fp64":
+        assert 2 * c1 < c0
 
 
 def test_fixed_velocities_rejected(W):

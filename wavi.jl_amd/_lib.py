@@ -106,6 +106,7 @@ class AdjointReport(ctypes.Structure):
     _fields_ = [
         ("n_picard", ctypes.c_int32), ("fwd_vcycles", ctypes.c_int32), ("adj_vcycles", ctypes.c_int32),
         ("unconverged", ctypes.c_int32), ("max_fwd_rel", ctypes.c_double), ("max_adj_rel", ctypes.c_double),
+        ("coarse_cg_iters", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -1209,6 +1209,13 @@ static int cg_chunk_len(int k, int prev)
 
 // Relative stop of the mixed-precision coarse CG: 2^-19 = 16 FP32 ulps.
 constexpr double CGM_FP32_RELTOL = 0x1p-19;
+// ... and of its Jacobi-preconditioned form in a linear solve's V-cycles,
+// in the D^-1 norm: its FP32 recurrences stall above 2^-19 (the scaling
+// spans the Haar levels' orders of magnitude), and the flexible CG around
+// the V-cycle needs no more.  Measured on the 1280 x 160 gradient: 48,178
+// coarse iterations at 2^-19, 22,134 at 2^-14, the same V-cycle count
+// within 7%.
+constexpr double CGM_FP32_JACOBI_RELTOL = 0x1p-14;
 
 // The coarse CG runs in one workgroup (k_cg_solve1, k_cgm_solve1) only when
 // wavi_set_coarse_cg asks for it: on MI355X one workgroup spends about 3 us
@@ -1323,15 +1330,26 @@ static CGMLayout cgm_layout(const wavi_ctx *ctx)
     return Lo;
 }
 
+template <int PAR, bool INIT, bool PRE>
+static void launch_cgm_iter_p(wavi_ctx *ctx, dim3 tb, const CGMIterArgs &a)
+{
+    switch (ctx->S.levels) {
+    case 0: hipLaunchKernelGGL((k_cgm_iter<PAR, INIT, 0, PRE>), tb, dim3(256), 0, ctx->stream, a); break;
+    case 1: hipLaunchKernelGGL((k_cgm_iter<PAR, INIT, 1, PRE>), tb, dim3(256), 0, ctx->stream, a); break;
+    case 2: hipLaunchKernelGGL((k_cgm_iter<PAR, INIT, 2, PRE>), tb, dim3(256), 0, ctx->stream, a); break;
+    default: hipLaunchKernelGGL((k_cgm_iter<PAR, INIT, 3, PRE>), tb, dim3(256), 0, ctx->stream, a); break;
+    }
+}
+
+// k_cgm_iter at the context's Haar level, Jacobi-preconditioned inside a
+// linear solve (ctx->cg_pre).
 template <int PAR, bool INIT>
 static void launch_cgm_iter(wavi_ctx *ctx, dim3 tb, const CGMIterArgs &a)
 {
-    switch (ctx->S.levels) {
-    case 0: hipLaunchKernelGGL((k_cgm_iter<PAR, INIT, 0>), tb, dim3(256), 0, ctx->stream, a); break;
-    case 1: hipLaunchKernelGGL((k_cgm_iter<PAR, INIT, 1>), tb, dim3(256), 0, ctx->stream, a); break;
-    case 2: hipLaunchKernelGGL((k_cgm_iter<PAR, INIT, 2>), tb, dim3(256), 0, ctx->stream, a); break;
-    default: hipLaunchKernelGGL((k_cgm_iter<PAR, INIT, 3>), tb, dim3(256), 0, ctx->stream, a); break;
-    }
+    if (ctx->cg_pre)
+        launch_cgm_iter_p<PAR, INIT, true>(ctx, tb, a);
+    else
+        launch_cgm_iter_p<PAR, INIT, false>(ctx, tb, a);
 }
 
 // The arguments of k_cgm_iter, fixed for the context.
@@ -1354,7 +1372,7 @@ static CGMIterArgs cgm_iter_args(wavi_ctx *ctx)
     // 16 ulps of ||r0||, and iterating past that point lets the recursive
     // residual drift from the true one and the iterate grow (the outer
     // Picard loop then stalls near 1e-8 instead of converging).
-    ia.reltol = std::max(std::sqrt(2.220446049250313e-16), CGM_FP32_RELTOL);
+    ia.reltol = ctx->cg_pre ? CGM_FP32_JACOBI_RELTOL : std::max(std::sqrt(2.220446049250313e-16), CGM_FP32_RELTOL);
     ia.maxiter = ctx->S.maxiter_coarse;
     ia.nblk = ctx->nblk;
     return ia;
@@ -1370,7 +1388,8 @@ static int cgm_prepare(wavi_ctx *ctx, const VSet<float> &w)
     const CGMLayout Lo = cgm_layout(ctx);
     const OpCoef<float> c = coef_of(ctx, w);
     hipLaunchKernelGGL(k_cgm_pack, flat(std::max(ctx->nh, std::max(ctx->nu, ctx->nv))), dim3(256), 0, ctx->stream, g,
-                       c, ctx->mwu, ctx->mwv, Lo);
+                       c, ctx->mwu, ctx->mwv, Lo, ctx->cg_pre ? (const double *)ctx->cdinv.u : nullptr,
+                       ctx->cg_pre ? (const double *)ctx->cdinv.v : nullptr);
     // P x0
     hipLaunchKernelGGL((k_tile_haar<float, float>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1,
                        w.xc.u, w.xc.v, ctx->mwu, ctx->mwv, w.ct.u, w.ct.v, ctx->ui, ctx->vi);
@@ -1394,7 +1413,10 @@ static int cgm_prepare(wavi_ctx *ctx, const VSet<float> &w)
     sa.nblk = ctx->nblk;
     ctx->h_flag->done = 0;
     ctx->h_flag->iters = 0;
-    hipLaunchKernelGGL(k_cgm_setup, tb, dim3(256), 0, ctx->stream, sa);
+    if (ctx->cg_pre)
+        hipLaunchKernelGGL(k_cgm_setup<true>, tb, dim3(256), 0, ctx->stream, sa);
+    else
+        hipLaunchKernelGGL(k_cgm_setup<false>, tb, dim3(256), 0, ctx->stream, sa);
     HIPCHK(hipGetLastError());
     return WAVI_OK;
 }
@@ -1416,11 +1438,16 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     const CGMIterArgs ia = cgm_iter_args(ctx);
     volatile CGMHostFlag *hf = ctx->h_flag;
     if (cg_one_workgroup(ctx)) {
-        switch (ctx->S.levels) {
-        case 0: hipLaunchKernelGGL(k_cgm_solve1<0>, dim3(1), dim3(256), 0, ctx->stream, ia); break;
-        case 1: hipLaunchKernelGGL(k_cgm_solve1<1>, dim3(1), dim3(256), 0, ctx->stream, ia); break;
-        case 2: hipLaunchKernelGGL(k_cgm_solve1<2>, dim3(1), dim3(256), 0, ctx->stream, ia); break;
-        default: hipLaunchKernelGGL(k_cgm_solve1<3>, dim3(1), dim3(256), 0, ctx->stream, ia); break;
+        const int sel = ctx->S.levels + (ctx->cg_pre ? 4 : 0);
+        switch (sel) {
+        case 0: hipLaunchKernelGGL((k_cgm_solve1<0, false>), dim3(1), dim3(256), 0, ctx->stream, ia); break;
+        case 1: hipLaunchKernelGGL((k_cgm_solve1<1, false>), dim3(1), dim3(256), 0, ctx->stream, ia); break;
+        case 2: hipLaunchKernelGGL((k_cgm_solve1<2, false>), dim3(1), dim3(256), 0, ctx->stream, ia); break;
+        case 3: hipLaunchKernelGGL((k_cgm_solve1<3, false>), dim3(1), dim3(256), 0, ctx->stream, ia); break;
+        case 4: hipLaunchKernelGGL((k_cgm_solve1<0, true>), dim3(1), dim3(256), 0, ctx->stream, ia); break;
+        case 5: hipLaunchKernelGGL((k_cgm_solve1<1, true>), dim3(1), dim3(256), 0, ctx->stream, ia); break;
+        case 6: hipLaunchKernelGGL((k_cgm_solve1<2, true>), dim3(1), dim3(256), 0, ctx->stream, ia); break;
+        default: hipLaunchKernelGGL((k_cgm_solve1<3, true>), dim3(1), dim3(256), 0, ctx->stream, ia); break;
         }
         HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(k_cgm_unpack, flat(std::max(ctx->nu, ctx->nv)), dim3(256), 0, ctx->stream, g, Lo, w.xc.u,
@@ -1447,7 +1474,7 @@ int do_coarse_cg<float>(wavi_ctx *ctx, const VSet<float> &w, int *iters)
     while (q <= qmax) {
         const int n = std::min(cg_chunk_len(k, ctx->cg_prev), qmax - q + 1);
         const int bev = n == CG_CHUNK ? burst_begin(ctx) : -1;
-        if (n == CG_CHUNK && use_graph && (q & 1)) {
+        if (n == CG_CHUNK && use_graph && (q & 1) && !ctx->cg_pre) {
             // the graph's launches have parities 1, 0, 1, ...: it replays a chunk that starts at an odd q
             if (!ctx->cgm_chunk) {
                 std::lock_guard<std::mutex> lock(g_graph_mu);
@@ -1753,7 +1780,7 @@ struct CgPreScope {
 };
 
 static int do_linear_solve(wavi_ctx *ctx, const DevVec &b, const DevVec &x, const LinScratch &s, double tol,
-                           int maxit, int *vcycles, double *rel_out, bool *converged)
+                           int maxit, int *vcycles, int64_t *cg_iters, double *rel_out, bool *converged)
 {
     const GridDims &g = ctx->g;
     const int nu = ctx->nu, nv = ctx->nv;
@@ -1762,15 +1789,18 @@ static int do_linear_solve(wavi_ctx *ctx, const DevVec &b, const DevVec &x, cons
     // V-cycle's coarse CG is scaled by the coarse operator's diagonal, which
     // the coarse problem (Haar levels of very different scale) needs far
     // more than the reference's plain CG can give it.
-    const bool jacobi = ctx->lin_jacobi && ctx->precision == WAVI_PRECISION_FP64;
+    const bool jacobi = ctx->lin_jacobi;
     if (jacobi) CHK(do_coarse_diag(ctx, nullptr));
     CgPreScope pre(ctx, jacobi);
+    int64_t cg_total = 0;
     auto vcycle = [&](int *cg) -> int {
         hipLaunchKernelGGL(k_fill<double>, flat(nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_WU_CORRECTION_COARSE],
                            0.0, nu);
         hipLaunchKernelGGL(k_fill<double>, flat(nv), dim3(256), 0, ctx->stream, ctx->f[WAVI_WV_CORRECTION_COARSE],
                            0.0, nv);
-        return do_vcycle(ctx, cg);
+        CHK(do_vcycle(ctx, cg));
+        cg_total += *cg;
+        return WAVI_OK;
     };
     hipLaunchKernelGGL(k_fill<double>, flat(nu), dim3(256), 0, ctx->stream, x.u, 0.0, nu);
     hipLaunchKernelGGL(k_fill<double>, flat(nv), dim3(256), 0, ctx->stream, x.v, 0.0, nv);
@@ -1837,6 +1867,7 @@ static int do_linear_solve(wavi_ctx *ctx, const DevVec &b, const DevVec &x, cons
         if (restart == 3) CHK(true_rel(&rel));
     }
     *vcycles += it;
+    *cg_iters += cg_total;
     *rel_out = rel;
     *converged = rel <= tol;
     return WAVI_OK;
@@ -1938,7 +1969,7 @@ static int do_misfit_gradient(wavi_ctx *ctx, int npic, double tol, int maxit, co
         CHK(do_rhs(ctx));
         double rel = 0.0;
         bool ok = false;
-        CHK(do_linear_solve(ctx, ctx->b, ctx->x, A.ls, tol, maxit, &rep->fwd_vcycles, &rel, &ok));
+        CHK(do_linear_solve(ctx, ctx->b, ctx->x, A.ls, tol, maxit, &rep->fwd_vcycles, &rep->coarse_cg_iters, &rel, &ok));
         if (!ok) rep->unconverged++;
         rep->max_fwd_rel = std::max(rep->max_fwd_rel, rel);
         hipLaunchKernelGGL(k_copy_masked<double>, flat(nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_U_U], ctx->x.u,
@@ -1977,7 +2008,7 @@ static int do_misfit_gradient(wavi_ctx *ctx, int npic, double tol, int maxit, co
         CHK(do_inner_update(ctx));
         double rel = 0.0;
         bool ok = false;
-        CHK(do_linear_solve(ctx, bar, lam, A.ls, tol, maxit, &rep->adj_vcycles, &rel, &ok));
+        CHK(do_linear_solve(ctx, bar, lam, A.ls, tol, maxit, &rep->adj_vcycles, &rep->coarse_cg_iters, &rel, &ok));
         if (!ok) rep->unconverged++;
         rep->max_adj_rel = std::max(rep->max_adj_rel, rel);
         AdjArgs a;
@@ -2098,15 +2129,25 @@ static int do_update_thickness(wavi_ctx *ctx, double dt)
 
 // ---------------------------------------------------------------------------
 // k_cgm_iter_batch at the Haar level of the members.
-template <int PAR, bool INIT>
-static void launch_cgm_batch(hipStream_t s, int levels, int nwg, const CGMBatch &B)
+template <int PAR, bool INIT, bool PRE>
+static void launch_cgm_batch_p(hipStream_t s, int levels, int nwg, const CGMBatch &B)
 {
     switch (levels) {
-    case 0: hipLaunchKernelGGL((k_cgm_iter_batch<PAR, INIT, 0>), dim3(nwg), dim3(256), 0, s, B); break;
-    case 1: hipLaunchKernelGGL((k_cgm_iter_batch<PAR, INIT, 1>), dim3(nwg), dim3(256), 0, s, B); break;
-    case 2: hipLaunchKernelGGL((k_cgm_iter_batch<PAR, INIT, 2>), dim3(nwg), dim3(256), 0, s, B); break;
-    default: hipLaunchKernelGGL((k_cgm_iter_batch<PAR, INIT, 3>), dim3(nwg), dim3(256), 0, s, B); break;
+    case 0: hipLaunchKernelGGL((k_cgm_iter_batch<PAR, INIT, 0, PRE>), dim3(nwg), dim3(256), 0, s, B); break;
+    case 1: hipLaunchKernelGGL((k_cgm_iter_batch<PAR, INIT, 1, PRE>), dim3(nwg), dim3(256), 0, s, B); break;
+    case 2: hipLaunchKernelGGL((k_cgm_iter_batch<PAR, INIT, 2, PRE>), dim3(nwg), dim3(256), 0, s, B); break;
+    default: hipLaunchKernelGGL((k_cgm_iter_batch<PAR, INIT, 3, PRE>), dim3(nwg), dim3(256), 0, s, B); break;
     }
+}
+
+// k_cgm_iter_batch, Jacobi-preconditioned when the members' CGs are (pre).
+template <int PAR, bool INIT>
+static void launch_cgm_batch(hipStream_t s, int levels, int nwg, const CGMBatch &B, bool pre)
+{
+    if (pre)
+        launch_cgm_batch_p<PAR, INIT, true>(s, levels, nwg, B);
+    else
+        launch_cgm_batch_p<PAR, INIT, false>(s, levels, nwg, B);
 }
 
 // Device tables of a lockstep batch and the events that order the members'
@@ -2181,15 +2222,20 @@ static int run_cg_batch(const std::vector<wavi_ctx *> &mem, const std::vector<hi
         for (int q = 0; q < ctx->nblk / 8; ++q) hg.push_back((int)j);
         nwg += ctx->nblk;
         qmax = std::max(qmax, ctx->S.maxiter_coarse + 1);
+        if (ctx->cg_pre != lead->cg_pre) {
+            ctx->err = "members of one coarse-CG batch disagree on the Jacobi-preconditioned coarse CG";
+            return WAVI_ESTATE;
+        }
     }
     *bad = 0;
     wavi_ctx *ctx = lead;
+    const bool jac = lead->cg_pre;
     HIPCHK(hipMemcpyAsync(R.members, hm.data(), hm.size() * sizeof(CGMIterArgs), hipMemcpyHostToDevice, cs));
     HIPCHK(hipMemcpyAsync(R.group, hg.data(), hg.size() * sizeof(int), hipMemcpyHostToDevice, cs));
     HIPCHK(hipMemcpyAsync(R.first, hf.data(), hf.size() * sizeof(int), hipMemcpyHostToDevice, cs));
     const CGMBatch B{R.members, R.group, R.first};
     const int levels = lead->S.levels;
-    launch_cgm_batch<0, true>(cs, levels, nwg, B);
+    launch_cgm_batch<0, true>(cs, levels, nwg, B, jac);
     HIPCHK(hipGetLastError());
     // launch q performs iteration q - 1 of every member whose CG has not stopped
     int q = 1, c = 0;
@@ -2197,9 +2243,9 @@ static int run_cg_batch(const std::vector<wavi_ctx *> &mem, const std::vector<hi
         const int n = std::min(cg_chunk_len(c, 0), qmax - q + 1);
         for (int s = 0; s < n; ++s, ++q) {
             if (q & 1)
-                launch_cgm_batch<1, false>(cs, levels, nwg, B);
+                launch_cgm_batch<1, false>(cs, levels, nwg, B, jac);
             else
-                launch_cgm_batch<0, false>(cs, levels, nwg, B);
+                launch_cgm_batch<0, false>(cs, levels, nwg, B, jac);
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(lead->cg_ev[c % 3], cs));

@@ -2237,7 +2237,8 @@ enum CGMArray {
     CGM_BU = 12,     // beta_eff on u / v points, 0 off the masks
     CGM_CELL = 14,   // per h cell: (h*eta_av on the h mask, dimp) pairs (two arrays' room)
     CGM_CORNER = 16, // per h cell: corner h*eta_av (0 off the c grid and mask)
-    CGM_NARRAY = 17
+    CGM_DINV = 17,   // Jacobi-preconditioned CG: 1 / |diag(A_c)| on u / v points, 0 off the masks
+    CGM_NARRAY = 19
 };
 
 // Ring images, compact per tile: for each parity, tile (ntile + 1 tiles, the
@@ -2285,7 +2286,8 @@ struct CGMLayout {
 };
 
 // Coefficients and masks into the arena (once per V-cycle); 0 off the masks.
-__global__ void k_cgm_pack(GridDims g, OpCoef<float> c, const uint8_t *mwu, const uint8_t *mwv, CGMLayout Lo)
+__global__ void k_cgm_pack(GridDims g, OpCoef<float> c, const uint8_t *mwu, const uint8_t *mwv, CGMLayout Lo,
+                           const double *dinvu, const double *dinvv)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int nh = g.nx * g.ny, nu = g.nxu * g.nyu, nv = g.nxv * g.nyv;
@@ -2307,12 +2309,14 @@ __global__ void k_cgm_pack(GridDims g, OpCoef<float> c, const uint8_t *mwu, cons
         F[CGM_HU * ns + o] = c.um[k] ? c.hu[k] : 0.f;
         F[CGM_BU * ns + o] = c.um[k] ? c.bu[k] : 0.f;
         Lo.M[o] = (uint8_t)((c.ui[k] ? 1 : 0) | (mwu[k] ? 2 : 0));
+        if (dinvu) F[CGM_DINV * ns + o] = (float)dinvu[k];
     }
     if (k < nv) {
         const unsigned o = Lo.at(k % g.nxv, k / g.nxv);
         F[(CGM_HU + 1) * ns + o] = c.vm[k] ? c.hv[k] : 0.f;
         F[(CGM_BU + 1) * ns + o] = c.vm[k] ? c.bv[k] : 0.f;
         Lo.M[ns + o] = (uint8_t)((c.vi[k] ? 1 : 0) | (mwv[k] ? 2 : 0));
+        if (dinvv) F[(CGM_DINV + 1) * ns + o] = (float)dinvv[k];
     }
 }
 
@@ -2343,6 +2347,7 @@ struct CGMArgs {
 // setup (P x0 computed beforehand): r0 = b - R A P x0, p = s = 0, x = x0 in
 // the arena, P r0 into the P r rings of set 0, partials of ||b||^2; resets
 // the scalars.
+template <bool PRE>
 __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
 {
     __shared__ OpTile<float> s;
@@ -2380,13 +2385,15 @@ __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
     haar_pm_tile2<float>(a.levels, 1, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
     float ru = 0.f, rv = 0.f;
     double b2u = 0.0, b2v = 0.0;
+    // Jacobi: the rings hold P D^-1 r and ||b||^2 is the D^-1 norm
+    const float dzu = PRE && pc.oku ? F[CGM_DINV * ns + o] : 1.f, dzv = PRE && pc.okv ? F[(CGM_DINV + 1) * ns + o] : 1.f;
     if (pc.oku) {
         ru = bu - (pc.mwu ? tu[ly][lx] : 0.f);
         F[CGM_R * ns + o] = ru;
         F[CGM_P * ns + o] = 0.f;
         F[CGM_S * ns + o] = 0.f;
         F[CGM_X * ns + o] = xu;
-        b2u = (double)bu * (double)bu;
+        b2u = PRE ? (double)bu * ((double)dzu * (double)bu) : (double)bu * (double)bu;
     }
     if (pc.okv) {
         rv = bv - (pc.mwv ? tv[ly][lx] : 0.f);
@@ -2394,11 +2401,11 @@ __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
         F[(CGM_P + 1) * ns + o] = 0.f;
         F[(CGM_S + 1) * ns + o] = 0.f;
         F[(CGM_X + 1) * ns + o] = xv;
-        b2v = (double)bv * (double)bv;
+        b2v = PRE ? (double)bv * ((double)dzv * (double)bv) : (double)bv * (double)bv;
     }
     // each thread rewrites only its own entry; the transform ended with a barrier
-    tu[ly][lx] = pc.mwu ? ru : 0.f;
-    tv[ly][lx] = pc.mwv ? rv : 0.f;
+    tu[ly][lx] = pc.mwu ? (PRE ? dzu * ru : ru) : 0.f;
+    tv[ly][lx] = pc.mwv ? (PRE ? dzv * rv : rv) : 0.f;
     __syncthreads();
     haar_pm_tile2<float>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
     if (pc.oku) ring_put(a.R, ring_at(0, g.nbx * g.nby, tile_id, 0, 0), lx, ly, pc.ui ? tu[ly][lx] : 0.f);
@@ -2566,7 +2573,7 @@ struct CGMIterArgs {
 // barriers: partial sums; block transforms of r and s (x pass, transpose,
 // y pass straight into the window); intermediates; operator and R x pass;
 // R y pass and P y pass; P x pass and the new partials.
-template <int PAR, bool INIT, int LEV>
+template <int PAR, bool INIT, int LEV, bool PRE>
 __device__ __forceinline__ bool cgm_iter_tile(const CGMIterArgs &a, const int L)
 {
     static_assert(!INIT || PAR == 0, "the initial launch reads set 0");
@@ -2576,6 +2583,7 @@ __device__ __forceinline__ bool cgm_iter_tile(const CGMIterArgs &a, const int L)
         float rxx[17][18], ryy[17][18], ext[17][18], rxy[17][18];
         float T[4][TILE][TILE + 1];      // transposes of the block transforms
         uint8_t mb[2][TILE][TILE + 1];   // mask bits of the tile
+        float dz[PRE ? 2 : 1][TILE][TILE + 1];  // Jacobi: 1 / |diag(A_c)| of the tile
         double red0[8];
         float red1[8];
         double red2[8];
@@ -2671,6 +2679,11 @@ __device__ __forceinline__ bool cgm_iter_tile(const CGMIterArgs &a, const int L)
     const float2 cc1 = *reinterpret_cast<const float2 *>(cell2 + 8u * c1o);
     const float ch0 = el(arr(CGM_CORNER), 4u * c0o), ch1 = el(arr(CGM_CORNER), 4u * c1o);
     const float bu0 = el(arr(CGM_BU), ob), bv0 = el(arr(CGM_BU + 1), ob);
+    float dz_u = 1.f, dz_v = 1.f;
+    if constexpr (PRE) {
+        dz_u = el(arr(CGM_DINV), ob);
+        dz_v = el(arr(CGM_DINV + 1), ob);
+    }
     const bool inu = mbu & 1, inv = mbv & 1, mwu = mbu & 2, mwv = mbv & 2;
     // ---- scalars from the previous launch's partials (FP64 across workgroups)
     float fa = 0.f, fb = 0.f, fxp = 0.f;
@@ -2733,13 +2746,15 @@ __device__ __forceinline__ bool cgm_iter_tile(const CGMIterArgs &a, const int L)
         fb = (float)beta;
         fxp = XUPD ? (float)(S.alpha + alpha * beta) : 0.f;
     }
-    // ---- vector updates on the tile (zero past the ends of the grids)
+    // ---- vector updates on the tile (zero past the ends of the grids); with
+    // Jacobi the direction and x take u = D^-1 r where they take r
     float s_u = 0.f, s_v = 0.f;
     if (!INIT) {
-        const float p_u = r_u + fb * p_u0, p_v = r_v + fb * p_v0;
+        const float z_u = PRE ? dz_u * r_u : r_u, z_v = PRE ? dz_v * r_v : r_v;
+        const float p_u = z_u + fb * p_u0, p_v = z_v + fb * p_v0;
         if (XUPD) {
-            st_nt(&el(arr(CGM_X), ob), (x_u0 + fxp * p_u0) + fa * r_u);
-            st_nt(&el(arr(CGM_X + 1), ob), (x_v0 + fxp * p_v0) + fa * r_v);
+            st_nt(&el(arr(CGM_X), ob), (x_u0 + fxp * p_u0) + fa * z_u);
+            st_nt(&el(arr(CGM_X + 1), ob), (x_v0 + fxp * p_v0) + fa * z_v);
         }
         s_u = w_u0 + fb * s_u0;
         s_v = w_v0 + fb * s_v0;
@@ -2752,14 +2767,15 @@ __device__ __forceinline__ bool cgm_iter_tile(const CGMIterArgs &a, const int L)
         st_nt(&el(arr(CGM_R), ob), r_u);
         st_nt(&el(arr(CGM_R + 1), ob), r_v);
     }
-    // ---- x pass of P r and P s; halo and thickness windows into LDS
+    // ---- x pass of P r and P s (Jacobi: of D^-1 r and D^-1 s); halo and
+    // thickness windows into LDS
     {
-        float a0 = mwu ? r_u : 0.f, a1 = mwv ? r_v : 0.f;
+        float a0 = mwu ? (PRE ? dz_u * r_u : r_u) : 0.f, a1 = mwv ? (PRE ? dz_v * r_v : r_v) : 0.f;
         haar_lines_P<LEV>(a0, a1, lx);
         sm.T[0][ly][lx] = a0;
         sm.T[1][ly][lx] = a1;
         if (!INIT) {
-            float a2 = mwu ? s_u : 0.f, a3 = mwv ? s_v : 0.f;
+            float a2 = mwu ? (PRE ? dz_u * s_u : s_u) : 0.f, a3 = mwv ? (PRE ? dz_v * s_v : s_v) : 0.f;
             haar_lines_P<LEV>(a2, a3, lx);
             sm.T[2][ly][lx] = a2;
             sm.T[3][ly][lx] = a3;
@@ -2767,6 +2783,10 @@ __device__ __forceinline__ bool cgm_iter_tile(const CGMIterArgs &a, const int L)
     }
     sm.mb[0][ly][lx] = mbu;
     sm.mb[1][ly][lx] = mbv;
+    if constexpr (PRE) {
+        sm.dz[0][ly][lx] = dz_u;
+        sm.dz[1][ly][lx] = dz_v;
+    }
     if (tid < 136) {
         const float t = INIT ? hr : hr - fa * (hw + fb * hs);
         if (!hcomp)
@@ -2848,6 +2868,10 @@ __device__ __forceinline__ bool cgm_iter_tile(const CGMIterArgs &a, const int L)
         const float w0 = (sm.mb[0][cy][cx] & 2) ? c0 : 0.f;
         const float w1 = (sm.mb[1][cy][cx] & 2) ? c1 : 0.f;
         float q0 = w0, q1v = w1;
+        if constexpr (PRE) {
+            q0 = sm.dz[0][cy][cx] * w0;
+            q1v = sm.dz[1][cy][cx] * w1;
+        }
         haar_lines_P<LEV>(q0, q1v, cy);
         sm.T[0][cy][cx] = w0;
         sm.T[1][cy][cx] = w1;
@@ -2867,8 +2891,10 @@ __device__ __forceinline__ bool cgm_iter_tile(const CGMIterArgs &a, const int L)
         rs.put(a.R, rb, inu ? tw_u : 0.f);
         rs.put(a.R, rb + 64, inv ? tw_v : 0.f);
     }
-    float gg = r_u * r_u + r_v * r_v;
-    float dd = wn_u * r_u + wn_v * r_v;
+    // (r, u) and (w, u) with u = D^-1 r under Jacobi, (r, r) and (w, r) without
+    const float zr_u = PRE ? dz_u * r_u : r_u, zr_v = PRE ? dz_v * r_v : r_v;
+    float gg = r_u * zr_u + r_v * zr_v;
+    float dd = wn_u * zr_u + wn_v * zr_v;
     gg = wave_total_rows_f(row_sum16f(gg));
     dd = wave_total_rows_f(row_sum16f(dd));
     if ((tid & 63) == 63) {
@@ -2884,10 +2910,10 @@ __device__ __forceinline__ bool cgm_iter_tile(const CGMIterArgs &a, const int L)
 
 // One coarse-CG iteration of one CG problem (the tile of workgroup
 // blockIdx.x, XCD-aware).
-template <int PAR, bool INIT, int LEV>
+template <int PAR, bool INIT, int LEV, bool PRE>
 __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
 {
-    (void)cgm_iter_tile<PAR, INIT, LEV>(a, xcd_tile(a.nblk));
+    (void)cgm_iter_tile<PAR, INIT, LEV, PRE>(a, xcd_tile(a.nblk));
 }
 
 // The whole mixed coarse CG of a small grid in one workgroup (after
@@ -2897,18 +2923,18 @@ __global__ void __launch_bounds__(256) k_cgm_iter(CGMIterArgs a)
 // previous iteration only, so the results are those of one launch per
 // iteration, bit for bit.  Every thread decides the stop alike (same
 // partials, same scalars); at most maxiter + 1 iterations.
-template <int LEV>
+template <int LEV, bool PRE>
 __global__ void __launch_bounds__(256) k_cgm_solve1(CGMIterArgs a)
 {
     const int ntile = a.g.nbx * a.g.nby;
     for (int L = 0; L < ntile; ++L) {
-        (void)cgm_iter_tile<0, true, LEV>(a, L);
+        (void)cgm_iter_tile<0, true, LEV, PRE>(a, L);
         __syncthreads();
     }
     for (int q = 1; q <= a.maxiter + 1; ++q) {
         bool stop = false;
         for (int L = 0; L < ntile; ++L) {
-            stop = (q & 1) ? cgm_iter_tile<1, false, LEV>(a, L) : cgm_iter_tile<0, false, LEV>(a, L);
+            stop = (q & 1) ? cgm_iter_tile<1, false, LEV, PRE>(a, L) : cgm_iter_tile<0, false, LEV, PRE>(a, L);
             __syncthreads();
         }
         if (stop) break;
@@ -2927,13 +2953,13 @@ struct CGMBatch {
     const int *first;
 };
 
-template <int PAR, bool INIT, int LEV>
+template <int PAR, bool INIT, int LEV, bool PRE>
 __global__ void __launch_bounds__(256) k_cgm_iter_batch(CGMBatch B)
 {
     const int m = B.group_member[blockIdx.x >> 3];
     const CGMIterArgs a = B.members[m];
     const int w = (int)blockIdx.x - B.first[m];
-    (void)cgm_iter_tile<PAR, INIT, LEV>(a, (w & 7) * (a.nblk >> 3) + (w >> 3));
+    (void)cgm_iter_tile<PAR, INIT, LEV, PRE>(a, (w & 7) * (a.nblk >> 3) + (w >> 3));
 }
 
 // out = mask_out ? H(mask_in ? in : 0) : 0 on the u grid (comp 0) and v grid

@@ -89,6 +89,7 @@ struct AdjointReport
     unconverged::Int32
     max_fwd_rel::Float64
     max_adj_rel::Float64
+    coarse_cg_iters::Int64
 end
 
 const PRECISION = Dict(:fp64 => Int32(0), :mixed => Int32(1))
