@@ -13,8 +13,9 @@ timed solves run without instrumentation; the dominant kernel's launch time
 comes from one more solve of each precision with HIP events on, after the
 timed loop.  The default line also carries the basal-drag gradient (adjoint
 ms, FP64 and mixed, checked by central differences), the 80 x 10 channel
-transient (200 steps) on the GPU beside the CPU oracle, and the CPU
-baseline.  When the mixed velocities are more than 1e-6 from FP64 (relative
+transient (200 steps) on the GPU beside the CPU oracle, the 64-member
+ensemble (forward solves and forward + adjoint gradients, members/s), and
+the CPU baseline.  When the mixed velocities are more than 1e-6 from FP64 (relative
 L2 or max |du| / max |u|) the line says "valid": false and the exit status
 is 1.  Prints one JSON line.  --precision fp64 or mixed times one of them
 alone (for profiling runs).
@@ -208,7 +209,8 @@ def timed_solves(W, nx, ny, precision, steps, warmup, picard=None, maxiter_coars
     return times, rep, vel, roof
 
 
-def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False, n_picard=10, max_nx=None):
+def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False, n_picard=10, max_nx=None,
+                 compact=False):
     """The seeded ensemble (channel solves of mixed sizes, nσ, Glen A and
     basal-drag fields) from the reset state, one batched call per step:
     forward solves (wavi_update_velocities_batch, 20 Picard iterations: the
@@ -218,7 +220,7 @@ def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False
     n_picard Picard iterations with converged solves; each member's
     observations are its own forward velocities times (1 + 0.01 N(0, 1))).
     Also timed member after member with one thread, for the concurrency
-    gain.  Returns the JSON line."""
+    gain, unless compact.  Returns the JSON line."""
     if max_nx:
         # the same seeded draw, keeping the first n members at most max_nx wide
         specs = [sp for sp in W.workloads.ensemble_specs(16 * n) if sp["nx"] <= max_nx][:n]
@@ -298,8 +300,9 @@ def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False
             assert rc == 0, lib.wavi_last_error(ctx)
         return time.perf_counter() - t0
 
-    serial = one_at_a_time()
-    log("[bench] ensemble one member at a time: %.3f s" % serial)
+    serial = None if compact else one_at_a_time()
+    if serial is not None:
+        log("[bench] ensemble one member at a time: %.3f s" % serial)
     total = float(sum(times))
 
     def velocities():
@@ -311,7 +314,7 @@ def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False
         return out
 
     lock = None
-    if not adjoint and precision == "mixed":
+    if not adjoint and precision == "mixed" and not compact:
         # the same solves with every member in one lockstep group (the
         # batch itself puts only the small members in one)
         ref = velocities()
@@ -330,8 +333,10 @@ def ensemble_run(W, n, nthreads, steps, warmup, precision="mixed", adjoint=False
                        % (n, "forward + adjoint gradients (%d Picard iterations, solves to 1e-13)" % n_picard
                           if adjoint else "solves (20 Picard iterations)",
                           ["%dx%d" % s for s in W.workloads.ENSEMBLE_SIZES if not max_nx or s[0] <= max_nx]),
-           "precision": precision, "host_threads": nthreads, "unknowns_total": unknowns,
-           "serial_members_per_s": n / serial, "batched_over_serial": serial / (total / steps)}
+           "precision": precision, "host_threads": nthreads, "unknowns_total": unknowns}
+    if serial is not None:
+        cfg["serial_members_per_s"] = n / serial
+        cfg["batched_over_serial"] = serial / (total / steps)
     if adjoint:
         cfg["unconverged_solves"] = int(sum(reps[k].unconverged for k in range(n)))
         cfg["max_rel_residual"] = float(max(max(reps[k].max_fwd_rel, reps[k].max_adj_rel) for k in range(n)))
@@ -663,6 +668,19 @@ def main():
         }
         log("[bench] transient 80x10, 200 steps")
         cfg["transient_80x10"] = transient_run(W)
+        log("[bench] ensemble of 64: forward solves, then forward + adjoint gradients")
+        ef = ensemble_run(W, 64, 8, 2, 1, compact=True)
+        ea = ensemble_run(W, 64, 8, 2, 1, adjoint=True, compact=True)
+        cfg["ensemble_64"] = {
+            "workload": ef["config"]["workload"] + "; then the same members' forward + adjoint gradients "
+                        "(10 Picard iterations, solves to 1e-13); mixed precision, one batched call per step, "
+                        "2 steps after 1 warm-up",
+            "forward_members_per_s": ef["value"], "forward_s_per_batch": ef["ms_per_step"] / 1e3,
+            "picard_iters_min_max": ef["config"]["picard_iters_min_max"],
+            "gradient_members_per_s": ea["value"], "gradient_s_per_batch": ea["ms_per_step"] / 1e3,
+            "gradient_unconverged_solves": ea["config"]["unconverged_solves"],
+            "unknowns_total": ef["config"]["unknowns_total"],
+        }
     out = {
         "metric": metric(nx, ny),
         "value": value,
