@@ -165,6 +165,33 @@ def test_jacobi_coarse_cg_gives_the_same_gradient_with_fewer_coarse_iterations(W
         assert 2 * c1 < c0
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_jacobi_coarse_cg_other_haar_level_and_one_workgroup(W, precision):
+    """The Jacobi-scaled coarse CG at Haar level 1 (blocks of 4): the same
+    gradient as the plain V-cycle to the solve tolerance; and the coarse CG
+    run whole in one workgroup (wavi_set_coarse_cg) gives the launched
+    Jacobi CG's gradient bit for bit."""
+    nx, ny = 48, 12
+    _step("Jacobi coarse CG, levels 1 and one workgroup, %s %dx%d" % (precision, nx, ny))
+    l0, uo, vo = _setup(W, nx, ny)
+    out = {}
+    for key, jac, mode in (("plain", 0, W._lib.WAVI_CG_LAUNCHES), ("jacobi", 1, W._lib.WAVI_CG_LAUNCHES),
+                           ("jacobi_wg", 1, W._lib.WAVI_CG_WORKGROUP)):
+        m = W.workloads.inversion_model(l0, precision, nσ=4, n_picard=N_PICARD)
+        m.solver_params.levels = 1
+        W.update_geometry(m)
+        spec = m.parallel_spec
+        assert spec._lib.wavi_set_linear_coarse_jacobi(spec.ctx, jac) == 0
+        assert spec._lib.wavi_set_coarse_cg(spec.ctx, mode) == 0
+        out[key] = W.misfit_gradient(m, N_PICARD, uo, vo, TOL_LIN)
+    (Jp, gp, rp), (Jj, gj, rj), (Jw, gw, rw) = out["plain"], out["jacobi"], out["jacobi_wg"]
+    assert rp["unconverged"] == 0 and rj["unconverged"] == 0
+    assert abs(Jj - Jp) <= 1e-11 * abs(Jp)
+    assert np.linalg.norm(gj - gp) <= 1e-9 * np.linalg.norm(gp)
+    assert rj["coarse_cg_iters"] < rp["coarse_cg_iters"]
+    assert Jw == Jj and np.array_equal(gw, gj) and rw == rj
+
+
 def test_fixed_velocities_rejected(W):
     _step("adjoint rejects fixed velocities")
     nx, ny = 32, 8
