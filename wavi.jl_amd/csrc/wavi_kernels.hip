@@ -2309,14 +2309,14 @@ __global__ void k_cgm_pack(GridDims g, OpCoef<float> c, const uint8_t *mwu, cons
         F[CGM_HU * ns + o] = c.um[k] ? c.hu[k] : 0.f;
         F[CGM_BU * ns + o] = c.um[k] ? c.bu[k] : 0.f;
         Lo.M[o] = (uint8_t)((c.ui[k] ? 1 : 0) | (mwu[k] ? 2 : 0));
-        if (dinvu) F[CGM_DINV * ns + o] = (float)dinvu[k];
+        if (dinvu) F[CGM_DINV * ns + o] = (float)fmin(dinvu[k], 1e30);  // finite in FP32
     }
     if (k < nv) {
         const unsigned o = Lo.at(k % g.nxv, k / g.nxv);
         F[(CGM_HU + 1) * ns + o] = c.vm[k] ? c.hv[k] : 0.f;
         F[(CGM_BU + 1) * ns + o] = c.vm[k] ? c.bv[k] : 0.f;
         Lo.M[ns + o] = (uint8_t)((c.vi[k] ? 1 : 0) | (mwv[k] ? 2 : 0));
-        if (dinvv) F[(CGM_DINV + 1) * ns + o] = (float)dinvv[k];
+        if (dinvv) F[(CGM_DINV + 1) * ns + o] = (float)fmin(dinvv[k], 1e30);  // finite in FP32
     }
 }
 
