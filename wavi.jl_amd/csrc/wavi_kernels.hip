@@ -1735,6 +1735,10 @@ __global__ void k_scale_uv(double *u, int nu, double *v, int nv, double a)
 // a kernel.  Scalars a kernel publishes are double-buffered by iteration
 // parity and every workgroup writes the same value; a kernel never reads a
 // scalar that another workgroup of the same kernel writes.
+// PRE = true (inside the flexible CG's linear solves only) runs
+// preconditioned CG with M = D^-1, D = |diag A_c| (k_coarse_diag): the
+// direction takes D^-1 r, alpha and beta take (r, D^-1 r), and the stop test
+// stays on ||r||.  PRE = false is cg! as the reference runs it.
 
 // Stop report of a coarse CG in host-mapped pinned memory: written once, by
 // the launch that stops, so the host learns of the stop without a copy on
@@ -1804,7 +1808,8 @@ __device__ __forceinline__ void tile_apply_restrict(const CGArgs<T> &a, int i0, 
 }
 
 // setup (t = P x0 computed beforehand): r = b - R A t (b arrives in r),
-// u = 0, partials of ||r0||^2 and ||b||^2; resets the scalars.
+// u = 0, partials of ||r0||^2 and ||b||^2 (PRE: also of (r0, D^-1 r0));
+// resets the scalars.
 template <typename T, bool PRE>
 __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
 {
@@ -1870,7 +1875,8 @@ __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
 }
 
 // part 1 (iteration parity par): residual norm and stop test, then
-// u = r + beta u with beta = res^2 / prev^2, t = P u.  The stop decision is
+// u = r + beta u with beta = res^2 / prev^2 (PRE: u = D^-1 r + beta u with
+// beta = rz / prev rz), t = P u.  The stop decision is
 // recomputed from the partials each time, so it needs no flag: after a stop
 // the partials are frozen and it[1 - par] repeats the count.
 template <typename T, bool PRE>
@@ -2001,8 +2007,9 @@ __device__ __forceinline__ void cg_apply_tile(const CGArgs<T> &a, const int L)
     }
 }
 
-// part 3: alpha = res^2 / dot(u, c), x += alpha u, r -= alpha c, partials of
-// ||r||^2 and the iteration count
+// part 3: alpha = res^2 / dot(u, c) (PRE: rz / dot(u, c)), x += alpha u,
+// r -= alpha c, partials of ||r||^2 (PRE: and of (r, D^-1 r)) and the
+// iteration count
 template <typename T, bool PRE>
 __device__ __forceinline__ void cg_update_tile(const CGArgs<T> &a, int par, const int L)
 {
@@ -2573,6 +2580,11 @@ struct CGMIterArgs {
 // barriers: partial sums; block transforms of r and s (x pass, transpose,
 // y pass straight into the window); intermediates; operator and R x pass;
 // R y pass and P y pass; P x pass and the new partials.
+// PRE (inside the flexible CG's linear solves only): the Jacobi-
+// preconditioned form with u = D^-1 r, w = A_c u, gamma = (r, u),
+// delta = (w, u); D^-1 is pointwise, so u, D^-1 s and D^-1 w are formed
+// where they are used and the rings hold P D^-1 r, P D^-1 s, P D^-1 w (the
+// halo of P u_new is then the same linear step).
 template <int PAR, bool INIT, int LEV, bool PRE>
 __device__ __forceinline__ bool cgm_iter_tile(const CGMIterArgs &a, const int L)
 {
