@@ -117,6 +117,9 @@ struct wavi_ctx {
     double *Pz = nullptr;           // partials of (r, D^-1 r) of the Jacobi-preconditioned coarse CG
     DevVec cdinv;                   // 1 / |diag(A_c)| for the Jacobi-preconditioned coarse CG
     bool cg_pre = false;            // the V-cycles of do_linear_solve run the Jacobi-preconditioned coarse CG
+    DevVec ws_keep;                 // the coarse warm start before a speculative V-cycle (do_precondition)
+    hipEvent_t ev_norms = nullptr;  // the Picard residual norms reached the host
+    double last_rel = INFINITY;     // the context's last Picard relative residual
     bool lin_jacobi = true;         // ... when set (wavi_set_linear_coarse_jacobi)
     double *quad = nullptr;  // 4 * nh pos_fraction quadrants
     double *norms = nullptr; // device: 2 doubles
@@ -637,6 +640,9 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
     CREATECHK(zalloc(&ctx->Pz, g.npart));
     CREATECHK(zalloc(&ctx->cdinv.u, ctx->nu));
     CREATECHK(zalloc(&ctx->cdinv.v, ctx->nv));
+    CREATECHK(zalloc(&ctx->ws_keep.u, ctx->nu));
+    CREATECHK(zalloc(&ctx->ws_keep.v, ctx->nv));
+    CREATECHK(hipEventCreateWithFlags(&ctx->ev_norms, hipEventDisableTiming));
     CREATECHK(zalloc(&ctx->quad, 4 * (size_t)ctx->nh));
     CREATECHK(zalloc(&ctx->norms, 2));
     CREATECHK(hipHostMalloc((void **)&ctx->h_norms, 2 * sizeof(double), hipHostMallocDefault));
@@ -709,7 +715,8 @@ static void ctx_free(wavi_ctx *ctx)
     for (double *p : ctx->saved)
         if (p) (void)hipFree(p);
     for (double *p : {ctx->zeta, ctx->qw, ctx->heta_h, ctx->heta_c, ctx->dimp, ctx->Pr0, ctx->Pr1, ctx->Pd, ctx->Pb,
-                      ctx->Pn, ctx->Pz, ctx->cdinv.u, ctx->cdinv.v, ctx->quad, ctx->norms})
+                      ctx->Pn, ctx->Pz, ctx->cdinv.u, ctx->cdinv.v, ctx->ws_keep.u, ctx->ws_keep.v, ctx->quad,
+                      ctx->norms})
         if (p) (void)hipFree(p);
     for (DevVec *dv : {&ctx->x, &ctx->b, &ctx->r, &ctx->e, &ctx->d, &ctx->rho, &ctx->t, &ctx->f3, &ctx->fixedv,
                        &ctx->cr, &ctx->cp, &ctx->cq, &ctx->ct}) {
@@ -731,7 +738,7 @@ static void ctx_free(wavi_ctx *ctx)
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     for (hipEvent_t e : ctx->cg_ev)
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {ctx->ev_ls, ctx->ev_ls_done, ctx->ev_tune[0], ctx->ev_tune[1]})
+    for (hipEvent_t e : {ctx->ev_ls, ctx->ev_ls_done, ctx->ev_tune[0], ctx->ev_tune[1], ctx->ev_norms})
         if (e) (void)hipEventDestroy(e);
     if (ctx->st) (void)hipFree(ctx->st);
     if (ctx->tab) (void)hipFree(ctx->tab);
@@ -1689,19 +1696,61 @@ static int precondition_update(wavi_ctx *ctx)
     return WAVI_OK;
 }
 
+// Grids of at most this many 16 x 16 tiles queue the V-cycle before the
+// host has read the residual norms (do_precondition).
+constexpr int SPECULATE_MAX_TILES = 64;
+
+// Whether do_precondition queues the V-cycle before the convergence test:
+// on small grids, where a V-cycle costs a few hundred microseconds and the
+// host's wait for the norms leaves the GPU idle between them, unless the
+// context's last residual says the test may pass.
+static bool speculate(const wavi_ctx *ctx)
+{
+    return !ctx->prof && ctx->g.nbx * ctx->g.nby <= SPECULATE_MAX_TILES &&
+           !(ctx->last_rel < 10.0 * ctx->S.tol_picard);
+}
+
 static int do_precondition(wavi_ctx *ctx, int *converged, double *rel, int *cg_iters)
 {
     CHK(precondition_resid(ctx));
-    CHK(precondition_decide(ctx, converged, rel));
     *cg_iters = 0;
-    if (!*converged) {
-        CHK(do_vcycle(ctx, cg_iters));
-        CHK(precondition_update(ctx));
+    if (speculate(ctx)) {
+        // V-cycle queued at once; if the test passes after all, nothing of
+        // it is kept: the velocities are left, and the coarse warm start (the
+        // one state the V-cycle writes) goes back to its value from before
+        HIPCHK(hipEventRecord(ctx->ev_norms, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->ws_keep.u, ctx->f[WAVI_WU_CORRECTION_COARSE], ctx->nu * sizeof(double),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->ws_keep.v, ctx->f[WAVI_WV_CORRECTION_COARSE], ctx->nv * sizeof(double),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+        int cg = 0;
+        CHK(do_vcycle(ctx, &cg));
+        HIPCHK(hipEventSynchronize(ctx->ev_norms));
+        *rel = ctx->h_norms[0] / ctx->h_norms[1];
+        *converged = *rel < ctx->S.tol_picard;
+        if (*converged) {
+            HIPCHK(hipMemcpyAsync(ctx->f[WAVI_WU_CORRECTION_COARSE], ctx->ws_keep.u, ctx->nu * sizeof(double),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(ctx->f[WAVI_WV_CORRECTION_COARSE], ctx->ws_keep.v, ctx->nv * sizeof(double),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+        } else {
+            *cg_iters = cg;
+            CHK(precondition_update(ctx));
+        }
+    } else {
+        CHK(precondition_decide(ctx, converged, rel));
+        if (!*converged) {
+            CHK(do_vcycle(ctx, cg_iters));
+            CHK(precondition_update(ctx));
+        }
     }
+    ctx->last_rel = *rel;
     return WAVI_OK;
 }
 
-static int do_update_velocities(wavi_ctx *ctx, wavi_solve_report *rep)
+// update_velocities!; wait = false leaves the work queued on the stream
+// (wavi_run_steps, whose later steps follow on the same stream).
+static int do_update_velocities(wavi_ctx *ctx, wavi_solve_report *rep, bool wait = true)
 {
     wavi_solve_report local;
     if (!rep) rep = &local;
@@ -1719,7 +1768,7 @@ static int do_update_velocities(wavi_ctx *ctx, wavi_solve_report *rep)
             rep->rel_resid_hist[it - 1] = rel;
         }
     }
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (wait) HIPCHK(hipStreamSynchronize(ctx->stream));
     rep->picard_iters = it;
     rep->converged = converged;
     rep->rel_resid = rel;
@@ -2112,10 +2161,10 @@ static int do_after_solve(wavi_ctx *ctx)
     return WAVI_OK;
 }
 
-static int do_update_state(wavi_ctx *ctx, wavi_solve_report *rep)
+static int do_update_state(wavi_ctx *ctx, wavi_solve_report *rep, bool wait = true)
 {
     CHK(do_update_geometry(ctx));
-    CHK(do_update_velocities(ctx, rep));
+    CHK(do_update_velocities(ctx, rep, wait));
     return do_after_solve(ctx);
 }
 
@@ -2959,7 +3008,7 @@ int wavi_run_steps(wavi_ctx *ctx, int nsteps, wavi_solve_report *last_report)
         return WAVI_EARG;
     }
     for (int n = 0; n < nsteps; ++n) {
-        CHK(do_update_state(ctx, last_report));
+        CHK(do_update_state(ctx, last_report, false));
         CHK(do_update_thickness(ctx, ctx->P.dt));
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
