@@ -300,20 +300,18 @@ def test_update_velocities_other_haar_levels(W, oracle, levels):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_state_that_meets_the_test_keeps_velocities_and_warm_start(W, oracle, precision):
     """update_velocities! on a state that already meets tol_picard returns at
-    its first test, with no V-cycle (update_velocities.jl:20-24).  On grids
-    of a few tiles the library queues that V-cycle before it reads the
-    residual norms and drops it when the test passes: the velocities and the
-    coarse warm start (the one field the V-cycle writes) must come out as
-    before.  The fields the first test recomputes from the velocities (bed
-    speed, drag, viscosity) follow the reference; in FP64 every field equals
-    the oracle's."""
+    its first test, with no V-cycle (update_velocities.jl:20-24): the
+    velocities and the coarse warm start (the one field a V-cycle writes)
+    come out as before.  The fields the first test recomputes from the
+    velocities (bed speed, drag, viscosity) follow the reference; in FP64
+    every field equals the oracle's."""
     _step("converged state, %s" % precision)
     m = channel(W, nx=40, ny=24, maxiter_picard=60, precision=precision)
     W.update_geometry(m)
     W.update_velocities(m)
     assert m.last_report["converged"]
     before = {fid: np.array(m._array(c, a), dtype=float, copy=True) for c, a, fid in W.model.FIELD_MAP}
-    m.parallel_spec.close()  # a new context: no residual history, so the V-cycle is queued
+    m.parallel_spec.close()  # a new context, as for a model loaded from a checkpoint
     o = oracle.OracleModel(m) if precision == "fp64" else None
     W.update_velocities(m)
     rep = m.last_report

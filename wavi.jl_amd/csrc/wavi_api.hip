@@ -117,13 +117,10 @@ struct wavi_ctx {
     double *Pz = nullptr;           // partials of (r, D^-1 r) of the Jacobi-preconditioned coarse CG
     DevVec cdinv;                   // 1 / |diag(A_c)| for the Jacobi-preconditioned coarse CG
     bool cg_pre = false;            // the V-cycles of do_linear_solve run the Jacobi-preconditioned coarse CG
-    DevVec ws_keep;                 // the coarse warm start before a speculative V-cycle (do_precondition)
-    hipEvent_t ev_norms = nullptr;  // the Picard residual norms reached the host
-    double last_rel = INFINITY;     // the context's last Picard relative residual
     bool lin_jacobi = true;         // ... when set (wavi_set_linear_coarse_jacobi)
     double *quad = nullptr;  // 4 * nh pos_fraction quadrants
-    double *norms = nullptr; // device: 2 doubles
-    double *h_norms = nullptr; // pinned host: 2 doubles
+    double *h_norms = nullptr; // pinned, host-mapped: 2 doubles the finishing kernels write
+    double *d_norms = nullptr; // its device address
     CGScalars *st = nullptr;
     CGScalars *h_st = nullptr;  // pinned host copy
     int nblk = 0;               // workgroups of the XCD-ordered tile kernels
@@ -640,12 +637,9 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
     CREATECHK(zalloc(&ctx->Pz, g.npart));
     CREATECHK(zalloc(&ctx->cdinv.u, ctx->nu));
     CREATECHK(zalloc(&ctx->cdinv.v, ctx->nv));
-    CREATECHK(zalloc(&ctx->ws_keep.u, ctx->nu));
-    CREATECHK(zalloc(&ctx->ws_keep.v, ctx->nv));
-    CREATECHK(hipEventCreateWithFlags(&ctx->ev_norms, hipEventDisableTiming));
     CREATECHK(zalloc(&ctx->quad, 4 * (size_t)ctx->nh));
-    CREATECHK(zalloc(&ctx->norms, 2));
-    CREATECHK(hipHostMalloc((void **)&ctx->h_norms, 2 * sizeof(double), hipHostMallocDefault));
+    CREATECHK(hipHostMalloc((void **)&ctx->h_norms, 2 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+    CREATECHK(hipHostGetDevicePointer((void **)&ctx->d_norms, ctx->h_norms, 0));
     CREATECHK(hipMalloc(&ctx->st, sizeof(CGScalars)));
     CREATECHK(hipMemset(ctx->st, 0, sizeof(CGScalars)));
     CREATECHK(hipHostMalloc((void **)&ctx->h_st, sizeof(CGScalars), hipHostMallocDefault));
@@ -715,8 +709,7 @@ static void ctx_free(wavi_ctx *ctx)
     for (double *p : ctx->saved)
         if (p) (void)hipFree(p);
     for (double *p : {ctx->zeta, ctx->qw, ctx->heta_h, ctx->heta_c, ctx->dimp, ctx->Pr0, ctx->Pr1, ctx->Pd, ctx->Pb,
-                      ctx->Pn, ctx->Pz, ctx->cdinv.u, ctx->cdinv.v, ctx->ws_keep.u, ctx->ws_keep.v, ctx->quad,
-                      ctx->norms})
+                      ctx->Pn, ctx->Pz, ctx->cdinv.u, ctx->cdinv.v, ctx->quad})
         if (p) (void)hipFree(p);
     for (DevVec *dv : {&ctx->x, &ctx->b, &ctx->r, &ctx->e, &ctx->d, &ctx->rho, &ctx->t, &ctx->f3, &ctx->fixedv,
                        &ctx->cr, &ctx->cp, &ctx->cq, &ctx->ct}) {
@@ -738,7 +731,7 @@ static void ctx_free(wavi_ctx *ctx)
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     for (hipEvent_t e : ctx->cg_ev)
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {ctx->ev_ls, ctx->ev_ls_done, ctx->ev_tune[0], ctx->ev_tune[1], ctx->ev_norms})
+    for (hipEvent_t e : {ctx->ev_ls, ctx->ev_ls_done, ctx->ev_tune[0], ctx->ev_tune[1]})
         if (e) (void)hipEventDestroy(e);
     if (ctx->st) (void)hipFree(ctx->st);
     if (ctx->tab) (void)hipFree(ctx->tab);
@@ -1668,12 +1661,12 @@ static int precondition_resid(wavi_ctx *ctx)
     hipLaunchKernelGGL(k_tile_resid<double>, dim3(band_blocks(g.nbx, g.nby)), dim3(256), 0, ctx->stream, g,
                        coef64(ctx), ctx->x.u, ctx->x.v, ctx->b.u, ctx->b.v, ctx->r.u, ctx->r.v, ctx->Pr0, ctx->Pb,
                        band_blocks(g.nbx, g.nby));
-    hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, ctx->stream, ctx->Pr0, ctx->Pb, g.npart, ctx->norms);
+    hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, ctx->stream, ctx->Pr0, ctx->Pb, g.npart,
+                       ctx->d_norms);
     // set_residual! (update_velocities.jl:428-433): residual fields at inner points
     hipLaunchKernelGGL(k_copy_masked_uv<double>, flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream,
                        ctx->f[WAVI_U_RESIDUAL], ctx->r.u, ctx->ui, ctx->nu, ctx->f[WAVI_V_RESIDUAL], ctx->r.v, ctx->vi,
                        ctx->nv);
-    HIPCHK(hipMemcpyAsync(ctx->h_norms, ctx->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipGetLastError());
     return WAVI_OK;
 }
@@ -1696,55 +1689,15 @@ static int precondition_update(wavi_ctx *ctx)
     return WAVI_OK;
 }
 
-// Grids of at most this many 16 x 16 tiles queue the V-cycle before the
-// host has read the residual norms (do_precondition).
-constexpr int SPECULATE_MAX_TILES = 64;
-
-// Whether do_precondition queues the V-cycle before the convergence test:
-// on small grids, where a V-cycle costs a few hundred microseconds and the
-// host's wait for the norms leaves the GPU idle between them, unless the
-// context's last residual says the test may pass.
-static bool speculate(const wavi_ctx *ctx)
-{
-    return !ctx->prof && ctx->g.nbx * ctx->g.nby <= SPECULATE_MAX_TILES &&
-           !(ctx->last_rel < 10.0 * ctx->S.tol_picard);
-}
-
 static int do_precondition(wavi_ctx *ctx, int *converged, double *rel, int *cg_iters)
 {
     CHK(precondition_resid(ctx));
+    CHK(precondition_decide(ctx, converged, rel));
     *cg_iters = 0;
-    if (speculate(ctx)) {
-        // V-cycle queued at once; if the test passes after all, nothing of
-        // it is kept: the velocities are left, and the coarse warm start (the
-        // one state the V-cycle writes) goes back to its value from before
-        HIPCHK(hipEventRecord(ctx->ev_norms, ctx->stream));
-        HIPCHK(hipMemcpyAsync(ctx->ws_keep.u, ctx->f[WAVI_WU_CORRECTION_COARSE], ctx->nu * sizeof(double),
-                              hipMemcpyDeviceToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(ctx->ws_keep.v, ctx->f[WAVI_WV_CORRECTION_COARSE], ctx->nv * sizeof(double),
-                              hipMemcpyDeviceToDevice, ctx->stream));
-        int cg = 0;
-        CHK(do_vcycle(ctx, &cg));
-        HIPCHK(hipEventSynchronize(ctx->ev_norms));
-        *rel = ctx->h_norms[0] / ctx->h_norms[1];
-        *converged = *rel < ctx->S.tol_picard;
-        if (*converged) {
-            HIPCHK(hipMemcpyAsync(ctx->f[WAVI_WU_CORRECTION_COARSE], ctx->ws_keep.u, ctx->nu * sizeof(double),
-                                  hipMemcpyDeviceToDevice, ctx->stream));
-            HIPCHK(hipMemcpyAsync(ctx->f[WAVI_WV_CORRECTION_COARSE], ctx->ws_keep.v, ctx->nv * sizeof(double),
-                                  hipMemcpyDeviceToDevice, ctx->stream));
-        } else {
-            *cg_iters = cg;
-            CHK(precondition_update(ctx));
-        }
-    } else {
-        CHK(precondition_decide(ctx, converged, rel));
-        if (!*converged) {
-            CHK(do_vcycle(ctx, cg_iters));
-            CHK(precondition_update(ctx));
-        }
+    if (!*converged) {
+        CHK(do_vcycle(ctx, cg_iters));
+        CHK(precondition_update(ctx));
     }
-    ctx->last_rel = *rel;
     return WAVI_OK;
 }
 
@@ -1794,8 +1747,9 @@ static int dots(wavi_ctx *ctx, const LinScratch &s, int count, const double *con
     for (int c = 0; c < count; ++c)
         hipLaunchKernelGGL(k_tile_dot<double>, tg, dim3(256), 0, ctx->stream, g, au[c], av[c], bu[c], bv[c],
                            s.part + (size_t)c * g.npart);
-    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(256), 0, ctx->stream, s.part, g.npart, count, ctx->norms);
-    HIPCHK(hipMemcpyAsync(ctx->h_norms, ctx->norms, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (count < 1 || count > 2) return WAVI_EARG;  // h_norms holds 2
+    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(256), 0, ctx->stream, s.part, g.npart, count, ctx->d_norms);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (int c = 0; c < count; ++c) out[c] = ctx->h_norms[c];
     return WAVI_OK;
@@ -1858,8 +1812,8 @@ static int do_linear_solve(wavi_ctx *ctx, const DevVec &b, const DevVec &x, cons
                            coef64(ctx), x.u, x.v, b.u, b.v, ctx->r.u, ctx->r.v, ctx->Pr0, ctx->Pb,
                            band_blocks(g.nbx, g.nby));
         hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, ctx->stream, ctx->Pr0, ctx->Pb, g.npart,
-                           ctx->norms);
-        HIPCHK(hipMemcpyAsync(ctx->h_norms, ctx->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+                           ctx->d_norms);
+        HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->stream));
         *rel = ctx->h_norms[1] > 0.0 ? ctx->h_norms[0] / ctx->h_norms[1] : 0.0;
         return WAVI_OK;
