@@ -351,14 +351,19 @@ int wavi_haar(wavi_ctx *ctx, int comp, int dir, const double *in, double *out);
  * runs the four colour sweeps of gauss_seidel_smoother!,
  * src/Wavelets/update_preconditioners.jl:133-150, on tiles of tx x ty
  * points).  The shapes are 16 x 16, 32 x 16 and 16 x 32; every shape gives
- * the same bits.  By default the library's autotuner picks the shape, and
- * whether the tiles are walked in bands of tile columns or of tile rows (one
- * band per XCD), on the context's first V-cycle: it times each of the six
- * variants on the context's own grid and keeps the fastest, and remembers
- * the choice for later contexts of the same grid size and precision in the
- * process.  wavi_set_smoother_tile fixes a shape, walked in column bands
- * (0, 0: tune again); wavi_smoother_tile reports the shape in use for the
- * context's V-cycle precision (0, 0 before the first V-cycle).
+ * the same bits.  A grid whose window (the grid and a border of one point)
+ * holds at most 2048 points and fits 64 KiB of LDS (80 x 10 in either
+ * precision) can also be smoothed whole in one workgroup, every smoothing
+ * iteration of a V-cycle in one launch: the shape -1 x -1.  By default the
+ * library's autotuner picks the variant on the context's first V-cycle: it
+ * times one smoothing phase of each (the three shapes walked in bands of
+ * tile columns or of tile rows, one band per XCD, and the whole grid where it
+ * fits) on the context's own grid, keeps the fastest, and remembers the
+ * choice for later contexts of the same grid size and precision in the
+ * process.  wavi_set_smoother_tile fixes a shape, walked in column bands, or
+ * -1 x -1 (WAVI_EARG if the grid does not fit); 0, 0 tunes again.
+ * wavi_smoother_tile reports the shape in use for the context's V-cycle
+ * precision (0, 0 before the first V-cycle).
  * No reference counterpart. */
 int wavi_set_smoother_tile(wavi_ctx *ctx, int tx, int ty);
 int wavi_smoother_tile(const wavi_ctx *ctx, int *tx, int *ty);

@@ -348,6 +348,60 @@ def test_smoother_tile_shapes_bit_equal(W, oracle, tile):
     assert (tx.value, ty.value) == tile
 
 
+@pytest.mark.parametrize("nx,ny", [(80, 10), (48, 12), (40, 24)])
+def test_whole_grid_smoother_bit_equal_to_oracle(W, oracle, nx, ny):
+    """The smoothing of a small grid in one workgroup (k_smooth_grid, every
+    iteration in one launch; wavi_set_smoother_tile(-1, -1)) gives the
+    oracle's bits: a 3-Picard FP64 solve on grids of one (48 x 12) and two
+    (80 x 10, 40 x 24) window points per thread."""
+    _step("whole-grid smoother %dx%d" % (nx, ny))
+    m = channel(W, nx=nx, ny=ny, maxiter_picard=3)
+    o = oracle.OracleModel(m)
+    spec = _gpu_model(W, m)
+    lib = spec._lib
+    assert lib.wavi_set_smoother_tile(spec.ctx, -1, -1) == 0
+    assert lib.wavi_update_geometry(spec.ctx) == 0
+    o.update_geometry()
+    rep = W._lib.SolveReport()
+    assert lib.wavi_update_velocities(spec.ctx, ctypes.byref(rep)) == 0
+    orep = o.update_velocities()
+    assert rep.as_dict() == orep
+    _assert_fields_equal(W, m, o)
+    tx, ty = ctypes.c_int(), ctypes.c_int()
+    assert lib.wavi_smoother_tile(spec.ctx, ctypes.byref(tx), ctypes.byref(ty)) == 0
+    assert (tx.value, ty.value) == (-1, -1)
+
+
+def test_whole_grid_smoother_mixed_and_autotuned(W):
+    """Mixed precision on 80 x 10: the whole-grid smoother, a tile shape and
+    the autotuner's choice (which may be either) give the same bits; a grid
+    whose window does not fit one workgroup is refused."""
+    runs = []
+    for tile in ((-1, -1), (16, 16), (0, 0)):
+        _step("mixed smoother %dx%d on 80x10" % tile)
+        m = channel(W, nx=80, ny=10, precision="mixed", maxiter_picard=4)
+        spec = _gpu_model(W, m)
+        lib = spec._lib
+        assert lib.wavi_set_smoother_tile(spec.ctx, *tile) == 0
+        assert lib.wavi_update_geometry(spec.ctx) == 0
+        rep = W._lib.SolveReport()
+        assert lib.wavi_update_velocities(spec.ctx, ctypes.byref(rep)) == 0
+        W.pull_fields(m)
+        tx, ty = ctypes.c_int(), ctypes.c_int()
+        assert lib.wavi_smoother_tile(spec.ctx, ctypes.byref(tx), ctypes.byref(ty)) == 0
+        if tile == (0, 0):
+            assert (tx.value, ty.value) in ((-1, -1), (16, 16), (32, 16), (16, 32))
+            _step("autotuned smoother %dx%d" % (tx.value, ty.value))
+        else:
+            assert (tx.value, ty.value) == tile
+        runs.append((m.fields.gu.u.copy(), m.fields.gv.v.copy(), rep.as_dict()))
+    for u, v, r in runs[1:]:
+        assert np.array_equal(u, runs[0][0]) and np.array_equal(v, runs[0][1]) and r == runs[0][2]
+    big = channel(W, nx=70, ny=40, precision="mixed")
+    spec = _gpu_model(W, big)
+    assert spec._lib.wavi_set_smoother_tile(spec.ctx, -1, -1) == W._lib.WAVI_EARG
+
+
 def test_smoother_autotuner_and_shapes_in_mixed_precision(W):
     """The mixed solve gives the same bits whatever the smoother tile shape;
     the autotuner picks one of the shapes; an unknown shape is rejected."""

@@ -41,6 +41,8 @@ static std::mutex g_graph_mu;
 // context creation on that GPU, and kept in g_scratch_ptr.
 static __device__ __align__(256) unsigned char scratch[WAVI_NOALLOC_SCRATCH_BYTES];
 constexpr int MAX_DEVICES = 64;
+// LDS the whole-grid smoother may take (at most the device's per-workgroup limit)
+constexpr size_t SMG_LDS_MAX = 65536;
 static std::mutex g_scratch_mu;
 static unsigned char *g_scratch_ptr[MAX_DEVICES] = {};
 
@@ -109,7 +111,8 @@ struct wavi_ctx {
     hipEvent_t cg_ev[3] = {};       // chunk completion events of the mixed coarse CG
     hipEvent_t ev_ls = nullptr, ev_ls_done = nullptr;  // set-up and batch end in a lockstep batch without allocation
     hipEvent_t ev_tune[2] = {};     // timing events of the smoother's tile-shape tuning
-    int smooth_shape[2] = {-1, -1}; // smoother tile shape by V-cycle precision (0 FP64, 1 FP32); -1: tune
+    int smooth_shape[2] = {-1, -1}; // smoother variant by V-cycle precision (0 FP64, 1 FP32); -1: tune
+    size_t lds_max = 0;             // dynamic LDS a workgroup may take (the whole-grid smoother)
     int cg_mode = WAVI_CG_AUTO;      // coarse CG: one launch per iteration or one workgroup (wavi_set_coarse_cg)
     int cg_prev = 0;                 // iterations of the context's previous coarse CG (0: none yet)
     hipGraphExec_t cgm_chunk = nullptr;  // 32 k_cgm_iter launches (parities 1, 0, ...) as one graph
@@ -533,6 +536,11 @@ int wavi_ctx_create(const wavi_grid_desc *grid, const wavi_phys_params *phys, co
         }                                                                         \
     } while (0)
     CREATECHK(hipGetDevice(&ctx->device));
+    {
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && lds > 0)
+            ctx->lds_max = std::min((size_t)lds, SMG_LDS_MAX);
+    }
     if (ctx->device >= 0 && ctx->device < MAX_DEVICES) {
         // the scratch address of the entry point without allocation, once per device
         std::lock_guard<std::mutex> slock(g_scratch_mu);
@@ -1035,6 +1043,10 @@ static int do_diag(wavi_ctx *ctx, const VSet<T> &w)
 // XCD bands of tile columns or of tile rows: variant v is shape v % 3,
 // column bands when v < 3.
 constexpr int SMOOTH_SHAPES = 3, SMOOTH_VARIANTS = 6;
+// Variant SMOOTH_GRID: the whole grid in one workgroup (k_smooth_grid), all
+// smoothing iterations in one launch, for grids whose window fits
+// SMG_MAX_SLOTS points per thread and the context's LDS limit.
+constexpr int SMOOTH_GRID = SMOOTH_VARIANTS;
 static const int SMOOTH_TX[SMOOTH_SHAPES] = {16, 32, 16};
 static const int SMOOTH_TY[SMOOTH_SHAPES] = {16, 16, 32};
 
@@ -1065,22 +1077,48 @@ static void launch_smooth(wavi_ctx *ctx, int variant, const OpCoef<T> &c, const 
     }
 }
 
+static bool smooth_grid_fits(const wavi_ctx *ctx, size_t elem_bytes)
+{
+    const SmoothGridDims d(ctx->g);
+    return d.points() <= SMG_THREADS * SMG_MAX_SLOTS && d.elems() * elem_bytes <= ctx->lds_max;
+}
+
+// nsmooth smoothing iterations of the whole grid in one launch, from x into y
+// (which may be x); the grid must fit (smooth_grid_fits).
+template <typename T>
+static void launch_smooth_grid(wavi_ctx *ctx, const OpCoef<T> &c, const VSet<T> &w, const T *xu, const T *xv, T *yu,
+                               T *yv, int packed, int nsmooth)
+{
+    const GridDims &g = ctx->g;
+    const SmoothGridDims d(g);
+    const size_t lds = d.elems() * sizeof(T);
+    const T om = (T)ctx->S.smoother_omega;
+    if (d.points() <= SMG_THREADS)
+        hipLaunchKernelGGL((k_smooth_grid<T, 1>), dim3(1), dim3(SMG_THREADS), lds, ctx->stream, g, c, xu, xv, yu, yv,
+                           w.r.u, w.r.v, w.d.u, w.d.v, om, packed, nsmooth);
+    else
+        hipLaunchKernelGGL((k_smooth_grid<T, 2>), dim3(1), dim3(SMG_THREADS), lds, ctx->stream, g, c, xu, xv, yu, yv,
+                           w.r.u, w.r.v, w.d.u, w.d.v, om, packed, nsmooth);
+}
+
 // The autotuner's results, shared by the contexts of one process: the
 // fastest smoother variant (tile shape and band order) per (nx, ny, V-cycle
 // precision).
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, int>, int> g_smooth_tuned;
 
-// The smoother tile shape of the context, chosen on first use: the shape
-// set by wavi_set_smoother_tile, else one tuned before for a grid of this
-// size and precision, else measured now: after a warm-up launch of each,
-// three timed runs of four launches of every candidate shape from the
+// The smoother variant of the context, chosen on first use: the one set by
+// wavi_set_smoother_tile, else one tuned before for a grid of this size and
+// precision, else measured now: after a warm-up of each, three timed runs of
+// one smoothing phase (nsmooth iterations) of every candidate from the
 // current iterate w.e into w.t (free while smoothing; the results are not
-// used), each run between two events; a shape's time is its fastest run
-// (launches of small grids take a few microseconds, and the fastest of
-// several runs filters out the jitter of the others), and the fastest shape
-// is kept.  Every shape gives the same bits, so the choice never changes a
-// result.
+// used), each run between two events.  The candidates are the three tile
+// shapes in column and in row bands (one launch per iteration) and, where
+// the grid fits, the whole grid in one workgroup (one launch per phase).  A
+// variant's time is its fastest run (launches of small grids take a few
+// microseconds, and the fastest of several runs filters out the jitter of the
+// others), and the fastest variant is kept.  Every variant gives the same
+// bits, so the choice never changes a result.
 template <typename T>
 static int smooth_shape(wavi_ctx *ctx, const VSet<T> &w, const OpCoef<T> &c, int packed)
 {
@@ -1092,14 +1130,23 @@ static int smooth_shape(wavi_ctx *ctx, const VSet<T> &w, const OpCoef<T> &c, int
         auto it = g_smooth_tuned.find(key);
         if (it != g_smooth_tuned.end()) return ctx->smooth_shape[pi] = it->second;
     }
+    const int n = ctx->S.nsmooth;
+    if (n <= 0) return 0;  // nothing to smooth, nothing to tune
+    const int candidates = SMOOTH_VARIANTS + (smooth_grid_fits(ctx, sizeof(T)) ? 1 : 0);
+    auto phase = [&](int sh) {
+        if (sh == SMOOTH_GRID)
+            launch_smooth_grid(ctx, c, w, w.e.u, w.e.v, w.t.u, w.t.v, packed, n);
+        else
+            for (int r = 0; r < n; ++r) launch_smooth(ctx, sh, c, w, w.e.u, w.e.v, w.t.u, w.t.v, packed);
+    };
     int best = 0;
     float best_ms = 0.f;
-    for (int sh = 0; sh < SMOOTH_VARIANTS; ++sh) {
-        launch_smooth(ctx, sh, c, w, w.e.u, w.e.v, w.t.u, w.t.v, packed);
+    for (int sh = 0; sh < candidates; ++sh) {
+        phase(sh);
         float ms = 0.f;
         for (int run = 0; run < 3; ++run) {
             if (hipEventRecord(ctx->ev_tune[0], ctx->stream) != hipSuccess) return 0;
-            for (int r = 0; r < 4; ++r) launch_smooth(ctx, sh, c, w, w.e.u, w.e.v, w.t.u, w.t.v, packed);
+            phase(sh);
             if (hipEventRecord(ctx->ev_tune[1], ctx->stream) != hipSuccess) return 0;
             if (hipEventSynchronize(ctx->ev_tune[1]) != hipSuccess) return 0;
             float t = 0.f;
@@ -1129,7 +1176,17 @@ static int do_smoother(wavi_ctx *ctx, const VSet<T> &w, const int order[4])
 {
     OpCoef<T> c = coef_of(ctx, w);
     const int packed = order[0] | order[1] << 4 | order[2] << 8 | order[3] << 12;
-    const int shape = smooth_shape(ctx, w, c, packed);
+    int shape = smooth_shape(ctx, w, c, packed);
+    if (shape == SMOOTH_GRID) {
+        if (smooth_grid_fits(ctx, sizeof(T))) {
+            ProfScope ps(ctx, WAVI_KT_COLOUR);
+            if (ctx->S.nsmooth > 0)
+                launch_smooth_grid(ctx, c, w, w.e.u, w.e.v, w.e.u, w.e.v, packed, ctx->S.nsmooth);
+            HIPCHK(hipGetLastError());
+            return WAVI_OK;
+        }
+        shape = 0;  // set for a V-cycle of the other precision, whose window does not fit
+    }
     T *xu = w.e.u, *xv = w.e.v, *yu = w.t.u, *yv = w.t.v;
     for (int it = 0; it < ctx->S.nsmooth; ++it) {
         ProfScope ps(ctx, WAVI_KT_COLOUR);
@@ -3097,8 +3154,16 @@ int wavi_set_smoother_tile(wavi_ctx *ctx, int tx, int ty)
     int shape = -1;
     for (int k = 0; k < SMOOTH_SHAPES; ++k)
         if (SMOOTH_TX[k] == tx && SMOOTH_TY[k] == ty) shape = k;
-    if (shape < 0 && !(tx == 0 && ty == 0)) {
-        ctx->err = "wavi_set_smoother_tile: the tile shapes are 16 x 16, 32 x 16 and 16 x 32 (0 x 0: tune)";
+    if (tx == -1 && ty == -1) {
+        const size_t w = ctx->precision == WAVI_PRECISION_MIXED ? sizeof(float) : sizeof(double);
+        if (!smooth_grid_fits(ctx, w)) {
+            ctx->err = "wavi_set_smoother_tile: the grid does not fit one workgroup (-1 x -1)";
+            return WAVI_EARG;
+        }
+        shape = SMOOTH_GRID;
+    } else if (shape < 0 && !(tx == 0 && ty == 0)) {
+        ctx->err = "wavi_set_smoother_tile: the tile shapes are 16 x 16, 32 x 16 and 16 x 32 (0 x 0: tune; -1 x -1: "
+                   "the whole grid in one workgroup)";
         return WAVI_EARG;
     }
     // a fixed shape is walked in column bands
@@ -3111,6 +3176,10 @@ int wavi_smoother_tile(const wavi_ctx *ctx, int *tx, int *ty)
     if (!ctx || !tx || !ty) return WAVI_EARG;
     const int pi = ctx->precision == WAVI_PRECISION_MIXED ? 1 : 0;
     const int sh = ctx->smooth_shape[pi];
+    if (sh == SMOOTH_GRID) {
+        *tx = *ty = -1;
+        return WAVI_OK;
+    }
     *tx = sh >= 0 ? SMOOTH_TX[sh % SMOOTH_SHAPES] : 0;
     *ty = sh >= 0 ? SMOOTH_TY[sh % SMOOTH_SHAPES] : 0;
     return WAVI_OK;

@@ -1591,6 +1591,159 @@ __global__ void __launch_bounds__(256) k_smooth4(GridDims g, OpCoef<T> c, const 
 }
 
 // ===========================================================================
+// The whole smoothing of a small grid in one workgroup: nsmooth iterations of
+// gauss_seidel_smoother! (update_preconditioners.jl:133-150), each the four
+// colour sweeps in `order`, with the grid staged in LDS once and block
+// barriers between the sweeps, so that a V-cycle's smoothing is one launch
+// instead of one per iteration.  The window is the grid with a border of one
+// absent point: window position (x, y) is the point (x - 1, y - 1), its value
+// at [y + 1][x + 1] and its cell at [y + 1][x + 1] as optile_u / optile_v
+// expect.  A point's update is that of k_smooth4 (the arithmetic of op_u /
+// op_v and of the one-colour sweep), so the bits are the same.  The result
+// goes to xu_out, xv_out, which may be xu, xv: every read of x precedes the
+// first barrier and every write follows the last.
+constexpr int SMG_THREADS = 1024;
+constexpr int SMG_MAX_SLOTS = 2;  // window points per thread
+
+// Rows of a 2-D array in dynamic LDS.
+template <typename T>
+struct LdsRows {
+    T *p;
+    int ld;
+    __device__ __forceinline__ T *operator[](int y) const { return p + y * ld; }
+};
+
+template <typename T>
+struct SmoothGridWin {
+    LdsRows<T> U, V, rxx, ryy, ext, rxy;
+};
+
+// Extents of the whole-grid window: PX x PY points and CX x CY cells, the
+// cell q = (q % CX - 1, q / CX - 1) at [1 + q / CX][1 + q % CX]; U and V have
+// PY + 1 rows of PX + 2, the cell arrays CY + 1 rows of CX + 2.
+struct SmoothGridDims {
+    int PX, PY, CX, CY;
+    __host__ __device__ explicit SmoothGridDims(const GridDims &g)
+        : PX((g.nxu > g.nxv ? g.nxu : g.nxv) + 2), PY((g.nyu > g.nyv ? g.nyu : g.nyv) + 2), CX(PX - 1), CY(PY - 1)
+    {
+    }
+    __host__ __device__ int points() const { return PX * PY; }
+    __host__ __device__ int cells() const { return CX * CY; }
+    // LDS elements of the six arrays
+    __host__ __device__ size_t elems() const
+    {
+        return 2 * (size_t)(PY + 1) * (PX + 2) + 4 * (size_t)(CY + 1) * (CX + 2);
+    }
+};
+
+template <typename T, int SLOTS>
+__global__ void __launch_bounds__(SMG_THREADS) k_smooth_grid(GridDims g, OpCoef<T> c, const T *xu, const T *xv,
+                                                            T *xu_out, T *xv_out, const T *bu, const T *bv,
+                                                            const T *du, const T *dv, T omega, int order, int nsmooth)
+{
+    extern __shared__ __align__(16) unsigned char smg_lds[];
+    const SmoothGridDims d(g);
+    const int PX = d.PX, PY = d.PY, CX = d.CX, CY = d.CY;
+    SmoothGridWin<T> s;
+    {
+        T *p = reinterpret_cast<T *>(smg_lds);
+        const int lp = PX + 2, lc = CX + 2;
+        s.U = {p, lp};
+        p += (size_t)(PY + 1) * lp;
+        s.V = {p, lp};
+        p += (size_t)(PY + 1) * lp;
+        s.rxx = {p, lc};
+        p += (size_t)(CY + 1) * lc;
+        s.ryy = {p, lc};
+        p += (size_t)(CY + 1) * lc;
+        s.ext = {p, lc};
+        p += (size_t)(CY + 1) * lc;
+        s.rxy = {p, lc};
+    }
+    // stage the grid and load the coefficients every sweep reads
+    SmoothPoint<T> sp[SLOTS];
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+        const int q = threadIdx.x + SMG_THREADS * k;
+        const int x = q % PX, y = q / PX, i = x - 1, j = y - 1;
+        PointCoef<T> &pc = sp[k].pc;
+        pc.oku = q < PX * PY && i >= 0 && j >= 0 && i < g.nxu && j < g.nyu;
+        pc.okv = q < PX * PY && i >= 0 && j >= 0 && i < g.nxv && j < g.nyv;
+        pc.ku = i + g.nxu * j;
+        pc.kv = i + g.nxv * j;
+        pc.ui = pc.um = pc.vi = pc.vm = pc.mwu = pc.mwv = false;
+        pc.bu = pc.hu = pc.bv = pc.hv = T(0);
+        sp[k].bu = sp[k].du = sp[k].bv = sp[k].dv = T(0);
+        T u = T(0), v = T(0);
+        if (pc.oku) {
+            u = xu[pc.ku];
+            pc.ui = c.ui[pc.ku];
+            pc.um = c.um[pc.ku];
+            pc.bu = c.bu[pc.ku];
+            pc.hu = c.hu[pc.ku];
+            sp[k].bu = bu[pc.ku];
+            sp[k].du = du[pc.ku];
+        }
+        if (pc.okv) {
+            v = xv[pc.kv];
+            pc.vi = c.vi[pc.kv];
+            pc.vm = c.vm[pc.kv];
+            pc.bv = c.bv[pc.kv];
+            pc.hv = c.hv[pc.kv];
+            sp[k].bv = bv[pc.kv];
+            sp[k].dv = dv[pc.kv];
+        }
+        if (q < PX * PY) {
+            s.U[y + 1][x + 1] = u;
+            s.V[y + 1][x + 1] = v;
+        }
+    }
+    CellCoef<T> cc[SLOTS];
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+        const int q = threadIdx.x + SMG_THREADS * k;
+        cell_prefetch_at(g, c, q % CX - 1, q / CX - 1, q < CX * CY, cc[k]);
+    }
+    for (int it = 0; it < nsmooth; ++it) {
+        for (int sw = 0; sw < 4; ++sw) {
+            const int colour = (order >> (4 * sw)) & 15;
+            __syncthreads();  // the window holds the values after the previous sweep
+#pragma unroll
+            for (int k = 0; k < SLOTS; ++k) {
+                const int q = threadIdx.x + SMG_THREADS * k;
+                if (q < CX * CY) cell_eval(c, cc[k], 1 + q % CX, 1 + q / CX, s);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < SLOTS; ++k) {
+                const int q = threadIdx.x + SMG_THREADS * k;
+                if (q >= PX * PY) continue;
+                const int x = q % PX, y = q / PX, i = x - 1, j = y - 1;
+                const PointCoef<T> &pc = sp[k].pc;
+                if (colour <= 2) {
+                    if (pc.oku && pc.ui && ((i - j) & 1) == colour - 1) {
+                        const T r = sp[k].bu - optile_u(c, s, pc, x, y);
+                        s.U[y + 1][x + 1] = s.U[y + 1][x + 1] + (omega * r) / sp[k].du;
+                    }
+                } else if (pc.okv && pc.vi && ((i - j) & 1) == colour - 3) {
+                    const T r = sp[k].bv - optile_v(c, s, pc, x, y);
+                    s.V[y + 1][x + 1] = s.V[y + 1][x + 1] + (omega * r) / sp[k].dv;
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+        const int q = threadIdx.x + SMG_THREADS * k;
+        if (q >= PX * PY) continue;
+        const int x = q % PX, y = q / PX;
+        if (sp[k].pc.oku) xu_out[sp[k].pc.ku] = s.U[y + 1][x + 1];
+        if (sp[k].pc.okv) xv_out[sp[k].pc.kv] = s.V[y + 1][x + 1];
+    }
+}
+
+// ===========================================================================
 // Restricted additive Schwarz (SharedMemorySpec, src/SharedMemorySpec.jl):
 // moving velocities between the full grid and a subdomain context on the
 // same device.  A subdomain covers the cells [i0, i0 + nxg) x [j0, j0 + nyg)
