@@ -196,6 +196,39 @@ def test_gradient_batch_two_pools(W):
         assert ra == rb
 
 
+def test_gradient_batch_members_with_either_coarse_cg(W):
+    """Small mixed members, two of them with the plain coarse CG in their
+    linear solves (wavi_set_linear_coarse_jacobi(ctx, 0)) and two with the
+    Jacobi-preconditioned one: only members of one kind share coarse-CG
+    launches, and every member is bit-identical to its own call."""
+    specs = _small_specs(W, 4, seed=19)
+    jacobi = [1, 0, 1, 0]
+    _step("ensemble gradients, plain and Jacobi coarse CGs: individual calls")
+
+    def members():
+        out = []
+        for sp, on in zip(specs, jacobi):
+            m = W.workloads.ensemble_member(sp, "mixed", maxiter_picard=2)
+            W.update_geometry(m)
+            spec = m.parallel_spec
+            assert spec._lib.wavi_set_linear_coarse_jacobi(spec.ctx, on) == 0
+            out.append(m)
+        return out
+
+    obs = []
+    for m, sp in zip(members(), specs):
+        nx, ny = sp["nx"], sp["ny"]
+        W.misfit_gradient(m, 2, np.zeros((nx + 1, ny)), np.zeros((nx, ny + 1)), gradient=False)
+        obs.append(W.workloads.inversion_observations(m.fields.gu.u, m.fields.gv.v, seed=sp["seed"]))
+    alone = [W.misfit_gradient(m, 2, uo, vo) for m, (uo, vo) in zip(members(), obs)]
+    _step("ensemble gradients, plain and Jacobi coarse CGs: one batched call")
+    batch = W.misfit_gradient_batch(members(), 2, obs, nthreads=2)
+    for (Ja, ga, ra), (Jb, gb, rb) in zip(alone, batch):
+        assert Ja == Jb
+        np.testing.assert_array_equal(ga, gb)
+        assert ra == rb
+
+
 def test_gradient_batch_more_members_than_rendezvous_threads(W):
     """36 small members: more than the 32 threads the meeting members get, so
     threads leave the rendezvous with one member and join with the next."""

@@ -2638,14 +2638,18 @@ static size_t arena_bytes(const wavi_ctx *ctx)
 }
 
 // The members of a batch that share coarse-CG launches, in member order.
-static std::vector<int> lock_group(wavi_ctx *const *ctxs, int n)
+// For gradients (linear = true) they also share the setting of
+// wavi_set_linear_coarse_jacobi, since every coarse CG of a gradient is one
+// of a linear solve and one launch serves one form of the CG.
+static std::vector<int> lock_group(wavi_ctx *const *ctxs, int n, bool linear = false)
 {
     std::vector<int> small;
     size_t bytes = 0;
     for (int k = 0; k < n; ++k) {
         const wavi_ctx *c = ctxs[k], *ref = small.empty() ? c : ctxs[small[0]];
         if (c->precision == WAVI_PRECISION_MIXED && c->g.nbx * c->g.nby <= LOCK_TILES && c->device == ref->device &&
-            c->S.levels == ref->S.levels && bytes + arena_bytes(c) <= LOCK_ARENA_BYTES) {
+            c->S.levels == ref->S.levels && (!linear || c->lin_jacobi == ref->lin_jacobi) &&
+            bytes + arena_bytes(c) <= LOCK_ARENA_BYTES) {
             small.push_back(k);
             bytes += arena_bytes(c);
         }
@@ -2741,7 +2745,7 @@ int wavi_misfit_gradient_batch(wavi_ctx **ctxs, int n, int nthreads, int n_picar
     if (batch_has_duplicates(ctxs, n)) return WAVI_EARG;
     // the small mixed-precision members meet at every coarse CG, so the
     // members that run at the same time share the CG launches
-    const std::vector<int> small = lock_group(ctxs, n);
+    const std::vector<int> small = lock_group(ctxs, n, true);
     std::unique_ptr<CgRendezvous> R;
     std::vector<int> slot(n, -1);
     if (small.size() >= 2) {
