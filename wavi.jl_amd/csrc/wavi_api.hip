@@ -1285,6 +1285,16 @@ static bool cg_one_workgroup(const wavi_ctx *ctx)
     return ctx->cg_mode == WAVI_CG_WORKGROUP;
 }
 
+// The three-kernel coarse CG takes its Haar rows loads first
+// (haar_pm_row_ld; the same bits) on grids of at most this many tiles,
+// where a launch costs one tile's chain of LDS waits.
+constexpr int HAAR_LD_MAX_TILES = 256;
+
+static bool haar_loads_first(const wavi_ctx *ctx)
+{
+    return ctx->g.nbx * ctx->g.nby <= HAAR_LD_MAX_TILES;
+}
+
 // coarse CG on b_c held in w.cr; solution in the warm start w.xc
 template <typename T>
 static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
@@ -1298,6 +1308,7 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
     ctx->h_flag->done = 0;
     ctx->h_flag->iters = 0;
     const bool pre = ctx->cg_pre;
+    const bool ld = haar_loads_first(ctx);
     if (pre)
         hipLaunchKernelGGL((k_cg_setup<T, true>), tb, dim3(256), 0, ctx->stream, a);
     else
@@ -1334,14 +1345,21 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
             const int par = q & 1;
             {
                 ProfScope ps(ctx, WAVI_KT_CG_DIR, sampled);
-                if (pre)
-                    hipLaunchKernelGGL((k_cg_dir<T, true>), tb, dim3(256), 0, ctx->stream, a, par);
+                if (pre && ld)
+                    hipLaunchKernelGGL((k_cg_dir<T, true, true>), tb, dim3(256), 0, ctx->stream, a, par);
+                else if (pre)
+                    hipLaunchKernelGGL((k_cg_dir<T, true, false>), tb, dim3(256), 0, ctx->stream, a, par);
+                else if (ld)
+                    hipLaunchKernelGGL((k_cg_dir<T, false, true>), tb, dim3(256), 0, ctx->stream, a, par);
                 else
-                    hipLaunchKernelGGL((k_cg_dir<T, false>), tb, dim3(256), 0, ctx->stream, a, par);
+                    hipLaunchKernelGGL((k_cg_dir<T, false, false>), tb, dim3(256), 0, ctx->stream, a, par);
             }
             {
                 ProfScope ps(ctx, WAVI_KT_CG_APPLY, sampled);
-                hipLaunchKernelGGL(k_cg_apply<T>, tb, dim3(256), 0, ctx->stream, a);
+                if (ld)
+                    hipLaunchKernelGGL((k_cg_apply<T, true>), tb, dim3(256), 0, ctx->stream, a);
+                else
+                    hipLaunchKernelGGL((k_cg_apply<T, false>), tb, dim3(256), 0, ctx->stream, a);
             }
             {
                 ProfScope ps(ctx, WAVI_KT_CG_UPDATE, sampled);

@@ -1019,6 +1019,47 @@ __device__ __forceinline__ T haar_pm_row(uint32_t nz, uint32_t neg, const T *src
     return acc;
 }
 
+// haar_pm_row with the B = 2 << levels entries of the row's block loaded
+// first, then added in increasing column order through selects: the same
+// additions as haar_pm_row, so the same bits, with one LDS wait instead of
+// one per term.  The block lies inside the 16 x 16 tile, so every load is in
+// bounds; entries outside the row's terms are loaded and discarded.  Shorter
+// where a tile's latency chain is the launch's cost (grids of few tiles);
+// the extra VALU work costs more than it saves when many tiles share a CU.
+template <typename T, int B>
+__device__ __forceinline__ T haar_pm_row_ld(uint32_t nz, uint32_t neg, const T *src, int stride)
+{
+    T v[B];
+#pragma unroll
+    for (int c = 0; c < B; ++c) v[c] = src[c * stride];
+    T acc = T(0);
+#pragma unroll
+    for (int c = 0; c < B; ++c) {
+        const T t = ((neg >> c) & 1u) ? acc - v[c] : acc + v[c];
+        acc = ((nz >> c) & 1u) ? t : acc;
+    }
+    return acc;
+}
+
+template <typename T, bool LD>
+__device__ __forceinline__ T haar_pm_row_sel(int levels, uint32_t nz, uint32_t neg, const T *src, int stride)
+{
+    if constexpr (!LD) {
+        return haar_pm_row<T>(nz, neg, src, stride);
+    } else {
+        switch (levels) {
+        case 0:
+            return haar_pm_row_ld<T, 2>(nz, neg, src, stride);
+        case 1:
+            return haar_pm_row_ld<T, 4>(nz, neg, src, stride);
+        case 2:
+            return haar_pm_row_ld<T, 8>(nz, neg, src, stride);
+        default:
+            return haar_pm_row_ld<T, 16>(nz, neg, src, stride);
+        }
+    }
+}
+
 // tile[ly][lx] <- (Wy (x) Wx) tile with W the inverse (dir 0) or its
 // transpose (dir 1): y pass into tmp, x pass back into tile.
 template <typename T>
@@ -1045,8 +1086,9 @@ __device__ __forceinline__ void haar_pm_tile(int levels, int dir, T (*tile)[TILE
     __syncthreads();
 }
 
-// The same transform on the u tile and the v tile of one block at once.
-template <typename T>
+// The same transform on the u tile and the v tile of one block at once (LD:
+// the rows as haar_pm_row_ld).
+template <typename T, bool LD = false>
 __device__ __forceinline__ void haar_pm_tile2(int levels, int dir, T (*tu)[TILE + 1], T (*mu)[TILE + 1],
                                               T (*tv)[TILE + 1], T (*mv)[TILE + 1], int i0, int j0, const GridDims &g,
                                               bool has_u, bool has_v)
@@ -1058,20 +1100,24 @@ __device__ __forceinline__ void haar_pm_tile2(int levels, int dir, T (*tu)[TILE 
     const int ry = ly - jb, rx = lx - ib;
     if (has_u && i < g.nxu && j < g.nyu) {
         const int m = min(g.nyu - (j0 + jb), B);
-        mu[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][ry], c_haar[levels].neg[dir][m][ry], &tu[jb][lx], TILE + 1);
+        mu[ly][lx] = haar_pm_row_sel<T, LD>(levels, c_haar[levels].nz[dir][m][ry], c_haar[levels].neg[dir][m][ry],
+                                            &tu[jb][lx], TILE + 1);
     }
     if (has_v && i < g.nxv && j < g.nyv) {
         const int m = min(g.nyv - (j0 + jb), B);
-        mv[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][ry], c_haar[levels].neg[dir][m][ry], &tv[jb][lx], TILE + 1);
+        mv[ly][lx] = haar_pm_row_sel<T, LD>(levels, c_haar[levels].nz[dir][m][ry], c_haar[levels].neg[dir][m][ry],
+                                            &tv[jb][lx], TILE + 1);
     }
     __syncthreads();
     if (has_u && i < g.nxu && j < g.nyu) {
         const int m = min(g.nxu - (i0 + ib), B);
-        tu[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][rx], c_haar[levels].neg[dir][m][rx], &mu[ly][ib], 1);
+        tu[ly][lx] = haar_pm_row_sel<T, LD>(levels, c_haar[levels].nz[dir][m][rx], c_haar[levels].neg[dir][m][rx],
+                                            &mu[ly][ib], 1);
     }
     if (has_v && i < g.nxv && j < g.nyv) {
         const int m = min(g.nxv - (i0 + ib), B);
-        tv[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][rx], c_haar[levels].neg[dir][m][rx], &mv[ly][ib], 1);
+        tv[ly][lx] = haar_pm_row_sel<T, LD>(levels, c_haar[levels].nz[dir][m][rx], c_haar[levels].neg[dir][m][rx],
+                                            &mv[ly][ib], 1);
     }
     __syncthreads();
 }
@@ -1951,13 +1997,13 @@ __device__ __forceinline__ int band_tile(const GridDims &g, int L, int &bx, int 
 
 // c = R A t for the tile: A into tu / tv, then the transposed inverse Haar,
 // zero off the wavelet masks.
-template <typename T>
+template <typename T, bool LD = false>
 __device__ __forceinline__ void tile_apply_restrict(const CGArgs<T> &a, int i0, int j0, bool has_u, bool has_v,
                                                     const PointCoef<T> &pc, OpTile<T> &s, T (*tu)[TILE + 1],
                                                     T (*mu)[TILE + 1], T (*tv)[TILE + 1], T (*mv)[TILE + 1])
 {
     optile_apply(a.g, a.c, a.tu, a.tv, i0, j0, pc, s, tu, tv);
-    haar_pm_tile2<T>(a.levels, 1, tu, mu, tv, mv, i0, j0, a.g, has_u, has_v);
+    haar_pm_tile2<T, LD>(a.levels, 1, tu, mu, tv, mv, i0, j0, a.g, has_u, has_v);
 }
 
 // setup (t = P x0 computed beforehand): r = b - R A t (b arrives in r),
@@ -2031,8 +2077,9 @@ __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
 // u = r + beta u with beta = res^2 / prev^2 (PRE: u = D^-1 r + beta u with
 // beta = rz / prev rz), t = P u.  The stop decision is
 // recomputed from the partials each time, so it needs no flag: after a stop
-// the partials are frozen and it[1 - par] repeats the count.
-template <typename T, bool PRE>
+// the partials are frozen and it[1 - par] repeats the count.  LD: the Haar
+// rows loads first (haar_pm_row_ld), the same bits.
+template <typename T, bool PRE, bool LD = false>
 __device__ __forceinline__ bool cg_dir_tile(const CGArgs<T> &a, int par, const int L)
 {
     __shared__ T tu[TILE][TILE + 1], mu[TILE][TILE + 1], tv[TILE][TILE + 1], mv[TILE][TILE + 1];
@@ -2117,14 +2164,14 @@ __device__ __forceinline__ bool cg_dir_tile(const CGArgs<T> &a, int par, const i
     tu[ly][lx] = mwu ? unu : T(0);
     tv[ly][lx] = mwv ? unv : T(0);
     __syncthreads();
-    haar_pm_tile2<T>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
+    haar_pm_tile2<T, LD>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
     if (oku) a.tu[ku] = iu ? tu[ly][lx] : T(0);
     if (okv) a.tv[kv] = iv ? tv[ly][lx] : T(0);
     return false;
 }
 
 // part 2: c = R A t and partials of dot(u, c)
-template <typename T>
+template <typename T, bool LD = false>
 __device__ __forceinline__ void cg_apply_tile(const CGArgs<T> &a, const int L)
 {
     __shared__ OpTile<T> s;
@@ -2141,7 +2188,7 @@ __device__ __forceinline__ void cg_apply_tile(const CGArgs<T> &a, const int L)
     PointCoef<T> pc;
     point_prefetch(g, a.c, a.mwu, a.mwv, i0 + lx, j0 + ly, has_u, has_v, pc);
     const T pu = pc.oku ? a.pu[pc.ku] : T(0), pv = pc.okv ? a.pv[pc.kv] : T(0);
-    tile_apply_restrict<T>(a, i0, j0, has_u, has_v, pc, s, tu, mu, tv, mv);
+    tile_apply_restrict<T, LD>(a, i0, j0, has_u, has_v, pc, s, tu, mu, tv, mv);
     double du = 0.0, dv = 0.0;
     if (pc.oku) {
         const T cv = pc.mwu ? tu[ly][lx] : T(0);
@@ -2221,16 +2268,16 @@ __device__ __forceinline__ void cg_update_tile(const CGArgs<T> &a, int par, cons
     }
 }
 
-template <typename T, bool PRE>
+template <typename T, bool PRE, bool LD>
 __global__ void __launch_bounds__(256) k_cg_dir(CGArgs<T> a, int par)
 {
-    (void)cg_dir_tile<T, PRE>(a, par, xcd_tile(a.nblk));
+    (void)cg_dir_tile<T, PRE, LD>(a, par, xcd_tile(a.nblk));
 }
 
-template <typename T>
+template <typename T, bool LD>
 __global__ void __launch_bounds__(256) k_cg_apply(CGArgs<T> a)
 {
-    cg_apply_tile<T>(a, xcd_tile(a.nblk));
+    cg_apply_tile<T, LD>(a, xcd_tile(a.nblk));
 }
 
 template <typename T, bool PRE>
