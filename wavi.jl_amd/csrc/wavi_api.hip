@@ -1201,6 +1201,9 @@ static int do_smoother(wavi_ctx *ctx, const VSet<T> &w, const int order[4])
     return WAVI_OK;
 }
 
+// Longest chunk of coarse-CG launches queued at once (cg_chunk_len).
+constexpr int CG_CHUNK = 32;
+
 template <typename T>
 static CGArgs<T> cg_args(wavi_ctx *ctx, const VSet<T> &w)
 {
@@ -1257,7 +1260,6 @@ static bool use_cg_graph()
 // transient's CGs change length slowly from step to step), chunk 0 holds
 // prev + 1, at most 32 (the launches up to and including the one that
 // stops, if this CG is as long), and chunk k >= 1 holds min(32, 2^k).
-constexpr int CG_CHUNK = 32;
 static int cg_chunk_len(int k, int prev)
 {
     if (prev > 0) return k == 0 ? std::min(prev + 1, CG_CHUNK) : (k >= 5 ? CG_CHUNK : 1 << k);
@@ -1309,6 +1311,11 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
     ctx->h_flag->iters = 0;
     const bool pre = ctx->cg_pre;
     const bool ld = haar_loads_first(ctx);
+    // k_cg_apply / k_cg_update return on the stop flag at once after a short
+    // previous CG (one that fitted its first chunk): such CGs stop within a
+    // chunk, so the launches queued past the stop are many.  Long CGs have
+    // about one, and their launches do without the flag's round trip.
+    const bool early = ctx->cg_prev < CG_CHUNK;
     if (pre)
         hipLaunchKernelGGL((k_cg_setup<T, true>), tb, dim3(256), 0, ctx->stream, a);
     else
@@ -1356,17 +1363,25 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
             }
             {
                 ProfScope ps(ctx, WAVI_KT_CG_APPLY, sampled);
-                if (ld)
-                    hipLaunchKernelGGL((k_cg_apply<T, true>), tb, dim3(256), 0, ctx->stream, a);
+                if (ld && early)
+                    hipLaunchKernelGGL((k_cg_apply<T, true, true>), tb, dim3(256), 0, ctx->stream, a);
+                else if (ld)
+                    hipLaunchKernelGGL((k_cg_apply<T, true, false>), tb, dim3(256), 0, ctx->stream, a);
+                else if (early)
+                    hipLaunchKernelGGL((k_cg_apply<T, false, true>), tb, dim3(256), 0, ctx->stream, a);
                 else
-                    hipLaunchKernelGGL((k_cg_apply<T, false>), tb, dim3(256), 0, ctx->stream, a);
+                    hipLaunchKernelGGL((k_cg_apply<T, false, false>), tb, dim3(256), 0, ctx->stream, a);
             }
             {
                 ProfScope ps(ctx, WAVI_KT_CG_UPDATE, sampled);
-                if (pre)
-                    hipLaunchKernelGGL((k_cg_update<T, true>), tb, dim3(256), 0, ctx->stream, a, par);
+                if (pre && early)
+                    hipLaunchKernelGGL((k_cg_update<T, true, true>), tb, dim3(256), 0, ctx->stream, a, par);
+                else if (pre)
+                    hipLaunchKernelGGL((k_cg_update<T, true, false>), tb, dim3(256), 0, ctx->stream, a, par);
+                else if (early)
+                    hipLaunchKernelGGL((k_cg_update<T, false, true>), tb, dim3(256), 0, ctx->stream, a, par);
                 else
-                    hipLaunchKernelGGL((k_cg_update<T, false>), tb, dim3(256), 0, ctx->stream, a, par);
+                    hipLaunchKernelGGL((k_cg_update<T, false, false>), tb, dim3(256), 0, ctx->stream, a, par);
             }
         }
         burst_end(ctx, bev, q - 1, n);

@@ -1086,40 +1086,88 @@ __device__ __forceinline__ void haar_pm_tile(int levels, int dir, T (*tile)[TILE
     __syncthreads();
 }
 
-// The same transform on the u tile and the v tile of one block at once (LD:
-// the rows as haar_pm_row_ld).
-template <typename T, bool LD = false>
-__device__ __forceinline__ void haar_pm_tile2(int levels, int dir, T (*tu)[TILE + 1], T (*mu)[TILE + 1],
-                                              T (*tv)[TILE + 1], T (*mv)[TILE + 1], int i0, int j0, const GridDims &g,
-                                              bool has_u, bool has_v)
+// The row masks one thread reads in the two passes of a transform of the u
+// and the v tile of block (i0, j0), all loaded at once (so that they can be
+// loaded with a kernel's other inputs instead of one round trip per pass).
+struct HaarRowMasks {
+    uint32_t yz_u, yn_u, yz_v, yn_v;  // y pass: terms and signs, u and v tile
+    uint32_t xz_u, xn_u, xz_v, xn_v;  // x pass
+};
+
+// p[k] where ok, else 0, with the load issued unconditionally (index 0
+// where !ok).
+__device__ __forceinline__ uint32_t haar_mask_or0(bool ok, const uint16_t *p, int k)
+{
+    const uint32_t v = p[ok ? k : 0];
+    return ok ? v : 0u;
+}
+
+__device__ __forceinline__ HaarRowMasks haar_row_masks(int levels, int dir, int i0, int j0, const GridDims &g,
+                                                       bool has_u, bool has_v)
 {
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     const int B = 2 << levels;
     const int i = i0 + lx, j = j0 + ly;
     const int jb = ly & ~(B - 1), ib = lx & ~(B - 1);
     const int ry = ly - jb, rx = lx - ib;
-    if (has_u && i < g.nxu && j < g.nyu) {
-        const int m = min(g.nyu - (j0 + jb), B);
-        mu[ly][lx] = haar_pm_row_sel<T, LD>(levels, c_haar[levels].nz[dir][m][ry], c_haar[levels].neg[dir][m][ry],
-                                            &tu[jb][lx], TILE + 1);
-    }
-    if (has_v && i < g.nxv && j < g.nyv) {
-        const int m = min(g.nyv - (j0 + jb), B);
-        mv[ly][lx] = haar_pm_row_sel<T, LD>(levels, c_haar[levels].nz[dir][m][ry], c_haar[levels].neg[dir][m][ry],
-                                            &tv[jb][lx], TILE + 1);
-    }
+    const bool ou = has_u && i < g.nxu && j < g.nyu, ov = has_v && i < g.nxv && j < g.nyv;
+    const uint16_t *nz = &c_haar[levels].nz[dir][0][0], *ng = &c_haar[levels].neg[dir][0][0];
+    const int yu = min(g.nyu - (j0 + jb), B) * 16 + ry, yv = min(g.nyv - (j0 + jb), B) * 16 + ry;
+    const int xu = min(g.nxu - (i0 + ib), B) * 16 + rx, xv = min(g.nxv - (i0 + ib), B) * 16 + rx;
+    HaarRowMasks m;
+    m.yz_u = haar_mask_or0(ou, nz, yu);
+    m.yn_u = haar_mask_or0(ou, ng, yu);
+    m.yz_v = haar_mask_or0(ov, nz, yv);
+    m.yn_v = haar_mask_or0(ov, ng, yv);
+    m.xz_u = haar_mask_or0(ou, nz, xu);
+    m.xn_u = haar_mask_or0(ou, ng, xu);
+    m.xz_v = haar_mask_or0(ov, nz, xv);
+    m.xn_v = haar_mask_or0(ov, ng, xv);
+    return m;
+}
+
+// The same transform on the u tile and the v tile of one block at once, with
+// the row masks given (haar_row_masks; LD: the rows as haar_pm_row_ld).
+template <typename T, bool LD = false>
+__device__ __forceinline__ void haar_pm_tile2m(int levels, const HaarRowMasks &hm, T (*tu)[TILE + 1],
+                                               T (*mu)[TILE + 1], T (*tv)[TILE + 1], T (*mv)[TILE + 1], int i0,
+                                               int j0, const GridDims &g, bool has_u, bool has_v)
+{
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int B = 2 << levels;
+    const int i = i0 + lx, j = j0 + ly;
+    const int jb = ly & ~(B - 1), ib = lx & ~(B - 1);
+    if (has_u && i < g.nxu && j < g.nyu)
+        mu[ly][lx] = haar_pm_row_sel<T, LD>(levels, hm.yz_u, hm.yn_u, &tu[jb][lx], TILE + 1);
+    if (has_v && i < g.nxv && j < g.nyv)
+        mv[ly][lx] = haar_pm_row_sel<T, LD>(levels, hm.yz_v, hm.yn_v, &tv[jb][lx], TILE + 1);
     __syncthreads();
-    if (has_u && i < g.nxu && j < g.nyu) {
-        const int m = min(g.nxu - (i0 + ib), B);
-        tu[ly][lx] = haar_pm_row_sel<T, LD>(levels, c_haar[levels].nz[dir][m][rx], c_haar[levels].neg[dir][m][rx],
-                                            &mu[ly][ib], 1);
-    }
-    if (has_v && i < g.nxv && j < g.nyv) {
-        const int m = min(g.nxv - (i0 + ib), B);
-        tv[ly][lx] = haar_pm_row_sel<T, LD>(levels, c_haar[levels].nz[dir][m][rx], c_haar[levels].neg[dir][m][rx],
-                                            &mv[ly][ib], 1);
-    }
+    if (has_u && i < g.nxu && j < g.nyu)
+        tu[ly][lx] = haar_pm_row_sel<T, LD>(levels, hm.xz_u, hm.xn_u, &mu[ly][ib], 1);
+    if (has_v && i < g.nxv && j < g.nyv)
+        tv[ly][lx] = haar_pm_row_sel<T, LD>(levels, hm.xz_v, hm.xn_v, &mv[ly][ib], 1);
     __syncthreads();
+}
+
+// haar_pm_tile2m with the masks loaded here.
+template <typename T, bool LD = false>
+__device__ __forceinline__ void haar_pm_tile2(int levels, int dir, T (*tu)[TILE + 1], T (*mu)[TILE + 1],
+                                              T (*tv)[TILE + 1], T (*mv)[TILE + 1], int i0, int j0, const GridDims &g,
+                                              bool has_u, bool has_v)
+{
+    const HaarRowMasks hm = haar_row_masks(levels, dir, i0, j0, g, has_u, has_v);
+    haar_pm_tile2m<T, LD>(levels, hm, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
+}
+
+// p[k] where ok, else dflt, with the load issued unconditionally (at index 0
+// where !ok; every array here holds at least one element, the grid being at
+// least 2 x 2): a guarded load becomes a branch with a wait of its own, so
+// that a launch's loads would go out one round trip after another.
+template <typename V>
+__device__ __forceinline__ V ld_or(bool ok, const V *p, int k, V dflt)
+{
+    const V v = p[ok ? k : 0];
+    return ok ? v : dflt;
 }
 
 // ===========================================================================
@@ -1169,24 +1217,26 @@ __device__ __forceinline__ void cell_prefetch_at(const GridDims &g, const OpCoef
     const int nx = g.nx, ny = g.ny, nxu = g.nxu;
     cc.cell = in_window && a >= 0 && a < nx && b >= 0 && b < ny;
     cc.corner = in_window && a >= 0 && a < nx - 1 && b >= 0 && b < ny - 1;
-    if (cc.cell) {
+    // loads issued unconditionally (ld_or); absent entries are never read
+    {
+        const bool ok = cc.cell;
         const int k = a + nx * b, ku0 = a + nxu * b, kv0 = a + nx * b, kv1 = kv0 + nx;
-        cc.hm = c.hm[k];
-        cc.heta = c.heta_h[k];
-        cc.dimp = c.dimp[k];
-        cc.um0 = c.um[ku0];
-        cc.um1 = c.um[ku0 + 1];
-        cc.hu0 = c.hu[ku0];
-        cc.hu1 = c.hu[ku0 + 1];
-        cc.vm0 = c.vm[kv0];
-        cc.vm1 = c.vm[kv1];
-        cc.hv0 = c.hv[kv0];
-        cc.hv1 = c.hv[kv1];
+        cc.hm = ld_or(ok, c.hm, k, (uint8_t)0);
+        cc.heta = ld_or(ok, c.heta_h, k, T(0));
+        cc.dimp = ld_or(ok, c.dimp, k, T(0));
+        cc.um0 = ld_or(ok, c.um, ku0, (uint8_t)0);
+        cc.um1 = ld_or(ok, c.um, ku0 + 1, (uint8_t)0);
+        cc.hu0 = ld_or(ok, c.hu, ku0, T(0));
+        cc.hu1 = ld_or(ok, c.hu, ku0 + 1, T(0));
+        cc.vm0 = ld_or(ok, c.vm, kv0, (uint8_t)0);
+        cc.vm1 = ld_or(ok, c.vm, kv1, (uint8_t)0);
+        cc.hv0 = ld_or(ok, c.hv, kv0, T(0));
+        cc.hv1 = ld_or(ok, c.hv, kv1, T(0));
     }
-    if (cc.corner) {
+    {
         const int kc = a + (nx - 1) * b;
-        cc.cm = c.cm[kc];
-        cc.hetac = c.heta_c[kc];
+        cc.cm = ld_or(cc.corner, c.cm, kc, (uint8_t)0);
+        cc.hetac = ld_or(cc.corner, c.heta_c, kc, T(0));
     }
 }
 
@@ -1254,22 +1304,17 @@ __device__ __forceinline__ void point_prefetch(const GridDims &g, const OpCoef<T
     pc.okv = has_v && i < g.nxv && j < g.nyv;
     pc.ku = i + g.nxu * j;
     pc.kv = i + g.nxv * j;
-    pc.ui = pc.um = pc.vi = pc.vm = pc.mwu = pc.mwv = false;
-    pc.bu = pc.hu = pc.bv = pc.hv = T(0);
-    if (pc.oku) {
-        pc.ui = c.ui[pc.ku];
-        pc.um = c.um[pc.ku];
-        pc.bu = c.bu[pc.ku];
-        pc.hu = c.hu[pc.ku];
-        pc.mwu = mwu[pc.ku];
-    }
-    if (pc.okv) {
-        pc.vi = c.vi[pc.kv];
-        pc.vm = c.vm[pc.kv];
-        pc.bv = c.bv[pc.kv];
-        pc.hv = c.hv[pc.kv];
-        pc.mwv = mwv[pc.kv];
-    }
+    // loads issued unconditionally (ld_or)
+    pc.ui = ld_or(pc.oku, c.ui, pc.ku, (uint8_t)0) != 0;
+    pc.um = ld_or(pc.oku, c.um, pc.ku, (uint8_t)0) != 0;
+    pc.bu = ld_or(pc.oku, c.bu, pc.ku, T(0));
+    pc.hu = ld_or(pc.oku, c.hu, pc.ku, T(0));
+    pc.mwu = ld_or(pc.oku, mwu, pc.ku, (uint8_t)0) != 0;
+    pc.vi = ld_or(pc.okv, c.vi, pc.kv, (uint8_t)0) != 0;
+    pc.vm = ld_or(pc.okv, c.vm, pc.kv, (uint8_t)0) != 0;
+    pc.bv = ld_or(pc.okv, c.bv, pc.kv, T(0));
+    pc.hv = ld_or(pc.okv, c.hv, pc.kv, T(0));
+    pc.mwv = ld_or(pc.okv, mwv, pc.kv, (uint8_t)0) != 0;
 }
 
 // fx at the u point (i0 + x, j0 + y), whose value is s.U[y + 1][x + 1] and
@@ -1309,14 +1354,14 @@ __device__ __forceinline__ void optile_fetch(const GridDims &g, const T *U, cons
     const int q0 = threadIdx.x, q1 = threadIdx.x + 256;
     {
         const int x = q0 % 18, y = q0 / 18, i = i0 - 1 + x, j = j0 - 1 + y;
-        r.u0 = (i >= 0 && i < g.nxu && j >= 0 && j < g.nyu) ? U[i + g.nxu * j] : T(0);
-        r.v0 = (i >= 0 && i < g.nxv && j >= 0 && j < g.nyv) ? V[i + g.nxv * j] : T(0);
+        r.u0 = ld_or(i >= 0 && i < g.nxu && j >= 0 && j < g.nyu, U, i + g.nxu * j, T(0));
+        r.v0 = ld_or(i >= 0 && i < g.nxv && j >= 0 && j < g.nyv, V, i + g.nxv * j, T(0));
     }
-    r.u1 = r.v1 = T(0);
-    if (q1 < 324) {
+    {
         const int x = q1 % 18, y = q1 / 18, i = i0 - 1 + x, j = j0 - 1 + y;
-        r.u1 = (i >= 0 && i < g.nxu && j >= 0 && j < g.nyu) ? U[i + g.nxu * j] : T(0);
-        r.v1 = (i >= 0 && i < g.nxv && j >= 0 && j < g.nyv) ? V[i + g.nxv * j] : T(0);
+        const bool in = q1 < 324;
+        r.u1 = ld_or(in && i >= 0 && i < g.nxu && j >= 0 && j < g.nyu, U, i + g.nxu * j, T(0));
+        r.v1 = ld_or(in && i >= 0 && i < g.nxv && j >= 0 && j < g.nyv, V, i + g.nxv * j, T(0));
     }
 }
 
@@ -2002,8 +2047,9 @@ __device__ __forceinline__ void tile_apply_restrict(const CGArgs<T> &a, int i0, 
                                                     const PointCoef<T> &pc, OpTile<T> &s, T (*tu)[TILE + 1],
                                                     T (*mu)[TILE + 1], T (*tv)[TILE + 1], T (*mv)[TILE + 1])
 {
+    const HaarRowMasks hm = haar_row_masks(a.levels, 1, i0, j0, a.g, has_u, has_v);
     optile_apply(a.g, a.c, a.tu, a.tv, i0, j0, pc, s, tu, tv);
-    haar_pm_tile2<T, LD>(a.levels, 1, tu, mu, tv, mv, i0, j0, a.g, has_u, has_v);
+    haar_pm_tile2m<T, LD>(a.levels, hm, tu, mu, tv, mv, i0, j0, a.g, has_u, has_v);
 }
 
 // setup (t = P x0 computed beforehand): r = b - R A t (b arrives in r),
@@ -2097,14 +2143,15 @@ __device__ __forceinline__ bool cg_dir_tile(const CGArgs<T> &a, int par, const i
     // independent loads first
     const int it = a.st->it[par];
     const double told = a.st->tol, prev_res = a.st->res[1 - par];
-    const T ru = oku ? a.ru[ku] : T(0), pu = oku ? a.pu[ku] : T(0);
-    const T rv = okv ? a.rv[kv] : T(0), pv = okv ? a.pv[kv] : T(0);
-    const bool mwu = oku && a.mwu[ku], mwv = okv && a.mwv[kv];
-    const bool iu = oku && a.ui[ku], iv = okv && a.vi[kv];
+    const T ru = ld_or(oku, a.ru, ku, T(0)), pu = ld_or(oku, a.pu, ku, T(0));
+    const T rv = ld_or(okv, a.rv, kv, T(0)), pv = ld_or(okv, a.pv, kv, T(0));
+    const bool mwu = ld_or(oku, a.mwu, ku, (uint8_t)0) != 0, mwv = ld_or(okv, a.mwv, kv, (uint8_t)0) != 0;
+    const bool iu = ld_or(oku, a.ui, ku, (uint8_t)0) != 0, iv = ld_or(okv, a.vi, kv, (uint8_t)0) != 0;
+    const HaarRowMasks hm = haar_row_masks(a.levels, 0, i0, j0, g, has_u, has_v);
     double dzu = 0.0, dzv = 0.0, rz = 0.0, prev_rz = 0.0;
     if constexpr (PRE) {
-        dzu = oku ? a.dinvu[ku] : 0.0;
-        dzv = okv ? a.dinvv[kv] : 0.0;
+        dzu = ld_or(oku, a.dinvu, ku, 0.0);
+        dzv = ld_or(okv, a.dinvv, kv, 0.0);
         prev_rz = a.st->rz[1 - par];
     }
     double res;
@@ -2164,14 +2211,18 @@ __device__ __forceinline__ bool cg_dir_tile(const CGArgs<T> &a, int par, const i
     tu[ly][lx] = mwu ? unu : T(0);
     tv[ly][lx] = mwv ? unv : T(0);
     __syncthreads();
-    haar_pm_tile2<T, LD>(a.levels, 0, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
+    haar_pm_tile2m<T, LD>(a.levels, hm, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
     if (oku) a.tu[ku] = iu ? tu[ly][lx] : T(0);
     if (okv) a.tv[kv] = iv ? tv[ly][lx] : T(0);
     return false;
 }
 
-// part 2: c = R A t and partials of dot(u, c)
-template <typename T, bool LD = false>
+// part 2: c = R A t and partials of dot(u, c).  EARLY: return at once after
+// a stop (written only by earlier kernels).  Otherwise the launch runs and
+// stores in full: after a stop only k_cg_update reads c and the partials, and
+// it changes nothing then; a return on the flag would put its round trip
+// before every load of the launch.
+template <typename T, bool LD = false, bool EARLY = true>
 __device__ __forceinline__ void cg_apply_tile(const CGArgs<T> &a, const int L)
 {
     __shared__ OpTile<T> s;
@@ -2179,7 +2230,7 @@ __device__ __forceinline__ void cg_apply_tile(const CGArgs<T> &a, const int L)
     __shared__ double lds[8];
     const GridDims &g = a.g;
     if (L >= g.nbx * g.nby) return;
-    if (a.st->done) return;  // written only by earlier kernels
+    if (EARLY && a.st->done) return;
     int bx, by;
     const int tile_id = band_tile(g, L, bx, by);
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
@@ -2187,7 +2238,7 @@ __device__ __forceinline__ void cg_apply_tile(const CGArgs<T> &a, const int L)
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     PointCoef<T> pc;
     point_prefetch(g, a.c, a.mwu, a.mwv, i0 + lx, j0 + ly, has_u, has_v, pc);
-    const T pu = pc.oku ? a.pu[pc.ku] : T(0), pv = pc.okv ? a.pv[pc.kv] : T(0);
+    const T pu = ld_or(pc.oku, a.pu, pc.ku, T(0)), pv = ld_or(pc.okv, a.pv, pc.kv, T(0));
     tile_apply_restrict<T, LD>(a, i0, j0, has_u, has_v, pc, s, tu, mu, tv, mv);
     double du = 0.0, dv = 0.0;
     if (pc.oku) {
@@ -2209,14 +2260,18 @@ __device__ __forceinline__ void cg_apply_tile(const CGArgs<T> &a, const int L)
 
 // part 3: alpha = res^2 / dot(u, c) (PRE: rz / dot(u, c)), x += alpha u,
 // r -= alpha c, partials of ||r||^2 (PRE: and of (r, D^-1 r)) and the
-// iteration count
-template <typename T, bool PRE>
+// iteration count.  EARLY: return at once after a stop.  Otherwise a launch
+// after a stop stores x, r, their partials and the count unchanged (selected
+// by the flag, which is then needed only at the stores), so that the flag's
+// round trip does not precede the loads.
+template <typename T, bool PRE, bool EARLY = true>
 __device__ __forceinline__ void cg_update_tile(const CGArgs<T> &a, int par, const int L)
 {
     __shared__ double lds[8];
     const GridDims &g = a.g;
     if (L >= g.nbx * g.nby) return;
-    if (a.st->done) return;  // written only by earlier kernels
+    if (EARLY && a.st->done) return;
+    const bool stopped = !EARLY && a.st->done != 0;
     int bx, by;
     const int tile_id = band_tile(g, L, bx, by);
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
@@ -2226,12 +2281,14 @@ __device__ __forceinline__ void cg_update_tile(const CGArgs<T> &a, int par, cons
     const int ku = i + g.nxu * j, kv = i + g.nxv * j;
     const int it = a.st->it[par];
     const double res = a.st->res[par];
-    const T xu = oku ? a.xu[ku] : T(0), pu = oku ? a.pu[ku] : T(0), ru = oku ? a.ru[ku] : T(0), qu = oku ? a.qu[ku] : T(0);
-    const T xv = okv ? a.xv[kv] : T(0), pv = okv ? a.pv[kv] : T(0), rv = okv ? a.rv[kv] : T(0), qv = okv ? a.qv[kv] : T(0);
+    const T xu = ld_or(oku, a.xu, ku, T(0)), pu = ld_or(oku, a.pu, ku, T(0)), ru = ld_or(oku, a.ru, ku, T(0)),
+            qu = ld_or(oku, a.qu, ku, T(0));
+    const T xv = ld_or(okv, a.xv, kv, T(0)), pv = ld_or(okv, a.pv, kv, T(0)), rv = ld_or(okv, a.rv, kv, T(0)),
+            qv = ld_or(okv, a.qv, kv, T(0));
     double dzu = 0.0, dzv = 0.0, rzc = 0.0;
     if constexpr (PRE) {
-        dzu = oku ? a.dinvu[ku] : 0.0;
-        dzv = okv ? a.dinvv[kv] : 0.0;
+        dzu = ld_or(oku, a.dinvu, ku, 0.0);
+        dzv = ld_or(okv, a.dinvv, kv, 0.0);
         rzc = a.st->rz[par];
     }
     const double dot = lanes_sum(a.Pd, g.npart, lds);
@@ -2240,18 +2297,20 @@ __device__ __forceinline__ void cg_update_tile(const CGArgs<T> &a, int par, cons
         alpha = (T)(rzc / dot);
     else
         alpha = (T)((res * res) / dot);
-    if (threadIdx.x == 0) a.st->it[1 - par] = it + 1;
+    if (threadIdx.x == 0) a.st->it[1 - par] = stopped ? it : it + 1;
     double r2u = 0.0, r2v = 0.0, z2u = 0.0, z2v = 0.0;
     if (oku) {
-        a.xu[ku] = xu + alpha * pu;
-        const T r = ru - alpha * qu;
+        const T x = xu + alpha * pu, rn = ru - alpha * qu;
+        a.xu[ku] = stopped ? xu : x;
+        const T r = stopped ? ru : rn;
         a.ru[ku] = r;
         r2u = (double)r * (double)r;
         if constexpr (PRE) z2u = (double)r * (dzu * (double)r);
     }
     if (okv) {
-        a.xv[kv] = xv + alpha * pv;
-        const T r = rv - alpha * qv;
+        const T x = xv + alpha * pv, rn = rv - alpha * qv;
+        a.xv[kv] = stopped ? xv : x;
+        const T r = stopped ? rv : rn;
         a.rv[kv] = r;
         r2v = (double)r * (double)r;
         if constexpr (PRE) z2v = (double)r * (dzv * (double)r);
@@ -2274,16 +2333,16 @@ __global__ void __launch_bounds__(256) k_cg_dir(CGArgs<T> a, int par)
     (void)cg_dir_tile<T, PRE, LD>(a, par, xcd_tile(a.nblk));
 }
 
-template <typename T, bool LD>
+template <typename T, bool LD, bool EARLY>
 __global__ void __launch_bounds__(256) k_cg_apply(CGArgs<T> a)
 {
-    cg_apply_tile<T, LD>(a, xcd_tile(a.nblk));
+    cg_apply_tile<T, LD, EARLY>(a, xcd_tile(a.nblk));
 }
 
-template <typename T, bool PRE>
+template <typename T, bool PRE, bool EARLY>
 __global__ void __launch_bounds__(256) k_cg_update(CGArgs<T> a, int par)
 {
-    cg_update_tile<T, PRE>(a, par, xcd_tile(a.nblk));
+    cg_update_tile<T, PRE, EARLY>(a, par, xcd_tile(a.nblk));
 }
 
 // Diagonal of the coarse-CG operator A_c = M_w R M_i A M_i P M_w
