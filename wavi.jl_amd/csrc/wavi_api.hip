@@ -1297,6 +1297,22 @@ static bool haar_loads_first(const wavi_ctx *ctx)
     return ctx->g.nbx * ctx->g.nby <= HAAR_LD_MAX_TILES;
 }
 
+// A +-1 Haar transform (dir 1: P, 2: R) of the u and v vectors on the tiles,
+// with the Haar rows loads first where haar_loads_first says so.
+template <typename T>
+static void launch_tile_haar(wavi_ctx *ctx, int dir, const T *inu, const T *inv, const uint8_t *min_u,
+                             const uint8_t *min_v, T *outu, T *outv, const uint8_t *mout_u, const uint8_t *mout_v)
+{
+    const GridDims &g = ctx->g;
+    const dim3 tg(g.nbx, g.nby);
+    if (haar_loads_first(ctx))
+        hipLaunchKernelGGL((k_tile_haar<T, T, true>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, dir,
+                           inu, inv, min_u, min_v, outu, outv, mout_u, mout_v);
+    else
+        hipLaunchKernelGGL((k_tile_haar<T, T, false>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, dir,
+                           inu, inv, min_u, min_v, outu, outv, mout_u, mout_v);
+}
+
 // coarse CG on b_c held in w.cr; solution in the warm start w.xc
 template <typename T>
 static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
@@ -1305,8 +1321,7 @@ static int do_coarse_cg(wavi_ctx *ctx, const VSet<T> &w, int *iters)
     dim3 tg(g.nbx, g.nby), tb(ctx->nblk);
     CGArgs<T> a = cg_args(ctx, w);
     // t = P x0
-    hipLaunchKernelGGL((k_tile_haar<T, T>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1, a.xu, a.xv,
-                       ctx->mwu, ctx->mwv, a.tu, a.tv, ctx->ui, ctx->vi);
+    launch_tile_haar<T>(ctx, 1, a.xu, a.xv, ctx->mwu, ctx->mwv, a.tu, a.tv, ctx->ui, ctx->vi);
     ctx->h_flag->done = 0;
     ctx->h_flag->iters = 0;
     const bool pre = ctx->cg_pre;
@@ -1481,8 +1496,7 @@ static int cgm_prepare(wavi_ctx *ctx, const VSet<float> &w)
                        c, ctx->mwu, ctx->mwv, Lo, ctx->cg_pre ? (const double *)ctx->cdinv.u : nullptr,
                        ctx->cg_pre ? (const double *)ctx->cdinv.v : nullptr);
     // P x0
-    hipLaunchKernelGGL((k_tile_haar<float, float>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1,
-                       w.xc.u, w.xc.v, ctx->mwu, ctx->mwv, w.ct.u, w.ct.v, ctx->ui, ctx->vi);
+    launch_tile_haar<float>(ctx, 1, w.xc.u, w.xc.v, ctx->mwu, ctx->mwv, w.ct.u, w.ct.v, ctx->ui, ctx->vi);
     CGMArgs sa;
     sa.g = g;
     sa.c = c;
@@ -1639,8 +1653,7 @@ static int vcycle_pre(wavi_ctx *ctx, const VSet<T> &w)
                        w.e.u, w.e.v, w.r.u, w.r.v, w.rho.u, w.rho.v, ctx->Pn, (double *)nullptr,
                        band_blocks(g.nbx, g.nby));
     // b_c = R rho, into the CG residual buffer
-    hipLaunchKernelGGL((k_tile_haar<T, T>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 2, w.rho.u,
-                       w.rho.v, ctx->ui, ctx->vi, w.cr.u, w.cr.v, ctx->mwu, ctx->mwv);
+    launch_tile_haar<T>(ctx, 2, w.rho.u, w.rho.v, ctx->ui, ctx->vi, w.cr.u, w.cr.v, ctx->mwu, ctx->mwv);
     HIPCHK(hipGetLastError());
     return WAVI_OK;
 }
@@ -1653,8 +1666,7 @@ static int vcycle_post(wavi_ctx *ctx, const VSet<T> &w)
     const GridDims &g = ctx->g;
     dim3 tg(g.nbx, g.nby);
     // e += P x_c
-    hipLaunchKernelGGL((k_tile_haar<T, T>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, 1, w.xc.u,
-                       w.xc.v, ctx->mwu, ctx->mwv, w.t.u, w.t.v, ctx->ui, ctx->vi);
+    launch_tile_haar<T>(ctx, 1, w.xc.u, w.xc.v, ctx->mwu, ctx->mwv, w.t.u, w.t.v, ctx->ui, ctx->vi);
     hipLaunchKernelGGL(k_add_masked_uv<T>, flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream, w.e.u, w.t.u,
                        ctx->ui, ctx->nu, w.e.v, w.t.v, ctx->vi, ctx->nv);
     CHK(do_smoother(ctx, w, rev));

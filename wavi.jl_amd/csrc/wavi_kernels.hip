@@ -17,6 +17,17 @@ namespace wavi {
 // ===========================================================================
 // small utilities
 
+// p[k] where ok, else dflt, with the load issued unconditionally (at index 0
+// where !ok; every array here holds at least one element, the grid being at
+// least 2 x 2): a guarded load becomes a branch with a wait of its own, so
+// that a launch's loads would go out one round trip after another.
+template <typename V>
+__device__ __forceinline__ V ld_or(bool ok, const V *p, int k, V dflt)
+{
+    const V v = p[ok ? k : 0];
+    return ok ? v : dflt;
+}
+
 template <typename T>
 __global__ void k_fill(T *a, T v, int n)
 {
@@ -266,17 +277,13 @@ __global__ void k_surface_haf(GridDims g, GeomArgs a)
     a.haf[k] = h - (a.rw / a.ri) * (a.sl - b);
 }
 
-// pos_fraction quadrant areas (src/utilities.jl:263-360) for one cell.
-__device__ double pos_fraction_quadrant(int M, int N, const double *z1, const uint8_t *mask, int i, int j, int quad)
+// pos_fraction quadrant area (src/utilities.jl:263-360) of one cell from the
+// values at the cell (v1, its mask m1) and at the quadrant's two neighbours
+// (z2, z3 with masks m2, m3; an unmasked neighbour takes the cell's value).
+__device__ __forceinline__ double pos_fraction_area(double v1, bool m1, double z2, bool m2, double z3, bool m3)
 {
-    int i2, j2, i3, j3;
-    if (quad == 1) { i2 = i + 1 < M ? i + 1 : M - 1; j2 = j; i3 = i; j3 = j + 1 < N ? j + 1 : N - 1; }
-    else if (quad == 2) { i2 = i; j2 = j + 1 < N ? j + 1 : N - 1; i3 = i > 0 ? i - 1 : 0; j3 = j; }
-    else if (quad == 3) { i2 = i > 0 ? i - 1 : 0; j2 = j; i3 = i; j3 = j > 0 ? j - 1 : 0; }
-    else { i2 = i; j2 = j > 0 ? j - 1 : 0; i3 = i + 1 < M ? i + 1 : M - 1; j3 = j; }
-    double v1 = z1[i + M * j];
-    double v2 = mask[i2 + M * j2] ? z1[i2 + M * j2] : v1;
-    double v3 = mask[i3 + M * j3] ? z1[i3 + M * j3] : v1;
+    double v2 = m2 ? z2 : v1;
+    double v3 = m3 ? z3 : v1;
     double s1 = (v1 > 0) - (v1 < 0), s2 = (v2 > 0) - (v2 < 0), s3 = (v3 > 0) - (v3 < 0);
     double area = 0.0;
     if (!((s1 == s2) && (s1 == s3))) {
@@ -312,9 +319,10 @@ __device__ double pos_fraction_quadrant(int M, int N, const double *z1, const ui
     }
     if (v1 > 0.0 && v2 > 0.0 && v3 > 0.0) area = 1.0;
     if (v1 <= 0.0 && v2 <= 0.0 && v3 <= 0.0) area = 0.0;
-    if (!mask[i + M * j]) area = 0.0;
+    if (!m1) area = 0.0;
     return area;
 }
+
 
 // update_grounded_fraction (:92-99) on h, then accumulation, melt,
 // weertman_c and dsdh (:106-147)
@@ -324,16 +332,28 @@ __global__ void k_gf_h(GridDims g, GeomArgs a)
     const int nh = g.nx * g.ny;
     if (k >= nh) return;
     int i = k % g.nx, j = k / g.nx;
+    // the cell and its four neighbours (clamped), every load before the
+    // first store
+    const int M = g.nx, N = g.ny;
+    const int kE = (i + 1 < M ? i + 1 : M - 1) + M * j, kW = (i > 0 ? i - 1 : 0) + M * j;
+    const int kN = i + M * (j + 1 < N ? j + 1 : N - 1), kS = i + M * (j > 0 ? j - 1 : 0);
+    const double zc = a.haf[k], zE = a.haf[kE], zN = a.haf[kN], zW = a.haf[kW], zS = a.haf[kS];
+    const bool mc = a.hm[k], mE = a.hm[kE], mN = a.hm[kN], mW = a.hm[kW], mS = a.hm[kS];
+    const double accp = a.acc_param[k], wcp = a.wc_param[k];
+    // quadrants 1..4 take points (2, 3) = (E, N), (N, W), (W, S), (S, E),
+    // each clamped to the cell at the grid's edge (utilities.jl:263-360)
+    const double q[4] = {pos_fraction_area(zc, mc, zE, mE, zN, mN), pos_fraction_area(zc, mc, zN, mN, zW, mW),
+                         pos_fraction_area(zc, mc, zW, mW, zS, mS), pos_fraction_area(zc, mc, zS, mS, zE, mE)};
     double af = 0.0;
+#pragma unroll
     for (int quad = 1; quad <= 4; ++quad) {
-        double q = pos_fraction_quadrant(g.nx, g.ny, a.haf, a.hm, i, j, quad);
-        a.q[(quad - 1) * nh + k] = q;
-        af = af + 0.25 * q;
+        a.q[(quad - 1) * nh + k] = q[quad - 1];
+        af = af + 0.25 * q[quad - 1];
     }
     a.gfh[k] = af;
-    a.acc[k] = a.acc_param[k];
+    a.acc[k] = accp;
     a.mel[k] = a.melt;
-    a.wc[k] = a.wc_param[k] * af;
+    a.wc[k] = wcp * af;
     a.dsdh[k] = (1.0 - a.ri / a.rw) + (a.ri / a.rw) * af;
 }
 
@@ -344,66 +364,44 @@ __global__ void k_geom_uv(GridDims g, GeomArgs a)
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nu = g.nxu * g.nyu, nv = g.nxv * g.nyv, nh = g.nx * g.ny;
     const int nx = g.nx, ny = g.ny;
-    if (t < nu) {
-        int i = t % g.nxu, j = t / g.nxu;
-        // grounded fraction on u: quadrants 1,4 from cell i-1, 2,3 from cell i
-        double gf = 0.0;
-        double q1 = i >= 1 ? a.q[0 * nh + (i - 1) + nx * j] : 0.0;
-        double q2 = i < nx ? a.q[1 * nh + i + nx * j] : 0.0;
-        double q3 = i < nx ? a.q[2 * nh + i + nx * j] : 0.0;
-        double q4 = i >= 1 ? a.q[3 * nh + (i - 1) + nx * j] : 0.0;
-        gf = gf + 0.25 * q1;
-        gf = gf + 0.25 * q2;
-        gf = gf + 0.25 * q3;
-        gf = gf + 0.25 * q4;
-        a.gfu[t] = gf;
-        if (a.um[t]) {
-            double den = 0.0, nh_ = 0.0, ns_ = 0.0;
-            if (i >= 1) {
-                int kk = (i - 1) + nx * j;
-                den = den + 0.5 * (double)a.hm[kk];
-                nh_ = nh_ + 0.5 * (a.hm[kk] ? a.h[kk] : 0.0);
-                ns_ = ns_ + 0.5 * (a.hm[kk] ? a.s[kk] : 0.0);
-            }
-            if (i <= nx - 1) {
-                int kk = i + nx * j;
-                den = den + 0.5 * (double)a.hm[kk];
-                nh_ = nh_ + 0.5 * (a.hm[kk] ? a.h[kk] : 0.0);
-                ns_ = ns_ + 0.5 * (a.hm[kk] ? a.s[kk] : 0.0);
-            }
-            a.hu[t] = nh_ / den;
-            a.su[t] = ns_ / den;
+    if (t >= nu + nv) return;
+    // the face between h cells ka (left / below) and kb; every load issued
+    // before the first is used (ld_or), the sums in the same order as before
+    const bool v = t >= nu;
+    const int tt = v ? t - nu : t;
+    const int i = v ? tt % g.nxv : tt % g.nxu, j = v ? tt / g.nxv : tt / g.nxu;
+    const bool oka = v ? j >= 1 : i >= 1, okb = v ? j < ny : i < nx;
+    const int ka = v ? i + nx * (j - 1) : (i - 1) + nx * j, kb = i + nx * j;
+    // grounded fraction: on u quadrants 1, 4 from cell i - 1 and 2, 3 from
+    // cell i; on v quadrants 1, 2 from cell j - 1 and 3, 4 from cell j
+    const double q1 = ld_or(oka, a.q, ka, 0.0);
+    const double q2 = v ? ld_or(oka, a.q, nh + ka, 0.0) : ld_or(okb, a.q, nh + kb, 0.0);
+    const double q3 = ld_or(okb, a.q, 2 * nh + kb, 0.0);
+    const double q4 = v ? ld_or(okb, a.q, 3 * nh + kb, 0.0) : ld_or(oka, a.q, 3 * nh + ka, 0.0);
+    const bool fm = (v ? a.vm : a.um)[tt];
+    const uint8_t ha = ld_or(oka, a.hm, ka, (uint8_t)0), hb = ld_or(okb, a.hm, kb, (uint8_t)0);
+    const double hha = ld_or(oka, a.h, ka, 0.0), hhb = ld_or(okb, a.h, kb, 0.0);
+    const double sa = ld_or(oka, a.s, ka, 0.0), sb = ld_or(okb, a.s, kb, 0.0);
+    double gf = 0.0;
+    gf = gf + 0.25 * q1;
+    gf = gf + 0.25 * q2;
+    gf = gf + 0.25 * q3;
+    gf = gf + 0.25 * q4;
+    (v ? a.gfv : a.gfu)[tt] = gf;
+    if (fm) {
+        double den = 0.0, nh_ = 0.0, ns_ = 0.0;
+        if (oka) {
+            den = den + 0.5 * (double)ha;
+            nh_ = nh_ + 0.5 * (ha ? hha : 0.0);
+            ns_ = ns_ + 0.5 * (ha ? sa : 0.0);
         }
-    } else if (t < nu + nv) {
-        int tv = t - nu;
-        int i = tv % g.nxv, j = tv / g.nxv;
-        double gf = 0.0;
-        double q1 = j >= 1 ? a.q[0 * nh + i + nx * (j - 1)] : 0.0;
-        double q2 = j >= 1 ? a.q[1 * nh + i + nx * (j - 1)] : 0.0;
-        double q3 = j < ny ? a.q[2 * nh + i + nx * j] : 0.0;
-        double q4 = j < ny ? a.q[3 * nh + i + nx * j] : 0.0;
-        gf = gf + 0.25 * q1;
-        gf = gf + 0.25 * q2;
-        gf = gf + 0.25 * q3;
-        gf = gf + 0.25 * q4;
-        a.gfv[tv] = gf;
-        if (a.vm[tv]) {
-            double den = 0.0, nh_ = 0.0, ns_ = 0.0;
-            if (j >= 1) {
-                int kk = i + nx * (j - 1);
-                den = den + 0.5 * (double)a.hm[kk];
-                nh_ = nh_ + 0.5 * (a.hm[kk] ? a.h[kk] : 0.0);
-                ns_ = ns_ + 0.5 * (a.hm[kk] ? a.s[kk] : 0.0);
-            }
-            if (j <= ny - 1) {
-                int kk = i + nx * j;
-                den = den + 0.5 * (double)a.hm[kk];
-                nh_ = nh_ + 0.5 * (a.hm[kk] ? a.h[kk] : 0.0);
-                ns_ = ns_ + 0.5 * (a.hm[kk] ? a.s[kk] : 0.0);
-            }
-            a.hv[tv] = nh_ / den;
-            a.sv[tv] = ns_ / den;
+        if (okb) {
+            den = den + 0.5 * (double)hb;
+            nh_ = nh_ + 0.5 * (hb ? hhb : 0.0);
+            ns_ = ns_ + 0.5 * (hb ? sb : 0.0);
         }
+        (v ? a.hv : a.hu)[tt] = nh_ / den;
+        (v ? a.sv : a.su)[tt] = ns_ / den;
     }
 }
 
@@ -427,47 +425,55 @@ struct InnerArgs {
 __device__ __forceinline__ double corner_shear(const GridDims &g, const InnerArgs &p, int a, int b)
 {
     const int nxc = g.nx - 1, nyc = g.ny - 1;
-    if (a < 0 || a >= nxc || b < 0 || b >= nyc) return 0.0;
-    if (!p.cm[a + nxc * b]) return 0.0;
+    const bool in = a >= 0 && a < nxc && b >= 0 && b < nyc;
     const int nxu = g.nxu, nx = g.nx;
-    double u1 = p.um[(a + 1) + nxu * (b + 1)] ? p.U[(a + 1) + nxu * (b + 1)] : 0.0;
-    double u0 = p.um[(a + 1) + nxu * b] ? p.U[(a + 1) + nxu * b] : 0.0;
-    double v1 = p.vm[(a + 1) + nx * (b + 1)] ? p.V[(a + 1) + nx * (b + 1)] : 0.0;
-    double v0 = p.vm[a + nx * (b + 1)] ? p.V[a + nx * (b + 1)] : 0.0;
-    double dudy = p.rdy * u1 - p.rdy * u0;
-    double dvdx = p.rdx * v1 - p.rdx * v0;
-    return dudy + dvdx;
+    // every load issued at once (ld_or), values selected
+    const int ku1 = (a + 1) + nxu * (b + 1), ku0 = (a + 1) + nxu * b;
+    const int kv1 = (a + 1) + nx * (b + 1), kv0 = a + nx * (b + 1);
+    const bool cm = ld_or(in, p.cm, a + nxc * b, (uint8_t)0) != 0;
+    const bool mu1 = ld_or(in, p.um, ku1, (uint8_t)0) != 0, mu0 = ld_or(in, p.um, ku0, (uint8_t)0) != 0;
+    const bool mv1 = ld_or(in, p.vm, kv1, (uint8_t)0) != 0, mv0 = ld_or(in, p.vm, kv0, (uint8_t)0) != 0;
+    const double U1 = ld_or(in, p.U, ku1, 0.0), U0 = ld_or(in, p.U, ku0, 0.0);
+    const double V1 = ld_or(in, p.V, kv1, 0.0), V0 = ld_or(in, p.V, kv0, 0.0);
+    const double u1 = mu1 ? U1 : 0.0, u0 = mu0 ? U0 : 0.0;
+    const double v1 = mv1 ? V1 : 0.0, v0 = mv0 ? V0 : 0.0;
+    const double dudy = p.rdy * u1 - p.rdy * u0;
+    const double dvdx = p.rdx * v1 - p.rdx * v0;
+    return (in && cm) ? dudy + dvdx : 0.0;
 }
 
 // gc.cent (CGrid.jl:45) at h cell (i, j) in the reference's factor order
 __device__ __forceinline__ double cent_c_to_h(const GridDims &g, const InnerArgs &p, int i, int j)
 {
     const int nxc = g.nx - 1, nyc = g.ny - 1;
+    // the four corners' loads first, then the sums in the reference's order
+    const double s00 = corner_shear(g, p, i - 1, j - 1), s10 = corner_shear(g, p, i, j - 1);
+    const double s01 = corner_shear(g, p, i - 1, j), s11 = corner_shear(g, p, i, j);
     double out = 0.0;
     if (p.xfirst_c2h) {
         if (j - 1 >= 0 && j - 1 < nyc) {
             double t0 = 0.0;
-            if (i - 1 >= 0 && i - 1 < nxc) t0 = t0 + 0.5 * corner_shear(g, p, i - 1, j - 1);
-            if (i >= 0 && i < nxc) t0 = t0 + 0.5 * corner_shear(g, p, i, j - 1);
+            if (i - 1 >= 0 && i - 1 < nxc) t0 = t0 + 0.5 * s00;
+            if (i >= 0 && i < nxc) t0 = t0 + 0.5 * s10;
             out = out + 0.5 * t0;
         }
         if (j >= 0 && j < nyc) {
             double t1 = 0.0;
-            if (i - 1 >= 0 && i - 1 < nxc) t1 = t1 + 0.5 * corner_shear(g, p, i - 1, j);
-            if (i >= 0 && i < nxc) t1 = t1 + 0.5 * corner_shear(g, p, i, j);
+            if (i - 1 >= 0 && i - 1 < nxc) t1 = t1 + 0.5 * s01;
+            if (i >= 0 && i < nxc) t1 = t1 + 0.5 * s11;
             out = out + 0.5 * t1;
         }
     } else {
         if (i - 1 >= 0 && i - 1 < nxc) {
             double t0 = 0.0;
-            if (j - 1 >= 0 && j - 1 < nyc) t0 = t0 + 0.5 * corner_shear(g, p, i - 1, j - 1);
-            if (j >= 0 && j < nyc) t0 = t0 + 0.5 * corner_shear(g, p, i - 1, j);
+            if (j - 1 >= 0 && j - 1 < nyc) t0 = t0 + 0.5 * s00;
+            if (j >= 0 && j < nyc) t0 = t0 + 0.5 * s01;
             out = out + 0.5 * t0;
         }
         if (i >= 0 && i < nxc) {
             double t1 = 0.0;
-            if (j - 1 >= 0 && j - 1 < nyc) t1 = t1 + 0.5 * corner_shear(g, p, i, j - 1);
-            if (j >= 0 && j < nyc) t1 = t1 + 0.5 * corner_shear(g, p, i, j);
+            if (j - 1 >= 0 && j - 1 < nyc) t1 = t1 + 0.5 * s10;
+            if (j >= 0 && j < nyc) t1 = t1 + 0.5 * s11;
             out = out + 0.5 * t1;
         }
     }
@@ -477,6 +483,8 @@ __device__ __forceinline__ double cent_c_to_h(const GridDims &g, const InnerArgs
 // All h-cell updates of inner_update!: strain rate, speeds, beta, basal drag,
 // viscosity on the sigma levels, quadratures, beta_eff and the h-grid
 // rheological coefficients.
+constexpr int INNER_NS_PREFETCH = 8;  // sigma levels whose eta and glen_b k_inner_h loads up front
+
 __global__ void k_inner_h(GridDims g, InnerArgs p)
 {
     const int nh = g.nx * g.ny;
@@ -484,19 +492,37 @@ __global__ void k_inner_h(GridDims g, InnerArgs p)
     if (k >= nh) return;
     const int i = k % g.nx, j = k / g.nx;
     const int nxu = g.nxu, nx = g.nx;
+    // every load of the cell issued before the first is used: masks with
+    // their fields (the values selected), the shear's corners, and eta and
+    // glen_b on up to INNER_NS_PREFETCH levels
     const bool hm = p.hm[k];
+    const double Ua = p.U[i + nxu * j], Ub = p.U[(i + 1) + nxu * j];
+    const double Va = p.V[i + nx * j], Vb = p.V[i + nx * (j + 1)];
+    const bool ma = p.um[i + nxu * j], mb = p.um[(i + 1) + nxu * j];
+    const bool na = p.vm[i + nx * j], nb = p.vm[i + nx * (j + 1)];
+    const double shc = cent_c_to_h(g, p, i, j);
+    double eta_k[INNER_NS_PREFETCH], gb_k[INNER_NS_PREFETCH], zeta_k[INNER_NS_PREFETCH], qw_k[INNER_NS_PREFETCH];
+#pragma unroll
+    for (int ks = 0; ks < INNER_NS_PREFETCH; ++ks) {
+        eta_k[ks] = ld_or(ks < p.ns, p.eta, k + nh * ks, 0.0);
+        gb_k[ks] = ld_or(ks < p.ns, p.glen_b, k + nh * ks, 0.0);
+        zeta_k[ks] = ld_or(ks < p.ns, p.zeta, ks, 0.0);
+        qw_k[ks] = ld_or(ks < p.ns, p.qw, ks, 0.0);
+    }
+    // (and before the first store, which the compiler cannot move them past)
+    const double f2_old = p.f2[k], beta_old = p.beta[k], wck = p.wc[k], hk = p.h[k], dsdhk = p.dsdh[k];
     // update_shelf_strain_rate! (:202-209)
     double ux = 0.0, vy = 0.0, sh = 0.0;
     double ub = 0.0, vb = 0.0;
     {
-        double u0 = p.um[i + nxu * j] ? p.U[i + nxu * j] : 0.0;
-        double u1 = p.um[(i + 1) + nxu * j] ? p.U[(i + 1) + nxu * j] : 0.0;
-        double v0 = p.vm[i + nx * j] ? p.V[i + nx * j] : 0.0;
-        double v1 = p.vm[i + nx * (j + 1)] ? p.V[i + nx * (j + 1)] : 0.0;
+        double u0 = ma ? Ua : 0.0;
+        double u1 = mb ? Ub : 0.0;
+        double v0 = na ? Va : 0.0;
+        double v1 = nb ? Vb : 0.0;
         if (hm) {
             ux = p.rdx * u1 - p.rdx * u0;
             vy = p.rdy * v1 - p.rdy * v0;
-            sh = cent_c_to_h(g, p, i, j);
+            sh = shc;
             ub = 0.5 * u0 + 0.5 * u1;
             vb = 0.5 * v0 + 0.5 * v1;
         }
@@ -507,33 +533,33 @@ __global__ void k_inner_h(GridDims g, InnerArgs p)
     double avs = sqrt(ub * ub + vb * vb);
     p.avs[k] = avs;
     // update_bed_speed!, update_β!, update_basal_drag! (:227-255)
-    double f2_old = p.f2[k], beta_old = p.beta[k];
     double bs = avs / (1.0 + f2_old * beta_old);
-    double beta = p.wc[k] * det_pow(sqrt(bs * bs + p.wreg2), p.wexp);
+    double beta = wck * det_pow(sqrt(bs * bs + p.wreg2), p.wexp);
     double tau = beta * bs;
     p.bs[k] = bs;
     p.beta[k] = beta;
     p.tau[k] = tau;
     // inner_update_viscosity!, update_av_viscosity!, update_quadrature_falpha! (:264-328)
     double av = 0.0, q1 = 0.0, q2 = 0.0;
-    const double hk = p.h[k];
     if (hm) {
-        for (int ks = 0; ks < p.ns; ++ks) {
-            const int k3 = k + nh * ks;
-            double eta = p.eta[k3];
-            const double zeta = p.zeta[ks];
-            const double gb = p.glen_b[k3];
+        // level ks from its eta, glen_b, zeta and weight (the levels in
+        // increasing order: the prefetched ones unrolled, then any others)
+        auto level = [&](int ks, double eta, double gb, double zeta, double w) {
             for (int it = 0; it < p.niter_visc; ++it) {
                 double q = (tau * zeta) / eta;
                 double arg = sqrt((sr * sr + 0.25 * (q * q)) + p.greg2);
                 eta = (0.5 * gb) * det_pow(arg, p.gexp);
             }
-            p.eta[k3] = eta;
-            const double w = p.qw[ks];
+            p.eta[k + nh * ks] = eta;
             av = av + w * eta;
             q1 = q1 + ((w * hk) * zeta) / eta;
             q2 = q2 + ((w * hk) * (zeta * zeta)) / eta;
-        }
+        };
+#pragma unroll
+        for (int ks = 0; ks < INNER_NS_PREFETCH; ++ks)
+            if (ks < p.ns) level(ks, eta_k[ks], gb_k[ks], zeta_k[ks], qw_k[ks]);
+        for (int ks = INNER_NS_PREFETCH; ks < p.ns; ++ks)
+            level(ks, p.eta[k + nh * ks], p.glen_b[k + nh * ks], p.zeta[ks], p.qw[ks]);
     }
     p.etaav[k] = av;
     p.f1[k] = q1;
@@ -542,7 +568,7 @@ __global__ void k_inner_h(GridDims g, InnerArgs p)
     p.beff[k] = beta / (1.0 + q2 * beta);
     // update_rheological_operators! (:374-383), h-grid parts
     p.heta_h[k] = hm ? hk * av : 0.0;
-    p.dimp[k] = hm ? p.c0 * p.dsdh[k] : 0.0;
+    p.dimp[k] = hm ? p.c0 * dsdhk : 0.0;
 }
 
 // update_βeff_on_uv_grids! (:348-366) and the c-grid coefficient
@@ -552,59 +578,51 @@ __global__ void k_inner_uvc(GridDims g, InnerArgs p)
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nu = g.nxu * g.nyu, nv = g.nxv * g.nyv, nc = (g.nx - 1) * (g.ny - 1);
     const int nx = g.nx;
-    if (t < nu) {
-        if (!p.um[t]) return;
-        int i = t % g.nxu, j = t / g.nxu;
+    if (t < nu + nv) {
+        // the face between h cells ka (left / below) and kb; every load
+        // issued before the first is used (ld_or), the sums as before
+        const bool v = t >= nu;
+        const int tt = v ? t - nu : t;
+        const int i = v ? tt % g.nxv : tt % g.nxu, j = v ? tt / g.nxv : tt / g.nxu;
+        const bool oka = v ? j >= 1 : i >= 1, okb = v ? j <= g.ny - 1 : i <= nx - 1;
+        const int ka = v ? i + nx * (j - 1) : (i - 1) + nx * j, kb = i + nx * j;
+        const bool fm = (v ? p.vm : p.um)[tt];
+        const double gff = (v ? p.gfv : p.gfu)[tt];
+        const uint8_t ha = ld_or(oka, p.hm, ka, (uint8_t)0), hb = ld_or(okb, p.hm, kb, (uint8_t)0);
+        const double ba = ld_or(oka, p.beff, ka, 0.0), bb = ld_or(okb, p.beff, kb, 0.0);
+        const double ga = ld_or(oka, p.gfh, ka, 0.0), gb = ld_or(okb, p.gfh, kb, 0.0);
+        if (!fm) return;
         double den = 0.0, nb = 0.0, ng = 0.0;
-        if (i >= 1) {
-            int kk = (i - 1) + nx * j;
-            den = den + 0.5 * (double)p.hm[kk];
-            nb = nb + 0.5 * (p.hm[kk] ? p.beff[kk] : 0.0);
-            ng = ng + 0.5 * (p.hm[kk] ? p.gfh[kk] : 0.0);
+        if (oka) {
+            den = den + 0.5 * (double)ha;
+            nb = nb + 0.5 * (ha ? ba : 0.0);
+            ng = ng + 0.5 * (ha ? ga : 0.0);
         }
-        if (i <= nx - 1) {
-            int kk = i + nx * j;
-            den = den + 0.5 * (double)p.hm[kk];
-            nb = nb + 0.5 * (p.hm[kk] ? p.beff[kk] : 0.0);
-            ng = ng + 0.5 * (p.hm[kk] ? p.gfh[kk] : 0.0);
+        if (okb) {
+            den = den + 0.5 * (double)hb;
+            nb = nb + 0.5 * (hb ? bb : 0.0);
+            ng = ng + 0.5 * (hb ? gb : 0.0);
         }
         double b = nb / den;
         double ip = ng / den;
-        if (ip > 0.0) b = (b * p.gfu[t]) / ip;
-        p.bu[t] = b;
-    } else if (t < nu + nv) {
-        int tv = t - nu;
-        if (!p.vm[tv]) return;
-        int i = tv % g.nxv, j = tv / g.nxv;
-        double den = 0.0, nb = 0.0, ng = 0.0;
-        if (j >= 1) {
-            int kk = i + nx * (j - 1);
-            den = den + 0.5 * (double)p.hm[kk];
-            nb = nb + 0.5 * (p.hm[kk] ? p.beff[kk] : 0.0);
-            ng = ng + 0.5 * (p.hm[kk] ? p.gfh[kk] : 0.0);
-        }
-        if (j <= g.ny - 1) {
-            int kk = i + nx * j;
-            den = den + 0.5 * (double)p.hm[kk];
-            nb = nb + 0.5 * (p.hm[kk] ? p.beff[kk] : 0.0);
-            ng = ng + 0.5 * (p.hm[kk] ? p.gfh[kk] : 0.0);
-        }
-        double b = nb / den;
-        double ip = ng / den;
-        if (ip > 0.0) b = (b * p.gfv[tv]) / ip;
-        p.bv[tv] = b;
+        if (ip > 0.0) b = (b * gff) / ip;
+        (v ? p.bv : p.bu)[tt] = b;
     } else if (t < nu + nv + nc) {
         int tc = t - nu - nv;
         int i = tc % (nx - 1), j = tc / (nx - 1);
+        // the four cells' h and eta_av loaded with the corner's mask
+        const int k00 = i + nx * j, k10 = k00 + 1, k01 = k00 + nx, k11 = k01 + 1;
+        const double h00 = p.h[k00], h10 = p.h[k10], h01 = p.h[k01], h11 = p.h[k11];
+        const double e00 = p.etaav[k00], e10 = p.etaav[k10], e01 = p.etaav[k01], e11 = p.etaav[k11];
         if (!p.cm[tc]) {
             p.heta_c[tc] = 0.0;
             return;
         }
         // gh.cent_xy (HGrid.jl:203) of h*eta_av in the reference's factor order
-        double a00 = p.h[i + nx * j] * p.etaav[i + nx * j];
-        double a10 = p.h[(i + 1) + nx * j] * p.etaav[(i + 1) + nx * j];
-        double a01 = p.h[i + nx * (j + 1)] * p.etaav[i + nx * (j + 1)];
-        double a11 = p.h[(i + 1) + nx * (j + 1)] * p.etaav[(i + 1) + nx * (j + 1)];
+        double a00 = h00 * e00;
+        double a10 = h10 * e10;
+        double a01 = h01 * e01;
+        double a11 = h11 * e11;
         double out;
         if (p.yfirst_h2c) {
             double t0 = 0.5 * a00 + 0.5 * a01;
@@ -842,52 +860,47 @@ struct RhsArgs {
     double rdx, rdy, g, ri, rw, sl, tdt;
 };
 
-__device__ __forceinline__ double surf_adj(const RhsArgs &a, int k)
-{
-    if (!a.hm[k]) return 0.0;
-    return a.s[k] + (a.tdt * a.dsdh[k]) * (a.acc[k] - a.mel[k]);
-}
-
 __global__ void k_rhs(GridDims g, RhsArgs a)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nu = g.nxu * g.nyu, nv = g.nxv * g.nyv;
     const int nx = g.nx, ny = g.ny;
     const double rg = a.ri * a.g, hri = 0.5 * a.ri, hrw = 0.5 * a.rw;
-    if (t < nu) {
-        if (!a.ui[t]) { a.bu[t] = 0.0; return; }
-        int i = t % g.nxu, j = t / g.nxu;
-        double a1 = i < nx ? surf_adj(a, i + nx * j) : 0.0;
-        double a0 = i > 0 ? surf_adj(a, (i - 1) + nx * j) : 0.0;
-        double tt = a.rdx * a1 - a.rdx * a0;
-        double f1 = (rg * a.hu[t]) * tt;
-        double m1 = i < nx ? (a.hm[i + nx * j] ? 1.0 : 0.0) : 0.0;
-        double m0 = i > 0 ? (a.hm[(i - 1) + nx * j] ? 1.0 : 0.0) : 0.0;
-        double tm = a.rdx * m1 - a.rdx * m0;
-        double h = a.hu[t], s = a.su[t];
-        double d = h - (s - a.sl);
-        if (!(d > 0.0)) d = 0.0;
-        double X = (hri * (h * h) - hrw * (d * d)) - (a.ri * h) * s;
-        double f2 = (tm * a.g) * X;
-        a.bu[t] = (f1 + f2) + (-a.f3u[t]);
-    } else if (t < nu + nv) {
-        int tv = t - nu;
-        if (!a.vi[tv]) { a.bv[tv] = 0.0; return; }
-        int i = tv % g.nxv, j = tv / g.nxv;
-        double a1 = j < ny ? surf_adj(a, i + nx * j) : 0.0;
-        double a0 = j > 0 ? surf_adj(a, i + nx * (j - 1)) : 0.0;
-        double tt = a.rdy * a1 - a.rdy * a0;
-        double f1 = (rg * a.hv[tv]) * tt;
-        double m1 = j < ny ? (a.hm[i + nx * j] ? 1.0 : 0.0) : 0.0;
-        double m0 = j > 0 ? (a.hm[i + nx * (j - 1)] ? 1.0 : 0.0) : 0.0;
-        double tm = a.rdy * m1 - a.rdy * m0;
-        double h = a.hv[tv], s = a.sv[tv];
-        double d = h - (s - a.sl);
-        if (!(d > 0.0)) d = 0.0;
-        double X = (hri * (h * h) - hrw * (d * d)) - (a.ri * h) * s;
-        double f2 = (tm * a.g) * X;
-        a.bv[tv] = (f1 + f2) + (-a.f3v[tv]);
+    if (t >= nu + nv) return;
+    // a face between h cells ka (left / below) and kb: every load issued
+    // before the first is used (ld_or), then the reference's f1, f2, f3 terms
+    const bool v = t >= nu;
+    const int tt = v ? t - nu : t;
+    const int i = v ? tt % g.nxv : tt % g.nxu, j = v ? tt / g.nxv : tt / g.nxu;
+    const bool oka = v ? j > 0 : i > 0, okb = v ? j < ny : i < nx;
+    const int ka = v ? i + nx * (j - 1) : (i - 1) + nx * j, kb = i + nx * j;
+    const bool inner = (v ? a.vi : a.ui)[tt];
+    const uint8_t ha = ld_or(oka, a.hm, ka, (uint8_t)0), hb = ld_or(okb, a.hm, kb, (uint8_t)0);
+    const double sa = ld_or(oka, a.s, ka, 0.0), sb = ld_or(okb, a.s, kb, 0.0);
+    const double da = ld_or(oka, a.dsdh, ka, 0.0), db = ld_or(okb, a.dsdh, kb, 0.0);
+    const double aa = ld_or(oka, a.acc, ka, 0.0), ab = ld_or(okb, a.acc, kb, 0.0);
+    const double ma = ld_or(oka, a.mel, ka, 0.0), mb = ld_or(okb, a.mel, kb, 0.0);
+    const double h = (v ? a.hv : a.hu)[tt], s = (v ? a.sv : a.su)[tt], f3 = (v ? a.f3v : a.f3u)[tt];
+    double *out = v ? a.bv : a.bu;
+    if (!inner) {
+        out[tt] = 0.0;
+        return;
     }
+    const double rd = v ? a.rdy : a.rdx;
+    // the surface adjusted for the implicit step, s + theta dt dsdh (a - m),
+    // on the two cells (0 off the mask)
+    double a1 = okb ? (hb ? sb + (a.tdt * db) * (ab - mb) : 0.0) : 0.0;
+    double a0 = oka ? (ha ? sa + (a.tdt * da) * (aa - ma) : 0.0) : 0.0;
+    double ts = rd * a1 - rd * a0;
+    double f1 = (rg * h) * ts;
+    double m1 = okb ? (hb ? 1.0 : 0.0) : 0.0;
+    double m0 = oka ? (ha ? 1.0 : 0.0) : 0.0;
+    double tm = rd * m1 - rd * m0;
+    double d = h - (s - a.sl);
+    if (!(d > 0.0)) d = 0.0;
+    double X = (hri * (h * h) - hrw * (d * d)) - (a.ri * h) * s;
+    double f2 = (tm * a.g) * X;
+    out[tt] = (f1 + f2) + (-f3);
 }
 
 // ===========================================================================
@@ -918,6 +931,28 @@ __global__ void k_diag(GridDims g, OpCoef<T> c, T *du, T *dv)
 // mul! takes for square factors); each output sums its terms in increasing
 // source index.
 
+// One composed row of a table-driven transform: sum over the row's n terms
+// of coef * src[col * stride], in the table's order.  The row's whole table
+// entries are loaded first (MAX_HAAR_NNZ of each; those past n are not used),
+// so that a term waits on its LDS read only, not on two table loads.
+template <typename T>
+__device__ __forceinline__ T haar_table_row(const HaarTable *tab, int dir, int m, int r, const T *src, int stride)
+{
+    const int n = tab->nnz[dir][m][r];
+    float cf[MAX_HAAR_NNZ];
+    int cl[MAX_HAAR_NNZ];
+#pragma unroll
+    for (int q = 0; q < MAX_HAAR_NNZ; ++q) {
+        cf[q] = tab->coef[dir][m][r][q];
+        cl[q] = tab->col[dir][m][r][q];
+    }
+    T acc = T(0);
+#pragma unroll
+    for (int q = 0; q < MAX_HAAR_NNZ; ++q)
+        if (q < n) acc = acc + (T)cf[q] * src[cl[q] * stride];
+    return acc;
+}
+
 template <typename T>
 __device__ __forceinline__ void haar_tile(const HaarTable *tab, int dir, T (*tile)[TILE + 1], T (*tmp)[TILE + 1],
                                           int i0, int j0, int nxw, int nyw)
@@ -931,11 +966,7 @@ __device__ __forceinline__ void haar_tile(const HaarTable *tab, int dir, T (*til
         int m = nyw - (j0 + jb);
         m = m < B ? m : B;
         int r = ly - jb;
-        int n = tab->nnz[dir][m][r];
-        T acc = T(0);
-        for (int q = 0; q < n; ++q)
-            acc = acc + (T)tab->coef[dir][m][r][q] * tile[jb + tab->col[dir][m][r][q]][lx];
-        tmp[ly][lx] = acc;
+        tmp[ly][lx] = haar_table_row<T>(tab, dir, m, r, &tile[jb][lx], TILE + 1);
     }
     __syncthreads();
     // x pass
@@ -944,11 +975,7 @@ __device__ __forceinline__ void haar_tile(const HaarTable *tab, int dir, T (*til
         int m = nxw - (i0 + ib);
         m = m < B ? m : B;
         int r = lx - ib;
-        int n = tab->nnz[dir][m][r];
-        T acc = T(0);
-        for (int q = 0; q < n; ++q)
-            acc = acc + (T)tab->coef[dir][m][r][q] * tmp[ly][ib + tab->col[dir][m][r][q]];
-        tile[ly][lx] = acc;
+        tile[ly][lx] = haar_table_row<T>(tab, dir, m, r, &tmp[ly][ib], 1);
     }
     __syncthreads();
 }
@@ -968,7 +995,8 @@ __global__ void k_tile_wavelets(GridDims g, const HaarTable *tab, const double *
     if (bx < g.nbxu && by < g.nbyu) {
         int k = i + g.nxu * j;
         bool ok = i < g.nxu && j < g.nyu;
-        tile[ly][lx] = (ok && um[k]) ? U[k] : 0.0;
+        const double x = ld_or(ok, U, k, 0.0);
+        tile[ly][lx] = (ld_or(ok, um, k, (uint8_t)0) != 0) ? x : 0.0;
         __syncthreads();
         haar_tile<double>(tab, 0, tile, tmp, i0, j0, g.nxu, g.nyu);
         if (ok) {
@@ -982,7 +1010,8 @@ __global__ void k_tile_wavelets(GridDims g, const HaarTable *tab, const double *
     if (bx < g.nbxv && by < g.nbyv) {
         int k = i + g.nxv * j;
         bool ok = i < g.nxv && j < g.nyv;
-        tile[ly][lx] = (ok && vm[k]) ? V[k] : 0.0;
+        const double x = ld_or(ok, V, k, 0.0);
+        tile[ly][lx] = (ld_or(ok, vm, k, (uint8_t)0) != 0) ? x : 0.0;
         __syncthreads();
         haar_tile<double>(tab, 0, tile, tmp, i0, j0, g.nxv, g.nyv);
         if (ok) {
@@ -1060,32 +1089,6 @@ __device__ __forceinline__ T haar_pm_row_sel(int levels, uint32_t nz, uint32_t n
     }
 }
 
-// tile[ly][lx] <- (Wy (x) Wx) tile with W the inverse (dir 0) or its
-// transpose (dir 1): y pass into tmp, x pass back into tile.
-template <typename T>
-__device__ __forceinline__ void haar_pm_tile(int levels, int dir, T (*tile)[TILE + 1], T (*tmp)[TILE + 1], int i0,
-                                             int j0, int nxw, int nyw)
-{
-    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int B = 2 << levels;
-    const int i = i0 + lx, j = j0 + ly;
-    if (i < nxw && j < nyw) {
-        const int jb = ly & ~(B - 1);
-        const int m = min(nyw - (j0 + jb), B);
-        const int r = ly - jb;
-        tmp[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][r], c_haar[levels].neg[dir][m][r], &tile[jb][lx],
-                                     TILE + 1);
-    }
-    __syncthreads();
-    if (i < nxw && j < nyw) {
-        const int ib = lx & ~(B - 1);
-        const int m = min(nxw - (i0 + ib), B);
-        const int r = lx - ib;
-        tile[ly][lx] = haar_pm_row<T>(c_haar[levels].nz[dir][m][r], c_haar[levels].neg[dir][m][r], &tmp[ly][ib], 1);
-    }
-    __syncthreads();
-}
-
 // The row masks one thread reads in the two passes of a transform of the u
 // and the v tile of block (i0, j0), all loaded at once (so that they can be
 // loaded with a kernel's other inputs instead of one round trip per pass).
@@ -1157,17 +1160,6 @@ __device__ __forceinline__ void haar_pm_tile2(int levels, int dir, T (*tu)[TILE 
 {
     const HaarRowMasks hm = haar_row_masks(levels, dir, i0, j0, g, has_u, has_v);
     haar_pm_tile2m<T, LD>(levels, hm, tu, mu, tv, mv, i0, j0, g, has_u, has_v);
-}
-
-// p[k] where ok, else dflt, with the load issued unconditionally (at index 0
-// where !ok; every array here holds at least one element, the grid being at
-// least 2 x 2): a guarded load becomes a branch with a wait of its own, so
-// that a launch's loads would go out one round trip after another.
-template <typename V>
-__device__ __forceinline__ V ld_or(bool ok, const V *p, int k, V dflt)
-{
-    const V v = p[ok ? k : 0];
-    return ok ? v : dflt;
 }
 
 // ===========================================================================
@@ -2076,7 +2068,7 @@ __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
         a.hflag->iters = 0;
     }
     int bx, by;
-    const int tile_id = band_tile(g, L, bx, by);
+    band_tile(g, L, bx, by);
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
     const int i0 = bx * TILE, j0 = by * TILE;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
@@ -2133,7 +2125,7 @@ __device__ __forceinline__ bool cg_dir_tile(const CGArgs<T> &a, int par, const i
     const GridDims &g = a.g;
     if (L >= g.nbx * g.nby) return false;
     int bx, by;
-    const int tile_id = band_tile(g, L, bx, by);
+    band_tile(g, L, bx, by);
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     const int i0 = bx * TILE, j0 = by * TILE;
@@ -2232,7 +2224,7 @@ __device__ __forceinline__ void cg_apply_tile(const CGArgs<T> &a, const int L)
     if (L >= g.nbx * g.nby) return;
     if (EARLY && a.st->done) return;
     int bx, by;
-    const int tile_id = band_tile(g, L, bx, by);
+    band_tile(g, L, bx, by);
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
     const int i0 = bx * TILE, j0 = by * TILE;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
@@ -2273,7 +2265,7 @@ __device__ __forceinline__ void cg_update_tile(const CGArgs<T> &a, int par, cons
     if (EARLY && a.st->done) return;
     const bool stopped = !EARLY && a.st->done != 0;
     int bx, by;
-    const int tile_id = band_tile(g, L, bx, by);
+    band_tile(g, L, bx, by);
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     const int i = bx * TILE + lx, j = by * TILE + ly;
@@ -2559,30 +2551,40 @@ __global__ void k_cgm_pack(GridDims g, OpCoef<float> c, const uint8_t *mwu, cons
     const int nh = g.nx * g.ny, nu = g.nxu * g.nyu, nv = g.nxv * g.nyv;
     float *F = Lo.F;
     const unsigned ns = Lo.ns;
-    if (k < nh) {
-        const int a = k % g.nx, b = k / g.nx;
-        float hc = 0.f;
-        if (a < g.nx - 1 && b < g.ny - 1) {
-            const int kc = a + (g.nx - 1) * b;
-            hc = c.cm[kc] ? c.heta_c[kc] : 0.f;
-        }
+    // every load first (ld_or): the stores into the arena come after, since
+    // the compiler cannot move a load past them
+    const bool kh = k < nh, ku = k < nu, kv = k < nv;
+    const int a = k % g.nx, b = k / g.nx;
+    const bool okc = kh && a < g.nx - 1 && b < g.ny - 1;
+    const int kc = a + (g.nx - 1) * b;
+    const bool cmc = ld_or(okc, c.cm, kc, (uint8_t)0) != 0, hmk = ld_or(kh, c.hm, k, (uint8_t)0) != 0;
+    const float hcc = ld_or(okc, c.heta_c, kc, 0.f), hhk = ld_or(kh, c.heta_h, k, 0.f), dpk = ld_or(kh, c.dimp, k, 0.f);
+    const bool umk = ld_or(ku, c.um, k, (uint8_t)0) != 0, uik = ld_or(ku, c.ui, k, (uint8_t)0) != 0;
+    const bool mwuk = ld_or(ku, mwu, k, (uint8_t)0) != 0;
+    const float huk = ld_or(ku, c.hu, k, 0.f), buk = ld_or(ku, c.bu, k, 0.f);
+    const bool vmk = ld_or(kv, c.vm, k, (uint8_t)0) != 0, vik = ld_or(kv, c.vi, k, (uint8_t)0) != 0;
+    const bool mwvk = ld_or(kv, mwv, k, (uint8_t)0) != 0;
+    const float hvk = ld_or(kv, c.hv, k, 0.f), bvk = ld_or(kv, c.bv, k, 0.f);
+    const double dzu = dinvu ? ld_or(ku, dinvu, k, 0.0) : 0.0, dzv = dinvv ? ld_or(kv, dinvv, k, 0.0) : 0.0;
+    if (kh) {
+        const float hc = okc ? (cmc ? hcc : 0.f) : 0.f;
         const unsigned o = Lo.at(a, b);
-        reinterpret_cast<float2 *>(F + CGM_CELL * ns)[o] = make_float2(c.hm[k] ? c.heta_h[k] : 0.f, c.dimp[k]);
+        reinterpret_cast<float2 *>(F + CGM_CELL * ns)[o] = make_float2(hmk ? hhk : 0.f, dpk);
         F[CGM_CORNER * ns + o] = hc;
     }
-    if (k < nu) {
+    if (ku) {
         const unsigned o = Lo.at(k % g.nxu, k / g.nxu);
-        F[CGM_HU * ns + o] = c.um[k] ? c.hu[k] : 0.f;
-        F[CGM_BU * ns + o] = c.um[k] ? c.bu[k] : 0.f;
-        Lo.M[o] = (uint8_t)((c.ui[k] ? 1 : 0) | (mwu[k] ? 2 : 0));
-        if (dinvu) F[CGM_DINV * ns + o] = (float)fmin(dinvu[k], 1e30);  // finite in FP32
+        F[CGM_HU * ns + o] = umk ? huk : 0.f;
+        F[CGM_BU * ns + o] = umk ? buk : 0.f;
+        Lo.M[o] = (uint8_t)((uik ? 1 : 0) | (mwuk ? 2 : 0));
+        if (dinvu) F[CGM_DINV * ns + o] = (float)fmin(dzu, 1e30);  // finite in FP32
     }
-    if (k < nv) {
+    if (kv) {
         const unsigned o = Lo.at(k % g.nxv, k / g.nxv);
-        F[(CGM_HU + 1) * ns + o] = c.vm[k] ? c.hv[k] : 0.f;
-        F[(CGM_BU + 1) * ns + o] = c.vm[k] ? c.bv[k] : 0.f;
-        Lo.M[ns + o] = (uint8_t)((c.vi[k] ? 1 : 0) | (mwv[k] ? 2 : 0));
-        if (dinvv) F[(CGM_DINV + 1) * ns + o] = (float)fmin(dinvv[k], 1e30);  // finite in FP32
+        F[(CGM_HU + 1) * ns + o] = vmk ? hvk : 0.f;
+        F[(CGM_BU + 1) * ns + o] = vmk ? bvk : 0.f;
+        Lo.M[ns + o] = (uint8_t)((vik ? 1 : 0) | (mwvk ? 2 : 0));
+        if (dinvv) F[(CGM_DINV + 1) * ns + o] = (float)fmin(dzv, 1e30);  // finite in FP32
     }
 }
 
@@ -3234,34 +3236,64 @@ __global__ void __launch_bounds__(256) k_cgm_iter_batch(CGMBatch B)
 }
 
 // out = mask_out ? H(mask_in ? in : 0) : 0 on the u grid (comp 0) and v grid
-template <typename Tin, typename Tout>
+// mask[k] != 0 at a point ok of a tile, true there without a mask (mask ==
+// nullptr), false where !ok.  The load is unconditional (a branch on the
+// pointer would give it a wait of its own): without a mask it reads byte 0
+// of `any`, a valid buffer, and ignores it.
+__device__ __forceinline__ bool mask_or_all(const uint8_t *mask, const void *any, bool ok, int k)
+{
+    const bool has = mask != nullptr;
+    const uint8_t *p = has ? mask : static_cast<const uint8_t *>(any);
+    const uint8_t b = p[(ok & has) ? k : 0];
+    return ok & (!has | (b != 0));  // no short circuit, which would branch around the load
+}
+
+template <typename Tin, typename Tout, bool LD = false>
 __global__ void k_tile_haar(GridDims g, const HaarTable *tab, int levels, int dir,
                             const Tin *inu, const Tin *inv, const uint8_t *min_u, const uint8_t *min_v,
                             Tout *outu, Tout *outv, const uint8_t *mout_u, const uint8_t *mout_v)
 {
     __shared__ Tout tile[TILE][TILE + 1];
     __shared__ Tout tmp[TILE][TILE + 1];
+    __shared__ Tout tile_v[TILE][TILE + 1];
+    __shared__ Tout tmp_v[TILE][TILE + 1];
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     const int bx = blockIdx.x, by = blockIdx.y;
     const int i0 = bx * TILE, j0 = by * TILE;
     const int i = i0 + lx, j = j0 + ly;
-    if (bx < g.nbxu && by < g.nbyu) {
+    const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
+    if (dir != 0) {
+        // the +-1 transforms: the u and the v tile together, every input,
+        // mask and row mask loaded first (ld_or, haar_row_masks)
+        const int ku = i + g.nxu * j, kv = i + g.nxv * j;
+        const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
+        const Tin xu = ld_or(oku, inu, ku, Tin(0)), xv = ld_or(okv, inv, kv, Tin(0));
+        const bool iu = mask_or_all(min_u, inu, oku, ku), iv = mask_or_all(min_v, inv, okv, kv);
+        const bool ou = mask_or_all(mout_u, inu, oku, ku), ov = mask_or_all(mout_v, inv, okv, kv);
+        const HaarRowMasks hm = haar_row_masks(levels, dir - 1, i0, j0, g, has_u, has_v);
+        tile[ly][lx] = (oku && iu) ? (Tout)xu : Tout(0);
+        tile_v[ly][lx] = (okv && iv) ? (Tout)xv : Tout(0);
+        __syncthreads();
+        haar_pm_tile2m<Tout, LD>(levels, hm, tile, tmp, tile_v, tmp_v, i0, j0, g, has_u, has_v);
+        if (oku) outu[ku] = ou ? tile[ly][lx] : Tout(0);
+        if (okv) outv[kv] = ov ? tile_v[ly][lx] : Tout(0);
+        return;
+    }
+    if (has_u) {
         int k = i + g.nxu * j;
         bool ok = i < g.nxu && j < g.nyu;
         tile[ly][lx] = (ok && (!min_u || min_u[k])) ? (Tout)inu[k] : Tout(0);
         __syncthreads();
-        if (dir == 0) haar_tile<Tout>(tab, 0, tile, tmp, i0, j0, g.nxu, g.nyu);
-        else haar_pm_tile<Tout>(levels, dir - 1, tile, tmp, i0, j0, g.nxu, g.nyu);
+        haar_tile<Tout>(tab, 0, tile, tmp, i0, j0, g.nxu, g.nyu);
         if (ok) outu[k] = (!mout_u || mout_u[k]) ? tile[ly][lx] : Tout(0);
         __syncthreads();
     }
-    if (bx < g.nbxv && by < g.nbyv) {
+    if (has_v) {
         int k = i + g.nxv * j;
         bool ok = i < g.nxv && j < g.nyv;
         tile[ly][lx] = (ok && (!min_v || min_v[k])) ? (Tout)inv[k] : Tout(0);
         __syncthreads();
-        if (dir == 0) haar_tile<Tout>(tab, 0, tile, tmp, i0, j0, g.nxv, g.nyv);
-        else haar_pm_tile<Tout>(levels, dir - 1, tile, tmp, i0, j0, g.nxv, g.nyv);
+        haar_tile<Tout>(tab, 0, tile, tmp, i0, j0, g.nxv, g.nyv);
         if (ok) outv[k] = (!mout_v || mout_v[k]) ? tile[ly][lx] : Tout(0);
         __syncthreads();
     }
@@ -3279,24 +3311,30 @@ __global__ void k_post_solve(GridDims g, const uint8_t *hm, const uint8_t *um, c
     if (k >= g.nx * g.ny) return;
     const int i = k % g.nx, j = k / g.nx;
     const int nxu = g.nxu, nx = g.nx;
-    double u_ = 0.5 * U[i + nxu * j] + 0.5 * U[(i + 1) + nxu * j];
-    double v_ = 0.5 * V[i + nx * j] + 0.5 * V[i + nx * (j + 1)];
+    // every load issued before the first store (which the compiler cannot
+    // move them past) and before the mask is tested
+    const int u1 = (i + 1) + nxu * j, u0 = i + nxu * j, v1 = i + nx * (j + 1), v0 = i + nx * j;
+    const double U0 = U[u0], U1 = U[u1], V0 = V[v0], V1 = V[v1];
+    const double bk = beta[k], f1k = f1[k], f2k = f2[k], acck = acc[k], melk = mel[k];
+    const double hu0 = hu[u0], hu1 = hu[u1], hv0 = hv[v0], hv1 = hv[v1];
+    const bool hk = hm[k], mu0 = um[u0], mu1 = um[u1], mv0 = vm[v0], mv1 = vm[v1];
+    double u_ = 0.5 * U0 + 0.5 * U1;
+    double v_ = 0.5 * V0 + 0.5 * V1;
     uh[k] = u_;
     vh[k] = v_;
-    double ub = u_ / (1.0 + beta[k] * f2[k]);
-    double vb = v_ / (1.0 + beta[k] * f2[k]);
+    double ub = u_ / (1.0 + bk * f2k);
+    double vb = v_ / (1.0 + bk * f2k);
     ubo[k] = ub;
     vbo[k] = vb;
-    uso[k] = ub * (1.0 + beta[k] * f1[k]);
-    vso[k] = vb * (1.0 + beta[k] * f1[k]);
-    if (hm[k]) {
-        int u1 = (i + 1) + nxu * j, u0 = i + nxu * j, v1 = i + nx * (j + 1), v0 = i + nx * j;
-        double q1 = um[u1] ? hu[u1] * U[u1] : 0.0;
-        double q0 = um[u0] ? hu[u0] * U[u0] : 0.0;
-        double p1 = vm[v1] ? hv[v1] * V[v1] : 0.0;
-        double p0 = vm[v0] ? hv[v0] * V[v0] : 0.0;
+    uso[k] = ub * (1.0 + bk * f1k);
+    vso[k] = vb * (1.0 + bk * f1k);
+    if (hk) {
+        double q1 = mu1 ? hu1 * U1 : 0.0;
+        double q0 = mu0 ? hu0 * U0 : 0.0;
+        double p1 = mv1 ? hv1 * V1 : 0.0;
+        double p0 = mv0 ? hv0 * V0 : 0.0;
         double div = (rdx * q1 - rdx * q0) + (rdy * p1 - rdy * p0);
-        dhdt[k] = (acc[k] - mel[k]) - div;
+        dhdt[k] = (acck - melk) - div;
     }
 }
 
