@@ -1299,18 +1299,21 @@ static bool haar_loads_first(const wavi_ctx *ctx)
 
 // A +-1 Haar transform (dir 1: P, 2: R) of the u and v vectors on the tiles,
 // with the Haar rows loads first where haar_loads_first says so.
+// (accu != nullptr: the result added into accu, accv at the output masks'
+// points instead, outu, outv untouched)
 template <typename T>
 static void launch_tile_haar(wavi_ctx *ctx, int dir, const T *inu, const T *inv, const uint8_t *min_u,
-                             const uint8_t *min_v, T *outu, T *outv, const uint8_t *mout_u, const uint8_t *mout_v)
+                             const uint8_t *min_v, T *outu, T *outv, const uint8_t *mout_u, const uint8_t *mout_v,
+                             T *accu = nullptr, T *accv = nullptr)
 {
     const GridDims &g = ctx->g;
     const dim3 tg(g.nbx, g.nby);
     if (haar_loads_first(ctx))
         hipLaunchKernelGGL((k_tile_haar<T, T, true>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, dir,
-                           inu, inv, min_u, min_v, outu, outv, mout_u, mout_v);
+                           inu, inv, min_u, min_v, outu, outv, mout_u, mout_v, accu, accv);
     else
         hipLaunchKernelGGL((k_tile_haar<T, T, false>), tg, dim3(256), 0, ctx->stream, g, ctx->tab, ctx->S.levels, dir,
-                           inu, inv, min_u, min_v, outu, outv, mout_u, mout_v);
+                           inu, inv, min_u, min_v, outu, outv, mout_u, mout_v, accu, accv);
 }
 
 // coarse CG on b_c held in w.cr; solution in the warm start w.xc
@@ -1666,9 +1669,7 @@ static int vcycle_post(wavi_ctx *ctx, const VSet<T> &w)
     const GridDims &g = ctx->g;
     dim3 tg(g.nbx, g.nby);
     // e += P x_c
-    launch_tile_haar<T>(ctx, 1, w.xc.u, w.xc.v, ctx->mwu, ctx->mwv, w.t.u, w.t.v, ctx->ui, ctx->vi);
-    hipLaunchKernelGGL(k_add_masked_uv<T>, flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream, w.e.u, w.t.u,
-                       ctx->ui, ctx->nu, w.e.v, w.t.v, ctx->vi, ctx->nv);
+    launch_tile_haar<T>(ctx, 1, w.xc.u, w.xc.v, ctx->mwu, ctx->mwv, w.t.u, w.t.v, ctx->ui, ctx->vi, w.e.u, w.e.v);
     CHK(do_smoother(ctx, w, rev));
     HIPCHK(hipGetLastError());
     return WAVI_OK;
@@ -1760,15 +1761,13 @@ static int precondition_resid(wavi_ctx *ctx)
     hipLaunchKernelGGL((k_select_uv<double, double>), flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream,
                        ctx->f[WAVI_U_U], ctx->ui, ctx->x.u, ctx->nu, ctx->f[WAVI_V_V], ctx->vi, ctx->x.v, ctx->nv);
     CHK(do_rhs(ctx));
+    // r = b - A x, with set_residual! (update_velocities.jl:428-433): the
+    // residual fields at inner points
     hipLaunchKernelGGL(k_tile_resid<double>, dim3(band_blocks(g.nbx, g.nby)), dim3(256), 0, ctx->stream, g,
                        coef64(ctx), ctx->x.u, ctx->x.v, ctx->b.u, ctx->b.v, ctx->r.u, ctx->r.v, ctx->Pr0, ctx->Pb,
-                       band_blocks(g.nbx, g.nby));
+                       band_blocks(g.nbx, g.nby), ctx->f[WAVI_U_RESIDUAL], ctx->f[WAVI_V_RESIDUAL]);
     hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, ctx->stream, ctx->Pr0, ctx->Pb, g.npart,
                        ctx->d_norms);
-    // set_residual! (update_velocities.jl:428-433): residual fields at inner points
-    hipLaunchKernelGGL(k_copy_masked_uv<double>, flat_uv(ctx->nu, ctx->nv), dim3(256), 0, ctx->stream,
-                       ctx->f[WAVI_U_RESIDUAL], ctx->r.u, ctx->ui, ctx->nu, ctx->f[WAVI_V_RESIDUAL], ctx->r.v, ctx->vi,
-                       ctx->nv);
     HIPCHK(hipGetLastError());
     return WAVI_OK;
 }

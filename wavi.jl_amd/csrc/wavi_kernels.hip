@@ -1425,20 +1425,16 @@ __device__ __forceinline__ void point_prefetch_op(const GridDims &g, const OpCoe
     pc.okv = has_v && i < g.nxv && j < g.nyv;
     pc.ku = i + g.nxu * j;
     pc.kv = i + g.nxv * j;
-    pc.ui = pc.um = pc.vi = pc.vm = pc.mwu = pc.mwv = false;
-    pc.bu = pc.hu = pc.bv = pc.hv = T(0);
-    if (pc.oku) {
-        pc.ui = c.ui[pc.ku];
-        pc.um = c.um[pc.ku];
-        pc.bu = c.bu[pc.ku];
-        pc.hu = c.hu[pc.ku];
-    }
-    if (pc.okv) {
-        pc.vi = c.vi[pc.kv];
-        pc.vm = c.vm[pc.kv];
-        pc.bv = c.bv[pc.kv];
-        pc.hv = c.hv[pc.kv];
-    }
+    pc.mwu = pc.mwv = false;
+    // loads issued unconditionally (ld_or)
+    pc.ui = ld_or(pc.oku, c.ui, pc.ku, (uint8_t)0) != 0;
+    pc.um = ld_or(pc.oku, c.um, pc.ku, (uint8_t)0) != 0;
+    pc.bu = ld_or(pc.oku, c.bu, pc.ku, T(0));
+    pc.hu = ld_or(pc.oku, c.hu, pc.ku, T(0));
+    pc.vi = ld_or(pc.okv, c.vi, pc.kv, (uint8_t)0) != 0;
+    pc.vm = ld_or(pc.okv, c.vm, pc.kv, (uint8_t)0) != 0;
+    pc.bv = ld_or(pc.okv, c.bv, pc.kv, T(0));
+    pc.hv = ld_or(pc.okv, c.hv, pc.kv, T(0));
 }
 
 // Stage the tile's window of x and its intermediates (ends with a barrier).
@@ -1477,10 +1473,13 @@ __global__ void __launch_bounds__(256) k_op(GridDims g, OpCoef<T> c, const T *xu
 
 // r = b - A x at inner points (get_resid!, utilities.jl:549-552); per-tile
 // partials of sum r^2 into Pr and, if Pb != nullptr, of sum b^2 into Pb (u
-// tiles first, then v tiles).
+// tiles first, then v tiles); if fu != nullptr, r also into fu, fv at the
+// inner points (the residual fields of set_residual!,
+// update_velocities.jl:428-433; their other points are left as they are).
 template <typename T>
 __global__ void __launch_bounds__(256) k_tile_resid(GridDims g, OpCoef<T> c, const T *xu, const T *xv, const T *bu,
-                                                    const T *bv, T *ru, T *rv, double *Pr, double *Pb, int nblk)
+                                                    const T *bv, T *ru, T *rv, double *Pr, double *Pb, int nblk,
+                                                    T *fu = nullptr, T *fv = nullptr)
 {
     __shared__ OpTile<T> s;
     __shared__ double lds[8];
@@ -1490,13 +1489,14 @@ __global__ void __launch_bounds__(256) k_tile_resid(GridDims g, OpCoef<T> c, con
     const int i0 = bx * TILE, j0 = by * TILE, lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     PointCoef<T> pc;
     point_prefetch_op(g, c, i0 + lx, j0 + ly, has_u, has_v, pc);
-    const T b_u = pc.oku ? bu[pc.ku] : T(0), b_v = pc.okv ? bv[pc.kv] : T(0);
+    const T b_u = ld_or(pc.oku, bu, pc.ku, T(0)), b_v = ld_or(pc.okv, bv, pc.kv, T(0));
     optile_prepare(g, c, xu, xv, i0, j0, s);
     if (has_u) {
         double r2 = 0.0, b2 = 0.0;
         if (pc.oku) {
             const T r = pc.ui ? b_u - optile_u(c, s, pc, lx, ly) : T(0);
             ru[pc.ku] = r;
+            if (fu && pc.ui) fu[pc.ku] = r;
             r2 = (double)r * (double)r;
             b2 = (double)b_u * (double)b_u;
         }
@@ -1511,6 +1511,7 @@ __global__ void __launch_bounds__(256) k_tile_resid(GridDims g, OpCoef<T> c, con
         if (pc.okv) {
             const T r = pc.vi ? b_v - optile_v(c, s, pc, lx, ly) : T(0);
             rv[pc.kv] = r;
+            if (fv && pc.vi) fv[pc.kv] = r;
             r2 = (double)r * (double)r;
             b2 = (double)b_v * (double)b_v;
         }
@@ -3248,10 +3249,16 @@ __device__ __forceinline__ bool mask_or_all(const uint8_t *mask, const void *any
     return ok & (!has | (b != 0));  // no short circuit, which would branch around the load
 }
 
+// Haar transform (dir 0 forward, 1 inverse, 2 transposed inverse) of the
+// masked u and v inputs into the masked outputs.  With accu != nullptr (a
+// +-1 transform only) the result is added into accu, accv at the points of
+// the output masks instead (x += P y, as an add of the masked output would
+// do), and outu, outv are not written.
 template <typename Tin, typename Tout, bool LD = false>
 __global__ void k_tile_haar(GridDims g, const HaarTable *tab, int levels, int dir,
                             const Tin *inu, const Tin *inv, const uint8_t *min_u, const uint8_t *min_v,
-                            Tout *outu, Tout *outv, const uint8_t *mout_u, const uint8_t *mout_v)
+                            Tout *outu, Tout *outv, const uint8_t *mout_u, const uint8_t *mout_v,
+                            Tout *accu = nullptr, Tout *accv = nullptr)
 {
     __shared__ Tout tile[TILE][TILE + 1];
     __shared__ Tout tmp[TILE][TILE + 1];
@@ -3271,12 +3278,21 @@ __global__ void k_tile_haar(GridDims g, const HaarTable *tab, int levels, int di
         const bool iu = mask_or_all(min_u, inu, oku, ku), iv = mask_or_all(min_v, inv, okv, kv);
         const bool ou = mask_or_all(mout_u, inu, oku, ku), ov = mask_or_all(mout_v, inv, okv, kv);
         const HaarRowMasks hm = haar_row_masks(levels, dir - 1, i0, j0, g, has_u, has_v);
+        // the accumulation target's values, loaded with the inputs (through
+        // outu, outv, valid buffers, when there is none)
+        const bool acc = accu != nullptr;
+        const Tout au = ld_or(oku, acc ? accu : outu, ku, Tout(0)), av = ld_or(okv, acc ? accv : outv, kv, Tout(0));
         tile[ly][lx] = (oku && iu) ? (Tout)xu : Tout(0);
         tile_v[ly][lx] = (okv && iv) ? (Tout)xv : Tout(0);
         __syncthreads();
         haar_pm_tile2m<Tout, LD>(levels, hm, tile, tmp, tile_v, tmp_v, i0, j0, g, has_u, has_v);
-        if (oku) outu[ku] = ou ? tile[ly][lx] : Tout(0);
-        if (okv) outv[kv] = ov ? tile_v[ly][lx] : Tout(0);
+        if (acc) {
+            if (oku && ou) accu[ku] = au + tile[ly][lx];
+            if (okv && ov) accv[kv] = av + tile_v[ly][lx];
+        } else {
+            if (oku) outu[ku] = ou ? tile[ly][lx] : Tout(0);
+            if (okv) outv[kv] = ov ? tile_v[ly][lx] : Tout(0);
+        }
         return;
     }
     if (has_u) {
