@@ -2112,6 +2112,20 @@ __global__ void __launch_bounds__(256) k_cg_setup(CGArgs<T> a)
     }
 }
 
+// A CG scalar written by an earlier kernel, read as a vector load: a scalar
+// load of it comes back under the same wait as the kernel's arguments, which
+// the launch's later vector loads (the partials) need first, so its round
+// trip would come before theirs.  The index is a zero the compiler cannot
+// see through (a register move, not a load), which makes the load a vector
+// load, waited for where its value is used.
+template <typename V>
+__device__ __forceinline__ V cg_scalar(const V *p)
+{
+    int z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return p[z];
+}
+
 // part 1 (iteration parity par): residual norm and stop test, then
 // u = r + beta u with beta = res^2 / prev^2 (PRE: u = D^-1 r + beta u with
 // beta = rz / prev rz), t = P u.  The stop decision is
@@ -2134,8 +2148,8 @@ __device__ __forceinline__ bool cg_dir_tile(const CGArgs<T> &a, int par, const i
     const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
     const int ku = i + g.nxu * j, kv = i + g.nxv * j;
     // independent loads first
-    const int it = a.st->it[par];
-    const double told = a.st->tol, prev_res = a.st->res[1 - par];
+    const int it = cg_scalar(&a.st->it[par]);
+    const double told = cg_scalar(&a.st->tol), prev_res = cg_scalar(&a.st->res[1 - par]);
     const T ru = ld_or(oku, a.ru, ku, T(0)), pu = ld_or(oku, a.pu, ku, T(0));
     const T rv = ld_or(okv, a.rv, kv, T(0)), pv = ld_or(okv, a.pv, kv, T(0));
     const bool mwu = ld_or(oku, a.mwu, ku, (uint8_t)0) != 0, mwv = ld_or(okv, a.mwv, kv, (uint8_t)0) != 0;
@@ -2145,7 +2159,7 @@ __device__ __forceinline__ bool cg_dir_tile(const CGArgs<T> &a, int par, const i
     if constexpr (PRE) {
         dzu = ld_or(oku, a.dinvu, ku, 0.0);
         dzv = ld_or(okv, a.dinvv, kv, 0.0);
-        prev_rz = a.st->rz[1 - par];
+        prev_rz = cg_scalar(&a.st->rz[1 - par]);
     }
     double res;
     if constexpr (PRE) {
@@ -2264,7 +2278,7 @@ __device__ __forceinline__ void cg_update_tile(const CGArgs<T> &a, int par, cons
     const GridDims &g = a.g;
     if (L >= g.nbx * g.nby) return;
     if (EARLY && a.st->done) return;
-    const bool stopped = !EARLY && a.st->done != 0;
+    const bool stopped = !EARLY && cg_scalar(&a.st->done) != 0;
     int bx, by;
     band_tile(g, L, bx, by);
     const bool has_u = bx < g.nbxu && by < g.nbyu, has_v = bx < g.nbxv && by < g.nbyv;
@@ -2272,8 +2286,8 @@ __device__ __forceinline__ void cg_update_tile(const CGArgs<T> &a, int par, cons
     const int i = bx * TILE + lx, j = by * TILE + ly;
     const bool oku = has_u && i < g.nxu && j < g.nyu, okv = has_v && i < g.nxv && j < g.nyv;
     const int ku = i + g.nxu * j, kv = i + g.nxv * j;
-    const int it = a.st->it[par];
-    const double res = a.st->res[par];
+    const int it = cg_scalar(&a.st->it[par]);
+    const double res = cg_scalar(&a.st->res[par]);
     const T xu = ld_or(oku, a.xu, ku, T(0)), pu = ld_or(oku, a.pu, ku, T(0)), ru = ld_or(oku, a.ru, ku, T(0)),
             qu = ld_or(oku, a.qu, ku, T(0));
     const T xv = ld_or(okv, a.xv, kv, T(0)), pv = ld_or(okv, a.pv, kv, T(0)), rv = ld_or(okv, a.rv, kv, T(0)),
@@ -2282,7 +2296,7 @@ __device__ __forceinline__ void cg_update_tile(const CGArgs<T> &a, int par, cons
     if constexpr (PRE) {
         dzu = ld_or(oku, a.dinvu, ku, 0.0);
         dzv = ld_or(okv, a.dinvv, kv, 0.0);
-        rzc = a.st->rz[par];
+        rzc = cg_scalar(&a.st->rz[par]);
     }
     const double dot = lanes_sum(a.Pd, g.npart, lds);
     T alpha;
