@@ -3292,10 +3292,14 @@ __global__ void k_tile_haar(GridDims g, const HaarTable *tab, int levels, int di
         const bool iu = mask_or_all(min_u, inu, oku, ku), iv = mask_or_all(min_v, inv, okv, kv);
         const bool ou = mask_or_all(mout_u, inu, oku, ku), ov = mask_or_all(mout_v, inv, okv, kv);
         const HaarRowMasks hm = haar_row_masks(levels, dir - 1, i0, j0, g, has_u, has_v);
-        // the accumulation target's values, loaded with the inputs (through
-        // outu, outv, valid buffers, when there is none)
+        // the accumulation target's values, loaded with the inputs (a uniform
+        // branch: their first use is after the transform)
         const bool acc = accu != nullptr;
-        const Tout au = ld_or(oku, acc ? accu : outu, ku, Tout(0)), av = ld_or(okv, acc ? accv : outv, kv, Tout(0));
+        Tout au = Tout(0), av = Tout(0);
+        if (acc) {
+            au = ld_or(oku, accu, ku, Tout(0));
+            av = ld_or(okv, accv, kv, Tout(0));
+        }
         tile[ly][lx] = (oku && iu) ? (Tout)xu : Tout(0);
         tile_v[ly][lx] = (okv && iv) ? (Tout)xv : Tout(0);
         __syncthreads();
