@@ -372,6 +372,25 @@ def test_whole_grid_smoother_bit_equal_to_oracle(W, oracle, nx, ny):
     assert (tx.value, ty.value) == (-1, -1)
 
 
+def test_whole_grid_smoother_ragged_domain_bit_equal_to_oracle(W, oracle):
+    """The whole-grid smoother on the irregular domain (moving calving front,
+    interior holes, interior Dirichlet points; 40 x 24): a 3-Picard FP64
+    solve gives the oracle's bits."""
+    _step("whole-grid smoother, ragged 40x24")
+    m = ragged(W, nx=40, ny=24, maxiter_picard=3)
+    o = oracle.OracleModel(m)
+    spec = _gpu_model(W, m)
+    lib = spec._lib
+    assert lib.wavi_set_smoother_tile(spec.ctx, -1, -1) == 0
+    assert lib.wavi_update_geometry(spec.ctx) == 0
+    o.update_geometry()
+    rep = W._lib.SolveReport()
+    assert lib.wavi_update_velocities(spec.ctx, ctypes.byref(rep)) == 0
+    orep = o.update_velocities()
+    assert rep.as_dict() == orep
+    _assert_fields_equal(W, m, o)
+
+
 def test_whole_grid_smoother_mixed_and_autotuned(W):
     """Mixed precision on 80 x 10: the whole-grid smoother, a tile shape and
     the autotuner's choice (which may be either) give the same bits; a grid
