@@ -1092,6 +1092,11 @@ __device__ __forceinline__ T haar_pm_row_sel(int levels, uint32_t nz, uint32_t n
 // The row masks one thread reads in the two passes of a transform of the u
 // and the v tile of block (i0, j0), all loaded at once (so that they can be
 // loaded with a kernel's other inputs instead of one round trip per pass).
+// The y pass computes element (ly, lx) = (tid >> 4, tid & 15), the x pass
+// element (tid & 15, tid >> 4): the lanes of a wave then share the row of
+// their y pass and the column of their x pass, and with it the number of
+// terms, so the long rows (16 terms in the transposed inverse at 3 levels)
+// fall in one wave of four in both passes.
 struct HaarRowMasks {
     uint32_t yz_u, yn_u, yz_v, yn_v;  // y pass: terms and signs, u and v tile
     uint32_t xz_u, xn_u, xz_v, xn_v;  // x pass
@@ -1108,24 +1113,28 @@ __device__ __forceinline__ uint32_t haar_mask_or0(bool ok, const uint16_t *p, in
 __device__ __forceinline__ HaarRowMasks haar_row_masks(int levels, int dir, int i0, int j0, const GridDims &g,
                                                        bool has_u, bool has_v)
 {
+    // the y pass's element (ly, lx)
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     const int B = 2 << levels;
     const int i = i0 + lx, j = j0 + ly;
-    const int jb = ly & ~(B - 1), ib = lx & ~(B - 1);
-    const int ry = ly - jb, rx = lx - ib;
+    const int jb = ly & ~(B - 1), ry = ly - jb;
     const bool ou = has_u && i < g.nxu && j < g.nyu, ov = has_v && i < g.nxv && j < g.nyv;
     const uint16_t *nz = &c_haar[levels].nz[dir][0][0], *ng = &c_haar[levels].neg[dir][0][0];
     const int yu = min(g.nyu - (j0 + jb), B) * 16 + ry, yv = min(g.nyv - (j0 + jb), B) * 16 + ry;
-    const int xu = min(g.nxu - (i0 + ib), B) * 16 + rx, xv = min(g.nxv - (i0 + ib), B) * 16 + rx;
+    // the x pass's element (lyx, lxx)
+    const int lyx = threadIdx.x & 15, lxx = threadIdx.x >> 4;
+    const int ix = i0 + lxx, jx = j0 + lyx, ibx = lxx & ~(B - 1), rxx = lxx - ibx;
+    const bool oux = has_u && ix < g.nxu && jx < g.nyu, ovx = has_v && ix < g.nxv && jx < g.nyv;
+    const int xux = min(g.nxu - (i0 + ibx), B) * 16 + rxx, xvx = min(g.nxv - (i0 + ibx), B) * 16 + rxx;
     HaarRowMasks m;
     m.yz_u = haar_mask_or0(ou, nz, yu);
     m.yn_u = haar_mask_or0(ou, ng, yu);
     m.yz_v = haar_mask_or0(ov, nz, yv);
     m.yn_v = haar_mask_or0(ov, ng, yv);
-    m.xz_u = haar_mask_or0(ou, nz, xu);
-    m.xn_u = haar_mask_or0(ou, ng, xu);
-    m.xz_v = haar_mask_or0(ov, nz, xv);
-    m.xn_v = haar_mask_or0(ov, ng, xv);
+    m.xz_u = haar_mask_or0(oux, nz, xux);
+    m.xn_u = haar_mask_or0(oux, ng, xux);
+    m.xz_v = haar_mask_or0(ovx, nz, xvx);
+    m.xn_v = haar_mask_or0(ovx, ng, xvx);
     return m;
 }
 
@@ -1139,16 +1148,19 @@ __device__ __forceinline__ void haar_pm_tile2m(int levels, const HaarRowMasks &h
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     const int B = 2 << levels;
     const int i = i0 + lx, j = j0 + ly;
-    const int jb = ly & ~(B - 1), ib = lx & ~(B - 1);
+    const int jb = ly & ~(B - 1);
     if (has_u && i < g.nxu && j < g.nyu)
         mu[ly][lx] = haar_pm_row_sel<T, LD>(levels, hm.yz_u, hm.yn_u, &tu[jb][lx], TILE + 1);
     if (has_v && i < g.nxv && j < g.nyv)
         mv[ly][lx] = haar_pm_row_sel<T, LD>(levels, hm.yz_v, hm.yn_v, &tv[jb][lx], TILE + 1);
     __syncthreads();
-    if (has_u && i < g.nxu && j < g.nyu)
-        tu[ly][lx] = haar_pm_row_sel<T, LD>(levels, hm.xz_u, hm.xn_u, &mu[ly][ib], 1);
-    if (has_v && i < g.nxv && j < g.nyv)
-        tv[ly][lx] = haar_pm_row_sel<T, LD>(levels, hm.xz_v, hm.xn_v, &mv[ly][ib], 1);
+    // the x pass with the transposed element mapping (haar_row_masks)
+    const int lyx = threadIdx.x & 15, lxx = threadIdx.x >> 4;
+    const int ix = i0 + lxx, jx = j0 + lyx, ibx = lxx & ~(B - 1);
+    if (has_u && ix < g.nxu && jx < g.nyu)
+        tu[lyx][lxx] = haar_pm_row_sel<T, LD>(levels, hm.xz_u, hm.xn_u, &mu[lyx][ibx], 1);
+    if (has_v && ix < g.nxv && jx < g.nyv)
+        tv[lyx][lxx] = haar_pm_row_sel<T, LD>(levels, hm.xz_v, hm.xn_v, &mv[lyx][ibx], 1);
     __syncthreads();
 }
 
