@@ -969,13 +969,15 @@ __device__ __forceinline__ void haar_tile(const HaarTable *tab, int dir, T (*til
         tmp[ly][lx] = haar_table_row<T>(tab, dir, m, r, &tile[jb][lx], TILE + 1);
     }
     __syncthreads();
-    // x pass
-    if (i < nxw && j < nyw) {
-        int ib = lx - (lx % B);
+    // x pass, with the transposed element mapping of haar_pm_tile2m (the
+    // lanes of a wave share the column, so rows of one length)
+    const int lyx = threadIdx.x & 15, lxx = threadIdx.x >> 4;
+    if (i0 + lxx < nxw && j0 + lyx < nyw) {
+        int ib = lxx - (lxx % B);
         int m = nxw - (i0 + ib);
         m = m < B ? m : B;
-        int r = lx - ib;
-        tile[ly][lx] = haar_table_row<T>(tab, dir, m, r, &tmp[ly][ib], 1);
+        int r = lxx - ib;
+        tile[lyx][lxx] = haar_table_row<T>(tab, dir, m, r, &tmp[lyx][ib], 1);
     }
     __syncthreads();
 }
