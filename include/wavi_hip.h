@@ -48,7 +48,11 @@ extern "C" {
                                   * FP64 residual b - A(u)u and applies the FP32 correction.
                                   * The FP32 coarse CG's relative stop is 2^-19 ||r0|| (FP32's
                                   * attainable accuracy) in place of sqrt(eps), so its
-                                  * coarse_cg_iters can be lower than FP64's */
+                                  * coarse_cg_iters can be lower than FP64's.  Once the
+                                  * Picard residual is below 1e-7 the coarse CG starts
+                                  * from zero instead of the carried warm start
+                                  * (*_CORRECTION_COARSE), whose FP32 null-space content
+                                  * would otherwise hold the residual near 2e-9 */
 
 /* Grid: the reference's Grid(;...) keyword arguments (src/Grid.jl:66-82).
  * Masks are 0/1 bytes.  u_iszero/v_iszero remove points from the u/v masks

@@ -1790,12 +1790,36 @@ static int precondition_update(wavi_ctx *ctx)
     return WAVI_OK;
 }
 
+// The coarse warm start of a mixed-precision V-cycle in the Picard loop
+// (set/get_correction_coarse, update_preconditioners.jl:195-216): carried
+// over from the previous Picard iteration while the relative residual is at
+// least CGM_ZERO_START_REL, zero below it.  The coarse operator is singular
+// (the Haar space spans points the fine operator crops); carried in FP32,
+// the warm start gathers components in that null space, whose rounding
+// feeds back into r0 and holds the Picard residual near 2e-9 on irregular
+// domains.  A zero start has none.  Far from convergence the carried start
+// keeps the mixed solve on the FP64 one's trajectory, so it stays there.
+// FP64 solves always carry it, as the reference does.
+constexpr double CGM_ZERO_START_REL = 1e-7;
+
+static int coarse_start(wavi_ctx *ctx, double rel)
+{
+    if (ctx->precision != WAVI_PRECISION_MIXED || !(rel < CGM_ZERO_START_REL)) return WAVI_OK;
+    hipLaunchKernelGGL(k_fill<double>, flat(ctx->nu), dim3(256), 0, ctx->stream, ctx->f[WAVI_WU_CORRECTION_COARSE],
+                       0.0, ctx->nu);
+    hipLaunchKernelGGL(k_fill<double>, flat(ctx->nv), dim3(256), 0, ctx->stream, ctx->f[WAVI_WV_CORRECTION_COARSE],
+                       0.0, ctx->nv);
+    HIPCHK(hipGetLastError());
+    return WAVI_OK;
+}
+
 static int do_precondition(wavi_ctx *ctx, int *converged, double *rel, int *cg_iters)
 {
     CHK(precondition_resid(ctx));
     CHK(precondition_decide(ctx, converged, rel));
     *cg_iters = 0;
     if (!*converged) {
+        CHK(coarse_start(ctx, *rel));
         CHK(do_vcycle(ctx, cg_iters));
         CHK(precondition_update(ctx));
     }
@@ -2490,6 +2514,7 @@ static int lockstep_picard(std::vector<LockMember> &M, LockTables &R, int it, in
         for (int k : cgm) {
             *bad = k;
             wavi_ctx *ctx = M[k].ctx;
+            CHK(coarse_start(ctx, M[k].rel));
             CHK(vcycle_narrow(ctx));
             CHK(vcycle_pre(ctx, ctx->v32));
             CHK(cgm_prepare(ctx, ctx->v32));
