@@ -295,8 +295,8 @@ def test_iceberg_analytic(W, oracle):
 
 
 def _butterfly(x, levels, transposed):
-    """The lane butterflies of the mixed-precision kernels (haar_line_P /
-    haar_line_R) on a zero-padded 16-point line: pairs (o, o + h), o a
+    """The lane butterflies of the mixed-precision kernels (haar_lines_P /
+    haar_lines_R) on a zero-padded 16-point line: pairs (o, o + h), o a
     multiple of 2h; P runs h = B/2 .. 1, R runs h = 1 .. B/2."""
     v = np.zeros(16)
     v[:len(x)] = x

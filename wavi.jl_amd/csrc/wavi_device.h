@@ -200,10 +200,8 @@ __device__ __forceinline__ void st_nt(T *p, T v)
 // the wavefront must be active).
 constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1,0,3,2]: lane ^ 1
 constexpr int DPP_XOR2 = 0x4E;         // quad_perm [2,3,0,1]: lane ^ 2
-constexpr int DPP_QUAD_BCAST0 = 0x00;  // quad_perm [0,0,0,0]: lane & ~3
 constexpr int DPP_MIRROR = 0x140;      // row_mirror: lane ^ 15
 constexpr int DPP_HALF_MIRROR = 0x141; // row_half_mirror: lane ^ 7
-constexpr int DPP_ROR8 = 0x128;        // row_ror 8: lane ^ 8
 
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v)
@@ -218,17 +216,6 @@ __device__ __forceinline__ double dpp_d(double v)
     const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
     const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-// Value of the lane at row position pos ^ H (H = 1, 2, 4, 8); for H = 4 it is
-// valid in the lanes whose position is a multiple of 4.
-template <int H>
-__device__ __forceinline__ float row_partner(float v)
-{
-    if constexpr (H == 1) return dpp_f<DPP_XOR1>(v);
-    else if constexpr (H == 2) return dpp_f<DPP_XOR2>(v);
-    else if constexpr (H == 4) return dpp_f<DPP_HALF_MIRROR>(dpp_f<DPP_QUAD_BCAST0>(v));
-    else return dpp_f<DPP_ROR8>(v);
 }
 
 // Sum over a 16-lane row, left in every lane of the row.  Each stage adds a
@@ -308,14 +295,6 @@ struct OpCoef {
     const uint8_t *hm, *um, *vm, *cm, *ui, *vi;
     int nx, ny;
     T rdx, rdy;
-};
-
-template <typename T>
-struct GlobalIn {
-    const T *U, *V;
-    int nxu, nx;
-    __device__ __forceinline__ T u(int i, int j) const { return U[i + nxu * j]; }
-    __device__ __forceinline__ T v(int i, int j) const { return V[i + nx * j]; }
 };
 
 // Unit vector at one u (or v) point: the probe vector of get_op_diag seen

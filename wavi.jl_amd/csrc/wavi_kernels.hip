@@ -76,19 +76,6 @@ __global__ void k_set_velocities_uv(double *U, const double *xu, const double *e
 }
 
 template <typename T>
-__global__ void k_add_masked_uv(T *au, const T *bu, const uint8_t *mu, int nu, T *av, const T *bv,
-                                const uint8_t *mv, int nv)
-{
-    const bool v = blockIdx.y != 0;
-    T *a = v ? av : au;
-    const T *b = v ? bv : bu;
-    const uint8_t *mask = v ? mv : mu;
-    const int n = v ? nv : nu;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n && mask[k]) a[k] = a[k] + b[k];
-}
-
-template <typename T>
 __global__ void k_fill_uv(T *au, int nu, T *av, int nv, T val)
 {
     const bool v = blockIdx.y != 0;
@@ -123,35 +110,12 @@ __global__ void k_select(const Tin *in, const uint8_t *mask, Tout *out, int n)
     if (k < n) out[k] = mask[k] ? (Tout)in[k] : Tout(0);
 }
 
-// out = (Tout)in
-template <typename Tin, typename Tout>
-__global__ void k_convert(const Tin *in, Tout *out, int n)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) out[k] = (Tout)in[k];
-}
-
-// a = a + b where mask (inner points)
-template <typename T>
-__global__ void k_add_masked(T *a, const T *b, const uint8_t *mask, int n)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n && mask[k]) a[k] = a[k] + b[k];
-}
-
 // dst = src where mask
 template <typename T>
 __global__ void k_copy_masked(T *dst, const T *src, const uint8_t *mask, int n)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n && mask[k]) dst[k] = src[k];
-}
-
-// U[inner] = x + e (set_velocities! after precondition!, update_velocities.jl:79-80)
-__global__ void k_set_velocities(double *U, const double *x, const double *e, const uint8_t *inner, int n)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n && inner[k]) U[k] = x[k] + e[k];
 }
 
 // mask as 0.0/1.0 <-> uint8
@@ -1192,17 +1156,6 @@ struct OpTile {
     T ext[17][18];
     T rxy[17][18];
 };
-
-template <typename T>
-__device__ __forceinline__ void optile_load(const GridDims &g, const T *U, const T *V, int i0, int j0, OpTile<T> &s)
-{
-    for (int q = threadIdx.x; q < 18 * 18; q += 256) {
-        const int x = q % 18, y = q / 18;
-        const int i = i0 - 1 + x, j = j0 - 1 + y;
-        s.U[y][x] = (i >= 0 && i < g.nxu && j >= 0 && j < g.nyu) ? U[i + g.nxu * j] : T(0);
-        s.V[y][x] = (i >= 0 && i < g.nxv && j >= 0 && j < g.nyv) ? V[i + g.nxv * j] : T(0);
-    }
-}
 
 // Coefficients of one h cell and its c corner (the cell (i0-1+x, j0-1+y) of
 // the 17 x 17 window, q = x + 17 y), loaded before they are needed so that a
@@ -2714,22 +2667,6 @@ __global__ void __launch_bounds__(256) k_cgm_setup(CGMArgs a)
     }
 }
 
-// One stage as v + sign * partner with sign in {-1, 0, +1}: the product is
-// exact, so the fused multiply-add rounds exactly as the addition would.
-template <int H>
-__device__ __forceinline__ float haar_stage_P(float v, int pos)
-{
-    const float sg = ((pos & (H - 1)) == 0) ? ((pos & H) ? 1.f : -1.f) : 0.f;
-    return __builtin_fmaf(sg, row_partner<H>(v), v);
-}
-
-template <int H>
-__device__ __forceinline__ float haar_stage_R(float v, int pos)
-{
-    const float sg = ((pos & (H - 1)) == 0) ? ((pos & H) ? -1.f : 1.f) : 0.f;
-    return __builtin_fmaf(sg, row_partner<H>(v), v);
-}
-
 // Block Haar transforms of 16-point lines held one point per lane (pos = the
 // point's position in the line, blocks of B = 2^(LEV+1) points) as the
 // butterflies that compose wavelet_matrix (src/Wavelets/update_wavelets.jl:
@@ -2739,28 +2676,10 @@ __device__ __forceinline__ float haar_stage_R(float v, int pos)
 // Points past the end of the grid must hold zero: a pair whose e lies past
 // the end then leaves x_o unchanged, as wavelet_matrix does for a partial
 // block, and the garbage left in those points never reaches a point in range.
-template <int LEV>
-__device__ __forceinline__ float haar_line_P(float v, int pos)
-{
-    if constexpr (LEV >= 3) v = haar_stage_P<8>(v, pos);
-    if constexpr (LEV >= 2) v = haar_stage_P<4>(v, pos);
-    if constexpr (LEV >= 1) v = haar_stage_P<2>(v, pos);
-    return haar_stage_P<1>(v, pos);
-}
-
-template <int LEV>
-__device__ __forceinline__ float haar_line_R(float v, int pos)
-{
-    v = haar_stage_R<1>(v, pos);
-    if constexpr (LEV >= 1) v = haar_stage_R<2>(v, pos);
-    if constexpr (LEV >= 2) v = haar_stage_R<4>(v, pos);
-    if constexpr (LEV >= 3) v = haar_stage_R<8>(v, pos);
-    return v;
-}
-
-// The same transforms on N lines at once, each stage one v_fmac_f32 with a
-// DPP source per line (v += sign * partner(v)) instead of a DPP move and an
-// FMA.  Every block starts with two wait states: a DPP read of a register
+// Two lines at once (the u and the v tile), each stage one v_fmac_f32 with a
+// DPP source per line, v += sign * partner(v) with sign in {-1, 0, +1}: the
+// product is exact, so the fused multiply-add rounds exactly as the addition
+// would.  Every block starts with two wait states: a DPP read of a register
 // that the previous VALU instruction wrote needs them, and the compiler
 // does not see the VALU writes inside these blocks.  The H = 4 partner is
 // row_half_mirror of quad_perm [0,0,0,0], two DPP steps.
